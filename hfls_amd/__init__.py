@@ -1,0 +1,223 @@
+"""MI355X-native HTFETI hot path: Python host mirror of the reference's
+Driver/Cluster/Solver entry points over the C ABI of libhfls.so (include/hfls_gpu.h).
+
+The native library is mandatory: every call goes through the in-tree
+``libhfls.so`` (gfx950 HIP kernels); if it is missing the import of the binding
+raises instead of falling back to any CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhfls.so")
+
+EXPORTS = [
+    "hfls_last_error", "hfls_problem_create", "hfls_problem_destroy", "hfls_problem_sizes",
+    "hfls_problem_sub_info", "hfls_problem_sub_bf", "hfls_problem_sub_bdofs", "hfls_problem_sub_sset",
+    "hfls_problem_sub_l2g", "hfls_problem_sub_young", "hfls_problem_rows", "hfls_problem_dirichlet",
+    "hfls_problem_corners", "hfls_mesh_create", "hfls_mesh_destroy", "hfls_mesh_sizes",
+    "hfls_mesh_setup_times", "hfls_mesh_kernel", "hfls_mesh_load", "hfls_apply_F", "hfls_project",
+    "hfls_precondition", "hfls_kplus", "hfls_solve", "hfls_time_stream",
+]
+
+DEFAULTS = {
+    "Nx": "2", "Ny": "2", "Nz": "2", "nx": "5", "ny": "5", "nz": "5",
+    "Cx": "1", "Cy": "1", "Cz": "1",
+    "young_modulus": "1000", "ratio_mat": "1e4", "poissons_ratio": "0.3",
+    "material": "reference", "mat_sigma": "1.5", "mat_contrast": "1e3", "seed": "0",
+    "bc": "symmetry", "load": "reference", "body_force": "-1.0", "density": "2.0",
+    "typeBc": "ker", "Bc_fullRank": "true", "preconditioner": "Dirichlet_explicit",
+    "eps_iter": "1e-6", "max_iter": "200", "Ac_extended_by_kerGc": "true",
+    "slabs_x": "", "slabs_y": "", "slabs_z": "",
+    "linear_solver": "hip", "vtkWithinIter": "false", "print_matrices": "0",
+    "precision": "fp64", "nrhs": "1", "device": "0",
+}
+
+
+class HflsError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("pcpg_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libhfls.so (built by hfls_amd.build); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HflsError("libhfls.so not built: run python -m hfls_amd.build")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.hfls_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _opts(o):
+    o = {k: ("" if v is None else (",".join(str(int(x)) for x in v) if isinstance(v, (list, tuple)) else str(v)))
+         for k, v in o.items()}
+    keys = (ctypes.c_char_p * len(o))(*[k.encode() for k in o])
+    vals = (ctypes.c_char_p * len(o))(*[v.encode() for v in o.values()])
+    return len(o), keys, vals
+
+
+def _check(rc):
+    if rc != 0:
+        raise HflsError(load().hfls_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Problem:
+    """Host-side mesh, numbering and gluing (Mesh.cpp:69-289, Data.cpp:111-144,
+    Cluster.cpp:682-959) computed by the native library."""
+
+    def __init__(self, **opts):
+        lib = load()
+        n, k, v = _opts(opts)
+        self._h = ctypes.c_void_p()
+        _check(lib.hfls_problem_create(n, k, v, ctypes.byref(self._h)))
+        s = np.zeros(8, dtype=np.int64)
+        _check(lib.hfls_problem_sizes(self._h, _ptr(s)))
+        (self.nsub, self.nlam, self.n_glue, self.nPoints, self.ncl, self.nsub_cl, self.ndir,
+         self.ncorner) = (int(x) for x in s)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hfls_problem_destroy(self._h)
+            self._h = None
+
+    def sub_info(self, d):
+        info = np.zeros(8, dtype=np.int32)
+        _check(load().hfls_problem_sub_info(self._h, d, _ptr(info)))
+        return dict(zip(["n", "n_r", "n_s", "n_bs", "n_ss", "cluster", "nbf"], (int(x) for x in info[:7])))
+
+    def sub_bf(self, d):
+        m = self.sub_info(d)["nbf"]
+        rows, dofs, signs = (np.zeros(m, dtype=np.int32) for _ in range(3))
+        _check(load().hfls_problem_sub_bf(self._h, d, _ptr(rows), _ptr(dofs), _ptr(signs)))
+        return rows, dofs, signs
+
+    def sub_bdofs(self, d):
+        i = self.sub_info(d)
+        out = np.zeros(i["n_s"], dtype=np.int32)
+        _check(load().hfls_problem_sub_bdofs(self._h, d, _ptr(out)))
+        return out
+
+    def sub_sset(self, d):
+        i = self.sub_info(d)
+        out = np.zeros(i["n_ss"], dtype=np.int32)
+        _check(load().hfls_problem_sub_sset(self._h, d, _ptr(out)))
+        return out
+
+    def sub_l2g(self, d):
+        out = np.zeros(self.sub_info(d)["n"], dtype=np.int64)
+        _check(load().hfls_problem_sub_l2g(self._h, d, _ptr(out)))
+        return out
+
+    def sub_young(self, d):
+        e = ctypes.c_double()
+        _check(load().hfls_problem_sub_young(self._h, d, ctypes.byref(e)))
+        return e.value
+
+    def rows(self):
+        a, b = np.zeros(self.nlam, dtype=np.int32), np.zeros(self.nlam, dtype=np.int32)
+        dof, w = np.zeros(self.nlam, dtype=np.int64), np.zeros(self.nlam)
+        _check(load().hfls_problem_rows(self._h, _ptr(a), _ptr(b), _ptr(dof), _ptr(w)))
+        return a, b, dof, w
+
+    def dirichlet(self):
+        out = np.zeros(self.ndir, dtype=np.int64)
+        _check(load().hfls_problem_dirichlet(self._h, _ptr(out)))
+        return out
+
+    def corners(self):
+        out = np.zeros(self.ncorner, dtype=np.int64)
+        _check(load().hfls_problem_corners(self._h, _ptr(out)))
+        return out
+
+
+class Mesh:
+    """Device-resident, set-up HTFETI mesh (one GPU).  Mirrors the operator
+    methods of the reference Cluster (Cluster.hpp:106-121)."""
+
+    def __init__(self, device=0, **opts):
+        lib = load()
+        n, k, v = _opts(opts)
+        self._h = ctypes.c_void_p()
+        _check(lib.hfls_mesh_create(int(device), n, k, v, ctypes.byref(self._h)))
+        s = np.zeros(6, dtype=np.int64)
+        _check(lib.hfls_mesh_sizes(self._h, _ptr(s)))
+        self.nsub, self.nlam, self.ncol, self.npacked, self.nPoints, self.device_bytes = (int(x) for x in s)
+        t = np.zeros(2)
+        _check(lib.hfls_mesh_setup_times(self._h, _ptr(t)))
+        self.setup_ms, self.factor_ms = float(t[0]), float(t[1])
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hfls_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def kernel(self):
+        R = np.zeros(6 * self.npacked)
+        fix = np.zeros(6 * self.nsub, dtype=np.int32)
+        dfc = np.zeros(self.nsub, dtype=np.int32)
+        _check(load().hfls_mesh_kernel(self._h, _ptr(R), _ptr(fix), _ptr(dfc)))
+        return R, fix, dfc
+
+    def load_vector(self):
+        f = np.zeros(self.npacked)
+        _check(load().hfls_mesh_load(self._h, _ptr(f)))
+        return f
+
+    def mult_Ff(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = np.zeros(self.nlam)
+        _check(load().hfls_apply_F(self._h, _ptr(w), _ptr(out)))
+        return out
+
+    def projection(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        Px, alpha = np.zeros(self.nlam), np.zeros(self.ncol)
+        _check(load().hfls_project(self._h, _ptr(x), _ptr(Px), _ptr(alpha)))
+        return Px, alpha
+
+    def preconditioning(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.nlam)
+        _check(load().hfls_precondition(self._h, _ptr(x), _ptr(out)))
+        return out
+
+    def mult_Kplus_f(self, f_packed):
+        f = np.ascontiguousarray(f_packed, dtype=np.float64)
+        out = np.zeros(self.npacked)
+        _check(load().hfls_kplus(self._h, _ptr(f), _ptr(out)))
+        return out
+
+    def time_stream(self, which=0, reps=20):
+        ms, nbytes = ctypes.c_double(), ctypes.c_double()
+        _check(load().hfls_time_stream(self._h, int(which), int(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
+        return ms.value, nbytes.value
+
+    def pcpg(self, eps=1e-8, max_iter=200, f_packed=None, want_lambda=True):
+        u = np.zeros(3 * self.nPoints)
+        lam = np.zeros(self.nlam) if want_lambda else None
+        st = Stats()
+        f = None if f_packed is None else np.ascontiguousarray(f_packed, dtype=np.float64)
+        _check(load().hfls_solve(self._h, ctypes.c_double(eps), int(max_iter), _ptr(f), _ptr(u), _ptr(lam),
+                                 ctypes.byref(st)))
+        return {"u": u, "lambda": lam, "iterations": st.iterations, "converged": bool(st.converged),
+                "residual": st.residual, "solve_ms": st.solve_ms, "pcpg_ms": st.pcpg_ms}

@@ -1,0 +1,254 @@
+// C ABI (include/hfls_gpu.h) over the host problem and the device engine.
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/hfls_gpu.h"
+#include "engine.hpp"
+#include "problem.hpp"
+
+struct hfls_problem_s {
+  hfls::Problem* p;
+};
+struct hfls_mesh_s {
+  hfls::Engine* e;
+};
+
+static thread_local std::string g_err;
+
+extern "C" const char* hfls_last_error(void) { return g_err.c_str(); }
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -2;
+  }
+}
+
+static hfls::Options make_options(int n, const char* const* k, const char* const* v) {
+  hfls::Options o = hfls::default_options();
+  for (int i = 0; i < n; i++) {
+    if (!o.count(k[i])) throw std::runtime_error(std::string("unknown option ") + k[i]);
+    o[k[i]] = v[i];
+  }
+  return o;
+}
+
+extern "C" int hfls_problem_create(int nopt, const char* const* keys, const char* const* vals, hfls_problem* out) {
+  return guard([&] {
+    auto* h = new hfls_problem_s;
+    h->p = new hfls::Problem(make_options(nopt, keys, vals));
+    *out = h;
+  });
+}
+
+extern "C" int hfls_problem_destroy(hfls_problem p) {
+  return guard([&] {
+    if (p) {
+      delete p->p;
+      delete p;
+    }
+  });
+}
+
+extern "C" int hfls_problem_sizes(hfls_problem h, int64_t* s) {
+  return guard([&] {
+    const hfls::Problem& p = *h->p;
+    s[0] = p.nsub; s[1] = p.nlam; s[2] = p.n_glue; s[3] = p.nPoints; s[4] = p.ncl; s[5] = p.nsub_cl;
+    s[6] = (int64_t)p.dirichlet_dofs.size(); s[7] = (int64_t)p.corner_dofs.size();
+  });
+}
+
+static const hfls::Subdomain& sub(hfls_problem h, int d) {
+  if (d < 0 || d >= h->p->nsub) throw std::runtime_error("subdomain index out of range");
+  return h->p->subs[d];
+}
+
+extern "C" int hfls_problem_sub_info(hfls_problem h, int d, int* info) {
+  return guard([&] {
+    const hfls::Subdomain& s = sub(h, d);
+    info[0] = s.n; info[1] = s.n_r; info[2] = s.n_s; info[3] = s.n_bs; info[4] = s.n_ss; info[5] = s.cluster;
+    info[6] = (int)s.bf_row.size(); info[7] = 0;
+  });
+}
+
+extern "C" int hfls_problem_sub_bf(hfls_problem h, int d, int* rows, int* dofs, int* signs) {
+  return guard([&] {
+    const hfls::Subdomain& s = sub(h, d);
+    for (size_t i = 0; i < s.bf_row.size(); i++) {
+      rows[i] = s.bf_row[i];
+      dofs[i] = s.bf_dof[i];
+      signs[i] = s.bf_sign[i];
+    }
+  });
+}
+
+extern "C" int hfls_problem_sub_bdofs(hfls_problem h, int d, int* b) {
+  return guard([&] {
+    const hfls::Subdomain& s = sub(h, d);
+    std::memcpy(b, s.bdofs.data(), s.bdofs.size() * sizeof(int));
+  });
+}
+
+extern "C" int hfls_problem_sub_sset(hfls_problem h, int d, int* b) {
+  return guard([&] {
+    const hfls::Subdomain& s = sub(h, d);
+    std::memcpy(b, s.sset.data(), s.sset.size() * sizeof(int));
+  });
+}
+
+extern "C" int hfls_problem_sub_l2g(hfls_problem h, int d, int64_t* l2g) {
+  return guard([&] {
+    const hfls::Problem& p = *h->p;
+    const hfls::Subdomain& s = sub(h, d);
+    const int64_t nx1 = p.NE[0] + 1, nxy1 = nx1 * (p.NE[1] + 1);
+    for (int l = 0; l < s.n; l++) {
+      int ln = l / 3, c = l % 3;
+      int64_t ii = s.e0[0] + ln % s.nn[0], jj = s.e0[1] + (ln / s.nn[0]) % s.nn[1],
+              kk = s.e0[2] + ln / (s.nn[0] * s.nn[1]);
+      l2g[l] = 3 * (ii + jj * nx1 + kk * nxy1) + c;
+    }
+  });
+}
+
+extern "C" int hfls_problem_sub_young(hfls_problem h, int d, double* E) {
+  return guard([&] { *E = sub(h, d).E; });
+}
+
+extern "C" int hfls_problem_rows(hfls_problem h, int* a, int* b, int64_t* dof, double* w) {
+  return guard([&] {
+    const hfls::Problem& p = *h->p;
+    for (int64_t r = 0; r < p.nlam; r++) {
+      if (a) a[r] = p.row_sub_a[r];
+      if (b) b[r] = p.row_sub_b[r];
+      if (dof) dof[r] = p.row_dof[r];
+      if (w) w[r] = p.weight[r];
+    }
+  });
+}
+
+extern "C" int hfls_problem_dirichlet(hfls_problem h, int64_t* dofs) {
+  return guard([&] {
+    const auto& v = h->p->dirichlet_dofs;
+    std::memcpy(dofs, v.data(), v.size() * sizeof(int64_t));
+  });
+}
+
+extern "C" int hfls_problem_corners(hfls_problem h, int64_t* dofs) {
+  return guard([&] {
+    const auto& v = h->p->corner_dofs;
+    std::memcpy(dofs, v.data(), v.size() * sizeof(int64_t));
+  });
+}
+
+// ------------------------------------------------------------------ device mesh
+extern "C" int hfls_mesh_create(int device, int nopt, const char* const* keys, const char* const* vals,
+                                hfls_mesh* out) {
+  return guard([&] {
+    hfls::Options o = make_options(nopt, keys, vals);
+    auto* e = new hfls::Engine(o, device);
+    try {
+      e->setup();
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    auto* h = new hfls_mesh_s;
+    h->e = e;
+    *out = h;
+  });
+}
+
+extern "C" int hfls_mesh_destroy(hfls_mesh m) {
+  return guard([&] {
+    if (m) {
+      delete m->e;
+      delete m;
+    }
+  });
+}
+
+extern "C" int hfls_mesh_sizes(hfls_mesh m, int64_t* s) {
+  return guard([&] {
+    const hfls::Engine& e = *m->e;
+    s[0] = e.problem().nsub; s[1] = e.nlam(); s[2] = e.ncol_G(); s[3] = e.packed_n();
+    s[4] = e.problem().nPoints; s[5] = (int64_t)e.setup_info().device_bytes;
+  });
+}
+
+extern "C" int hfls_mesh_setup_times(hfls_mesh m, double* t) {
+  return guard([&] {
+    t[0] = m->e->setup_info().setup_ms;
+    t[1] = m->e->setup_info().factor_ms;
+  });
+}
+
+extern "C" int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect) {
+  return guard([&] {
+    std::vector<double> r;
+    std::vector<int> f, d;
+    if (R) {
+      m->e->get_R(r);
+      std::memcpy(R, r.data(), r.size() * sizeof(double));
+    }
+    if (fix) {
+      m->e->get_fix(f);
+      std::memcpy(fix, f.data(), f.size() * sizeof(int));
+    }
+    if (defect) {
+      m->e->get_defect(d);
+      std::memcpy(defect, d.data(), d.size() * sizeof(int));
+    }
+  });
+}
+
+extern "C" int hfls_mesh_load(hfls_mesh m, double* f) {
+  return guard([&] {
+    std::vector<double> v;
+    m->e->get_load(v);
+    std::memcpy(f, v.data(), v.size() * sizeof(double));
+  });
+}
+
+extern "C" int hfls_apply_F(hfls_mesh m, const double* w, double* Fw) {
+  return guard([&] { m->e->apply_F(w, Fw); });
+}
+
+extern "C" int hfls_project(hfls_mesh m, const double* x, double* Px, double* alpha) {
+  return guard([&] { m->e->project(x, Px, alpha); });
+}
+
+extern "C" int hfls_precondition(hfls_mesh m, const double* x, double* Mx) {
+  return guard([&] { m->e->precondition(x, Mx); });
+}
+
+extern "C" int hfls_kplus(hfls_mesh m, const double* f, double* x) {
+  return guard([&] { m->e->kplus_local(f, x); });
+}
+
+extern "C" int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f, double* u, double* lam,
+                          hfls_stats* stats) {
+  return guard([&] {
+    hfls::SolveInfo info;
+    m->e->solve(eps, max_iter, f, u, lam, &info);
+    if (stats) {
+      stats->iterations = info.iterations;
+      stats->converged = info.converged;
+      stats->residual = info.residual;
+      stats->solve_ms = info.solve_ms;
+      stats->pcpg_ms = info.pcpg_ms;
+    }
+  });
+}
+
+extern "C" int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes) {
+  return guard([&] { *ms = m->e->time_stream(which, reps, bytes); });
+}

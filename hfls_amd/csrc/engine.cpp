@@ -1,0 +1,1116 @@
+// Host orchestration of the device HTFETI path.
+//
+// Setup per subdomain i (interior-first order [r, b\s, s], r = DOFs untouched by
+// Bf, b = B-touched DOFs = j_col_cmpr of Cluster.cpp:756-765, s = kernel-detection
+// set inside b):
+//   M1 = K_i (dense, lower)      partial Cholesky over r  -> L_rr, W^T, S_bb
+//   S_i = S_bb                   explicit Dirichlet preconditioner (Matrix.cpp:1853-1870)
+//   M2 = [S_bb I; I 0]           partial Cholesky over b\s -> Schur of s
+//   kernel detection on s        -> R_s, R = [-K_rr^-1 K_rb R_b ; R_b] (Matrix.cpp:2393-2421)
+//   fixing DOFs in s, K_pp added -> K_reg (Matrix.cpp:1033-1046)
+//   continue over s              -> trailing block = -(S_bb + D)^-1 = -(K_reg^-1)_bb
+// so F_i = B_i K_reg^-1 B_i^T is applied as B_i (K+_bb (B_i^T w)), streaming the
+// n_s x n_s block instead of an nlam_i x nlam_i one.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace hfls {
+
+#define HC(x)                                                                                  \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +    \
+                               __FILE__ + ":" + std::to_string(__LINE__) + ": " #x);           \
+  } while (0)
+
+template <class T>
+void DVec<T>::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+template <class T>
+void DVec<T>::alloc(size_t count) {
+  release();
+  n = count;
+  if (count) HC(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+template <class T>
+void DVec<T>::upload(const std::vector<T>& v, hipStream_t s) {
+  alloc(v.size());
+  if (!v.empty()) HC(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  HC(hipStreamSynchronize(s));
+}
+template <class T>
+void DVec<T>::download(std::vector<T>& v, hipStream_t s) const {
+  v.resize(n);
+  if (n) HC(hipMemcpyAsync(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  HC(hipStreamSynchronize(s));
+}
+template struct DVec<double>;
+template struct DVec<int>;
+template struct DVec<int64_t>;
+template struct DVec<unsigned char>;
+template struct DVec<Scalars>;
+
+static int64_t align32(int64_t x) { return (x + 31) / 32 * 32; }
+static int round8(int x) { return (x + 7) / 8 * 8; }
+
+// ------------------------------------------------------------ small host dense LA
+// cyclic Jacobi eigen-decomposition of a symmetric n x n row-major matrix
+static void jacobi_eig(int n, std::vector<double>& A, std::vector<double>& V) {
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        double v = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+        tot += v;
+        if (i != j) off += v;
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        double tau = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2.0 * apq);
+        double t = tau >= 0 ? 1.0 / (tau + std::sqrt(1.0 + tau * tau)) : -1.0 / (-tau + std::sqrt(1.0 + tau * tau));
+        double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+          double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = c * vkp - s * vkq;
+          V[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+// in-place lower Cholesky of a row-major SPD matrix; returns false on failure
+static bool cholesky(int n, std::vector<double>& A) {
+  for (int j = 0; j < n; j++) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    A[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; i++) {
+      double v = A[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / d;
+    }
+  }
+  return true;
+}
+static void chol_solve(int n, const std::vector<double>& L, double* b) {
+  for (int i = 0; i < n; i++) {
+    double v = b[i];
+    for (int k = 0; k < i; k++) v -= L[(size_t)i * n + k] * b[k];
+    b[i] = v / L[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double v = b[i];
+    for (int k = i + 1; k < n; k++) v -= L[(size_t)k * n + i] * b[k];
+    b[i] = v / L[(size_t)i * n + i];
+  }
+}
+
+// Kernel of a small symmetric PSD matrix (the cluster G_c^T G_c) by the method of
+// Matrix::get_kernel_from_K (Matrix.cpp:1982-2448): diagonal scaling, Schur
+// complement on an s-set, eigen-jump 1e-5 over the lowest 20, R = [-K_rr^-1 K_rs R_s; R_s],
+// Gram-Schmidt.  The s-set is the 6 columns of up to 8 subdomains spread over the
+// cluster (deterministic instead of the reference's time-seeded shuffle).
+static std::vector<double> kernel_small(int n, const std::vector<double>& K, int nsub_cl, int* defect) {
+  std::vector<double> dg(n);
+  for (int i = 0; i < n; i++) dg[i] = std::sqrt(K[(size_t)i * n + i]);
+  std::vector<int> sub;
+  if (nsub_cl <= 8) {
+    for (int m = 0; m < nsub_cl; m++) sub.push_back(m);
+  } else {
+    for (int i = 0; i < 8; i++) {
+      int m = (int)std::lround((double)i * (nsub_cl - 1) / 7.0);
+      if (sub.empty() || sub.back() != m) sub.push_back(m);
+    }
+  }
+  std::vector<char> ins(n, 0);
+  std::vector<int> s, r;
+  for (int m : sub)
+    for (int k = 0; k < 6; k++) ins[6 * m + k] = 1;
+  for (int i = 0; i < n; i++) (ins[i] ? s : r).push_back(i);
+  int ns = (int)s.size(), nr = (int)r.size();
+  auto Ks = [&](int i, int j) { return K[(size_t)i * n + j] / (dg[i] * dg[j]); };
+  std::vector<double> Lrr((size_t)nr * nr);
+  for (int a = 0; a < nr; a++)
+    for (int b = 0; b < nr; b++) Lrr[(size_t)a * nr + b] = Ks(r[a], r[b]);
+  if (nr && !cholesky(nr, Lrr)) throw std::runtime_error("cluster kernel detection: K_rr not SPD");
+  std::vector<double> X((size_t)nr * ns);  // K_rr^-1 K_rs, column by column
+  std::vector<double> col(nr);
+  for (int j = 0; j < ns; j++) {
+    for (int a = 0; a < nr; a++) col[a] = Ks(r[a], s[j]);
+    if (nr) chol_solve(nr, Lrr, col.data());
+    for (int a = 0; a < nr; a++) X[(size_t)a * ns + j] = col[a];
+  }
+  std::vector<double> S((size_t)ns * ns);
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < ns; j++) {
+      double v = Ks(s[i], s[j]);
+      for (int a = 0; a < nr; a++) v -= Ks(r[a], s[i]) * X[(size_t)a * ns + j];
+      S[(size_t)i * ns + j] = v;
+    }
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < i; j++) {
+      double v = 0.5 * (S[(size_t)i * ns + j] + S[(size_t)j * ns + i]);
+      S[(size_t)i * ns + j] = S[(size_t)j * ns + i] = v;
+    }
+  std::vector<double> V;
+  jacobi_eig(ns, S, V);
+  std::vector<int> idx(ns);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return S[(size_t)a * ns + a] < S[(size_t)b * ns + b]; });
+  int itmax = std::min(20, ns), dfc = 0;
+  for (int i = itmax - 1; i > 0; i--)
+    if (std::fabs(S[(size_t)idx[i - 1] * ns + idx[i - 1]] / S[(size_t)idx[i] * ns + idx[i]]) < 1e-5) {
+      dfc = i;
+      break;
+    }
+  *defect = dfc;
+  std::vector<double> R((size_t)n * dfc, 0.0);  // row-major n x dfc
+  for (int k = 0; k < dfc; k++) {
+    for (int j = 0; j < ns; j++) R[(size_t)s[j] * dfc + k] = V[(size_t)j * ns + idx[k]];
+    for (int a = 0; a < nr; a++) {
+      double v = 0.0;
+      for (int j = 0; j < ns; j++) v += X[(size_t)a * ns + j] * V[(size_t)j * ns + idx[k]];
+      R[(size_t)r[a] * dfc + k] = -v;
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < dfc; k++) R[(size_t)i * dfc + k] /= dg[i];
+  for (int j = 0; j < dfc; j++) {  // modified Gram-Schmidt
+    for (int i = 0; i < j; i++) {
+      double d = 0.0;
+      for (int q = 0; q < n; q++) d += R[(size_t)q * dfc + j] * R[(size_t)q * dfc + i];
+      for (int q = 0; q < n; q++) R[(size_t)q * dfc + j] -= R[(size_t)q * dfc + i] * d;
+    }
+    double d = 0.0;
+    for (int q = 0; q < n; q++) d += R[(size_t)q * dfc + j] * R[(size_t)q * dfc + j];
+    d = std::sqrt(d);
+    for (int q = 0; q < n; q++) R[(size_t)q * dfc + j] /= d;
+  }
+  return R;
+}
+
+// rank test of a Bc block (Matrix.cpp:1538-1571): eigenvalues of A A^T (6x6),
+// descending; a ratio below 1e-4 means rank deficient
+static bool bc_full_rank(const std::vector<double>& AAt) {
+  std::vector<double> A = AAt, V;
+  jacobi_eig(6, A, V);
+  std::vector<double> sv(6);
+  for (int i = 0; i < 6; i++) sv[i] = std::fabs(A[i * 6 + i]);
+  std::sort(sv.begin(), sv.end(), std::greater<double>());
+  for (int i = 0; i < 5; i++)
+    if (std::fabs(sv[i + 1] / sv[i]) < 1e-4) return false;
+  return true;
+}
+
+// ------------------------------------------------------------ Engine
+Engine::Engine(const Options& o, int dev) : device(dev) {
+  HC(hipSetDevice(device));
+  HC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  P.reset(new Problem(o));
+  lumped = opt(o, "preconditioner") == "lumped";
+}
+
+Engine::~Engine() {
+  if (st) (void)hipStreamDestroy(st);
+}
+
+void Engine::build_meta() {
+  const int N = P->nsub;
+  h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N); h_cl.resize(N);
+  off_n.assign(N + 1, 0); off_b.assign(N + 1, 0);
+  off_K.assign(N + 1, 0); off_Lr.assign(N + 1, 0); off_R.assign(N + 1, 0); off_Rb.assign(N + 1, 0);
+  for (int d = 0; d < N; d++) {
+    const Subdomain& s = P->subs[d];
+    h_n[d] = s.n; h_nr[d] = s.n_r; h_ns[d] = s.n_s; h_nbs[d] = s.n_bs; h_nss[d] = s.n_ss;
+    h_ldb[d] = round8(s.n_s); h_cl[d] = s.cluster;
+    off_n[d + 1] = off_n[d] + s.n;
+    off_b[d + 1] = off_b[d] + s.n_s;
+    off_K[d + 1] = off_K[d] + align32((int64_t)h_ldb[d] * s.n_s);
+    off_Lr[d + 1] = off_Lr[d] + align32((int64_t)s.n * s.n_r);
+    off_R[d + 1] = off_R[d] + 6 * (int64_t)s.n;
+    off_Rb[d + 1] = off_Rb[d] + 6 * (int64_t)s.n_s;
+    maxn = std::max(maxn, s.n); maxns = std::max(maxns, s.n_s); maxnr = std::max(maxnr, s.n_r);
+  }
+  if (maxns > 8000) throw std::runtime_error("interface too large for the LDS-staged gather (n_s > 8000)");
+  NB = off_b[N];
+  d_n.upload(h_n, st); d_nr.upload(h_nr, st); d_ns.upload(h_ns, st); d_nbs.upload(h_nbs, st);
+  d_ldb.upload(h_ldb, st); d_cl.upload(h_cl, st);
+  d_ldn.upload(h_n, st);
+  d_off_n.upload(std::vector<int64_t>(off_n.begin(), off_n.end() - 1), st);
+  d_off_b.upload(std::vector<int64_t>(off_b.begin(), off_b.end() - 1), st);
+  d_off_K.upload(std::vector<int64_t>(off_K.begin(), off_K.end() - 1), st);
+  d_off_Lr.upload(std::vector<int64_t>(off_Lr.begin(), off_Lr.end() - 1), st);
+  d_off_R.upload(std::vector<int64_t>(off_R.begin(), off_R.end() - 1), st);
+  d_off_Rb.upload(std::vector<int64_t>(off_Rb.begin(), off_Rb.end() - 1), st);
+  std::vector<int64_t> xb(N), v6(N);
+  for (int d = 0; d < N; d++) { xb[d] = off_n[d] + h_nr[d]; v6[d] = 6 * (int64_t)d; }
+  d_xboff.upload(xb, st);
+  d_v6off.upload(v6, st);
+  d_ones.upload(std::vector<int>(N, 1), st);
+  d_sixes.upload(std::vector<int>(N, 6), st);
+  // permutation (interior-first order -> local DOF), global DOF ids, owners
+  std::vector<int> perm(off_n[N]), gdof(off_n[N]);
+  std::vector<unsigned char> owner(off_n[N]);
+  const int64_t nx1 = P->NE[0] + 1, nxy1 = nx1 * (P->NE[1] + 1);
+  for (int d = 0; d < N; d++) {
+    const Subdomain& s = P->subs[d];
+    int q = 0;
+    for (int l : s.rdofs) perm[off_n[d] + q++] = l;
+    for (int l : s.border) perm[off_n[d] + q++] = l;
+    for (int l = 0; l < s.n; l++) {
+      int ln = l / 3, c = l % 3;
+      int64_t ii = s.e0[0] + ln % s.nn[0], jj = s.e0[1] + (ln / s.nn[0]) % s.nn[1],
+              kk = s.e0[2] + ln / (s.nn[0] * s.nn[1]);
+      gdof[off_n[d] + l] = (int)(3 * (ii + jj * nx1 + kk * nxy1) + c);
+      owner[off_n[d] + l] = s.owner[l];
+    }
+  }
+  d_perm1.upload(perm, st);
+  d_gdof.upload(gdof, st);
+  d_owner.upload(owner, st);
+  // gather lists over packed b positions and dual-row entries
+  std::vector<std::vector<int>> lists(NB);
+  const int64_t nl = P->nlam;
+  std::vector<int> ra(nl, -1), rb(nl, -1), rsa(nl, -1), rsb(nl, -1), rpa(nl, 0), rpb(nl, 0);
+  for (int d = 0; d < N; d++) {
+    const Subdomain& s = P->subs[d];
+    for (size_t e = 0; e < s.bf_row.size(); e++) {
+      int r = s.bf_row[e], p = s.bpos[s.bf_dof[e]];
+      lists[off_b[d] + p].push_back(s.bf_sign[e] > 0 ? r : ~r);
+      if (s.bf_sign[e] > 0) { ra[r] = (int)(off_b[d] + p); rsa[r] = d; rpa[r] = p; }
+      else { rb[r] = (int)(off_b[d] + p); rsb[r] = d; rpb[r] = p; }
+    }
+  }
+  std::vector<int> gptr(NB + 1, 0), gidx;
+  for (int64_t q = 0; q < NB; q++) {
+    gptr[q + 1] = gptr[q] + (int)lists[q].size();
+    for (int c : lists[q]) gidx.push_back(c);
+  }
+  d_gptr.upload(gptr, st); d_gidx.upload(gidx, st);
+  d_ra.upload(ra, st); d_rb.upload(rb, st); d_rsa.upload(rsa, st); d_rsb.upload(rsb, st);
+  d_rpa.upload(rpa, st); d_rpb.upload(rpb, st);
+  d_wgt.upload(P->weight, st);
+}
+
+void Engine::partial_cholesky(double* M, const std::vector<int64_t>& moff, const std::vector<int>& ld,
+                              const std::vector<int>& start, const std::vector<int>& ncols,
+                              const std::vector<int>& end) {
+  const int nmat = (int)moff.size();
+  int nsteps = 0;
+  for (int i = 0; i < nmat; i++) nsteps = std::max(nsteps, (ncols[i] + 63) / 64);
+  if (nsteps == 0) return;
+  std::vector<PanelDesc> desc((size_t)nsteps * nmat);
+  std::vector<int> maxrows(nsteps, 0), maxpairs(nsteps, 0);
+  for (int p = 0; p < nsteps; p++)
+    for (int i = 0; i < nmat; i++) {
+      PanelDesc& pd = desc[(size_t)p * nmat + i];
+      int k0 = start[i] + 64 * p, w = std::min(64, start[i] + ncols[i] - k0);
+      pd.k0 = k0; pd.w = w > 0 ? w : 0; pd.end = end[i]; pd.active = w > 0;
+      if (w > 0) {
+        int m = end[i] - k0 - w;
+        maxrows[p] = std::max(maxrows[p], m);
+        int T = (m + 63) / 64;
+        maxpairs[p] = std::max(maxpairs[p], T * (T + 1) / 2);
+      }
+    }
+  DVec<PanelDesc> dd;
+  dd.alloc(desc.size());
+  HC(hipMemcpyAsync(dd.p, desc.data(), desc.size() * sizeof(PanelDesc), hipMemcpyHostToDevice, st));
+  DVec<int64_t> dm; dm.upload(moff, st);
+  DVec<int> dl; dl.upload(ld, st);
+  for (int p = 0; p < nsteps; p++) {
+    HC(launch_panel(st, nmat, maxrows[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M, d_err.p));
+    HC(launch_syrk(st, nmat, maxpairs[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M));
+  }
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::factor_subdomains() {
+  const int N = P->nsub;
+  const Options& o = P->o;
+  d_kdiag.alloc(off_n[N]);
+  d_f.alloc(off_n[N]);
+  HC(hipMemsetAsync(d_f.p, 0, off_n[N] * sizeof(double), st));
+  d_Lr.alloc(off_Lr[N]);
+  d_K.alloc(off_K[N]);
+  d_S.alloc(off_K[N]);
+  d_R.alloc(off_R[N]);
+  d_Rb.alloc(off_Rb[N]);
+  d_fixpos.alloc(6 * N);
+  d_defect.alloc(N);
+  double nu = optd(o, "poissons_ratio");
+  double acc[3] = {0, 0, 0};
+  std::string load = opt(o, "load");
+  if (load == "reference") acc[2] = 2.0 * 98.10;  // Data.cpp:315-318
+  else acc[2] = optd(o, "body_force");
+  // chunking by scratch bytes (M1: n^2, M2: 4 n_s^2, R1: 6n)
+  size_t freeb = 0, totb = 0;
+  HC(hipMemGetInfo(&freeb, &totb));
+  size_t budget = std::min<size_t>((size_t)48 << 30, freeb / 2);
+  int c0 = 0;
+  while (c0 < N) {
+    int c1 = c0;
+    size_t bytes = 0;
+    while (c1 < N) {
+      size_t b = 8 * (align32((int64_t)h_n[c1] * h_n[c1]) + align32(4 * (int64_t)h_ns[c1] * h_ns[c1]) + 6 * h_n[c1]);
+      if (c1 > c0 && bytes + b > budget) break;
+      bytes += b;
+      c1++;
+    }
+    const int nm = c1 - c0;
+    std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), lroff(nm), koff(nm), roff(nm), rboff(nm);
+    std::vector<int> ld1(nm), ld2(nm), nrv(nm), nsv(nm), nbsv(nm), nssv(nm), nv(nm), ldb(nm), zero(nm, 0), ones(nm, 1);
+    std::vector<int> start_bs(nm), prow0(nm), r0s(nm);
+    std::vector<AsmSub> asub(nm);
+    int64_t o1 = 0, o2 = 0, o3 = 0;
+    for (int i = 0; i < nm; i++) {
+      int d = c0 + i;
+      const Subdomain& s = P->subs[d];
+      m1off[i] = o1; o1 += align32((int64_t)s.n * s.n);
+      m2off[i] = o2; o2 += align32(4 * (int64_t)s.n_s * s.n_s);
+      r1off[i] = o3; o3 += 6 * (int64_t)s.n;
+      noff[i] = off_n[d]; lroff[i] = off_Lr[d]; koff[i] = off_K[d]; roff[i] = off_R[d]; rboff[i] = off_Rb[d];
+      ld1[i] = s.n; ld2[i] = 2 * s.n_s; nrv[i] = s.n_r; nsv[i] = s.n_s; nbsv[i] = s.n_bs; nssv[i] = s.n_ss;
+      nv[i] = s.n; ldb[i] = h_ldb[d];
+      prow0[i] = s.n_r + s.n_bs;
+      r0s[i] = s.n_r;
+      for (int a = 0; a < 3; a++) { asub[i].e0[a] = s.e0[a]; asub[i].ne[a] = s.ne[a]; }
+      asub[i].n = s.n; asub[i].pad = 0; asub[i].E = s.E;
+    }
+    DVec<double> M1, M2, R1;
+    M1.alloc(o1); M2.alloc(o2); R1.alloc(o3);
+    HC(hipMemsetAsync(M1.p, 0, o1 * sizeof(double), st));
+    HC(hipMemsetAsync(M2.p, 0, o2 * sizeof(double), st));
+    DVec<int64_t> dm1, dm2, dr1, dnoff, dlr, dk, dr, drb, dnoff_r, dr1_b, dr1_s;
+    DVec<int> dld1, dld2, dnr, dns, dnbs, dnss, dn, dldb, dprow0, dr0s, dpinv;
+    DVec<AsmSub> das;
+    dm1.upload(m1off, st); dm2.upload(m2off, st); dr1.upload(r1off, st); dnoff.upload(noff, st);
+    dlr.upload(lroff, st); dk.upload(koff, st); dr.upload(roff, st); drb.upload(rboff, st);
+    dld1.upload(ld1, st); dld2.upload(ld2, st); dnr.upload(nrv, st); dns.upload(nsv, st); dnbs.upload(nbsv, st);
+    dnss.upload(nssv, st); dn.upload(nv, st); dldb.upload(ldb, st); dprow0.upload(prow0, st); dr0s.upload(r0s, st);
+    das.alloc(nm);
+    HC(hipMemcpyAsync(das.p, asub.data(), nm * sizeof(AsmSub), hipMemcpyHostToDevice, st));
+    // pinv for the chunk (local DOF -> M1 position), packed with off_n offsets relative to chunk
+    std::vector<int> pinv(off_n[c1] - off_n[c0]);
+    std::vector<int64_t> pinv_off(nm);
+    for (int i = 0; i < nm; i++) {
+      const Subdomain& s = P->subs[c0 + i];
+      int64_t b = off_n[c0 + i] - off_n[c0];
+      pinv_off[i] = b;
+      int q = 0;
+      for (int l : s.rdofs) pinv[b + l] = q++;
+      for (int l : s.border) pinv[b + l] = q++;
+    }
+    dpinv.upload(pinv, st);
+    DVec<int64_t> dpinv_off; dpinv_off.upload(pinv_off, st);
+    // element assembly: M1 lower, loads and diagonal
+    HC(launch_assemble(st, nm, das.p, dm1.p, dld1.p, dpinv.p, dpinv_off.p, M1.p, d_f.p, dnoff.p, d_kdiag.p,
+                       P->dx[0], P->dx[1], P->dx[2], nu, acc[0], acc[1], acc[2]));
+    if (lumped)  // lumped preconditioner: K_bb (Cluster.cpp:1106-1107)
+      HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
+    auto t0 = std::chrono::steady_clock::now();
+    // M1: eliminate the interior
+    {
+      std::vector<int> st0(nm, 0);
+      partial_cholesky(M1.p, m1off, ld1, st0, nrv, nv);
+    }
+    if (!lumped) HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
+    HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, M2.p, dm2.p, dld2.p, 1.0, 1));
+    HC(launch_copy_cols(st, nm, maxnr, dn.p, dnr.p, M1.p, dm1.p, dld1.p, d_Lr.p, dlr.p, dld1.p));
+    // M2: eliminate b \ s
+    {
+      std::vector<int> st0(nm, 0), end2(nm);
+      for (int i = 0; i < nm; i++) end2[i] = 2 * nsv[i];
+      partial_cholesky(M2.p, m2off, ld2, st0, nbsv, end2);
+    }
+    // kernel detection on the s-set and R (rows in interior-first order in R1)
+    {
+      std::vector<int64_t> rs_off(nm), rb_off(nm);
+      for (int i = 0; i < nm; i++) { rs_off[i] = r1off[i] + prow0[i]; rb_off[i] = r1off[i] + nrv[i]; }
+      DVec<int64_t> drs, drbo, dr1o;
+      drs.upload(rs_off, st); drbo.upload(rb_off, st);
+      std::vector<int64_t> defoff(nm);
+      for (int i = 0; i < nm; i++) defoff[i] = c0 + i;
+      HC(launch_detect(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, d_kdiag.p, dnoff.p, d_perm1.p, dprow0.p, R1.p,
+                       drs.p, dld1.p, d_defect.p + c0));
+      // R_{b\s} = -L_A^{-T} (L_sA^T R_s)
+      HC(launch_coldot(st, nm, maxns, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, dnbs.p, R1.p, drs.p, dld1.p, R1.p, drbo.p,
+                       dld1.p, 6, -1.0, 0.0));
+      HC(launch_bwd(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, R1.p, drbo.p, dld1.p, 6));
+      // R_r = -L_rr^{-T} (W R_b)
+      HC(launch_coldot(st, nm, maxnr, M1.p, dm1.p, dld1.p, dnr.p, dns.p, dnr.p, R1.p, drbo.p, dld1.p, R1.p, dr1.p,
+                       dld1.p, 6, -1.0, 0.0));
+      HC(launch_bwd(st, nm, M1.p, dm1.p, dld1.p, dnr.p, R1.p, dr1.p, dld1.p, 6));
+      HC(launch_gs_pivots(st, nm, d_R.p, dr.p, dn.p, d_perm1.p, dnoff.p, dnr.p, dns.p, dnbs.p, d_Rb.p, drb.p, R1.p,
+                          dr1.p, d_kdiag.p, dnoff.p, M2.p, dm2.p, dld2.p, d_fixpos.p + 6 * c0));
+      HC(hipStreamSynchronize(st));
+    }
+    // M2: eliminate s (regularised) -> trailing block = -(K_reg^-1)_bb
+    {
+      std::vector<int> end2(nm);
+      for (int i = 0; i < nm; i++) end2[i] = 2 * nsv[i];
+      partial_cholesky(M2.p, m2off, ld2, nbsv, nssv, end2);
+    }
+    HC(launch_copy_block(st, nm, maxns, M2.p, dm2.p, dld2.p, dns.p, dns.p, d_K.p, dk.p, dldb.p, -1.0, 0));
+    HC(hipStreamSynchronize(st));
+    sinfo.factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c0 = c1;
+  }
+  int err = 0;
+  HC(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) throw std::runtime_error("non-positive pivot in the subdomain factorisation");
+  d_defect.download(h_def, st);
+  d_fixpos.download(h_fix, st);
+  sinfo.min_defect = *std::min_element(h_def.begin(), h_def.end());
+  sinfo.max_defect = *std::max_element(h_def.begin(), h_def.end());
+  if (sinfo.min_defect != 6 || sinfo.max_defect != 6)
+    throw std::runtime_error("kernel detection found a defect other than 6");
+}
+
+void Engine::build_bc_and_coarse() {
+  const int N = P->nsub;
+  const Options& o = P->o;
+  std::string typeBc = opt(o, "typeBc");
+  bool full_rank = opt(o, "Bc_fullRank") == "true";
+  std::vector<double> Rb;
+  d_Rb.download(Rb, st);
+  const int64_t nx1 = P->NE[0] + 1, nxy1 = nx1 * (P->NE[1] + 1);
+  // per-subdomain Bc rows: (cluster-local row, entries (bpos, val))
+  struct Row { int l; std::vector<std::pair<int, double>> e; };
+  std::vector<std::vector<Row>> brow(N);
+  std::vector<int> nlcl(P->ncl, 0);
+  auto Rbv = [&](int d, int p, int k) { return Rb[off_Rb[d] + p + (int64_t)k * h_ns[d]]; };
+  for (int cl = 0; cl < P->ncl; cl++) {
+    const int d0 = cl * P->nsub_cl, d1 = d0 + P->nsub_cl;
+    int cnt = 0;
+    if (typeBc == "ker") {  // Cluster.cpp:779-819
+      for (int i = d0; i < d1; i++) {
+        const Subdomain& a = P->subs[i];
+        for (int j = i + 1; j < d1; j++) {
+          const Subdomain& b = P->subs[j];
+          int lo[3], hi[3];
+          bool touch = true;
+          for (int x = 0; x < 3; x++) {
+            lo[x] = std::max(a.e0[x], b.e0[x]);
+            hi[x] = std::min(a.e0[x] + a.ne[x], b.e0[x] + b.ne[x]);
+            if (lo[x] > hi[x]) touch = false;
+          }
+          if (!touch) continue;
+          std::vector<int> pa, pb;  // b positions of shared DOFs, ascending global order
+          for (int kk = lo[2]; kk <= hi[2]; kk++)
+            for (int jj = lo[1]; jj <= hi[1]; jj++)
+              for (int ii = lo[0]; ii <= hi[0]; ii++)
+                for (int c = 0; c < 3; c++) {
+                  int la = 3 * ((ii - a.e0[0]) + (jj - a.e0[1]) * a.nn[0] + (kk - a.e0[2]) * a.nn[0] * a.nn[1]) + c;
+                  int lb = 3 * ((ii - b.e0[0]) + (jj - b.e0[1]) * b.nn[0] + (kk - b.e0[2]) * b.nn[0] * b.nn[1]) + c;
+                  pa.push_back(a.bpos[la]);
+                  pb.push_back(b.bpos[lb]);
+                }
+          for (size_t q = 0; q < pa.size(); q++)
+            if (pa[q] < 0 || pb[q] < 0) throw std::runtime_error("shared DOF outside the interface set");
+          std::vector<double> AAt(36, 0.0);
+          for (int k = 0; k < 6; k++)
+            for (int m = 0; m < 6; m++) {
+              double s = 0.0;
+              for (size_t q = 0; q < pa.size(); q++) s += Rbv(i, pa[q], k) * Rbv(i, pa[q], m);
+              AAt[k * 6 + m] = s;
+            }
+          if (full_rank && !bc_full_rank(AAt)) continue;
+          for (int k = 0; k < 6; k++) {
+            Row ri{cnt, {}}, rj{cnt, {}};
+            for (size_t q = 0; q < pa.size(); q++) {
+              double v = Rbv(i, pa[q], k);
+              ri.e.push_back({pa[q], v});
+              rj.e.push_back({pb[q], -v});
+            }
+            brow[i].push_back(ri);
+            brow[j].push_back(rj);
+            cnt++;
+          }
+        }
+      }
+    } else if (typeBc == "cor" || typeBc == "all") {  // Cluster.cpp:846-903, corners only
+      for (int d = d0; d < d1; d++) {
+        const Subdomain& s = P->subs[d];
+        for (int l = 0; l < s.n; l++) {
+          int ln = l / 3, c = l % 3;
+          int64_t ii = s.e0[0] + ln % s.nn[0], jj = s.e0[1] + (ln / s.nn[0]) % s.nn[1],
+                  kk = s.e0[2] + ln / (s.nn[0] * s.nn[1]);
+          auto isb = [](const std::vector<int>& st_, int64_t v) { return std::binary_search(st_.begin(), st_.end(), (int)v); };
+          if (!(isb(P->starts[0], ii) && isb(P->starts[1], jj) && isb(P->starts[2], kk))) continue;
+          bool cube = (ii == 0 || ii == P->NE[0]) && (jj == 0 || jj == P->NE[1]) && (kk == 0 || kk == P->NE[2]);
+          if (cube) continue;
+          int list[8];
+          int m = P->subs_of_node(ii, jj, kk, list);
+          if (m < 2) continue;
+          for (int q = 0; q < m; q++) {
+            int t = list[q];
+            if (t <= d || t >= d1) continue;
+            const Subdomain& u = P->subs[t];
+            int lu = 3 * (int)((ii - u.e0[0]) + (jj - u.e0[1]) * u.nn[0] + (kk - u.e0[2]) * u.nn[0] * u.nn[1]) + c;
+            brow[d].push_back(Row{cnt, {{s.bpos[l], 1.0}}});
+            brow[t].push_back(Row{cnt, {{u.bpos[lu], -1.0}}});
+            cnt++;
+            if (full_rank) break;
+          }
+        }
+      }
+    } else {
+      throw std::runtime_error("unknown typeBc " + typeBc);
+    }
+    nlcl[cl] = cnt;
+  }
+  // per-subdomain dense Bc^T (b-order, ld = ldb), row maps
+  h_nlc.assign(N, 0);
+  off_C.assign(N + 1, 0);
+  off_lc.assign(N + 1, 0);
+  for (int d = 0; d < N; d++) {
+    std::sort(brow[d].begin(), brow[d].end(), [](const Row& a, const Row& b) { return a.l < b.l; });
+    h_nlc[d] = (int)brow[d].size();
+    if (h_nlc[d] > 256) throw std::runtime_error("too many cluster constraints on one subdomain");
+    maxnlc = std::max(maxnlc, h_nlc[d]);
+    off_C[d + 1] = off_C[d] + align32((int64_t)h_ldb[d] * std::max(1, h_nlc[d]));
+    off_lc[d + 1] = off_lc[d] + h_nlc[d];
+  }
+  h_nlcl = nlcl;
+  h_la_off.assign(P->ncl + 1, 0);
+  h_Aoff.assign(P->ncl + 1, 0);
+  max_cl_len = 0;
+  for (int cl = 0; cl < P->ncl; cl++) {
+    int len = nlcl[cl] + 6 * P->nsub_cl;
+    h_la_off[cl + 1] = h_la_off[cl] + len;
+    h_Aoff[cl + 1] = h_Aoff[cl] + align32((int64_t)len * len);
+    max_cl_len = std::max(max_cl_len, len);
+  }
+  std::vector<double> BcT(off_C[N], 0.0);
+  std::vector<int> lcmap(off_lc[N]);
+  std::vector<int> gc_src;
+  std::vector<int64_t> gc_off(P->ncl);
+  std::vector<std::vector<int>> contrib(h_la_off[P->ncl]);
+  for (int d = 0; d < N; d++) {
+    int cl = h_cl[d];
+    for (int k = 0; k < h_nlc[d]; k++) {
+      const Row& r = brow[d][k];
+      for (auto& e : r.e) BcT[off_C[d] + e.first + (int64_t)k * h_ldb[d]] += e.second;
+      lcmap[off_lc[d] + k] = (int)(h_la_off[cl] + r.l);
+      contrib[h_la_off[cl] + r.l].push_back((int)(off_lc[d] + k));
+    }
+  }
+  int64_t gcount = 0;
+  for (int cl = 0; cl < P->ncl; cl++) {
+    gc_off[cl] = gcount;
+    for (int l = 0; l < nlcl[cl]; l++) {
+      auto& c = contrib[h_la_off[cl] + l];
+      if (c.empty() || c.size() > 2) throw std::runtime_error("cluster constraint with unexpected support");
+      gc_src.push_back(c[0]);
+      gc_src.push_back(c.size() > 1 ? c[1] : -1);
+      gcount++;
+    }
+  }
+  d_BcT.upload(BcT, st);
+  d_lcmap.upload(lcmap, st);
+  d_nlc.upload(h_nlc, st);
+  d_off_C.upload(std::vector<int64_t>(off_C.begin(), off_C.end() - 1), st);
+  d_off_lc.upload(std::vector<int64_t>(off_lc.begin(), off_lc.end() - 1), st);
+  if (gc_src.empty()) gc_src.push_back(-1);
+  d_gc_src.upload(gc_src, st);
+  d_gc_off.upload(gc_off, st);
+  std::vector<int> nrcl(P->ncl, 6 * P->nsub_cl), d0v(P->ncl), nlen(P->ncl);
+  for (int cl = 0; cl < P->ncl; cl++) { d0v[cl] = cl * P->nsub_cl; nlen[cl] = nlcl[cl] + 6 * P->nsub_cl; }
+  d_nlcl.upload(nlcl, st); d_nrcl.upload(nrcl, st); d_d0.upload(d0v, st); d_nlen.upload(nlen, st);
+  d_la_off.upload(std::vector<int64_t>(h_la_off.begin(), h_la_off.end() - 1), st);
+  d_Aoff.upload(std::vector<int64_t>(h_Aoff.begin(), h_Aoff.end() - 1), st);
+  std::vector<int64_t> aidx(N);
+  for (int d = 0; d < N; d++) aidx[d] = h_la_off[h_cl[d]] + nlcl[h_cl[d]] + 6 * (d - h_cl[d] * P->nsub_cl);
+  d_alpha_idx.upload(aidx, st);
+
+  // C_i = K+_bb Bc_i^T ; Fc_i = Bc_i C_i ; Gc_i = -Bc_i R_b
+  d_C.alloc(off_C[N]);
+  HC(hipMemsetAsync(d_C.p, 0, off_C[N] * sizeof(double), st));
+  HC(launch_gemm_nn(st, N, maxns, maxnlc, d_ns.p, d_nlc.p, d_K.p, d_off_K.p, d_ldb.p, d_BcT.p, d_off_C.p, d_ldb.p,
+                    d_C.p, d_off_C.p, d_ldb.p));
+  std::vector<int64_t> fcoff(N + 1, 0), gcoff(N + 1, 0);
+  for (int d = 0; d < N; d++) {
+    fcoff[d + 1] = fcoff[d] + (int64_t)h_nlc[d] * h_nlc[d];
+    gcoff[d + 1] = gcoff[d] + 6 * (int64_t)h_nlc[d];
+  }
+  DVec<double> dFc, dGc;
+  dFc.alloc(std::max<int64_t>(1, fcoff[N]));
+  dGc.alloc(std::max<int64_t>(1, gcoff[N]));
+  {
+    DVec<int64_t> dfo, dgo;
+    dfo.upload(std::vector<int64_t>(fcoff.begin(), fcoff.end() - 1), st);
+    dgo.upload(std::vector<int64_t>(gcoff.begin(), gcoff.end() - 1), st);
+    std::vector<int> six(N, 6);
+    DVec<int> dsix; dsix.upload(six, st);
+    HC(launch_tdots(st, N, maxnlc * maxnlc, d_ns.p, d_nlc.p, d_nlc.p, d_BcT.p, d_off_C.p, d_ldb.p, d_C.p, d_off_C.p,
+                    d_ldb.p, dFc.p, dfo.p, d_nlc.p, 1.0));
+    HC(launch_tdots(st, N, maxnlc * 6, d_ns.p, d_nlc.p, dsix.p, d_BcT.p, d_off_C.p, d_ldb.p, d_Rb.p, d_off_Rb.p,
+                    d_ns.p, dGc.p, dgo.p, d_nlc.p, -1.0));
+    HC(hipStreamSynchronize(st));
+  }
+  std::vector<double> Fcd, Gcd;
+  dFc.download(Fcd, st);
+  dGc.download(Gcd, st);
+  // cluster matrices, kernel H of Gc, augmented saddle-point matrices
+  std::vector<double> Hall((size_t)N * 36);
+  std::vector<double> Aaug;
+  std::vector<int64_t> aug_off(P->ncl);
+  std::vector<int> aug_ld(P->ncl), aug_nlc(P->ncl), aug_nr(P->ncl), aug_end(P->ncl);
+  int64_t aug_total = 0;
+  for (int cl = 0; cl < P->ncl; cl++) {
+    int L2 = 2 * (nlcl[cl] + 6 * P->nsub_cl);
+    aug_off[cl] = aug_total;
+    aug_total += align32((int64_t)L2 * L2);
+  }
+  Aaug.assign(aug_total, 0.0);
+  for (int cl = 0; cl < P->ncl; cl++) {
+    const int d0 = cl * P->nsub_cl, Ns = P->nsub_cl, nlc = nlcl[cl], nr = 6 * Ns;
+    std::vector<double> Fc((size_t)nlc * nlc, 0.0), Gc((size_t)nlc * nr, 0.0);
+    for (int m = 0; m < Ns; m++) {
+      int d = d0 + m;
+      for (int a = 0; a < h_nlc[d]; a++) {
+        int la_ = brow[d][a].l;
+        for (int b = 0; b < h_nlc[d]; b++) Fc[(size_t)la_ * nlc + brow[d][b].l] += Fcd[fcoff[d] + a + (int64_t)b * h_nlc[d]];
+        for (int k = 0; k < 6; k++) Gc[(size_t)la_ * nr + 6 * m + k] += Gcd[gcoff[d] + a + (int64_t)k * h_nlc[d]];
+      }
+    }
+    std::vector<double> GtG((size_t)nr * nr, 0.0);
+    for (int l = 0; l < nlc; l++) {
+      std::vector<int> nz;
+      for (int c = 0; c < nr; c++)
+        if (Gc[(size_t)l * nr + c] != 0.0) nz.push_back(c);
+      for (int a : nz)
+        for (int b : nz) GtG[(size_t)a * nr + b] += Gc[(size_t)l * nr + a] * Gc[(size_t)l * nr + b];
+    }
+    int dfc = 0;
+    std::vector<double> H = kernel_small(nr, GtG, Ns, &dfc);
+    if (dfc != 6) throw std::runtime_error("cluster kernel dimension is not 6");
+    for (int m = 0; m < Ns; m++)
+      for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) Hall[(size_t)(d0 + m) * 36 + a * 6 + b] = H[(size_t)(6 * m + a) * 6 + b];
+    // scale of the H H^T regularisation of the coarse Schur complement
+    double tf = 0.0, tg = 0.0;
+    for (int l = 0; l < nlc; l++) tf += Fc[(size_t)l * nlc + l];
+    for (int c = 0; c < nr; c++) tg += GtG[(size_t)c * nr + c];
+    double sig = (nlc > 0 && tf > 0) ? (tg / nr) / (tf / nlc) : 1.0;
+    const int L2 = 2 * (nlc + nr);
+    double* A = Aaug.data() + aug_off[cl];
+    auto at = [&](int r, int c) -> double& { return A[r + (int64_t)c * L2]; };  // lower triangle, column major
+    for (int i = 0; i < nlc; i++)
+      for (int j = 0; j <= i; j++) at(i, j) = 0.5 * (Fc[(size_t)i * nlc + j] + Fc[(size_t)j * nlc + i]);
+    for (int a = 0; a < nr; a++)
+      for (int l = 0; l < nlc; l++) at(nlc + a, l) = Gc[(size_t)l * nr + a];
+    for (int a = 0; a < nr; a++)
+      for (int b = 0; b <= a; b++) {
+        double hh = 0.0;
+        for (int k = 0; k < 6; k++) hh += H[(size_t)a * 6 + k] * H[(size_t)b * 6 + k];
+        at(nlc + a, nlc + b) = -sig * hh;                         // block (2,2): -sig H H^T
+        at(2 * nlc + nr + a, 2 * nlc + nr + b) = -hh / sig;       // block (4,4) start: -H H^T / sig
+      }
+    for (int l = 0; l < nlc; l++) at(nlc + nr + l, l) = 1.0;
+    for (int a = 0; a < nr; a++) at(2 * nlc + nr + a, nlc + a) = 1.0;
+    aug_ld[cl] = L2; aug_nlc[cl] = nlc; aug_nr[cl] = nr; aug_end[cl] = L2;
+  }
+  d_H.upload(Hall, st);
+  {
+    DVec<double> dA;
+    dA.upload(Aaug, st);
+    std::vector<int> zero(P->ncl, 0), startB(P->ncl), row0(P->ncl), nlenv(P->ncl);
+    for (int cl = 0; cl < P->ncl; cl++) { startB[cl] = aug_nlc[cl]; row0[cl] = aug_nlc[cl] + aug_nr[cl]; nlenv[cl] = row0[cl]; }
+    partial_cholesky(dA.p, aug_off, aug_ld, zero, aug_nlc, aug_end);
+    DVec<int64_t> dao; dao.upload(aug_off, st);
+    DVec<int> dald, dst0, dend; dald.upload(aug_ld, st); dst0.upload(startB, st); dend.upload(aug_end, st);
+    HC(launch_scale_lower(st, P->ncl, max_cl_len * 2, dst0.p, dend.p, dA.p, dao.p, dald.p, -1.0));
+    partial_cholesky(dA.p, aug_off, aug_ld, startB, aug_nr, aug_end);
+    d_Ainv.alloc(h_Aoff[P->ncl]);
+    DVec<int> drow0, dnl; drow0.upload(row0, st); dnl.upload(nlenv, st);
+    HC(launch_copy_block(st, P->ncl, max_cl_len, dA.p, dao.p, dald.p, drow0.p, dnl.p, d_Ainv.p, d_Aoff.p,
+                         d_nlen.p, 1.0, 0));
+    HC(hipStreamSynchronize(st));
+  }
+  // Gf^T Gf on the host from R_b and H, inverse on the device
+  const int ncol = 6 * P->ncl;
+  std::vector<double> GtG((size_t)ncol * ncol, 0.0);
+  {
+    std::vector<double> RH((size_t)NB * 6);  // R_b,i[p,:] H_i per packed b position
+    for (int d = 0; d < N; d++)
+      for (int p = 0; p < h_ns[d]; p++)
+        for (int b = 0; b < 6; b++) {
+          double s = 0.0;
+          for (int a = 0; a < 6; a++) s += Rbv(d, p, a) * Hall[(size_t)d * 36 + a * 6 + b];
+          RH[(size_t)(off_b[d] + p) * 6 + b] = s;
+        }
+    for (int64_t r = 0; r < P->nlam; r++) {
+      double row[12];
+      int cols[2];
+      int nc = 0;
+      for (int side = 0; side < 2; side++) {
+        int dsub = side == 0 ? P->row_sub_a[r] : P->row_sub_b[r];
+        if (dsub < 0) continue;
+        const Subdomain& s = P->subs[dsub];
+        // position of the row's DOF in this subdomain
+        int64_t g = P->row_dof[r];
+        int64_t node = g / 3, c = g % 3;
+        int64_t ii = node % nx1, jj = (node / nx1) % (P->NE[1] + 1), kk = node / nxy1;
+        int l = 3 * (int)((ii - s.e0[0]) + (jj - s.e0[1]) * s.nn[0] + (kk - s.e0[2]) * s.nn[0] * s.nn[1]) + (int)c;
+        int pos = s.bpos[l];
+        double sg = side == 0 ? 1.0 : -1.0;
+        int ccl = s.cluster;
+        int slot = -1;
+        for (int q = 0; q < nc; q++)
+          if (cols[q] == ccl) slot = q;
+        if (slot < 0) {
+          slot = nc++;
+          cols[slot] = ccl;
+          for (int b = 0; b < 6; b++) row[slot * 6 + b] = 0.0;
+        }
+        for (int b = 0; b < 6; b++) row[slot * 6 + b] += sg * RH[(size_t)(off_b[dsub] + pos) * 6 + b];
+      }
+      for (int qa = 0; qa < nc; qa++)
+        for (int qb = 0; qb < nc; qb++)
+          for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++)
+              GtG[(size_t)(6 * cols[qa] + a) * ncol + 6 * cols[qb] + b] += row[qa * 6 + a] * row[qb * 6 + b];
+    }
+  }
+  {
+    const int L2 = 2 * ncol;
+    std::vector<double> Ag((size_t)L2 * L2, 0.0);
+    for (int i = 0; i < ncol; i++) {
+      for (int j = 0; j <= i; j++) Ag[i + (size_t)j * L2] = 0.5 * (GtG[(size_t)i * ncol + j] + GtG[(size_t)j * ncol + i]);
+      Ag[(ncol + i) + (size_t)i * L2] = 1.0;
+    }
+    DVec<double> dA;
+    dA.upload(Ag, st);
+    partial_cholesky(dA.p, {0}, {L2}, {0}, {ncol}, {L2});
+    d_invGtG.alloc((size_t)ncol * ncol);
+    DVec<int64_t> z0; z0.upload(std::vector<int64_t>{0}, st);
+    DVec<int> dl, dr0, dn; dl.upload(std::vector<int>{L2}, st); dr0.upload(std::vector<int>{ncol}, st);
+    dn.upload(std::vector<int>{ncol}, st);
+    HC(launch_copy_block(st, 1, ncol, dA.p, z0.p, dl.p, dr0.p, dn.p, d_invGtG.p, z0.p, dn.p, -1.0, 0));
+    HC(hipStreamSynchronize(st));
+  }
+}
+
+void Engine::setup() {
+  auto t0 = std::chrono::steady_clock::now();
+  d_err.alloc(1);
+  HC(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
+  build_meta();
+  factor_subdomains();
+  build_bc_and_coarse();
+  // work vectors
+  const int64_t nl = P->nlam;
+  const int N = P->nsub;
+  for (DVec<double>* v : {&lam, &g, &w, &Fw, &Pg, &z, &Pz, &dvec}) v->alloc(nl);
+  yb.alloc(NB);
+  cvec.alloc(std::max<int64_t>(1, off_lc[N]));
+  avec.alloc(6 * N);
+  gc.alloc(h_la_off[P->ncl]);
+  la.alloc(h_la_off[P->ncl]);
+  v6.alloc(6 * N);
+  h.alloc(6 * N);
+  hg.alloc(6 * N);
+  alpha.alloc(6 * P->ncl);
+  nparts = (int)((nl + 255) / 256);
+  part.alloc(nparts);
+  X1.alloc(off_n[N]);
+  vloc.alloc(off_n[N]);
+  xloc.alloc(off_n[N]);
+  ug.alloc(3 * P->nPoints);
+  sc.alloc(1);
+  HC(hipStreamSynchronize(st));
+  sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  size_t freeb = 0, totb = 0;
+  HC(hipMemGetInfo(&freeb, &totb));
+  sinfo.device_bytes = (off_Lr[N] + 2 * off_K[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
+}
+
+// ------------------------------------------------------------ solve pieces
+void Engine::F_apply(const double* win, double* out, const double* sub_from, const double* dotw, const int* done) {
+  const int N = P->nsub;
+  HC(launch_gemv_gather(st, N, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, win,
+                        nullptr, nullptr, nullptr, yb.p, done));
+  HC(launch_corr_dots(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_Rb.p, d_off_Rb.p, d_ns.p,
+                      d_off_b.p, d_gptr.p, d_gidx.p, win, nullptr, cvec.p, avec.p, done));
+  HC(launch_cluster_gc(st, P->ncl, max_cl_len, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, cvec.p, avec.p,
+                       gc.p, d_la_off.p, done));
+  HC(launch_cluster_gemv(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, done));
+  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
+                       d_ns.p, d_off_b.p, d_alpha_idx.p, la.p, yb.p, done));
+  HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, nullptr, sub_from, out, dotw, part.p, done));
+}
+
+void Engine::proj(const double* x, double* out, double* hbuf, double* alpha_out, const double* dotv,
+                  const int* done) {
+  const int N = P->nsub;
+  HC(launch_proj_dots(st, N, d_Rb.p, d_off_Rb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, x, v6.p, done));
+  HC(launch_proj_coarse(st, N, 6 * P->ncl, v6.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, alpha_out, 1.0, done));
+  HC(launch_gh(st, P->nlam, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, hbuf, x, out, -1.0, dotv,
+               part.p, done));
+}
+
+void Engine::M_apply(const double* x, double* out, const int* done) {
+  const int N = P->nsub;
+  HC(launch_gemv_gather(st, N, maxns, d_S.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, x,
+                        d_wgt.p, nullptr, nullptr, yb.p, done));
+  HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, d_wgt.p, nullptr, out, nullptr, nullptr, done));
+}
+
+void Engine::kplus_b(const double* vin) {
+  const int N = P->nsub;
+  HC(launch_permute(st, N, maxn, 0, d_n.p, d_perm1.p, d_off_n.p, vin, d_off_n.p, X1.p, d_off_n.p, 1, 0, 0));
+  HC(launch_fwd(st, N, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, d_n.p, X1.p, d_off_n.p, d_ldn.p, 1));
+  HC(launch_gemv_gather(st, N, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr,
+                        nullptr, X1.p, d_xboff.p, yb.p, nullptr));
+}
+
+void Engine::kplus_full(const double* vin, double* xout) {
+  const int N = P->nsub;
+  kplus_b(vin);
+  // X1[0:nr] = y_r - W x_b ; then L_rr^T x_r = X1[0:nr]
+  HC(launch_coldot(st, N, maxnr, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, d_ns.p, d_nr.p, yb.p, d_off_b.p, d_ns.p, X1.p,
+                   d_off_n.p, d_ldn.p, 1, -1.0, 1.0));
+  HC(launch_bwd(st, N, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, X1.p, d_off_n.p, d_ldn.p, 1));
+  // X1[nr:n] = x_b
+  HC(launch_copy_cols(st, N, 1, d_ns.p, d_ones.p, yb.p, d_off_b.p, d_ns.p, X1.p, d_xboff.p, d_ns.p));
+  HC(launch_permute(st, N, maxn, 1, d_n.p, d_perm1.p, d_off_n.p, X1.p, d_off_n.p, xout, d_off_n.p, 1, 0, 0));
+}
+
+void Engine::cluster_solve_from_yb(const double* vin) {
+  const int N = P->nsub;
+  const DVec<int>& one_col = d_ones;
+  const DVec<int>& six_col = d_sixes;
+  // c_i = Bc_i (K^+ v)_b ; a_i = R_i^T v_i (full)
+  HC(launch_tdots(st, N, maxnlc, d_ns.p, d_nlc.p, one_col.p, d_BcT.p, d_off_C.p, d_ldb.p, yb.p, d_off_b.p, d_ns.p,
+                  cvec.p, d_off_lc.p, d_ns.p, 1.0));
+  HC(launch_tdots(st, N, 6, d_n.p, six_col.p, one_col.p, d_R.p, d_off_R.p, d_ldn.p, vin, d_off_n.p, d_ldn.p, avec.p,
+                  d_v6off.p, d_ns.p, 1.0));
+  HC(launch_cluster_gc(st, P->ncl, max_cl_len, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, cvec.p, avec.p,
+                       gc.p, d_la_off.p, nullptr));
+  HC(launch_cluster_gemv(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, nullptr));
+}
+
+// ------------------------------------------------------------ public operators
+void Engine::apply_F(const double* w_h, double* Fw_h) {
+  const int64_t nl = P->nlam;
+  HC(hipMemcpyAsync(w.p, w_h, nl * sizeof(double), hipMemcpyHostToDevice, st));
+  F_apply(w.p, Fw.p, nullptr, nullptr, nullptr);
+  HC(hipMemcpyAsync(Fw_h, Fw.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::project(const double* x_h, double* Px_h, double* alpha_h) {
+  const int64_t nl = P->nlam;
+  HC(hipMemcpyAsync(g.p, x_h, nl * sizeof(double), hipMemcpyHostToDevice, st));
+  proj(g.p, Pg.p, h.p, alpha.p, nullptr, nullptr);
+  HC(hipMemcpyAsync(Px_h, Pg.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+  std::vector<double> a(6 * P->ncl);
+  HC(hipMemcpyAsync(a.data(), alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  if (alpha_h)
+    for (size_t i = 0; i < a.size(); i++) alpha_h[i] = -a[i];  // reference alpha = -(G^T G)^-1 G^T x
+}
+
+void Engine::precondition(const double* x_h, double* Mx_h) {
+  const int64_t nl = P->nlam;
+  HC(hipMemcpyAsync(Pg.p, x_h, nl * sizeof(double), hipMemcpyHostToDevice, st));
+  M_apply(Pg.p, z.p, nullptr);
+  HC(hipMemcpyAsync(Mx_h, z.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::kplus_local(const double* f_h, double* x_h) {
+  const int N = P->nsub;
+  HC(hipMemcpyAsync(vloc.p, f_h, off_n[N] * sizeof(double), hipMemcpyHostToDevice, st));
+  kplus_b(vloc.p);
+  cluster_solve_from_yb(vloc.p);
+  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr, d_BcT.p,
+                   d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, la.p, vloc.p));
+  kplus_full(vloc.p, xloc.p);
+  // x += R alpha (h = 0)
+  HC(hipMemsetAsync(hg.p, 0, 6 * N * sizeof(double), st));
+  std::vector<int> gd;
+  // add R alpha through the recovery kernel into a packed copy: reuse xloc with owner-free path
+  DVec<int> iota_;
+  std::vector<int> io(off_n[N]);
+  std::iota(io.begin(), io.end(), 0);
+  iota_.upload(io, st);
+  DVec<unsigned char> all1;
+  all1.upload(std::vector<unsigned char>(off_n[N], 1), st);
+  DVec<double> out;
+  out.alloc(off_n[N]);
+  HC(launch_recover(st, N, maxn, d_n.p, d_off_n.p, d_R.p, d_off_R.p, d_alpha_idx.p, la.p, hg.p, xloc.p, iota_.p,
+                    all1.p, out.p));
+  HC(hipMemcpyAsync(x_h, out.p, off_n[N] * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+double Engine::time_stream(int which, int reps, double* bytes) {
+  const int N = P->nsub;
+  const int64_t nl = P->nlam;
+  std::vector<double> ones(nl, 1.0);
+  HC(hipMemcpyAsync(w.p, ones.data(), nl * sizeof(double), hipMemcpyHostToDevice, st));
+  const double* A = which == 0 ? d_K.p : d_S.p;
+  const double* wt = which == 0 ? nullptr : d_wgt.p;
+  HC(launch_gemv_gather(st, N, maxns, A, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, w.p, wt, nullptr,
+                        nullptr, yb.p, nullptr));
+  hipEvent_t a, b;
+  HC(hipEventCreate(&a));
+  HC(hipEventCreate(&b));
+  HC(hipEventRecord(a, st));
+  for (int i = 0; i < reps; i++)
+    HC(launch_gemv_gather(st, N, maxns, A, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, w.p, wt,
+                          nullptr, nullptr, yb.p, nullptr));
+  HC(hipEventRecord(b, st));
+  HC(hipEventSynchronize(b));
+  float ms = 0;
+  HC(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (bytes) {
+    // matrix stream + output + gathered dual values (8 B) and their row codes (4 B)
+    double m = 0, nnzg = 0;
+    for (int d = 0; d < N; d++) m += 8.0 * (double)h_ns[d] * h_ns[d];
+    for (int d = 0; d < N; d++)
+      for (size_t e = 0; e < P->subs[d].bf_row.size(); e++) nnzg += 1.0;
+    *bytes = m + 8.0 * (double)NB + 12.0 * nnzg + (which ? 8.0 * nnzg : 0.0);
+  }
+  return ms / reps;
+}
+
+void Engine::get_R(std::vector<double>& R) const {
+  std::vector<double> Rd;
+  d_R.download(Rd, st);
+  R = Rd;
+}
+void Engine::get_fix(std::vector<int>& fix) const {
+  fix.resize(6 * P->nsub);
+  for (int d = 0; d < P->nsub; d++)
+    for (int k = 0; k < 6; k++) fix[6 * d + k] = P->subs[d].border[h_fix[6 * d + k]];
+}
+void Engine::get_defect(std::vector<int>& d) const { d = h_def; }
+void Engine::get_load(std::vector<double>& f) const { d_f.download(f, st); }
+
+// ------------------------------------------------------------ PCPG (Cluster.cpp:1363-1458)
+void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, double* lam_h, SolveInfo* info) {
+  const int N = P->nsub;
+  const int64_t nl = P->nlam;
+  hipEvent_t e0, e1, e2;
+  HC(hipEventCreate(&e0)); HC(hipEventCreate(&e1)); HC(hipEventCreate(&e2));
+  const double* f = d_f.p;
+  DVec<double> fin;
+  if (f_h) {
+    fin.alloc(off_n[N]);
+    HC(hipMemcpyAsync(fin.p, f_h, off_n[N] * sizeof(double), hipMemcpyHostToDevice, st));
+    f = fin.p;
+  }
+  Scalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.eps = eps;
+  s0.max_iter = max_iter;
+  HC(hipMemcpyAsync(sc.p, &s0, sizeof(Scalars), hipMemcpyHostToDevice, st));
+  int* done = &sc.p->done;
+  HC(hipEventRecord(e0, st));
+  // d = Bf K_c^+ f  (Cluster.cpp:1385-1387)
+  kplus_b(f);
+  cluster_solve_from_yb(f);
+  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
+                       d_ns.p, d_off_b.p, d_alpha_idx.p, la.p, yb.p, nullptr));
+  HC(launch_scatter(st, nl, d_ra.p, d_rb.p, yb.p, nullptr, nullptr, dvec.p, nullptr, nullptr, nullptr));
+  // e = Rf^T f ; lambda0 = Gf (Gf^T Gf)^-1 e   (Cluster.cpp:1390-1394)
+  HC(launch_proj_coarse(st, N, 6 * P->ncl, avec.p, d_H.p, d_cl.p, d_invGtG.p, h.p, nullptr, 1.0, nullptr));
+  HC(launch_gh(st, nl, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, h.p, nullptr, lam.p, 1.0,
+               nullptr, nullptr, nullptr));
+  // g0 = F lambda0 - d
+  F_apply(lam.p, g.p, dvec.p, nullptr, nullptr);
+  proj(g.p, Pg.p, hg.p, alpha.p, nullptr, nullptr);
+  M_apply(Pg.p, z.p, nullptr);
+  proj(z.p, Pz.p, h.p, nullptr, g.p, nullptr);
+  HC(launch_reduce(st, nparts, part.p, sc.p, 2));
+  HC(hipMemcpyAsync(w.p, Pz.p, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HC(hipEventRecord(e1, st));
+  Scalars sh;
+  const int chunk = 8;
+  int launched = 0;
+  for (;;) {
+    for (int k = 0; k < chunk; k++) {
+      F_apply(w.p, Fw.p, nullptr, w.p, done);
+      HC(launch_reduce(st, nparts, part.p, sc.p, 0));
+      HC(launch_axpy2(st, nl, lam.p, g.p, w.p, Fw.p, sc.p));
+      proj(g.p, Pg.p, hg.p, alpha.p, nullptr, done);
+      M_apply(Pg.p, z.p, done);
+      proj(z.p, Pz.p, h.p, nullptr, g.p, done);
+      HC(launch_reduce(st, nparts, part.p, sc.p, 1));
+      HC(launch_update_w(st, nl, w.p, Pz.p, sc.p));
+    }
+    launched += chunk;
+    HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    if (sh.done || launched > max_iter + chunk) break;
+  }
+  HC(hipEventRecord(e2, st));
+  // recovery: u = K_c^+(f - Bf^T lambda) - Rf alpha  (Cluster.cpp:1648-1678)
+  HC(hipMemcpyAsync(vloc.p, f, off_n[N] * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, lam.p, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vloc.p));
+  kplus_b(vloc.p);
+  cluster_solve_from_yb(vloc.p);
+  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr, d_BcT.p,
+                   d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, la.p, vloc.p));
+  kplus_full(vloc.p, xloc.p);
+  HC(hipMemsetAsync(ug.p, 0, 3 * P->nPoints * sizeof(double), st));
+  HC(launch_recover(st, N, maxn, d_n.p, d_off_n.p, d_R.p, d_off_R.p, d_alpha_idx.p, la.p, hg.p, xloc.p, d_gdof.p,
+                    d_owner.p, ug.p));
+  hipEvent_t e3;
+  HC(hipEventCreate(&e3));
+  HC(hipEventRecord(e3, st));
+  if (u_h) HC(hipMemcpyAsync(u_h, ug.p, 3 * P->nPoints * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (lam_h) HC(hipMemcpyAsync(lam_h, lam.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  if (info) {
+    info->iterations = sh.it;
+    info->residual = sh.norm0 > 0 ? std::sqrt(std::fabs(sh.gPz)) / sh.norm0 : 0.0;
+    info->norm0 = sh.norm0;
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, e0, e3));
+    info->solve_ms = ms;
+    HC(hipEventElapsedTime(&ms, e1, e2));
+    info->pcpg_ms = ms;
+    info->converged = (sh.it < max_iter || std::sqrt(std::fabs(sh.gPz)) < eps * sh.norm0) ? 1 : 0;
+  }
+  int err = 0;
+  HC(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+  if (sh.err) throw std::runtime_error("PCPG produced a non-finite residual");
+}
+
+}  // namespace hfls

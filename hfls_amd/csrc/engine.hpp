@@ -1,0 +1,130 @@
+// Device-resident HTFETI mesh: setup (assembly, factorisation, kernel detection,
+// explicit K+_bb and S_i, HTFETI cluster inverse, projector) and PCPG solves.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "problem.hpp"
+
+namespace hfls {
+
+struct SolveInfo {
+  int iterations = 0;
+  double residual = 0.0;     // sqrt(g'Pz)/sqrt(g0'Pz0) at exit
+  double norm0 = 0.0;
+  double solve_ms = 0.0;     // device time of the whole load-case solve
+  double pcpg_ms = 0.0;      // device time of the iteration loop
+  double gemv_ms = 0.0;      // summed device time of the K+_bb stream kernels (F apply)
+  int gemv_launches = 0;
+  int converged = 0;
+};
+
+struct SetupInfo {
+  double setup_ms = 0.0;
+  double factor_ms = 0.0;
+  size_t device_bytes = 0;
+  int min_defect = 0, max_defect = 0;
+};
+
+template <class T>
+struct DVec {
+  T* p = nullptr;
+  size_t n = 0;
+  DVec() = default;
+  DVec(const DVec&) = delete;
+  DVec& operator=(const DVec&) = delete;
+  ~DVec() { release(); }
+  void release();
+  void alloc(size_t count);
+  void upload(const std::vector<T>& v, hipStream_t st);
+  void download(std::vector<T>& v, hipStream_t st) const;
+};
+
+class Engine {
+ public:
+  Engine(const Options& o, int device);
+  ~Engine();
+  void setup();
+  const Problem& problem() const { return *P; }
+  int64_t nlam() const { return P->nlam; }
+  int ncol_G() const { return 6 * P->ncl; }
+  const SetupInfo& setup_info() const { return sinfo; }
+
+  // Operators on host arrays (FP64).  Parity entry points of the C ABI.
+  void apply_F(const double* w, double* Fw);
+  void project(const double* x, double* Px, double* alpha);
+  void precondition(const double* x, double* Mx);
+  void kplus_local(const double* f_local, double* x_local);  // K_c^+ f (packed local vectors)
+
+  // One load case: body-force load assembled on the device, or the given packed
+  // local loads (sum over subdomains in l2g order).
+  void solve(double eps, int max_iter, const double* f_local, double* u_global, double* lam, SolveInfo* info);
+
+  // Time the dominant kernel (gather + GEMV streaming K+_bb (which=0) or S_i
+  // (which=1)) with HIP events on the library stream; returns ms per launch and
+  // the algorithmic bytes one launch moves.
+  double time_stream(int which, int reps, double* bytes);
+
+  // Accessors for tests: per-subdomain results of the setup (host copies).
+  void get_R(std::vector<double>& R) const;       // packed local, n x 6 column major per subdomain
+  void get_fix(std::vector<int>& fix) const;      // 6 local DOFs per subdomain
+  void get_defect(std::vector<int>& d) const;
+  void get_load(std::vector<double>& f) const;    // packed local body-force loads
+  int64_t packed_n() const { return off_n.back(); }
+  std::vector<int64_t> off_n, off_b;
+
+ private:
+  std::unique_ptr<Problem> P;
+  int device = 0;
+  hipStream_t st = nullptr;
+  SetupInfo sinfo;
+  int maxn = 0, maxns = 0, maxnr = 0, maxnlc = 0;
+  int64_t NB = 0;
+  // per-subdomain meta (host copies)
+  std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_cl, h_nlc;
+  std::vector<int64_t> off_K, off_Lr, off_R, off_Rb, off_C, off_lc;
+  // device meta
+  DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_cl, d_nlc, d_ldn, d_ones, d_sixes;
+  DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_Lr, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
+      d_v6off;
+  DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap;
+  DVec<unsigned char> d_owner;
+  DVec<double> d_kdiag, d_f, d_Lr, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
+  DVec<int> d_fixpos, d_defect, d_err;
+  // dual rows
+  DVec<int> d_ra, d_rb, d_rsa, d_rsb, d_rpa, d_rpb;
+  DVec<double> d_wgt;
+  // clusters
+  std::vector<int> h_nlcl;
+  std::vector<int64_t> h_la_off, h_Aoff;
+  DVec<int> d_gc_src, d_nlcl, d_nrcl, d_d0, d_nlen;
+  DVec<int64_t> d_gc_off, d_la_off, d_Aoff;
+  DVec<double> d_Ainv;
+  int max_cl_len = 0;
+  // work vectors
+  DVec<double> lam, g, w, Fw, Pg, z, Pz, dvec, yb, cvec, avec, gc, la, v6, h, hg, alpha, part, X1, vloc, xloc, ug;
+  DVec<Scalars> sc;
+  int nparts = 0;
+  bool lumped = false;
+  std::vector<int> h_fix, h_def;
+
+  void build_meta();
+  void factor_subdomains();
+  void build_bc_and_coarse();
+  void partial_cholesky(double* M, const std::vector<int64_t>& moff, const std::vector<int>& ld,
+                        const std::vector<int>& start, const std::vector<int>& ncols, const std::vector<int>& end);
+  // solve pieces (device vectors)
+  void F_apply(const double* w_in, double* out, const double* sub_from, const double* dotw, const int* done);
+  void proj(const double* x, double* out, double* hbuf, double* alpha_out, const double* dotv, const int* done);
+  void M_apply(const double* x, double* out, const int* done);
+  void kplus_b(const double* vloc_in);                     // yb = (K^+ v)_b
+  void kplus_full(const double* vloc_in, double* xout);    // xout = K^+ v (packed local)
+  void cluster_solve_from_yb(const double* vloc_in);       // la from yb and R^T v
+};
+
+}  // namespace hfls

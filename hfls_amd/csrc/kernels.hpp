@@ -1,0 +1,113 @@
+// Launch wrappers of the gfx950 kernels (kernels.hip, kernels2.hip).  Every
+// wrapper enqueues on the given stream and returns the launch status.  Arrays
+// named *off hold per-matrix (per-subdomain) offsets into packed buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hfls {
+
+struct PanelDesc {
+  int k0, w, end, active;  // panel columns [k0,k0+w), trailing rows/cols up to end
+};
+
+struct AsmSub {  // box subdomain for element assembly
+  int e0[3], ne[3], n, pad;
+  double E;
+};
+
+struct Scalars {  // PCPG scalars kept on the device
+  double rho, gamma, gPz, gPz_prev, wFw, norm0, eps, gPg0;
+  int it, done, max_iter, err;
+};
+
+// ---------------------------------------------------------------- setup
+hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const int64_t* moff, const int* ld,
+                           const int* pinv, const int64_t* pinv_off, double* M, double* f, const int64_t* f_off,
+                           double* kdiag, double dx, double dy, double dz, double nu, double ax, double ay,
+                           double az);
+hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* desc, const int64_t* moff,
+                        const int* ld, double* M, int* err);
+hipError_t launch_syrk(hipStream_t st, int nmat, int maxpairs, const PanelDesc* desc, const int64_t* moff,
+                       const int* ld, double* M);
+hipError_t launch_copy_block(hipStream_t st, int nmat, int maxn, const double* src, const int64_t* soff,
+                             const int* sld, const int* srow0, const int* n, double* dst, const int64_t* doff,
+                             const int* dld, double scale, int aug);
+hipError_t launch_detect(hipStream_t st, int nmat, const double* M, const int64_t* moff, const int* ld,
+                         const int* row0, const int* m, const double* kdiag, const int64_t* kd_off,
+                         const int* perm, const int* prow0, double* Rs, const int64_t* rs_off, const int* ldr,
+                         int* defect);
+hipError_t launch_coldot(hipStream_t st, int nmat, int maxcols, const double* L, const int64_t* loff,
+                         const int* ld, const int* row0, const int* mrows, const int* ncols, const double* Y,
+                         const int64_t* yoff, const int* ldy, double* out, const int64_t* ooff, const int* ldo,
+                         int nrhs, double alpha, double beta);
+hipError_t launch_bwd(hipStream_t st, int nmat, const double* L, const int64_t* loff, const int* ld,
+                      const int* m, double* X, const int64_t* xoff, const int* ldx, int nrhs);
+hipError_t launch_fwd(hipStream_t st, int nmat, const double* L, const int64_t* loff, const int* ld,
+                      const int* m, const int* mall, double* X, const int64_t* xoff, const int* ldx, int nrhs);
+hipError_t launch_gs_pivots(hipStream_t st, int nmat, double* R, const int64_t* roff, const int* n,
+                            const int* perm, const int64_t* perm_off, const int* nr, const int* ns,
+                            const int* nbs, double* Rb, const int64_t* rb_off, double* Rblk,
+                            const int64_t* rblk_off, const double* kdiag, const int64_t* kd_off, double* M2,
+                            const int64_t* m2off, const int* ld2, int* fixpos);
+hipError_t launch_permute(hipStream_t st, int nsub, int maxlen, int scatter, const int* len, const int* perm,
+                          const int64_t* poff, const double* src, const int64_t* soff, double* dst,
+                          const int64_t* doff, int nrhs, int64_t sstride, int64_t dstride);
+hipError_t launch_copy_cols(hipStream_t st, int nmat, int maxcols, const int* rows, const int* cols,
+                            const double* src, const int64_t* soff, const int* lds, double* dst,
+                            const int64_t* doff, const int* ldd);
+hipError_t launch_scale_lower(hipStream_t st, int nmat, int maxcols, const int* row0, const int* n, double* M,
+                              const int64_t* moff, const int* ld, double s);
+hipError_t launch_gemm_nn(hipStream_t st, int nmat, int maxn, int maxcol, const int* n, const int* ncol,
+                          const double* A, const int64_t* aoff, const int* lda, const double* X,
+                          const int64_t* xoff, const int* ldx, double* Y, const int64_t* yoff, const int* ldy);
+hipError_t launch_tdots(hipStream_t st, int nmat, int maxpairs, const int* np, const int* ni, const int* nj,
+                        const double* X, const int64_t* xoff, const int* ldx, const double* Y, const int64_t* yoff,
+                        const int* ldy, double* out, const int64_t* ooff, const int* ldo, double alpha);
+
+// ---------------------------------------------------------------- solve
+hipError_t launch_gemv_gather(hipStream_t st, int nsub, int maxrows, const double* A, const int64_t* aoff,
+                              const int* lda, const int* ns, const int64_t* boff, const int* gptr,
+                              const int* gidx, const double* v, const double* wgt, const double* xb,
+                              const int64_t* xboff, double* y, const int* done);
+hipError_t launch_corr_dots(hipStream_t st, int nsub, int maxns, const double* C, const int64_t* coff,
+                            const int* ldc, const int* nlc, const int64_t* lcoff, const double* Rb,
+                            const int64_t* rboff, const int* ns, const int64_t* boff, const int* gptr,
+                            const int* gidx, const double* v, const double* xb, double* cvec, double* avec,
+                            const int* done);
+hipError_t launch_cluster_gc(hipStream_t st, int ncl, int maxlen, const int* gc_src, const int64_t* gc_off,
+                             const int* nlcl, const int* nrcl, const int* d0, const double* cvec,
+                             const double* avec, double* gc, const int64_t* la_off, const int* done);
+hipError_t launch_cluster_gemv(hipStream_t st, int ncl, int maxlen, const double* Ainv, const int64_t* aoff,
+                               const int* nlen, const double* gc, const int64_t* la_off, double* la,
+                               const int* done);
+hipError_t launch_corr_apply(hipStream_t st, int nsub, int maxrows, const double* C, const int64_t* coff,
+                             const int* ldc, const int* nlc, const int64_t* lcoff, const int* lcmap,
+                             const double* Rb, const int64_t* rboff, const int* ns, const int64_t* boff,
+                             const int64_t* alpha_idx, const double* la, double* y, const int* done);
+hipError_t launch_scatter(hipStream_t st, int64_t nlam, const int* ra, const int* rb, const double* y,
+                          const double* wgt, const double* sub_from, double* out, const double* dotw,
+                          double* part, const int* done);
+hipError_t launch_reduce(hipStream_t st, int nparts, const double* part, Scalars* sc, int mode);
+hipError_t launch_axpy2(hipStream_t st, int64_t nlam, double* lam, double* g, const double* w,
+                        const double* Fw, const Scalars* sc);
+hipError_t launch_proj_dots(hipStream_t st, int nsub, const double* Rb, const int64_t* rboff, const int* ns,
+                            const int64_t* boff, const int* gptr, const int* gidx, const double* x,
+                            double* v, const int* done);
+hipError_t launch_proj_coarse(hipStream_t st, int nsub, int ncol, const double* v, const double* H,
+                              const int* cl, const double* invGtG, double* h, double* alpha, double sgn,
+                              const int* done);
+hipError_t launch_gh(hipStream_t st, int64_t nlam, const int* rsub_a, const int* rsub_b, const int* rpos_a,
+                     const int* rpos_b, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
+                     const double* x, double* out, double sgn, const double* dotv, double* part,
+                     const int* done);
+hipError_t launch_update_w(hipStream_t st, int64_t nlam, double* w, const double* Pz, const Scalars* sc);
+hipError_t launch_sub_bt(hipStream_t st, int nsub, int maxns, const int* nr, const int* ns, const int* perm,
+                         const int64_t* noff, const int64_t* boff, const int* gptr, const int* gidx,
+                         const double* lam, const double* BcT, const int64_t* coff, const int* ldc, const int* nlc,
+                         const int64_t* lcoff, const int* lcmap, const double* la, double* v);
+hipError_t launch_recover(hipStream_t st, int nsub, int maxn, const int* n, const int64_t* noff, const double* R,
+                          const int64_t* roff, const int64_t* alpha_idx, const double* la, const double* h,
+                          const double* x, const int* gdof, const unsigned char* owner, double* u);
+
+}  // namespace hfls

@@ -1,0 +1,95 @@
+/* C ABI of the MI355X HTFETI hot path (libhfls.so).
+ *
+ * Conventions (mirroring the reference's Matrix/Cluster layout):
+ *   - every function returns 0 on success and a negative code on failure; the
+ *     message is available from hfls_last_error() (thread local).  Nothing calls
+ *     exit(), unlike the reference's PARDISO wrapper (Matrix.cpp:1174-1178).
+ *   - options are the reference's options2 map (htfeti.cpp:12-57) passed as
+ *     parallel key/value string arrays; unknown keys are rejected.
+ *   - dual vectors (lambda, g, w) have nlam entries in the reference's Bf row
+ *     order (Cluster.cpp:846-924); coarse vectors have 6*n_clusters entries.
+ *   - "packed local" vectors concatenate per-subdomain vectors in the l2g order of
+ *     Data.cpp:111-144 (sorted global DOFs), subdomains ascending.
+ *   - all arrays are caller owned host memory; FP64.
+ *   - a handle is not thread safe; one handle per host thread.
+ */
+#ifndef HFLS_GPU_H
+#define HFLS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfls_problem_s* hfls_problem;  /* host-side mesh + gluing (integer work) */
+typedef struct hfls_mesh_s* hfls_mesh;        /* device-resident set-up mesh */
+
+typedef struct {
+  int iterations;    /* number of F applications in the PCPG loop (Cluster.cpp:1422-1450) */
+  int converged;     /* 1 if sqrt(g'Pz) < eps*sqrt(g0'Pz0) (Cluster.cpp:1425) */
+  double residual;   /* sqrt(g'Pz)/sqrt(g0'Pz0) at exit */
+  double solve_ms;   /* device time of the whole load-case solve */
+  double pcpg_ms;    /* device time of the iteration loop */
+} hfls_stats;
+
+const char* hfls_last_error(void);
+
+/* ---- host problem: Mesh::createMesh (Mesh.cpp:69-289), Data::buildMappingStruct
+ *      (Data.cpp:111-144), Cluster::create_cluster_constraints (Cluster.cpp:682-769) */
+int hfls_problem_create(int nopt, const char* const* keys, const char* const* vals, hfls_problem* out);
+int hfls_problem_destroy(hfls_problem p);
+/* sizes[0..7] = nsub, nlam, n_glue, nPoints, n_clusters, nsub_per_cluster, n_dirichlet, n_corner_dofs */
+int hfls_problem_sizes(hfls_problem p, int64_t* sizes);
+/* info[0..7] = n, n_r, n_s, n_bs, n_ss, cluster, n_bf_entries, E bits unused (0) */
+int hfls_problem_sub_info(hfls_problem p, int d, int* info);
+/* compressed Bf_d (Cluster.cpp:931-959): global row, local DOF, sign (+1/-1) */
+int hfls_problem_sub_bf(hfls_problem p, int d, int* rows, int* dofs, int* signs);
+/* j_col_cmpr (Cluster.cpp:756-765), s-set used by the kernel detection, l2g */
+int hfls_problem_sub_bdofs(hfls_problem p, int d, int* bdofs);
+int hfls_problem_sub_sset(hfls_problem p, int d, int* sset);
+int hfls_problem_sub_l2g(hfls_problem p, int d, int64_t* l2g);
+int hfls_problem_sub_young(hfls_problem p, int d, double* E);
+/* dual rows: subdomain of the +1 entry, of the -1 entry (-1 for Dirichlet rows),
+ * global DOF, multiplicity weight (Cluster.cpp:893-895,918) */
+int hfls_problem_rows(hfls_problem p, int* sub_a, int* sub_b, int64_t* dof, double* weight);
+int hfls_problem_dirichlet(hfls_problem p, int64_t* dofs);
+int hfls_problem_corners(hfls_problem p, int64_t* dofs);
+
+/* ---- device mesh: replaces Cluster::initialization's factorisation and coarse
+ *      setup (Cluster.cpp:68-352) and the PARDISO seam (Matrix.cpp:1029-1287) */
+int hfls_mesh_create(int device, int nopt, const char* const* keys, const char* const* vals, hfls_mesh* out);
+int hfls_mesh_destroy(hfls_mesh m);
+/* sizes[0..5] = nsub, nlam, ncol_G, packed_local_length, nPoints, device_bytes */
+int hfls_mesh_sizes(hfls_mesh m, int64_t* sizes);
+/* setup timings: [0] setup_ms, [1] factor_ms */
+int hfls_mesh_setup_times(hfls_mesh m, double* t);
+/* kernel R (packed local, n x 6 column major per subdomain), fixing DOFs (6 local
+ * DOFs per subdomain), detected defect per subdomain, body-force loads (packed local) */
+int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect);
+int hfls_mesh_load(hfls_mesh m, double* f);
+
+/* operators: Cluster::mult_Ff (Cluster.cpp:1114-1188), Projection (1314-1329,
+ * alpha as returned there), Preconditioning (1043-1112), mult_Kplus_f (967-1040) */
+int hfls_apply_F(hfls_mesh m, const double* w, double* Fw);
+int hfls_project(hfls_mesh m, const double* x, double* Px, double* alpha);
+int hfls_precondition(hfls_mesh m, const double* x, double* Mx);
+int hfls_kplus(hfls_mesh m, const double* f_local, double* x_local);
+
+/* one load case: Cluster::pcpg (Cluster.cpp:1363-1458) + the recovery of
+ * printVTK (Cluster.cpp:1648-1678).  f_local may be NULL (body force of the
+ * options); u has 3*nPoints entries, lambda nlam (either may be NULL). */
+int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f_local, double* u, double* lambda,
+               hfls_stats* stats);
+
+/* Measurement hook (no counterpart in the reference): average device time of
+ * the dominant kernel, the interface gather + dense GEMV that streams K+_bb
+ * (which=0, inside mult_Ff) or S_i (which=1, inside Preconditioning), over reps
+ * back-to-back launches timed with HIP events on the library's stream; bytes =
+ * algorithmic bytes of one launch. */
+int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFLS_GPU_H */
