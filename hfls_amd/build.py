@@ -48,6 +48,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, OUT)
+    # the reference's main() over the drop-in C++ facade (include/hfls/htfeti.hpp)
+    root = os.path.dirname(HERE)
+    exe = os.path.join(HERE, "htfeti")
+    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(root, "include"),
+           os.path.join(CSRC, "htfeti_main.cpp"), "-L" + HERE, "-lhfls",
+           "-Wl,-rpath,$ORIGIN", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("facade build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     if verbose:
         print("built", OUT)
     return OUT
