@@ -242,17 +242,20 @@ Engine::~Engine() {
 
 void Engine::build_meta() {
   const int N = P->nsub;
-  h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N); h_cl.resize(N);
+  h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N); h_ldt.resize(N);
+  h_cl.resize(N);
   off_n.assign(N + 1, 0); off_b.assign(N + 1, 0);
-  off_K.assign(N + 1, 0); off_Lr.assign(N + 1, 0); off_R.assign(N + 1, 0); off_Rb.assign(N + 1, 0);
+  off_K.assign(N + 1, 0); off_T.assign(N + 1, 0); off_Kr.assign(N + 1, 0); off_R.assign(N + 1, 0);
+  off_Rb.assign(N + 1, 0);
   for (int d = 0; d < N; d++) {
     const Subdomain& s = P->subs[d];
     h_n[d] = s.n; h_nr[d] = s.n_r; h_ns[d] = s.n_s; h_nbs[d] = s.n_bs; h_nss[d] = s.n_ss;
-    h_ldb[d] = round8(s.n_s); h_cl[d] = s.cluster;
+    h_ldb[d] = round8(s.n_s); h_ldt[d] = round8(s.n_r); h_cl[d] = s.cluster;
     off_n[d + 1] = off_n[d] + s.n;
     off_b[d + 1] = off_b[d] + s.n_s;
     off_K[d + 1] = off_K[d] + align32((int64_t)h_ldb[d] * s.n_s);
-    off_Lr[d + 1] = off_Lr[d] + align32((int64_t)s.n * s.n_r);
+    off_T[d + 1] = off_T[d] + align32((int64_t)h_ldt[d] * s.n_s);
+    off_Kr[d + 1] = off_Kr[d] + align32((int64_t)h_ldt[d] * s.n_r);
     off_R[d + 1] = off_R[d] + 6 * (int64_t)s.n;
     off_Rb[d + 1] = off_Rb[d] + 6 * (int64_t)s.n_s;
     maxn = std::max(maxn, s.n); maxns = std::max(maxns, s.n_s); maxnr = std::max(maxnr, s.n_r);
@@ -260,12 +263,13 @@ void Engine::build_meta() {
   if (maxns > 8000) throw std::runtime_error("interface too large for the LDS-staged gather (n_s > 8000)");
   NB = off_b[N];
   d_n.upload(h_n, st); d_nr.upload(h_nr, st); d_ns.upload(h_ns, st); d_nbs.upload(h_nbs, st);
-  d_ldb.upload(h_ldb, st); d_cl.upload(h_cl, st);
+  d_ldb.upload(h_ldb, st); d_ldt.upload(h_ldt, st); d_cl.upload(h_cl, st);
   d_ldn.upload(h_n, st);
   d_off_n.upload(std::vector<int64_t>(off_n.begin(), off_n.end() - 1), st);
   d_off_b.upload(std::vector<int64_t>(off_b.begin(), off_b.end() - 1), st);
   d_off_K.upload(std::vector<int64_t>(off_K.begin(), off_K.end() - 1), st);
-  d_off_Lr.upload(std::vector<int64_t>(off_Lr.begin(), off_Lr.end() - 1), st);
+  d_off_T.upload(std::vector<int64_t>(off_T.begin(), off_T.end() - 1), st);
+  d_off_Kr.upload(std::vector<int64_t>(off_Kr.begin(), off_Kr.end() - 1), st);
   d_off_R.upload(std::vector<int64_t>(off_R.begin(), off_R.end() - 1), st);
   d_off_Rb.upload(std::vector<int64_t>(off_Rb.begin(), off_Rb.end() - 1), st);
   std::vector<int64_t> xb(N), v6(N);
@@ -274,6 +278,7 @@ void Engine::build_meta() {
   d_v6off.upload(v6, st);
   d_ones.upload(std::vector<int>(N, 1), st);
   d_sixes.upload(std::vector<int>(N, 6), st);
+  d_zeros.upload(std::vector<int>(N, 0), st);
   // permutation (interior-first order -> local DOF), global DOF ids, owners
   std::vector<int> perm(off_n[N]), gdof(off_n[N]);
   std::vector<unsigned char> owner(off_n[N]);
@@ -357,7 +362,8 @@ void Engine::factor_subdomains() {
   d_kdiag.alloc(off_n[N]);
   d_f.alloc(off_n[N]);
   HC(hipMemsetAsync(d_f.p, 0, off_n[N] * sizeof(double), st));
-  d_Lr.alloc(off_Lr[N]);
+  d_T.alloc(off_T[N]);
+  d_Kinv.alloc(off_Kr[N]);
   d_K.alloc(off_K[N]);
   d_S.alloc(off_K[N]);
   d_R.alloc(off_R[N]);
@@ -378,42 +384,50 @@ void Engine::factor_subdomains() {
     int c1 = c0;
     size_t bytes = 0;
     while (c1 < N) {
-      size_t b = 8 * (align32((int64_t)h_n[c1] * h_n[c1]) + align32(4 * (int64_t)h_ns[c1] * h_ns[c1]) + 6 * h_n[c1]);
+      const int64_t l1 = h_n[c1] + h_nr[c1];
+      size_t b = 8 * (align32(l1 * l1) + align32(4 * (int64_t)h_ns[c1] * h_ns[c1]) + 6 * h_n[c1]);
       if (c1 > c0 && bytes + b > budget) break;
       bytes += b;
       c1++;
     }
     const int nm = c1 - c0;
-    std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), lroff(nm), koff(nm), roff(nm), rboff(nm);
-    std::vector<int> ld1(nm), ld2(nm), nrv(nm), nsv(nm), nbsv(nm), nssv(nm), nv(nm), ldb(nm), zero(nm, 0), ones(nm, 1);
+    std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), toff(nm), kroff(nm), koff(nm), roff(nm), rboff(nm),
+        t_src(nm);
+    std::vector<int> ld1(nm), ld2(nm), nrv(nm), nsv(nm), nbsv(nm), nssv(nm), nv(nm), ldb(nm), ldt(nm), end1(nm),
+        zero(nm, 0), ones(nm, 1), six(nm, 6), ldn(nm);
     std::vector<int> start_bs(nm), prow0(nm), r0s(nm);
     std::vector<AsmSub> asub(nm);
     int64_t o1 = 0, o2 = 0, o3 = 0;
     for (int i = 0; i < nm; i++) {
       int d = c0 + i;
       const Subdomain& s = P->subs[d];
-      m1off[i] = o1; o1 += align32((int64_t)s.n * s.n);
+      const int64_t l1 = s.n + s.n_r;
+      m1off[i] = o1; o1 += align32(l1 * l1);
       m2off[i] = o2; o2 += align32(4 * (int64_t)s.n_s * s.n_s);
       r1off[i] = o3; o3 += 6 * (int64_t)s.n;
-      noff[i] = off_n[d]; lroff[i] = off_Lr[d]; koff[i] = off_K[d]; roff[i] = off_R[d]; rboff[i] = off_Rb[d];
-      ld1[i] = s.n; ld2[i] = 2 * s.n_s; nrv[i] = s.n_r; nsv[i] = s.n_s; nbsv[i] = s.n_bs; nssv[i] = s.n_ss;
-      nv[i] = s.n; ldb[i] = h_ldb[d];
+      noff[i] = off_n[d]; toff[i] = off_T[d]; kroff[i] = off_Kr[d]; koff[i] = off_K[d]; roff[i] = off_R[d];
+      rboff[i] = off_Rb[d];
+      t_src[i] = m1off[i] + s.n + (int64_t)s.n_r * l1;  // block (n.., n_r..) = -K_rr^-1 K_rb
+      ld1[i] = (int)l1; ld2[i] = 2 * s.n_s; nrv[i] = s.n_r; nsv[i] = s.n_s; nbsv[i] = s.n_bs; nssv[i] = s.n_ss;
+      nv[i] = s.n; ldb[i] = h_ldb[d]; ldt[i] = h_ldt[d]; end1[i] = (int)l1; ldn[i] = s.n;
       prow0[i] = s.n_r + s.n_bs;
       r0s[i] = s.n_r;
       for (int a = 0; a < 3; a++) { asub[i].e0[a] = s.e0[a]; asub[i].ne[a] = s.ne[a]; }
-      asub[i].n = s.n; asub[i].pad = 0; asub[i].E = s.E;
+      asub[i].n = s.n; asub[i].nr = s.n_r; asub[i].E = s.E;
     }
     DVec<double> M1, M2, R1;
     M1.alloc(o1); M2.alloc(o2); R1.alloc(o3);
     HC(hipMemsetAsync(M1.p, 0, o1 * sizeof(double), st));
     HC(hipMemsetAsync(M2.p, 0, o2 * sizeof(double), st));
-    DVec<int64_t> dm1, dm2, dr1, dnoff, dlr, dk, dr, drb, dnoff_r, dr1_b, dr1_s;
-    DVec<int> dld1, dld2, dnr, dns, dnbs, dnss, dn, dldb, dprow0, dr0s, dpinv;
+    DVec<int64_t> dm1, dm2, dr1, dnoff, dto, dkro, dk, dr, drb, dtsrc;
+    DVec<int> dld1, dld2, dnr, dns, dnbs, dnss, dn, dldb, dldt, dprow0, dr0s, dpinv, dsix, dldn, dnrow0;
     DVec<AsmSub> das;
     dm1.upload(m1off, st); dm2.upload(m2off, st); dr1.upload(r1off, st); dnoff.upload(noff, st);
-    dlr.upload(lroff, st); dk.upload(koff, st); dr.upload(roff, st); drb.upload(rboff, st);
+    dto.upload(toff, st); dkro.upload(kroff, st); dk.upload(koff, st); dr.upload(roff, st); drb.upload(rboff, st);
+    dtsrc.upload(t_src, st);
     dld1.upload(ld1, st); dld2.upload(ld2, st); dnr.upload(nrv, st); dns.upload(nsv, st); dnbs.upload(nbsv, st);
-    dnss.upload(nssv, st); dn.upload(nv, st); dldb.upload(ldb, st); dprow0.upload(prow0, st); dr0s.upload(r0s, st);
+    dnss.upload(nssv, st); dn.upload(nv, st); dldb.upload(ldb, st); dldt.upload(ldt, st); dprow0.upload(prow0, st);
+    dr0s.upload(r0s, st); dsix.upload(six, st); dldn.upload(ldn, st); dnrow0.upload(nv, st);
     das.alloc(nm);
     HC(hipMemcpyAsync(das.p, asub.data(), nm * sizeof(AsmSub), hipMemcpyHostToDevice, st));
     // pinv for the chunk (local DOF -> M1 position), packed with off_n offsets relative to chunk
@@ -435,14 +449,16 @@ void Engine::factor_subdomains() {
     if (lumped)  // lumped preconditioner: K_bb (Cluster.cpp:1106-1107)
       HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
     auto t0 = std::chrono::steady_clock::now();
-    // M1: eliminate the interior
+    // M1 = [K_rr K_rb I; K_br K_bb 0; I 0 0]: eliminate the interior; the trailing
+    // block is [S_bb, -(K_rr^-1 K_rb)^T ; -(K_rr^-1 K_rb), -K_rr^-1]
     {
       std::vector<int> st0(nm, 0);
-      partial_cholesky(M1.p, m1off, ld1, st0, nrv, nv);
+      partial_cholesky(M1.p, m1off, ld1, st0, nrv, end1);
     }
     if (!lumped) HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
     HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, M2.p, dm2.p, dld2.p, 1.0, 1));
-    HC(launch_copy_cols(st, nm, maxnr, dn.p, dnr.p, M1.p, dm1.p, dld1.p, d_Lr.p, dlr.p, dld1.p));
+    HC(launch_copy_cols(st, nm, maxns, dnr.p, dns.p, M1.p, dtsrc.p, dld1.p, d_T.p, dto.p, dldt.p, -1.0));
+    HC(launch_copy_block(st, nm, maxnr, M1.p, dm1.p, dld1.p, dnrow0.p, dnr.p, d_Kinv.p, dkro.p, dldt.p, -1.0, 0));
     // M2: eliminate b \ s
     {
       std::vector<int> st0(nm, 0), end2(nm);
@@ -463,10 +479,9 @@ void Engine::factor_subdomains() {
       HC(launch_coldot(st, nm, maxns, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, dnbs.p, R1.p, drs.p, dld1.p, R1.p, drbo.p,
                        dld1.p, 6, -1.0, 0.0));
       HC(launch_bwd(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, R1.p, drbo.p, dld1.p, 6));
-      // R_r = -L_rr^{-T} (W R_b)
-      HC(launch_coldot(st, nm, maxnr, M1.p, dm1.p, dld1.p, dnr.p, dns.p, dnr.p, R1.p, drbo.p, dld1.p, R1.p, dr1.p,
-                       dld1.p, 6, -1.0, 0.0));
-      HC(launch_bwd(st, nm, M1.p, dm1.p, dld1.p, dnr.p, R1.p, dr1.p, dld1.p, 6));
+      // R_r = -(K_rr^-1 K_rb) R_b  (Matrix.cpp:2393-2421)
+      HC(launch_gemm_nn(st, nm, maxnr, 6, dnr.p, dns.p, dsix.p, d_T.p, dto.p, dldt.p, R1.p, drbo.p, dldn.p, R1.p,
+                        dr1.p, dldn.p, -1.0));
       HC(launch_gs_pivots(st, nm, d_R.p, dr.p, dn.p, d_perm1.p, dnoff.p, dnr.p, dns.p, dnbs.p, d_Rb.p, drb.p, R1.p,
                           dr1.p, d_kdiag.p, dnoff.p, M2.p, dm2.p, dld2.p, d_fixpos.p + 6 * c0));
       HC(hipStreamSynchronize(st));
@@ -653,8 +668,8 @@ void Engine::build_bc_and_coarse() {
   // C_i = K+_bb Bc_i^T ; Fc_i = Bc_i C_i ; Gc_i = -Bc_i R_b
   d_C.alloc(off_C[N]);
   HC(hipMemsetAsync(d_C.p, 0, off_C[N] * sizeof(double), st));
-  HC(launch_gemm_nn(st, N, maxns, maxnlc, d_ns.p, d_nlc.p, d_K.p, d_off_K.p, d_ldb.p, d_BcT.p, d_off_C.p, d_ldb.p,
-                    d_C.p, d_off_C.p, d_ldb.p));
+  HC(launch_gemm_nn(st, N, maxns, maxnlc, d_ns.p, nullptr, d_nlc.p, d_K.p, d_off_K.p, d_ldb.p, d_BcT.p, d_off_C.p,
+                    d_ldb.p, d_C.p, d_off_C.p, d_ldb.p, 1.0));
   std::vector<int64_t> fcoff(N + 1, 0), gcoff(N + 1, 0);
   for (int d = 0; d < N; d++) {
     fcoff[d + 1] = fcoff[d] + (int64_t)h_nlc[d] * h_nlc[d];
@@ -842,6 +857,7 @@ void Engine::setup() {
   nparts = (int)((nl + 255) / 256);
   part.alloc(nparts);
   X1.alloc(off_n[N]);
+  X2.alloc(off_n[N]);
   vloc.alloc(off_n[N]);
   xloc.alloc(off_n[N]);
   ug.alloc(3 * P->nPoints);
@@ -850,7 +866,7 @@ void Engine::setup() {
   sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   size_t freeb = 0, totb = 0;
   HC(hipMemGetInfo(&freeb, &totb));
-  sinfo.device_bytes = (off_Lr[N] + 2 * off_K[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
+  sinfo.device_bytes = (off_T[N] + off_Kr[N] + 2 * off_K[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
 }
 
 // ------------------------------------------------------------ solve pieces
@@ -862,7 +878,7 @@ void Engine::F_apply(const double* win, double* out, const double* sub_from, con
                       d_off_b.p, d_gptr.p, d_gidx.p, win, nullptr, cvec.p, avec.p, done));
   HC(launch_cluster_gc(st, P->ncl, max_cl_len, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, cvec.p, avec.p,
                        gc.p, d_la_off.p, done));
-  HC(launch_cluster_gemv(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, done));
+  HC(launch_sym_gemv_waves(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, done));
   HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
                        d_ns.p, d_off_b.p, d_alpha_idx.p, la.p, yb.p, done));
   HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, nullptr, sub_from, out, dotw, part.p, done));
@@ -884,38 +900,38 @@ void Engine::M_apply(const double* x, double* out, const int* done) {
   HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, d_wgt.p, nullptr, out, nullptr, nullptr, done));
 }
 
-void Engine::kplus_b(const double* vin) {
+void Engine::prep_t(const double* vin) {
   const int N = P->nsub;
   HC(launch_permute(st, N, maxn, 0, d_n.p, d_perm1.p, d_off_n.p, vin, d_off_n.p, X1.p, d_off_n.p, 1, 0, 0));
-  HC(launch_fwd(st, N, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, d_n.p, X1.p, d_off_n.p, d_ldn.p, 1));
-  HC(launch_gemv_gather(st, N, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr,
-                        nullptr, X1.p, d_xboff.p, yb.p, nullptr));
+  // X1[nr:n] = v_b - (K_rr^-1 K_rb)^T v_r
+  HC(launch_coldot(st, N, maxns, d_T.p, d_off_T.p, d_ldt.p, d_zeros.p, d_nr.p, d_ns.p, X1.p, d_off_n.p, d_ldn.p, X1.p,
+                   d_xboff.p, d_ldn.p, 1, -1.0, 1.0));
 }
 
-void Engine::kplus_full(const double* vin, double* xout) {
-  const int N = P->nsub;
-  kplus_b(vin);
-  // X1[0:nr] = y_r - W x_b ; then L_rr^T x_r = X1[0:nr]
-  HC(launch_coldot(st, N, maxnr, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, d_ns.p, d_nr.p, yb.p, d_off_b.p, d_ns.p, X1.p,
-                   d_off_n.p, d_ldn.p, 1, -1.0, 1.0));
-  HC(launch_bwd(st, N, d_Lr.p, d_off_Lr.p, d_ldn.p, d_nr.p, X1.p, d_off_n.p, d_ldn.p, 1));
-  // X1[nr:n] = x_b
-  HC(launch_copy_cols(st, N, 1, d_ns.p, d_ones.p, yb.p, d_off_b.p, d_ns.p, X1.p, d_xboff.p, d_ns.p));
-  HC(launch_permute(st, N, maxn, 1, d_n.p, d_perm1.p, d_off_n.p, X1.p, d_off_n.p, xout, d_off_n.p, 1, 0, 0));
+void Engine::kplus_b_from_t() {
+  HC(launch_gemv_gather(st, P->nsub, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p,
+                        nullptr, nullptr, X1.p, d_xboff.p, yb.p, nullptr));
 }
 
-void Engine::cluster_solve_from_yb(const double* vin) {
+void Engine::cluster_solve_t(const double* vin) {
   const int N = P->nsub;
-  const DVec<int>& one_col = d_ones;
-  const DVec<int>& six_col = d_sixes;
-  // c_i = Bc_i (K^+ v)_b ; a_i = R_i^T v_i (full)
-  HC(launch_tdots(st, N, maxnlc, d_ns.p, d_nlc.p, one_col.p, d_BcT.p, d_off_C.p, d_ldb.p, yb.p, d_off_b.p, d_ns.p,
-                  cvec.p, d_off_lc.p, d_ns.p, 1.0));
-  HC(launch_tdots(st, N, 6, d_n.p, six_col.p, one_col.p, d_R.p, d_off_R.p, d_ldn.p, vin, d_off_n.p, d_ldn.p, avec.p,
+  // c_i = Bc_i (K^+ v)_b = C_i^T t_i ; a_i = R_i^T v_i (full)
+  HC(launch_tdots(st, N, maxnlc, d_ns.p, d_nlc.p, d_ones.p, d_C.p, d_off_C.p, d_ldb.p, X1.p, d_xboff.p, d_ns.p, cvec.p,
+                  d_off_lc.p, d_ns.p, 1.0));
+  HC(launch_tdots(st, N, 6, d_n.p, d_sixes.p, d_ones.p, d_R.p, d_off_R.p, d_ldn.p, vin, d_off_n.p, d_ldn.p, avec.p,
                   d_v6off.p, d_ns.p, 1.0));
   HC(launch_cluster_gc(st, P->ncl, max_cl_len, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, cvec.p, avec.p,
                        gc.p, d_la_off.p, nullptr));
-  HC(launch_cluster_gemv(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, nullptr));
+  HC(launch_sym_gemv_waves(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, nullptr));
+}
+
+void Engine::interior(double* xout) {
+  const int N = P->nsub;
+  // X2[0:nr] = K_rr^-1 v_r - (K_rr^-1 K_rb) yb ; X2[nr:n] = yb
+  HC(launch_gemv_two(st, N, maxnr, std::max(maxnr, maxns), d_nr.p, d_nr.p, d_ns.p, d_Kinv.p, d_off_Kr.p, d_ldt.p,
+                     d_T.p, d_off_T.p, d_ldt.p, X1.p, d_off_n.p, yb.p, d_off_b.p, -1.0, X2.p, d_off_n.p));
+  HC(launch_copy_cols(st, N, 1, d_ns.p, d_ones.p, yb.p, d_off_b.p, d_ns.p, X2.p, d_xboff.p, d_ns.p, 1.0));
+  HC(launch_permute(st, N, maxn, 1, d_n.p, d_perm1.p, d_off_n.p, X2.p, d_off_n.p, xout, d_off_n.p, 1, 0, 0));
 }
 
 // ------------------------------------------------------------ public operators
@@ -950,15 +966,14 @@ void Engine::precondition(const double* x_h, double* Mx_h) {
 void Engine::kplus_local(const double* f_h, double* x_h) {
   const int N = P->nsub;
   HC(hipMemcpyAsync(vloc.p, f_h, off_n[N] * sizeof(double), hipMemcpyHostToDevice, st));
-  kplus_b(vloc.p);
-  cluster_solve_from_yb(vloc.p);
-  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr, d_BcT.p,
-                   d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, la.p, vloc.p));
-  kplus_full(vloc.p, xloc.p);
-  // x += R alpha (h = 0)
+  // K_c^+ f = K^+ (f - Bc^T lam_c) + R alpha  (Cluster.cpp:967-1040)
+  prep_t(vloc.p);
+  cluster_solve_t(vloc.p);
+  kplus_b_from_t();
+  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
+                       d_ns.p, d_off_b.p, nullptr, la.p, yb.p, nullptr));
+  interior(xloc.p);
   HC(hipMemsetAsync(hg.p, 0, 6 * N * sizeof(double), st));
-  std::vector<int> gd;
-  // add R alpha through the recovery kernel into a packed copy: reuse xloc with owner-free path
   DVec<int> iota_;
   std::vector<int> io(off_n[N]);
   std::iota(io.begin(), io.end(), 0);
@@ -1040,8 +1055,9 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   int* done = &sc.p->done;
   HC(hipEventRecord(e0, st));
   // d = Bf K_c^+ f  (Cluster.cpp:1385-1387)
-  kplus_b(f);
-  cluster_solve_from_yb(f);
+  prep_t(f);
+  kplus_b_from_t();
+  cluster_solve_t(f);
   HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
                        d_ns.p, d_off_b.p, d_alpha_idx.p, la.p, yb.p, nullptr));
   HC(launch_scatter(st, nl, d_ra.p, d_rb.p, yb.p, nullptr, nullptr, dvec.p, nullptr, nullptr, nullptr));
@@ -1081,11 +1097,12 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   HC(hipMemcpyAsync(vloc.p, f, off_n[N] * sizeof(double), hipMemcpyDeviceToDevice, st));
   HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, lam.p, nullptr,
                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vloc.p));
-  kplus_b(vloc.p);
-  cluster_solve_from_yb(vloc.p);
-  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, nullptr, d_BcT.p,
-                   d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, la.p, vloc.p));
-  kplus_full(vloc.p, xloc.p);
+  prep_t(vloc.p);
+  cluster_solve_t(vloc.p);
+  kplus_b_from_t();
+  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
+                       d_ns.p, d_off_b.p, nullptr, la.p, yb.p, nullptr));
+  interior(xloc.p);
   HC(hipMemsetAsync(ug.p, 0, 3 * P->nPoints * sizeof(double), st));
   HC(launch_recover(st, N, maxn, d_n.p, d_off_n.p, d_R.p, d_off_R.p, d_alpha_idx.p, la.p, hg.p, xloc.p, d_gdof.p,
                     d_owner.p, ug.p));

@@ -86,15 +86,15 @@ class Engine {
   int maxn = 0, maxns = 0, maxnr = 0, maxnlc = 0;
   int64_t NB = 0;
   // per-subdomain meta (host copies)
-  std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_cl, h_nlc;
-  std::vector<int64_t> off_K, off_Lr, off_R, off_Rb, off_C, off_lc;
+  std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_ldt, h_cl, h_nlc;
+  std::vector<int64_t> off_K, off_T, off_Kr, off_R, off_Rb, off_C, off_lc;
   // device meta
-  DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_cl, d_nlc, d_ldn, d_ones, d_sixes;
-  DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_Lr, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
+  DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_ldt, d_cl, d_nlc, d_ldn, d_ones, d_sixes, d_zeros;
+  DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_T, d_off_Kr, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
       d_v6off;
   DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap;
   DVec<unsigned char> d_owner;
-  DVec<double> d_kdiag, d_f, d_Lr, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
+  DVec<double> d_kdiag, d_f, d_T, d_Kinv, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
   DVec<int> d_fixpos, d_defect, d_err;
   // dual rows
   DVec<int> d_ra, d_rb, d_rsa, d_rsb, d_rpa, d_rpb;
@@ -107,7 +107,7 @@ class Engine {
   DVec<double> d_Ainv;
   int max_cl_len = 0;
   // work vectors
-  DVec<double> lam, g, w, Fw, Pg, z, Pz, dvec, yb, cvec, avec, gc, la, v6, h, hg, alpha, part, X1, vloc, xloc, ug;
+  DVec<double> lam, g, w, Fw, Pg, z, Pz, dvec, yb, cvec, avec, gc, la, v6, h, hg, alpha, part, X1, X2, vloc, xloc, ug;
   DVec<Scalars> sc;
   int nparts = 0;
   bool lumped = false;
@@ -122,9 +122,10 @@ class Engine {
   void F_apply(const double* w_in, double* out, const double* sub_from, const double* dotw, const int* done);
   void proj(const double* x, double* out, double* hbuf, double* alpha_out, const double* dotv, const int* done);
   void M_apply(const double* x, double* out, const int* done);
-  void kplus_b(const double* vloc_in);                     // yb = (K^+ v)_b
-  void kplus_full(const double* vloc_in, double* xout);    // xout = K^+ v (packed local)
-  void cluster_solve_from_yb(const double* vloc_in);       // la from yb and R^T v
+  void prep_t(const double* vloc_in);        // X1 = [v_r; t_b], t_b = v_b - K_br K_rr^-1 v_r
+  void kplus_b_from_t();                     // yb = K+_bb t_b = (K^+ v)_b
+  void cluster_solve_t(const double* vloc_in);  // la = Ac^-1 [C^T t; -R^T v]
+  void interior(double* xout);               // xout = [K_rr^-1 v_r - (K_rr^-1 K_rb) yb ; yb] in local order
 };
 
 }  // namespace hfls

@@ -180,6 +180,8 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmSub* __restrict__ sub
       kd[l] = A[p + (int64_t)p * lda];
     }
   }
+  // augmented interior block [K_rr K_rb I; K_br K_bb 0; I 0 0]
+  for (int i = threadIdx.x; i < s.nr; i += 256) A[(s.n + i) + (int64_t)i * lda] = 1.0;
 }
 
 hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const int64_t* moff, const int* ld,
@@ -947,6 +949,35 @@ hipError_t launch_cluster_gemv(hipStream_t st, int ncl, int maxlen, const double
   return hipGetLastError();
 }
 
+// la[o + i] = sum_j A[j + i*n] gc[o + j], A symmetric (column i = row i): one
+// wave per output, lanes over j (coalesced), fixed butterfly reduction
+__global__ __launch_bounds__(256) void k_sym_gemv_waves(const double* __restrict__ A,
+                                                        const int64_t* __restrict__ aoff,
+                                                        const int* __restrict__ nlen, const double* __restrict__ x,
+                                                        const int64_t* __restrict__ xoff, double* __restrict__ out,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
+  const int cl = blockIdx.y;
+  const int n = nlen[cl];
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* col = A + aoff[cl] + (int64_t)i * n;
+  const double* xv = x + xoff[cl];
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += col[j] * xv[j];
+  s = wave_sum(s);
+  if (lane == 0) out[xoff[cl] + i] = s;
+}
+
+hipError_t launch_sym_gemv_waves(hipStream_t st, int ncl, int maxlen, const double* A, const int64_t* aoff,
+                                 const int* nlen, const double* x, const int64_t* xoff, double* out,
+                                 const int* done) {
+  hipLaunchKernelGGL(k_sym_gemv_waves, dim3((maxlen + 3) / 4, ncl), dim3(256), 0, st, A, aoff, nlen, x, xoff, out,
+                     done);
+  return hipGetLastError();
+}
+
 // y_i += R_b,i alpha_i - C_i lam_c,i   (Cluster.cpp:1162-1181)
 __global__ __launch_bounds__(256) void k_corr_apply(const double* __restrict__ C, const int64_t* __restrict__ coff,
                                                     const int* __restrict__ ldc, const int* __restrict__ nlc,
@@ -964,7 +995,7 @@ __global__ __launch_bounds__(256) void k_corr_apply(const double* __restrict__ C
   const int nc = nlc[d];
   if ((int)(blockIdx.x * 256) >= n) return;
   for (int k = threadIdx.x; k < nc; k += 256) lc[k] = la[lcmap[lcoff[d] + k]];
-  if (threadIdx.x < 6) al[threadIdx.x] = la[alpha_idx[d] + threadIdx.x];
+  if (threadIdx.x < 6) al[threadIdx.x] = alpha_idx ? la[alpha_idx[d] + threadIdx.x] : 0.0;
   __syncthreads();
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
@@ -1111,51 +1142,50 @@ hipError_t launch_proj_dots(hipStream_t st, int nsub, const double* Rb, const in
 }
 
 // y = Gf^T x = [sum_{i in cl} H_i^T v_i]_cl ; alpha = (Gf^T Gf)^-1 y ;
-// h_i = sgn * H_i alpha_cl(i).  One workgroup.  (Cluster.cpp:1314-1329)
-__global__ __launch_bounds__(256) void k_proj_coarse(int nsub, int ncol, const double* __restrict__ v,
-                                                     const double* __restrict__ H, const int* __restrict__ cl,
-                                                     const double* __restrict__ invGtG, double* __restrict__ h,
-                                                     double* __restrict__ alpha, double sgn,
-                                                     const int* __restrict__ done) {
+// h_i = sgn * H_i alpha_cl(i).  One workgroup; subdomains of a cluster are the
+// contiguous ids [cl*nsub_cl, (cl+1)*nsub_cl).  (Cluster.cpp:1314-1329)
+__global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const double* __restrict__ v,
+                                                      const double* __restrict__ H, const int* __restrict__ cl,
+                                                      const double* __restrict__ invGtG, double* __restrict__ h,
+                                                      double* __restrict__ alpha, double sgn,
+                                                      const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ double y[1024];
   __shared__ double al[1024];
-  for (int c = threadIdx.x; c < ncol; c += 256) y[c] = 0.0;
-  __syncthreads();
-  // deterministic: thread per column, subdomains in ascending order
-  for (int c = threadIdx.x; c < ncol; c += 256) {
+  const int ncl = ncol / 6;
+  const int nsc = nsub / ncl;
+  // y[c]: thread per column, ascending subdomains of its cluster (fixed order)
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
     const int ccl = c / 6, k = c % 6;
     double s = 0.0;
-    for (int d = 0; d < nsub; d++) {
-      if (cl[d] != ccl) continue;
+    for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
       const double* Hd = H + 36 * d;
-      for (int a = 0; a < 6; a++) s += Hd[a * 6 + k] * v[6 * d + a];
+      const double* vd = v + 6 * d;
+      s += Hd[0 * 6 + k] * vd[0] + Hd[1 * 6 + k] * vd[1] + Hd[2 * 6 + k] * vd[2] + Hd[3 * 6 + k] * vd[3] +
+           Hd[4 * 6 + k] * vd[4] + Hd[5 * 6 + k] * vd[5];
     }
     y[c] = s;
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < ncol; c += 256) {
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
     double s = 0.0;
     for (int j = 0; j < ncol; j++) s += invGtG[c + (int64_t)j * ncol] * y[j];
     al[c] = s;
     if (alpha) alpha[c] = s;
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < nsub; d += 256) {
-    const double* Hd = H + 36 * d;
-    const int c0 = 6 * cl[d];
-    for (int a = 0; a < 6; a++) {
-      double s = 0.0;
-      for (int b = 0; b < 6; b++) s += Hd[a * 6 + b] * al[c0 + b];
-      h[6 * d + a] = sgn * s;
-    }
+  for (int q = threadIdx.x; q < 6 * nsub; q += blockDim.x) {
+    const int d = q / 6, a = q % 6;
+    const double* Hd = H + 36 * d + a * 6;
+    const double* ad = al + 6 * cl[d];
+    h[q] = sgn * (Hd[0] * ad[0] + Hd[1] * ad[1] + Hd[2] * ad[2] + Hd[3] * ad[3] + Hd[4] * ad[4] + Hd[5] * ad[5]);
   }
 }
 
 hipError_t launch_proj_coarse(hipStream_t st, int nsub, int ncol, const double* v, const double* H, const int* cl,
                               const double* invGtG, double* h, double* alpha, double sgn, const int* done) {
   if (ncol > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_proj_coarse, dim3(1), dim3(256), 0, st, nsub, ncol, v, H, cl, invGtG, h, alpha, sgn, done);
+  hipLaunchKernelGGL(k_proj_coarse, dim3(1), dim3(1024), 0, st, nsub, ncol, v, H, cl, invGtG, h, alpha, sgn, done);
   return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ struct PanelDesc {
 };
 
 struct AsmSub {  // box subdomain for element assembly
-  int e0[3], ne[3], n, pad;
+  int e0[3], ne[3], n, nr;  // nr > 0: also write the identity coupling M[n+i][i] = 1, i < nr
   double E;
 };
 
@@ -55,12 +55,18 @@ hipError_t launch_permute(hipStream_t st, int nsub, int maxlen, int scatter, con
                           const int64_t* doff, int nrhs, int64_t sstride, int64_t dstride);
 hipError_t launch_copy_cols(hipStream_t st, int nmat, int maxcols, const int* rows, const int* cols,
                             const double* src, const int64_t* soff, const int* lds, double* dst,
-                            const int64_t* doff, const int* ldd);
+                            const int64_t* doff, const int* ldd, double scale);
 hipError_t launch_scale_lower(hipStream_t st, int nmat, int maxcols, const int* row0, const int* n, double* M,
                               const int64_t* moff, const int* ld, double s);
-hipError_t launch_gemm_nn(hipStream_t st, int nmat, int maxn, int maxcol, const int* n, const int* ncol,
-                          const double* A, const int64_t* aoff, const int* lda, const double* X,
-                          const int64_t* xoff, const int* ldx, double* Y, const int64_t* yoff, const int* ldy);
+hipError_t launch_gemm_nn(hipStream_t st, int nmat, int maxn, int maxcol, const int* n, const int* nin,
+                          const int* ncol, const double* A, const int64_t* aoff, const int* lda, const double* X,
+                          const int64_t* xoff, const int* ldx, double* Y, const int64_t* yoff, const int* ldy,
+                          double alpha);
+// y_i = A_i x1_i + beta * B_i x2_i  (A_i: m x m1, B_i: m x m2, column major; x staged in LDS)
+hipError_t launch_gemv_two(hipStream_t st, int nsub, int maxm, int maxlds, const int* m, const int* m1,
+                           const int* m2, const double* A, const int64_t* aoff, const int* lda, const double* B,
+                           const int64_t* boff, const int* ldb, const double* x1, const int64_t* x1off,
+                           const double* x2, const int64_t* x2off, double beta, double* y, const int64_t* yoff);
 hipError_t launch_tdots(hipStream_t st, int nmat, int maxpairs, const int* np, const int* ni, const int* nj,
                         const double* X, const int64_t* xoff, const int* ldx, const double* Y, const int64_t* yoff,
                         const int* ldy, double* out, const int64_t* ooff, const int* ldo, double alpha);
@@ -85,6 +91,10 @@ hipError_t launch_corr_apply(hipStream_t st, int nsub, int maxrows, const double
                              const int* ldc, const int* nlc, const int64_t* lcoff, const int* lcmap,
                              const double* Rb, const int64_t* rboff, const int* ns, const int64_t* boff,
                              const int64_t* alpha_idx, const double* la, double* y, const int* done);
+// column dots with one wave per output: out[o + i] = alpha * sum_j A[j + i*n] x[xo + j]  (A symmetric, n x n)
+hipError_t launch_sym_gemv_waves(hipStream_t st, int ncl, int maxlen, const double* A, const int64_t* aoff,
+                                 const int* nlen, const double* x, const int64_t* xoff, double* out,
+                                 const int* done);
 hipError_t launch_scatter(hipStream_t st, int64_t nlam, const int* ra, const int* rb, const double* y,
                           const double* wgt, const double* sub_from, double* out, const double* dotw,
                           double* part, const int* done);

@@ -39,21 +39,22 @@ hipError_t launch_permute(hipStream_t st, int nsub, int maxlen, int scatter, con
 __global__ __launch_bounds__(256) void k_copy_cols(const int* __restrict__ rows, const int* __restrict__ cols,
                                                    const double* __restrict__ src, const int64_t* __restrict__ soff,
                                                    const int* __restrict__ lds, double* __restrict__ dst,
-                                                   const int64_t* __restrict__ doff, const int* __restrict__ ldd) {
+                                                   const int64_t* __restrict__ doff, const int* __restrict__ ldd,
+                                                   double scale) {
   const int mi = blockIdx.y;
   const int c = blockIdx.x;
   if (c >= cols[mi]) return;
   const double* S = src + soff[mi] + (int64_t)c * lds[mi];
   double* D = dst + doff[mi] + (int64_t)c * ldd[mi];
   const int nr = rows[mi];
-  for (int r = threadIdx.x; r < nr; r += 256) D[r] = S[r];
+  for (int r = threadIdx.x; r < nr; r += 256) D[r] = scale * S[r];
 }
 
 hipError_t launch_copy_cols(hipStream_t st, int nmat, int maxcols, const int* rows, const int* cols,
                             const double* src, const int64_t* soff, const int* lds, double* dst,
-                            const int64_t* doff, const int* ldd) {
+                            const int64_t* doff, const int* ldd, double scale) {
   hipLaunchKernelGGL(k_copy_cols, dim3(maxcols, nmat), dim3(256), 0, st, rows, cols, src, soff, lds, dst, doff,
-                     ldd);
+                     ldd, scale);
   return hipGetLastError();
 }
 
@@ -77,14 +78,15 @@ hipError_t launch_scale_lower(hipStream_t st, int nmat, int maxcols, const int* 
 
 // Y[p + k*ldy] = sum_q A[p + q*lda] X[q + k*ldx], p < n, k < ncol (8 per z slice);
 // the X entries are wave-uniform loads (cached), A is streamed down its columns.
-__global__ __launch_bounds__(256) void k_gemm_nn(const int* __restrict__ nn, const int* __restrict__ ncol,
+__global__ __launch_bounds__(256) void k_gemm_nn(const int* __restrict__ nn, const int* __restrict__ nin,
+                                                 const int* __restrict__ ncol,
                                                  const double* __restrict__ A, const int64_t* __restrict__ aoff,
                                                  const int* __restrict__ lda, const double* __restrict__ X,
                                                  const int64_t* __restrict__ xoff, const int* __restrict__ ldx,
                                                  double* __restrict__ Y, const int64_t* __restrict__ yoff,
-                                                 const int* __restrict__ ldy) {
+                                                 const int* __restrict__ ldy, double alpha) {
   const int mi = blockIdx.z;
-  const int n = nn[mi], nc = ncol[mi];
+  const int n = nn[mi], nc = ncol[mi], ni = nin ? nin[mi] : nn[mi];
   const int k0 = blockIdx.y * 8;
   if (k0 >= nc) return;
   const int kc = (nc - k0) < 8 ? (nc - k0) : 8;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(256) void k_gemm_nn(const int* __restrict__ nn, con
   const double* Am = A + aoff[mi];
   const int64_t la = lda[mi];
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int q = 0; q < n; q++) {
+  for (int q = 0; q < ni; q++) {
     const double a = Am[p + (int64_t)q * la];
 #pragma unroll
     for (int k = 0; k < 8; k++)
@@ -103,15 +105,80 @@ __global__ __launch_bounds__(256) void k_gemm_nn(const int* __restrict__ nn, con
   }
   double* Ym = Y + yoff[mi];
   const int64_t ly = ldy[mi];
-  for (int k = 0; k < kc; k++) Ym[p + (int64_t)(k0 + k) * ly] = acc[k];
+  for (int k = 0; k < kc; k++) Ym[p + (int64_t)(k0 + k) * ly] = alpha * acc[k];
 }
 
-hipError_t launch_gemm_nn(hipStream_t st, int nmat, int maxn, int maxcol, const int* n, const int* ncol,
-                          const double* A, const int64_t* aoff, const int* lda, const double* X,
-                          const int64_t* xoff, const int* ldx, double* Y, const int64_t* yoff, const int* ldy) {
+hipError_t launch_gemm_nn(hipStream_t st, int nmat, int maxn, int maxcol, const int* n, const int* nin,
+                          const int* ncol, const double* A, const int64_t* aoff, const int* lda, const double* X,
+                          const int64_t* xoff, const int* ldx, double* Y, const int64_t* yoff, const int* ldy,
+                          double alpha) {
   if (maxcol < 1) return hipSuccess;
-  hipLaunchKernelGGL(k_gemm_nn, dim3((maxn + 255) / 256, (maxcol + 7) / 8, nmat), dim3(256), 0, st, n, ncol, A, aoff,
-                     lda, X, xoff, ldx, Y, yoff, ldy);
+  hipLaunchKernelGGL(k_gemm_nn, dim3((maxn + 255) / 256, (maxcol + 7) / 8, nmat), dim3(256), 0, st, n, nin, ncol, A,
+                     aoff, lda, X, xoff, ldx, Y, yoff, ldy, alpha);
+  return hipGetLastError();
+}
+
+// y_i = A_i x1_i + beta * B_i x2_i with A_i (m x m1), B_i (m x m2) column major,
+// ld even; x1 then x2 staged through one LDS buffer; each thread owns two
+// consecutive rows.  Used for x_r = K_rr^-1 v_r - (K_rr^-1 K_rb) x_b.
+__global__ __launch_bounds__(256) void k_gemv_two(const int* __restrict__ mm, const int* __restrict__ mm1,
+                                                  const int* __restrict__ mm2, const double* __restrict__ A,
+                                                  const int64_t* __restrict__ aoff, const int* __restrict__ lda,
+                                                  const double* __restrict__ B, const int64_t* __restrict__ boff,
+                                                  const int* __restrict__ ldb, const double* __restrict__ x1,
+                                                  const int64_t* __restrict__ x1off, const double* __restrict__ x2,
+                                                  const int64_t* __restrict__ x2off, double beta,
+                                                  double* __restrict__ y, const int64_t* __restrict__ yoff) {
+  extern __shared__ double xs[];
+  const int d = blockIdx.y;
+  const int m = mm[d], m1 = mm1[d], m2 = mm2[d];
+  const int r0 = blockIdx.x * 512;
+  if (r0 >= m) return;
+  const int r = r0 + 2 * threadIdx.x;
+  const bool two = r + 1 < m, one = r < m;
+  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+  for (int p = threadIdx.x; p < m1; p += 256) xs[p] = x1[x1off[d] + p];
+  __syncthreads();
+  if (one) {
+    const double* Ad = A + aoff[d] + r;
+    const int64_t la = lda[d];
+    if (two) {
+      for (int j = 0; j < m1; j++) {
+        const double2 c = *reinterpret_cast<const double2*>(Ad + (int64_t)j * la);
+        a0 += c.x * xs[j];
+        a1 += c.y * xs[j];
+      }
+    } else {
+      for (int j = 0; j < m1; j++) a0 += Ad[(int64_t)j * la] * xs[j];
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < m2; p += 256) xs[p] = x2[x2off[d] + p];
+  __syncthreads();
+  if (!one) return;
+  const double* Bd = B + boff[d] + r;
+  const int64_t lb = ldb[d];
+  if (two) {
+    for (int j = 0; j < m2; j++) {
+      const double2 c = *reinterpret_cast<const double2*>(Bd + (int64_t)j * lb);
+      b0 += c.x * xs[j];
+      b1 += c.y * xs[j];
+    }
+    y[yoff[d] + r] = a0 + beta * b0;
+    y[yoff[d] + r + 1] = a1 + beta * b1;
+  } else {
+    for (int j = 0; j < m2; j++) b0 += Bd[(int64_t)j * lb] * xs[j];
+    y[yoff[d] + r] = a0 + beta * b0;
+  }
+}
+
+hipError_t launch_gemv_two(hipStream_t st, int nsub, int maxm, int maxlds, const int* m, const int* m1,
+                           const int* m2, const double* A, const int64_t* aoff, const int* lda, const double* B,
+                           const int64_t* boff, const int* ldb, const double* x1, const int64_t* x1off,
+                           const double* x2, const int64_t* x2off, double beta, double* y, const int64_t* yoff) {
+  if ((size_t)maxlds * sizeof(double) > 65536) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gemv_two, dim3((maxm + 511) / 512, nsub), dim3(256), (size_t)maxlds * sizeof(double), st, m, m1,
+                     m2, A, aoff, lda, B, boff, ldb, x1, x1off, x2, x2off, beta, y, yoff);
   return hipGetLastError();
 }
 
