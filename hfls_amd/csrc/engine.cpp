@@ -415,6 +415,28 @@ void Engine::factor_subdomains() {
       for (int a = 0; a < 3; a++) { asub[i].e0[a] = s.e0[a]; asub[i].ne[a] = s.ne[a]; }
       asub[i].n = s.n; asub[i].nr = s.n_r; asub[i].E = s.E;
     }
+    // host-side extent checks of every block a kernel of this chunk touches:
+    // (offset, ld, rows, cols) must stay inside its buffer
+    auto span = [](const char* what, int64_t off, int64_t ld, int64_t rows, int64_t cols, int64_t size) {
+      if (rows <= 0 || cols <= 0) return;
+      if (off < 0 || ld < rows || off + (cols - 1) * ld + rows > size)
+        throw std::runtime_error(std::string("block extent check failed: ") + what);
+    };
+    for (int i = 0; i < nm; i++) {
+      const int d = c0 + i;
+      const Subdomain& s = P->subs[d];
+      const int64_t l1 = ld1[i];
+      span("M1", m1off[i], l1, l1, l1, o1);
+      span("M2", m2off[i], ld2[i], ld2[i], ld2[i], o2);
+      span("R1", r1off[i], s.n, s.n, 6, o3);
+      span("R1 s-rows", r1off[i] + prow0[i], s.n, s.n_ss, 6, o3);
+      span("T source", t_src[i], l1, s.n_r, s.n_s, o1);
+      span("T", off_T[d], h_ldt[d], s.n_r, s.n_s, off_T[N]);
+      span("Kinv", off_Kr[d], h_ldt[d], s.n_r, s.n_r, off_Kr[N]);
+      span("K+", off_K[d], h_ldb[d], s.n_s, s.n_s, off_K[N]);
+      span("R", off_R[d], s.n, s.n, 6, off_R[N]);
+      span("Rb", off_Rb[d], s.n_s, s.n_s, 6, off_Rb[N]);
+    }
     DVec<double> M1, M2, R1;
     M1.alloc(o1); M2.alloc(o2); R1.alloc(o3);
     HC(hipMemsetAsync(M1.p, 0, o1 * sizeof(double), st));
@@ -473,12 +495,13 @@ void Engine::factor_subdomains() {
       drs.upload(rs_off, st); drbo.upload(rb_off, st);
       std::vector<int64_t> defoff(nm);
       for (int i = 0; i < nm; i++) defoff[i] = c0 + i;
+      // R1 is n x 6 per matrix with leading dimension n (dldn), not M1's ld
       HC(launch_detect(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, d_kdiag.p, dnoff.p, d_perm1.p, dprow0.p, R1.p,
-                       drs.p, dld1.p, d_defect.p + c0));
+                       drs.p, dldn.p, d_defect.p + c0));
       // R_{b\s} = -L_A^{-T} (L_sA^T R_s)
-      HC(launch_coldot(st, nm, maxns, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, dnbs.p, R1.p, drs.p, dld1.p, R1.p, drbo.p,
-                       dld1.p, 6, -1.0, 0.0));
-      HC(launch_bwd(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, R1.p, drbo.p, dld1.p, 6));
+      HC(launch_coldot(st, nm, maxns, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, dnbs.p, R1.p, drs.p, dldn.p, R1.p, drbo.p,
+                       dldn.p, 6, -1.0, 0.0));
+      HC(launch_bwd(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, R1.p, drbo.p, dldn.p, 6));
       // R_r = -(K_rr^-1 K_rb) R_b  (Matrix.cpp:2393-2421)
       HC(launch_gemm_nn(st, nm, maxnr, 6, dnr.p, dns.p, dsix.p, d_T.p, dto.p, dldt.p, R1.p, drbo.p, dldn.p, R1.p,
                         dr1.p, dldn.p, -1.0));
