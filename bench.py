@@ -140,7 +140,7 @@ def main():
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_gemv_gather (K+_bb)", "kernel_ms": ms, "bytes_per_launch": nbytes,
+                     "kernel": "k_symv_tiles (K+_bb, packed lower tiles)", "kernel_ms": ms, "bytes_per_launch": nbytes,
                      "S_stream_GBs": nbytes_s / (ms_s * 1e-3) / 1e9},
     }
     mesh.close()

@@ -23,6 +23,7 @@ HOST_FLAGS = ["-x", "c++", "-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__
 SOURCES = [
     ("kernels.hip", HIP_FLAGS),
     ("kernels2.hip", HIP_FLAGS),
+    ("symv.hip", HIP_FLAGS),
     ("problem.cpp", HOST_FLAGS + ["-ffp-contract=off"]),
     ("engine.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),

@@ -247,6 +247,8 @@ void Engine::build_meta() {
   off_n.assign(N + 1, 0); off_b.assign(N + 1, 0);
   off_K.assign(N + 1, 0); off_T.assign(N + 1, 0); off_Kr.assign(N + 1, 0); off_R.assign(N + 1, 0);
   off_Rb.assign(N + 1, 0);
+  off_P.assign(N + 1, 0);
+  off_part.assign(N + 1, 0);
   for (int d = 0; d < N; d++) {
     const Subdomain& s = P->subs[d];
     h_n[d] = s.n; h_nr[d] = s.n_r; h_ns[d] = s.n_s; h_nbs[d] = s.n_bs; h_nss[d] = s.n_ss;
@@ -258,6 +260,11 @@ void Engine::build_meta() {
     off_Kr[d + 1] = off_Kr[d] + align32((int64_t)h_ldt[d] * s.n_r);
     off_R[d + 1] = off_R[d] + 6 * (int64_t)s.n;
     off_Rb[d + 1] = off_Rb[d] + 6 * (int64_t)s.n_s;
+    const int T = (s.n_s + 63) / 64;
+    off_P[d + 1] = off_P[d] + (int64_t)T * (T + 1) / 2 * 4096;
+    off_part[d + 1] = off_part[d] + (int64_t)T * (T + 1) / 2 * 64;
+    maxT = std::max(maxT, T);
+    maxtiles = std::max(maxtiles, T * (T + 1) / 2);
     maxn = std::max(maxn, s.n); maxns = std::max(maxns, s.n_s); maxnr = std::max(maxnr, s.n_r);
   }
   if (maxns > 8000) throw std::runtime_error("interface too large for the LDS-staged gather (n_s > 8000)");
@@ -272,6 +279,14 @@ void Engine::build_meta() {
   d_off_Kr.upload(std::vector<int64_t>(off_Kr.begin(), off_Kr.end() - 1), st);
   d_off_R.upload(std::vector<int64_t>(off_R.begin(), off_R.end() - 1), st);
   d_off_Rb.upload(std::vector<int64_t>(off_Rb.begin(), off_Rb.end() - 1), st);
+  d_off_P.upload(std::vector<int64_t>(off_P.begin(), off_P.end() - 1), st);
+  d_off_part.upload(std::vector<int64_t>(off_part.begin(), off_part.end() - 1), st);
+  {
+    std::vector<int> bs(off_b[N]);
+    for (int d = 0; d < N; d++)
+      for (int64_t q = off_b[d]; q < off_b[d + 1]; q++) bs[q] = d;
+    d_bsub.upload(bs, st);
+  }
   std::vector<int64_t> xb(N), v6(N);
   for (int d = 0; d < N; d++) { xb[d] = off_n[d] + h_nr[d]; v6[d] = 6 * (int64_t)d; }
   d_xboff.upload(xb, st);
@@ -716,6 +731,14 @@ void Engine::build_bc_and_coarse() {
   std::vector<double> Fcd, Gcd;
   dFc.download(Fcd, st);
   dGc.download(Gcd, st);
+  // pack K+_bb and S_i into lower 64x64 tiles; the full copies are no longer needed
+  d_Kt.alloc(off_P[N]);
+  d_St.alloc(off_P[N]);
+  HC(launch_pack_tiles(st, N, maxtiles, d_ns.p, d_K.p, d_off_K.p, d_ldb.p, d_Kt.p, d_off_P.p));
+  HC(launch_pack_tiles(st, N, maxtiles, d_ns.p, d_S.p, d_off_K.p, d_ldb.p, d_St.p, d_off_P.p));
+  HC(hipStreamSynchronize(st));
+  d_K.release();
+  d_S.release();
   // cluster matrices, kernel H of Gc, augmented saddle-point matrices
   std::vector<double> Hall((size_t)N * 36);
   std::vector<double> Aaug;
@@ -881,6 +904,8 @@ void Engine::setup() {
   part.alloc(nparts);
   X1.alloc(off_n[N]);
   X2.alloc(off_n[N]);
+  partbuf.alloc(off_part[N]);
+  xb.alloc(NB);
   vloc.alloc(off_n[N]);
   xloc.alloc(off_n[N]);
   ug.alloc(3 * P->nPoints);
@@ -889,16 +914,16 @@ void Engine::setup() {
   sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   size_t freeb = 0, totb = 0;
   HC(hipMemGetInfo(&freeb, &totb));
-  sinfo.device_bytes = (off_T[N] + off_Kr[N] + 2 * off_K[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
+  sinfo.device_bytes = (off_T[N] + off_Kr[N] + 2 * off_P[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
 }
 
 // ------------------------------------------------------------ solve pieces
 void Engine::F_apply(const double* win, double* out, const double* sub_from, const double* dotw, const int* done) {
   const int N = P->nsub;
-  HC(launch_gemv_gather(st, N, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, win,
-                        nullptr, nullptr, nullptr, yb.p, done));
+  HC(launch_gather(st, NB, d_gptr.p, d_gidx.p, win, nullptr, xb.p, done));
+  symv(d_Kt.p, xb.p, d_off_b.p, done);
   HC(launch_corr_dots(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_Rb.p, d_off_Rb.p, d_ns.p,
-                      d_off_b.p, d_gptr.p, d_gidx.p, win, nullptr, cvec.p, avec.p, done));
+                      d_off_b.p, d_gptr.p, d_gidx.p, nullptr, xb.p, cvec.p, avec.p, done));
   HC(launch_cluster_gc(st, P->ncl, max_cl_len, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, cvec.p, avec.p,
                        gc.p, d_la_off.p, done));
   HC(launch_sym_gemv_waves(st, P->ncl, max_cl_len, d_Ainv.p, d_Aoff.p, d_nlen.p, gc.p, d_la_off.p, la.p, done));
@@ -918,8 +943,8 @@ void Engine::proj(const double* x, double* out, double* hbuf, double* alpha_out,
 
 void Engine::M_apply(const double* x, double* out, const int* done) {
   const int N = P->nsub;
-  HC(launch_gemv_gather(st, N, maxns, d_S.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, x,
-                        d_wgt.p, nullptr, nullptr, yb.p, done));
+  HC(launch_gather(st, NB, d_gptr.p, d_gidx.p, x, d_wgt.p, xb.p, done));
+  symv(d_St.p, xb.p, d_off_b.p, done);
   HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, d_wgt.p, nullptr, out, nullptr, nullptr, done));
 }
 
@@ -931,10 +956,12 @@ void Engine::prep_t(const double* vin) {
                    d_xboff.p, d_ldn.p, 1, -1.0, 1.0));
 }
 
-void Engine::kplus_b_from_t() {
-  HC(launch_gemv_gather(st, P->nsub, maxns, d_K.p, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p,
-                        nullptr, nullptr, X1.p, d_xboff.p, yb.p, nullptr));
+void Engine::symv(const double* tiles, const double* x, const int64_t* xoff, const int* done) {
+  HC(launch_symv_tiles(st, P->nsub, maxT, d_ns.p, tiles, d_off_P.p, x, xoff, partbuf.p, d_off_part.p, done));
+  HC(launch_symv_reduce(st, NB, d_bsub.p, d_ns.p, d_off_b.p, partbuf.p, d_off_part.p, yb.p, done));
 }
+
+void Engine::kplus_b_from_t() { symv(d_Kt.p, X1.p, d_xboff.p, nullptr); }
 
 void Engine::cluster_solve_t(const double* vin) {
   const int N = P->nsub;
@@ -1016,17 +1043,15 @@ double Engine::time_stream(int which, int reps, double* bytes) {
   const int64_t nl = P->nlam;
   std::vector<double> ones(nl, 1.0);
   HC(hipMemcpyAsync(w.p, ones.data(), nl * sizeof(double), hipMemcpyHostToDevice, st));
-  const double* A = which == 0 ? d_K.p : d_S.p;
-  const double* wt = which == 0 ? nullptr : d_wgt.p;
-  HC(launch_gemv_gather(st, N, maxns, A, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, w.p, wt, nullptr,
-                        nullptr, yb.p, nullptr));
+  HC(launch_gather(st, NB, d_gptr.p, d_gidx.p, w.p, which ? d_wgt.p : nullptr, xb.p, nullptr));
+  const double* A = which == 0 ? d_Kt.p : d_St.p;
+  HC(launch_symv_tiles(st, N, maxT, d_ns.p, A, d_off_P.p, xb.p, d_off_b.p, partbuf.p, d_off_part.p, nullptr));
   hipEvent_t a, b;
   HC(hipEventCreate(&a));
   HC(hipEventCreate(&b));
   HC(hipEventRecord(a, st));
   for (int i = 0; i < reps; i++)
-    HC(launch_gemv_gather(st, N, maxns, A, d_off_K.p, d_ldb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, w.p, wt,
-                          nullptr, nullptr, yb.p, nullptr));
+    HC(launch_symv_tiles(st, N, maxT, d_ns.p, A, d_off_P.p, xb.p, d_off_b.p, partbuf.p, d_off_part.p, nullptr));
   HC(hipEventRecord(b, st));
   HC(hipEventSynchronize(b));
   float ms = 0;
@@ -1034,12 +1059,15 @@ double Engine::time_stream(int which, int reps, double* bytes) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (bytes) {
-    // matrix stream + output + gathered dual values (8 B) and their row codes (4 B)
-    double m = 0, nnzg = 0;
-    for (int d = 0; d < N; d++) m += 8.0 * (double)h_ns[d] * h_ns[d];
-    for (int d = 0; d < N; d++)
-      for (size_t e = 0; e < P->subs[d].bf_row.size(); e++) nnzg += 1.0;
-    *bytes = m + 8.0 * (double)NB + 12.0 * nnzg + (which ? 8.0 * nnzg : 0.0);
+    // algorithmic: the lower half of each symmetric n_s x n_s block (8 B each),
+    // the interface vector and the tile-row partials written
+    double m = 0, xv = 0, pw = 0;
+    for (int d = 0; d < N; d++) {
+      m += 8.0 * (double)h_ns[d] * (h_ns[d] + 1) / 2.0;
+      xv += 8.0 * h_ns[d];
+      pw += 8.0 * (double)(off_part[d + 1] - off_part[d]);
+    }
+    *bytes = m + xv + pw;
   }
   return ms / reps;
 }

@@ -87,12 +87,15 @@ class Engine {
   int64_t NB = 0;
   // per-subdomain meta (host copies)
   std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_ldt, h_cl, h_nlc;
-  std::vector<int64_t> off_K, off_T, off_Kr, off_R, off_Rb, off_C, off_lc;
+  std::vector<int64_t> off_K, off_T, off_Kr, off_R, off_Rb, off_C, off_lc, off_P, off_part;
   // device meta
   DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_ldt, d_cl, d_nlc, d_ldn, d_ones, d_sixes, d_zeros;
+  DVec<int64_t> d_off_P, d_off_part;
   DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_T, d_off_Kr, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
       d_v6off;
-  DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap;
+  DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap, d_bsub;
+  DVec<double> d_Kt, d_St, partbuf, xb;  // packed lower tiles of K+_bb and S_i, tile partials
+  int maxT = 0, maxtiles = 0;
   DVec<unsigned char> d_owner;
   DVec<double> d_kdiag, d_f, d_T, d_Kinv, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
   DVec<int> d_fixpos, d_defect, d_err;
@@ -122,6 +125,7 @@ class Engine {
   void F_apply(const double* w_in, double* out, const double* sub_from, const double* dotw, const int* done);
   void proj(const double* x, double* out, double* hbuf, double* alpha_out, const double* dotv, const int* done);
   void M_apply(const double* x, double* out, const int* done);
+  void symv(const double* tiles, const double* x, const int64_t* xoff, const int* done);  // yb = A x
   void prep_t(const double* vloc_in);        // X1 = [v_r; t_b], t_b = v_b - K_br K_rr^-1 v_r
   void kplus_b_from_t();                     // yb = K+_bb t_b = (K^+ v)_b
   void cluster_solve_t(const double* vloc_in);  // la = Ac^-1 [C^T t; -R^T v]

@@ -120,4 +120,15 @@ hipError_t launch_recover(hipStream_t st, int nsub, int maxn, const int* n, cons
                           const int64_t* roff, const int64_t* alpha_idx, const double* la, const double* h,
                           const double* x, const int* gdof, const unsigned char* owner, double* u);
 
+// ---------------------------------------------------------------- symmetric tiles (symv.hip)
+hipError_t launch_pack_tiles(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* A,
+                             const int64_t* aoff, const int* lda, double* P, const int64_t* poff);
+hipError_t launch_gather(hipStream_t st, int64_t nb, const int* gptr, const int* gidx, const double* v,
+                         const double* wgt, double* xb, const int* done);
+hipError_t launch_symv_tiles(hipStream_t st, int nsub, int maxT, const int* ns, const double* P, const int64_t* poff,
+                             const double* x, const int64_t* xoff, double* part, const int64_t* partoff,
+                             const int* done);
+hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const int* ns, const int64_t* boff,
+                              const double* part, const int64_t* partoff, double* y, const int* done);
+
 }  // namespace hfls
