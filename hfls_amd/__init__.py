@@ -19,7 +19,8 @@ EXPORTS = [
     "hfls_problem_sub_l2g", "hfls_problem_sub_young", "hfls_problem_rows", "hfls_problem_dirichlet",
     "hfls_problem_corners", "hfls_mesh_create", "hfls_mesh_destroy", "hfls_mesh_sizes",
     "hfls_mesh_setup_times", "hfls_mesh_kernel", "hfls_mesh_load", "hfls_apply_F", "hfls_project",
-    "hfls_precondition", "hfls_kplus", "hfls_solve", "hfls_time_stream",
+    "hfls_precondition", "hfls_kplus", "hfls_solve", "hfls_time_stream", "hfls_solve_block",
+    "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction",
 ]
 
 DEFAULTS = {
@@ -43,6 +44,15 @@ class HflsError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("pcpg_ms", ctypes.c_double)]
+
+
+class CaseStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("residual", ctypes.c_double)]
+
+
+class BlockStats(ctypes.Structure):
+    _fields_ = [("solve_ms", ctypes.c_double), ("init_ms", ctypes.c_double), ("pcpg_ms", ctypes.c_double),
+                ("recover_ms", ctypes.c_double), ("sweeps", ctypes.c_int), ("cert_steps", ctypes.c_int)]
 
 
 _lib = None
@@ -146,6 +156,15 @@ class Problem:
         _check(load().hfls_problem_corners(self._h, _ptr(out)))
         return out
 
+    def npacked(self):
+        return sum(self.sub_info(d)["n"] for d in range(self.nsub))
+
+    def traction(self, seed, case0, k):
+        """k seeded smoothed tractions on the x = 1 face, packed local loads (k, npacked)."""
+        out = np.zeros((k, self.npacked()))
+        _check(load().hfls_problem_traction(self._h, ctypes.c_uint64(seed), int(case0), int(k), _ptr(out)))
+        return out
+
 
 class Mesh:
     """Device-resident, set-up HTFETI mesh (one GPU).  Mirrors the operator
@@ -221,3 +240,45 @@ class Mesh:
                                  ctypes.byref(st)))
         return {"u": u, "lambda": lam, "iterations": st.iterations, "converged": bool(st.converged),
                 "residual": st.residual, "solve_ms": st.solve_ms, "pcpg_ms": st.pcpg_ms}
+
+    def traction(self, seed, case0, k):
+        """k seeded smoothed tractions on the x = 1 face, packed local loads (k, npacked)."""
+        out = np.zeros((k, self.npacked))
+        _check(load().hfls_mesh_traction(self._h, ctypes.c_uint64(seed), int(case0), int(k), _ptr(out)))
+        return out
+
+    def apply_block(self, op, W, precision=64):
+        """op 0 F, 1 M, 2 P applied to the rows of W (k, nlam) at precision 64 or 32."""
+        W = np.ascontiguousarray(np.atleast_2d(W), dtype=np.float64)
+        out = np.zeros_like(W)
+        _check(load().hfls_apply_block(self._h, int(op), W.shape[0], int(precision), _ptr(W), _ptr(out)))
+        return out
+
+    def mult_Ff_block(self, W, precision=64):
+        return self.apply_block(0, W, precision)
+
+    def preconditioning_block(self, W, precision=64):
+        return self.apply_block(1, W, precision)
+
+    def projection_block(self, W):
+        return self.apply_block(2, W, 64)
+
+    def pcpg_block(self, F=None, k=None, eps=1e-8, max_iter=200, precision=32, want_lambda=False):
+        """Block of load cases: F (k, npacked) packed local loads, or None for k
+        copies of the body-force load.  Returns u (k, 3*nPoints), per-case
+        iterations/status/residual and the block timings."""
+        if F is not None:
+            F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+            k = F.shape[0]
+        k = int(k or 1)
+        u = np.zeros((k, 3 * self.nPoints))
+        lam = np.zeros((k, self.nlam)) if want_lambda else None
+        cs = (CaseStats * k)()
+        bs = BlockStats()
+        _check(load().hfls_solve_block(self._h, ctypes.c_double(eps), int(max_iter), k, int(precision), _ptr(F),
+                                       _ptr(u), _ptr(lam), cs, ctypes.byref(bs)))
+        return {"u": u, "lambda": lam, "iterations": np.array([c.iterations for c in cs]),
+                "status": np.array([c.status for c in cs]), "residual": np.array([c.residual for c in cs]),
+                "converged": np.array([c.status == 2 for c in cs]), "solve_ms": bs.solve_ms, "init_ms": bs.init_ms,
+                "pcpg_ms": bs.pcpg_ms, "recover_ms": bs.recover_ms, "sweeps": bs.sweeps,
+                "cert_steps": bs.cert_steps}

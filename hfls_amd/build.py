@@ -24,8 +24,11 @@ SOURCES = [
     ("kernels.hip", HIP_FLAGS),
     ("kernels2.hip", HIP_FLAGS),
     ("symv.hip", HIP_FLAGS),
+    ("tgemm.hip", HIP_FLAGS),
+    ("block.hip", HIP_FLAGS),
     ("problem.cpp", HOST_FLAGS + ["-ffp-contract=off"]),
     ("engine.cpp", HOST_FLAGS),
+    ("engine_block.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
 ]
 
@@ -41,7 +44,7 @@ def _compile(src, flags):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 4)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
         objs = list(ex.map(lambda s: _compile(*s), SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs

@@ -1,0 +1,613 @@
+// Dual-space and small kernels of the block PCPG: k load cases advanced together
+// (Cluster.cpp:1363-1458 per column), each with its own scalars and stopping
+// test.  Layouts:
+//   dual block    v[r*k + j]               (row r of the dual space, column j < k)
+//   b-block       X[(obrow[d] + p)*kp + j]  rows padded to 64 per subdomain
+//   cluster block la[(gcrow[cl] + l)*kp + j]
+//   local block   V[j*npacked + noff[d] + l] (column major packed local vectors)
+// A column whose status is not 0 is frozen: its scalars and dual iterates are not
+// updated, so every column's arithmetic is the same whether it is solved alone or
+// inside a block.
+#include "kernels.hpp"
+
+namespace hfls {
+
+static __device__ __forceinline__ double wsum64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// xb[(obrow[d] + p)*kp + j] = sum_e sign_e * wgt[r_e] * v[r_e*k + j]; optional FP32 copy
+__global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, const int* __restrict__ bsub,
+                                                 const int64_t* __restrict__ boff, const int64_t* __restrict__ obrow,
+                                                 const int* __restrict__ gptr, const int* __restrict__ gidx,
+                                                 const double* __restrict__ v, const double* __restrict__ wgt,
+                                                 double* __restrict__ xb, float* __restrict__ xb32, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t q = t / k;
+  const int j = (int)(t - q * k);
+  if (q >= nb) return;
+  const int d = bsub[q];
+  double s = 0.0;
+  const int e1 = gptr[q + 1];
+  for (int e = gptr[q]; e < e1; e++) {
+    const int code = gidx[e];
+    const int r = code >= 0 ? code : ~code;
+    double val = v[(int64_t)r * k + j];
+    if (wgt) val *= wgt[r];
+    s += code >= 0 ? val : -val;
+  }
+  const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
+  if (xb) xb[o] = s;
+  if (xb32) xb32[o] = (float)s;
+}
+
+hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
+                            const int64_t* obrow, const int* gptr, const int* gidx, const double* v,
+                            const double* wgt, double* xb, float* xb32, const int* gate) {
+  const int64_t n = nb * k;
+  hipLaunchKernelGGL(kb_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nb, k, kp, bsub, boff, obrow,
+                     gptr, gidx, v, wgt, xb, xb32, gate);
+  return hipGetLastError();
+}
+
+// yb = (double) y32 on the real rows and columns of a b-block
+__global__ __launch_bounds__(256) void kb_widen(int64_t nb, int k, int kp, const int* __restrict__ bsub,
+                                                const int64_t* __restrict__ boff, const int64_t* __restrict__ obrow,
+                                                const float* __restrict__ y32, double* __restrict__ yb, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t q = t / k;
+  const int j = (int)(t - q * k);
+  if (q >= nb) return;
+  const int d = bsub[q];
+  const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
+  yb[o] = (double)y32[o];
+}
+
+hipError_t launch_kb_widen(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
+                           const int64_t* obrow, const float* y32, double* yb, const int* gate) {
+  const int64_t n = nb * k;
+  hipLaunchKernelGGL(kb_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nb, k, kp, bsub, boff, obrow, y32,
+                     yb, gate);
+  return hipGetLastError();
+}
+
+// elementwise FP64 -> FP32 copy (tile stores of the same layout)
+__global__ __launch_bounds__(256) void kb_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) b[t] = (float)a[t];
+}
+
+hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b) {
+  hipLaunchKernelGGL(kb_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
+  return hipGetLastError();
+}
+
+// a[(64d + m)*kp + j] = sum_l R[l + m*n] * V[j*npacked + noff_d + l]   (R_i^T v_i, full local)
+__global__ __launch_bounds__(256) void kb_rtf(int k, int kp, int64_t npacked, const int* __restrict__ nn,
+                                              const int64_t* __restrict__ noff, const double* __restrict__ R,
+                                              const int64_t* __restrict__ roff, const double* __restrict__ V,
+                                              double* __restrict__ a) {
+  const int d = blockIdx.y;
+  const int m = blockIdx.x;  // 0..5
+  const int j = blockIdx.z;
+  const int n = nn[d];
+  __shared__ double sh[4];
+  const double* col = R + roff[d] + (int64_t)m * n;
+  const double* v = V + (int64_t)j * npacked + noff[d];
+  double s = 0.0;
+  for (int l = threadIdx.x; l < n; l += 256) s += col[l] * v[l];
+  s = wsum64(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) a[((int64_t)64 * d + m) * kp + j] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
+                         const double* R, const int64_t* roff, const double* V, double* a) {
+  hipLaunchKernelGGL(kb_rtf, dim3(6, nsub, k), dim3(256), 0, st, k, kp, npacked, n, noff, R, roff, V, a);
+  return hipGetLastError();
+}
+
+// gc[(gcrow[cl] + i)*kp + j]:  i < nl: c rows (cluster constraint sums), else -a rows
+//   c rows: cr[lcpos[src]*kp + j] (+ second contribution), a rows: a[(abase[d] + m)*kp + j]
+__global__ __launch_bounds__(256) void kb_cluster_gc(int k, int kp, const int* __restrict__ gc_src,
+                                                     const int64_t* __restrict__ gc_off, const int* __restrict__ nlcl,
+                                                     const int* __restrict__ nrcl, const int* __restrict__ d0,
+                                                     const double* __restrict__ cr, const int64_t* __restrict__ lcpos,
+                                                     const double* __restrict__ a, const int64_t* __restrict__ abase,
+                                                     double* __restrict__ gc, const int64_t* __restrict__ gcrow, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int cl = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int i = (int)(t / k), j = (int)(t % k);
+  const int nl = nlcl[cl], nr = nrcl[cl];
+  if (i >= nl + nr) return;
+  double v;
+  if (i < nl) {
+    const int* s = gc_src + 2 * (gc_off[cl] + i);
+    v = cr[lcpos[s[0]] * kp + j];
+    if (s[1] >= 0) v += cr[lcpos[s[1]] * kp + j];
+  } else {
+    const int m = i - nl;
+    const int d = d0[cl] + m / 6;
+    v = -a[(abase[d] + m % 6) * kp + j];
+  }
+  gc[(gcrow[cl] + i) * kp + j] = v;
+}
+
+hipError_t launch_kb_cluster_gc(hipStream_t st, int ncl, int maxlen, int k, int kp, const int* gc_src,
+                                const int64_t* gc_off, const int* nlcl, const int* nrcl, const int* d0,
+                                const double* cr, const int64_t* lcpos, const double* a, const int64_t* abase,
+                                double* gc, const int64_t* gcrow, const int* gate) {
+  const int64_t n = (int64_t)maxlen * k;
+  hipLaunchKernelGGL(kb_cluster_gc, dim3((unsigned)((n + 255) / 256), ncl), dim3(256), 0, st, k, kp, gc_src, gc_off,
+                     nlcl, nrcl, d0, cr, lcpos, a, abase, gc, gcrow, gate);
+  return hipGetLastError();
+}
+
+// y[(obrow[d] + p)*kp + j] += with_R * R_b[p,:] la[arow_d.., j] - C[p,:] la[lcrow.., j]
+//                              (+ Q[p,:] (Yh^T x)[:, j] + Yh[p,:] (Q^T x)[:, j] if defl)
+// C, R_b, Q, Yh are the columns 0.., nlc.., nlc+6.., nlc+12.. of the compact
+// n_s x (nlc + 18) block CRQ (ld ldc); the products CRQ^T x sit in rows
+// prow[d] + column of cr.
+__global__ __launch_bounds__(256) void kb_corr_apply(int k, int kp, const int* __restrict__ ns,
+                                                     const int64_t* __restrict__ obrow, const double* __restrict__ CRQ,
+                                                     const int64_t* __restrict__ coff, const int* __restrict__ ldc,
+                                                     const int* __restrict__ nlc, const int64_t* __restrict__ lcoff,
+                                                     const int64_t* __restrict__ lcrow,
+                                                     const int64_t* __restrict__ arow, const double* __restrict__ la,
+                                                     int with_R, int defl, const double* __restrict__ cr,
+                                                     const int64_t* __restrict__ prow, double* __restrict__ y, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int d = blockIdx.y;
+  const int n = ns[d];
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int p = (int)(t / k), j = (int)(t % k);
+  if (p >= n) return;
+  const double* Cd = CRQ + coff[d] + p;
+  const int64_t l = ldc[d];
+  const int nc = nlc[d];
+  double s = 0.0;
+  if (with_R) {
+    const int64_t ar = arow[d];
+    for (int m = 0; m < 6; m++) s += Cd[(int64_t)(nc + m) * l] * la[(ar + m) * kp + j];
+  }
+  const int64_t* lr = lcrow + lcoff[d];
+  for (int m = 0; m < nc; m++) s -= Cd[(int64_t)m * l] * la[lr[m] * kp + j];
+  if (defl) {
+    const int64_t pr = prow[d] + nc + 6;
+    for (int m = 0; m < 6; m++)
+      s += Cd[(int64_t)(nc + 6 + m) * l] * cr[(pr + 6 + m) * kp + j] +
+           Cd[(int64_t)(nc + 12 + m) * l] * cr[(pr + m) * kp + j];
+  }
+  y[(obrow[d] + p) * kp + j] += s;
+}
+
+hipError_t launch_kb_corr_apply(hipStream_t st, int nsub, int maxns, int k, int kp, const int* ns,
+                                const int64_t* obrow, const double* CRQ, const int64_t* coff, const int* ldc,
+                                const int* nlc, const int64_t* lcoff, const int64_t* lcrow, const int64_t* arow,
+                                const double* la, int with_R, int defl, const double* cr, const int64_t* prow,
+                                double* y, const int* gate) {
+  const int64_t n = (int64_t)maxns * k;
+  hipLaunchKernelGGL(kb_corr_apply, dim3((unsigned)((n + 255) / 256), nsub), dim3(256), 0, st, k, kp, ns, obrow, CRQ,
+                     coff, ldc, nlc, lcoff, lcrow, arow, la, with_R, defl, cr, prow, y, gate);
+  return hipGetLastError();
+}
+
+// X[(obrow[d] + p)*kp + m] = A[p + m*lda] (dir 0) or the reverse (dir 1), p < n_s, m < ncol
+__global__ __launch_bounds__(256) void kb_cols(int dir, int ncol, int kp, const int* __restrict__ ns,
+                                               const int64_t* __restrict__ obrow, double* __restrict__ A,
+                                               const int64_t* __restrict__ aoff, const int* __restrict__ lda,
+                                               double* __restrict__ X) {
+  const int d = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int p = (int)(t / ncol), m = (int)(t % ncol);
+  if (p >= ns[d]) return;
+  double* a = A + aoff[d] + p + (int64_t)m * lda[d];
+  double* x = X + (obrow[d] + p) * kp + m;
+  if (dir == 0) *x = *a;
+  else *a = *x;
+}
+
+hipError_t launch_kb_cols(hipStream_t st, int nsub, int maxns, int dir, int ncol, int kp, const int* ns,
+                          const int64_t* obrow, double* A, const int64_t* aoff, const int* lda, double* X) {
+  const int64_t n = (int64_t)maxns * ncol;
+  hipLaunchKernelGGL(kb_cols, dim3((unsigned)((n + 255) / 256), nsub), dim3(256), 0, st, dir, ncol, kp, ns, obrow, A,
+                     aoff, lda, X);
+  return hipGetLastError();
+}
+
+// FP32 tiles of the deflated symmetric block A - Q Yh^T - Yh Q^T (packed lower
+// tiles as the FP64 source); Q, Yh: columns nlc+6.., nlc+12.. of CRQ
+__global__ __launch_bounds__(256) void kb_deflate_f32(const int* __restrict__ ns, const double* __restrict__ P,
+                                                      const int64_t* __restrict__ poff, const double* __restrict__ CRQ,
+                                                      const int64_t* __restrict__ coff, const int* __restrict__ ldc,
+                                                      const int* __restrict__ nlc, float* __restrict__ P32) {
+  const int d = blockIdx.y;
+  const int n = ns[d];
+  const int T = (n + 63) / 64;
+  const int q = blockIdx.x;
+  if (q >= T * (T + 1) / 2) return;
+  int tj = 0, rem = q;
+  while (rem >= T - tj) {
+    rem -= T - tj;
+    tj++;
+  }
+  const int ti = tj + rem;
+  const int64_t idx = ((int64_t)tj * T - (int64_t)tj * (tj - 1) / 2 + (ti - tj)) * 4096;
+  const double* src = P + poff[d] + idx;
+  float* dst = P32 + poff[d] + idx;
+  const double* Q = CRQ + coff[d] + (int64_t)(nlc[d] + 6) * ldc[d];
+  const double* Y = CRQ + coff[d] + (int64_t)(nlc[d] + 12) * ldc[d];
+  const int64_t l = ldc[d];
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = 64 * ti + (e & 63), c = 64 * tj + (e >> 6);
+    double v = 0.0;
+    if (r < n && c < n) {
+      v = src[e];
+      for (int m = 0; m < 6; m++) v -= Q[r + m * l] * Y[c + m * l] + Y[r + m * l] * Q[c + m * l];
+    }
+    dst[e] = (float)v;
+  }
+}
+
+hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
+                                 const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
+                                 const int* nlc, float* P32) {
+  hipLaunchKernelGGL(kb_deflate_f32, dim3(maxtiles, nsub), dim3(256), 0, st, ns, P, poff, CRQ, coff, ldc, nlc, P32);
+  return hipGetLastError();
+}
+
+// out[r*k+j] = wgt[r]*(y_a[pos_a][j] - y_b[pos_b][j]) - sub[r*k+j]; per-column
+// partial dots with dotw, 64 dual rows per workgroup: part[blk*k + j]
+__global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, const int* __restrict__ rsa,
+                                                  const int* __restrict__ rsb, const int* __restrict__ rpa,
+                                                  const int* __restrict__ rpb, const int64_t* __restrict__ obrow,
+                                                  const double* __restrict__ y, const double* __restrict__ wgt,
+                                                  const double* __restrict__ sub, double* __restrict__ out,
+                                                  const double* __restrict__ dotw, double* __restrict__ part, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double red[4 * 64];
+  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  double s = 0.0;
+  if (j < k) {
+    for (int q = 0; q < 16; q++) {
+      const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
+      if (r >= nlam) break;
+      const int a = rsa[r], b = rsb[r];
+      double v = y[(obrow[a] + rpa[r]) * kp + j];
+      if (b >= 0) v -= y[(obrow[b] + rpb[r]) * kp + j];
+      if (wgt) v *= wgt[r];
+      if (sub) v -= sub[r * k + j];
+      out[r * k + j] = v;
+      if (dotw) s += v * dotw[r * k + j];
+    }
+  }
+  if (dotw) {
+    red[g * 64 + j] = s;
+    __syncthreads();
+    if (g == 0 && j < k) part[(int64_t)blockIdx.x * k + j] = red[j] + red[64 + j] + red[128 + j] + red[192 + j];
+  }
+}
+
+hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
+                             const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
+                             const double* sub, double* out, const double* dotw, double* part, const int* gate) {
+  if (k > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_scatter, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb,
+                     obrow, y, wgt, sub, out, dotw, part, gate);
+  return hipGetLastError();
+}
+
+// out[r*k+j] = x[r*k+j] + sgn * (R_b,a[pos_a,:] h_a[:,j] - R_b,b[pos_b,:] h_b[:,j]); partial dots
+__global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __restrict__ rsa,
+                                             const int* __restrict__ rsb, const int* __restrict__ rpa,
+                                             const int* __restrict__ rpb, const double* __restrict__ Rb,
+                                             const int64_t* __restrict__ rboff, const int* __restrict__ ns,
+                                             const double* __restrict__ h, const double* __restrict__ x,
+                                             double* __restrict__ out, double sgn, const double* __restrict__ dotv,
+                                             double* __restrict__ part, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double red[4 * 64];
+  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  double s = 0.0;
+  if (j < k) {
+    for (int q = 0; q < 16; q++) {
+      const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
+      if (r >= nlam) break;
+      const int a = rsa[r];
+      const int na = ns[a];
+      const double* Ra = Rb + rboff[a] + rpa[r];
+      double v = 0.0;
+      for (int m = 0; m < 6; m++) v += Ra[(int64_t)m * na] * h[(int64_t)(6 * a + m) * k + j];
+      const int b = rsb[r];
+      if (b >= 0) {
+        const int nb = ns[b];
+        const double* Rbb = Rb + rboff[b] + rpb[r];
+        double vb = 0.0;
+        for (int m = 0; m < 6; m++) vb += Rbb[(int64_t)m * nb] * h[(int64_t)(6 * b + m) * k + j];
+        v -= vb;
+      }
+      const double o = (x ? x[r * k + j] : 0.0) + sgn * v;
+      out[r * k + j] = o;
+      if (dotv) s += o * dotv[r * k + j];
+    }
+  }
+  if (dotv) {
+    red[g * 64 + j] = s;
+    __syncthreads();
+    if (g == 0 && j < k) part[(int64_t)blockIdx.x * k + j] = red[j] + red[64 + j] + red[128 + j] + red[192 + j];
+  }
+}
+
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
+                        const int* rpb, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
+                        const double* x, double* out, double sgn, const double* dotv, double* part, const int* gate) {
+  if (k > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, rpa, rpb, Rb,
+                     rboff, ns, h, x, out, sgn, dotv, part, gate);
+  return hipGetLastError();
+}
+
+// per column j: y = sum_{i in cl} H_i^T v_i ; alpha = invGtG y ; h_i = sgn * H_i alpha_cl(i)
+// v_i rows at v[(64 i + a)*kp + j]; one workgroup per column
+__global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k, int kp, const double* __restrict__ v,
+                                                      const double* __restrict__ H, const int* __restrict__ cl,
+                                                      const double* __restrict__ invGtG, double* __restrict__ h,
+                                                      double* __restrict__ alpha, double sgn, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int j = blockIdx.x;
+  __shared__ double y[1024];
+  __shared__ double al[1024];
+  const int ncl = ncol / 6, nsc = nsub / ncl;
+  for (int c = threadIdx.x; c < ncol; c += 256) {
+    const int ccl = c / 6, kk = c % 6;
+    double s = 0.0;
+    for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
+      const double* Hd = H + 36 * d;
+      for (int a = 0; a < 6; a++) s += Hd[a * 6 + kk] * v[((int64_t)64 * d + a) * kp + j];
+    }
+    y[c] = s;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncol; c += 256) {
+    double s = 0.0;
+    for (int q = 0; q < ncol; q++) s += invGtG[c + (int64_t)q * ncol] * y[q];
+    al[c] = s;
+    if (alpha) alpha[(int64_t)c * k + j] = s;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 6 * nsub; q += 256) {
+    const int d = q / 6, a = q % 6;
+    const double* Hd = H + 36 * d + a * 6;
+    const double* ad = al + 6 * cl[d];
+    double s = 0.0;
+    for (int b = 0; b < 6; b++) s += Hd[b] * ad[b];
+    h[(int64_t)q * k + j] = sgn * s;
+  }
+}
+
+hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
+                                 const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate) {
+  if (ncol > 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_proj_coarse, dim3(k), dim3(256), 0, st, nsub, ncol, k, kp, v, H, cl, invGtG, h, alpha, sgn, gate);
+  return hipGetLastError();
+}
+
+// Per-column reductions of the partial dots (fixed order) and the PCPG scalar
+// logic (Cluster.cpp:1398-1448).  Column status: 0 running, 1 waiting for an FP64
+// true-residual evaluation, 2 converged, 3 iteration limit, 4 empty slot.
+//   mode 0: wFw, rho = -gPz / wFw
+//   mode 1: gamma, iteration count; the stopping test sqrt(gPz) < eps*norm0 gives
+//           status 2 (status 1 when certification is on), a recursive residual
+//           below rtol times the last true one gives status 1 (refinement point),
+//           the iteration limit status 3; pending = some column has status 1
+//   mode 2: initial gPz, norm0
+//   mode 3: (only while pending) for status-1 columns, from the FP64 true residual:
+//           converged -> 2; limit -> 3; otherwise the iteration restarts from the
+//           true residual (repl = 1: g <- g_true, w <- P M P g_true); pending stays
+//           set for the gated launches that follow and is renewed by mode 1
+//   mode 4: record the true residual of every column (res)
+__global__ __launch_bounds__(64) void kb_reduce(int nparts, int k, const double* __restrict__ part, BScalars* sc,
+                                                int mode) {
+  const int j = threadIdx.x;
+  __shared__ int any;
+  if (mode == 3 && sc->pending == 0) return;  // uniform over the workgroup
+  if (j == 0) any = 0;
+  __syncthreads();
+  if (j < k) {
+    double s = 0.0;
+    for (int b = 0; b < nparts; b++) s += part[(int64_t)b * k + j];
+    const int stt = sc->done[j];
+    if (mode == 0) {
+      if (!stt) {
+        sc->wFw[j] = s;
+        sc->rho[j] = -sc->gPz[j] / s;
+      }
+    } else if (mode == 1) {
+      sc->repl[j] = 0;
+      if (!stt) {
+        sc->gPz_prev[j] = sc->gPz[j];
+        sc->gPz[j] = s;
+        sc->gamma[j] = s / sc->gPz_prev[j];
+        sc->it[j] += 1;
+        if (!(s == s)) sc->err = 1;
+        const double r = sqrt(fabs(s));
+        if (!(r >= sc->eps * sc->norm0[j])) sc->done[j] = sc->cert ? 1 : 2;
+        else if (r < sc->rtol * sc->ref[j]) sc->done[j] = 1;
+        else if (sc->it[j] >= sc->max_iter) sc->done[j] = 3;
+        if (sc->done[j] == 1) any = 1;
+      }
+    } else if (mode == 2) {
+      if (stt != 4) {
+        sc->gPz[j] = s;
+        sc->norm0[j] = sqrt(fabs(s));
+        sc->ref[j] = sc->norm0[j];
+        sc->it[j] = 0;
+        sc->res[j] = 1.0;
+        sc->repl[j] = 0;
+        sc->ncert[j] = 0;
+        if (!(s == s)) sc->err = 1;
+        sc->done[j] = !(s > 0.0) ? 2 : (sc->max_iter <= 0 ? 3 : 0);
+      }
+    } else if (mode == 3) {
+      sc->repl[j] = 0;
+      if (stt == 1) {
+        const double r = sqrt(fabs(s));
+        sc->res[j] = sc->norm0[j] > 0 ? r / sc->norm0[j] : 0.0;
+        sc->ncert[j] += 1;
+        if (r < sc->eps * sc->norm0[j]) {
+          sc->done[j] = 2;
+        } else if (sc->it[j] >= sc->max_iter) {
+          sc->done[j] = 3;
+        } else {
+          sc->repl[j] = 1;
+          sc->gPz[j] = s;
+          sc->ref[j] = r;
+          sc->done[j] = 0;
+        }
+      }
+    } else {
+      if (stt != 4) sc->res[j] = sc->norm0[j] > 0 ? sqrt(fabs(s)) / sc->norm0[j] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (j == 0) {
+    if (mode == 1) sc->pending = any;
+    if (mode == 2) sc->pending = 0;
+  }
+}
+
+hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode) {
+  if (k > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, nparts, k, part, sc, mode);
+  return hipGetLastError();
+}
+
+// mode 0: lam += rho w ; g += rho Fw      mode 1: w = Pz + gamma w      (running columns)
+// mode 2: g = g_true, w = P M P g_true where repl[j]                     (restart)
+__global__ __launch_bounds__(256) void kb_update(int64_t nlam, int k, int mode, double* __restrict__ a,
+                                                 double* __restrict__ b, const double* __restrict__ c,
+                                                 const double* __restrict__ e, const BScalars* __restrict__ sc,
+                                                 const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nlam * k) return;
+  const int j = (int)(t % k);
+  if (mode == 2) {
+    if (sc->repl[j]) {
+      a[t] = c[t];
+      b[t] = e[t];
+    }
+    return;
+  }
+  if (sc->done[j]) return;
+  if (mode == 0) {
+    const double rho = sc->rho[j];
+    a[t] += rho * c[t];
+    b[t] += rho * e[t];
+  } else {
+    a[t] = c[t] + sc->gamma[j] * a[t];
+  }
+}
+
+hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
+                            const double* e, const BScalars* sc, const int* gate) {
+  const int64_t n = nlam * k;
+  hipLaunchKernelGGL(kb_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nlam, k, mode, a, b, c, e, sc,
+                     gate);
+  return hipGetLastError();
+}
+
+// packed local block (column j at j*npacked) <-> interior-first padded blocks
+//   dir 0: Vr[(orrow[d] + i)*kp + j] = V[j][perm[i]] (i < nr), Vb[(obrow[d] + p)*kp + j] = V[j][perm[nr+p]]
+//   dir 1: the reverse
+__global__ __launch_bounds__(256) void kb_permute(int dir, int k, int kp, int64_t npacked, const int* __restrict__ nn,
+                                                  const int* __restrict__ nr, const int* __restrict__ perm,
+                                                  const int64_t* __restrict__ noff, double* __restrict__ V,
+                                                  double* __restrict__ Vr, const int64_t* __restrict__ orrow,
+                                                  double* __restrict__ Vb, const int64_t* __restrict__ obrow) {
+  const int d = blockIdx.y;
+  const int n = nn[d], r = nr[d];
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int i = (int)(t / k), j = (int)(t % k);
+  if (i >= n) return;
+  const int l = perm[noff[d] + i];
+  double* v = V + (int64_t)j * npacked + noff[d] + l;
+  double* s = i < r ? Vr + (orrow[d] + i) * kp + j : Vb + (obrow[d] + (i - r)) * kp + j;
+  if (dir == 0) *s = *v;
+  else *v = *s;
+}
+
+hipError_t launch_kb_permute(hipStream_t st, int nsub, int maxn, int dir, int k, int kp, int64_t npacked,
+                             const int* n, const int* nr, const int* perm, const int64_t* noff, double* V, double* Vr,
+                             const int64_t* orrow, double* Vb, const int64_t* obrow) {
+  const int64_t tot = (int64_t)maxn * k;
+  hipLaunchKernelGGL(kb_permute, dim3((unsigned)((tot + 255) / 256), nsub), dim3(256), 0, st, dir, k, kp, npacked, n,
+                     nr, perm, noff, V, Vr, orrow, Vb, obrow);
+  return hipGetLastError();
+}
+
+// V[j][perm[nr+p]] -= (Bf^T lam)[p][j] at the interface DOFs of every subdomain
+__global__ __launch_bounds__(256) void kb_sub_bt(int k, int64_t npacked, const int* __restrict__ nr,
+                                                 const int* __restrict__ ns, const int* __restrict__ perm,
+                                                 const int64_t* __restrict__ noff, const int64_t* __restrict__ boff,
+                                                 const int* __restrict__ gptr, const int* __restrict__ gidx,
+                                                 const double* __restrict__ lam, double* __restrict__ V) {
+  const int d = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int p = (int)(t / k), j = (int)(t % k);
+  if (p >= ns[d]) return;
+  double s = 0.0;
+  const int64_t q = boff[d] + p;
+  for (int e = gptr[q]; e < gptr[q + 1]; e++) {
+    const int code = gidx[e];
+    const int r = code >= 0 ? code : ~code;
+    s += code >= 0 ? lam[(int64_t)r * k + j] : -lam[(int64_t)r * k + j];
+  }
+  V[(int64_t)j * npacked + noff[d] + perm[noff[d] + nr[d] + p]] -= s;
+}
+
+hipError_t launch_kb_sub_bt(hipStream_t st, int nsub, int maxns, int k, int64_t npacked, const int* nr, const int* ns,
+                            const int* perm, const int64_t* noff, const int64_t* boff, const int* gptr,
+                            const int* gidx, const double* lam, double* V) {
+  const int64_t tot = (int64_t)maxns * k;
+  hipLaunchKernelGGL(kb_sub_bt, dim3((unsigned)((tot + 255) / 256), nsub), dim3(256), 0, st, k, npacked, nr, ns, perm,
+                     noff, boff, gptr, gidx, lam, V);
+  return hipGetLastError();
+}
+
+// U[j*nglob + gdof] = X[j][l] + R_i[l,:] (la[arow_d + m][j] + h[6d+m][j]) for owned DOFs
+__global__ __launch_bounds__(256) void kb_recover(int k, int kp, int64_t npacked, int64_t nglob,
+                                                  const int* __restrict__ nn, const int64_t* __restrict__ noff,
+                                                  const double* __restrict__ R, const int64_t* __restrict__ roff,
+                                                  const int64_t* __restrict__ arow, const double* __restrict__ la,
+                                                  const double* __restrict__ h, const double* __restrict__ X,
+                                                  const int* __restrict__ gdof, const unsigned char* __restrict__ owner,
+                                                  double* __restrict__ U) {
+  const int d = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int l = (int)(t / k), j = (int)(t % k);
+  const int n = nn[d];
+  if (l >= n) return;
+  if (!owner[noff[d] + l]) return;
+  const double* Rd = R + roff[d] + l;
+  double s = X[(int64_t)j * npacked + noff[d] + l];
+  for (int m = 0; m < 6; m++) s += Rd[(int64_t)m * n] * (la[(arow[d] + m) * kp + j] + h[(int64_t)(6 * d + m) * k + j]);
+  U[(int64_t)j * nglob + gdof[noff[d] + l]] = s;
+}
+
+hipError_t launch_kb_recover(hipStream_t st, int nsub, int maxn, int k, int kp, int64_t npacked, int64_t nglob,
+                             const int* n, const int64_t* noff, const double* R, const int64_t* roff,
+                             const int64_t* arow, const double* la, const double* h, const double* X, const int* gdof,
+                             const unsigned char* owner, double* U) {
+  const int64_t tot = (int64_t)maxn * k;
+  hipLaunchKernelGGL(kb_recover, dim3((unsigned)((tot + 255) / 256), nsub), dim3(256), 0, st, k, kp, npacked, nglob, n,
+                     noff, R, roff, arow, la, h, X, gdof, owner, U);
+  return hipGetLastError();
+}
+
+}  // namespace hfls

@@ -252,3 +252,46 @@ extern "C" int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f
 extern "C" int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes) {
   return guard([&] { *ms = m->e->time_stream(which, reps, bytes); });
 }
+
+extern "C" int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, int precision, const double* f,
+                                double* u, double* lam, hfls_case_stats* cases, hfls_block_stats* stats) {
+  return guard([&] {
+    if (k < 1 || k > hfls::KMAX) throw std::runtime_error("k must be in 1..64");
+    std::vector<int> it(k), stt(k);
+    std::vector<double> res(k);
+    hfls::BlockInfo info;
+    m->e->solve_block(eps, max_iter, k, precision, f, u, lam, it.data(), stt.data(), res.data(), &info);
+    if (cases)
+      for (int j = 0; j < k; j++) {
+        cases[j].iterations = it[j];
+        cases[j].status = stt[j];
+        cases[j].residual = res[j];
+      }
+    if (stats) {
+      stats->solve_ms = info.solve_ms;
+      stats->init_ms = info.init_ms;
+      stats->pcpg_ms = info.pcpg_ms;
+      stats->recover_ms = info.recover_ms;
+      stats->sweeps = info.sweeps;
+      stats->cert_steps = info.cert_steps;
+    }
+  });
+}
+
+extern "C" int hfls_apply_block(hfls_mesh m, int op, int k, int precision, const double* x, double* y) {
+  return guard([&] { m->e->apply_block(op, k, precision, x, y); });
+}
+
+extern "C" int hfls_problem_traction(hfls_problem p, uint64_t seed, int case0, int k, double* f) {
+  return guard([&] {
+    if (k < 1 || case0 < 0) throw std::runtime_error("invalid load-case range");
+    hfls::traction_loads(*p->p, seed, case0, k, f);
+  });
+}
+
+extern "C" int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f) {
+  return guard([&] {
+    if (k < 1 || case0 < 0) throw std::runtime_error("invalid load-case range");
+    hfls::traction_loads(m->e->problem(), seed, case0, k, f);
+  });
+}

@@ -23,14 +23,6 @@
 
 namespace hfls {
 
-#define HC(x)                                                                                  \
-  do {                                                                                         \
-    hipError_t e_ = (x);                                                                       \
-    if (e_ != hipSuccess)                                                                      \
-      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +    \
-                               __FILE__ + ":" + std::to_string(__LINE__) + ": " #x);           \
-  } while (0)
-
 template <class T>
 void DVec<T>::release() {
   if (p) (void)hipFree(p);
@@ -60,6 +52,8 @@ template struct DVec<int>;
 template struct DVec<int64_t>;
 template struct DVec<unsigned char>;
 template struct DVec<Scalars>;
+template struct DVec<float>;
+template struct DVec<BScalars>;
 
 static int64_t align32(int64_t x) { return (x + 31) / 32 * 32; }
 static int round8(int x) { return (x + 7) / 8 * 8; }

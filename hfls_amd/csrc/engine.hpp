@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,15 @@
 #include "problem.hpp"
 
 namespace hfls {
+
+// throw on a failed HIP call, naming the call and its place
+#define HC(x)                                                                                  \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +    \
+                               __FILE__ + ":" + std::to_string(__LINE__) + ": " #x);           \
+  } while (0)
 
 struct SolveInfo {
   int iterations = 0;
@@ -29,6 +39,13 @@ struct SetupInfo {
   double factor_ms = 0.0;
   size_t device_bytes = 0;
   int min_defect = 0, max_defect = 0;
+};
+
+// Block solve of k <= 64 load cases (one column each).
+struct BlockInfo {
+  double solve_ms = 0.0, init_ms = 0.0, pcpg_ms = 0.0, recover_ms = 0.0, cert_ms = 0.0;
+  int sweeps = 0;        // block iterations executed
+  int cert_steps = 0;    // FP64 true-residual evaluations summed over the columns
 };
 
 template <class T>
@@ -69,6 +86,19 @@ class Engine {
   // (which=1)) with HIP events on the library stream; returns ms per launch and
   // the algorithmic bytes one launch moves.
   double time_stream(int which, int reps, double* bytes);
+
+  // Block of k load cases (column major: column j of f_local at j*packed_n(), of u
+  // at j*3*nPoints, of lam at j*nlam).  precision 64: F_i and S_i applied in FP64;
+  // 32: F_i and S_i applied in FP32 on the matrix cores with FP32 accumulation,
+  // everything else FP64, and each column's stopping test certified by its FP64
+  // true residual.  Per column: iterations, status (2 converged, 3 limit) and the
+  // final FP64 residual sqrt(g'PMPg)/sqrt(g0'PMPg0).
+  void solve_block(double eps, int max_iter, int k, int precision, const double* f_local, double* u, double* lam,
+                   int* iters, int* status, double* res, BlockInfo* info);
+  // op 0 F, 1 M (preconditioner), 2 P (projector) applied to k dual vectors
+  // (column major, ld nlam); precision selects the F_i / S_i arithmetic
+  void apply_block(int op, int k, int precision, const double* x, double* y);
+  size_t block_bytes() const;
 
   // Accessors for tests: per-subdomain results of the setup (host copies).
   void get_R(std::vector<double>& R) const;       // packed local, n x 6 column major per subdomain
@@ -115,6 +145,26 @@ class Engine {
   int nparts = 0;
   bool lumped = false;
   std::vector<int> h_fix, h_def;
+
+  // ---- block state (allocated on the first block call)
+  bool blk_ready = false;
+  int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
+  int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
+  DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
+      b_v6off, b_v6row, b_crrow;
+  DVec<int> b_crn;
+  DVec<double> b_CR;
+  DVec<float> b_Kt32, b_St32, b_xb32, b_y32;
+  DVec<double> b_xb, b_yb, b_cr, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
+      b_Vr, b_Xr, b_V, b_X, b_U;
+  DVec<BScalars> b_sc;
+  int b_nparts = 0;
+  void block_prepare();
+  void bF(int k, int prec, const double* win, double* out, const double* sub, const double* dotw,
+          const int* gate = nullptr);
+  void bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate = nullptr);
+  void bM(int k, int prec, const double* x, double* out, const int* gate = nullptr);
+  void b_kplus(int k, bool interior_too);  // from b_V: yb = (K_c^+ v)_b-ish pieces; see engine_block.cpp
 
   void build_meta();
   void factor_subdomains();

@@ -1,0 +1,456 @@
+// Block PCPG: k load cases of one mesh solved together (Cluster.cpp:1363-1458 run
+// per column, recovery as Cluster.cpp:1648-1678).
+//
+// The dense local operators become GEMMs over a block of right-hand sides on the
+// matrix cores (tgemm.hip): K+_bb and S_i (packed symmetric 64x64 tiles) in FP32
+// with FP32 accumulation inside the iteration (precision 32) or FP64 (precision
+// 64); the correction blocks [C_i | R_b,i], the cluster inverse, the interior
+// blocks K_rr^-1 and K_rr^-1 K_rb of the initialisation and the recovery always in
+// FP64.  Dual vectors, dot products, the projector and the coarse problem are FP64.
+//
+// Mixed precision: the recursive residual g is advanced with the FP32 operator.
+// When a column meets the stopping test, its FP64 true residual g = F lam - d is
+// evaluated and the test repeated on it (certification); a column that fails has
+// its recursive residual replaced by the true one and resumes.  The stopping test
+// is checked every `chunk` iterations from the column's own start, so a column's
+// iterates do not depend on the other columns of the block.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace hfls {
+
+static constexpr int KP = KMAX;  // leading dimension of row-major blocks
+static int64_t r64(int64_t x) { return (x + 63) / 64 * 64; }
+static int64_t a32(int64_t x) { return (x + 31) / 32 * 32; }
+
+template <class T>
+static void zero_alloc(DVec<T>& v, size_t n, hipStream_t st) {
+  v.alloc(std::max<size_t>(n, 1));
+  HC(hipMemsetAsync(v.p, 0, v.n * sizeof(T), st));
+}
+
+void Engine::block_prepare() {
+  if (blk_ready) return;
+  const int N = P->nsub, NC = P->ncl;
+  const int64_t nl = P->nlam, np = off_n[N], ng = 3 * (int64_t)P->nPoints;
+  std::vector<int64_t> obrow(N), orrow(N), xoff(N), roff(N), croff(N), cro(N), cro_rb(N), cro_q(N), cro_y(N),
+      arow(N), abase(N), v6row(N), v6off(N);
+  std::vector<int> crn(N);
+  int64_t rb = 0, rr = 0, rc = 0, co = 0;
+  maxTb = maxTr = maxTcr = maxTcl = 0;
+  for (int d = 0; d < N; d++) {
+    obrow[d] = rb;
+    rb += r64(h_ns[d]);
+    maxTb = std::max(maxTb, (h_ns[d] + 63) / 64);
+    orrow[d] = rr;
+    rr += r64(h_nr[d]);
+    maxTr = std::max(maxTr, (h_nr[d] + 63) / 64);
+    crn[d] = h_nlc[d] + 18;
+    croff[d] = rc;
+    rc += r64(crn[d]);
+    maxTcr = std::max(maxTcr, (crn[d] + 63) / 64);
+    cro[d] = co;
+    cro_rb[d] = co + (int64_t)h_nlc[d] * h_ldb[d];
+    cro_q[d] = cro_rb[d] + 6 * (int64_t)h_ldb[d];
+    cro_y[d] = cro_q[d] + 6 * (int64_t)h_ldb[d];
+    co += a32((int64_t)h_ldb[d] * crn[d]);
+    xoff[d] = obrow[d] * KP;
+    roff[d] = orrow[d] * KP;
+    v6row[d] = 64 * (int64_t)d;
+    v6off[d] = v6row[d] * KP;
+  }
+  NBP = rb;
+  NRP = rr;
+  CRP = rc;
+  std::vector<int64_t> gcrow(NC), gcoff(NC), croff_el(N);
+  int64_t gr = 0;
+  for (int cl = 0; cl < NC; cl++) {
+    const int nlen = h_nlcl[cl] + 6 * P->nsub_cl;
+    gcrow[cl] = gr;
+    gcoff[cl] = gr * KP;
+    gr += r64(nlen);
+    maxTcl = std::max(maxTcl, (nlen + 63) / 64);
+  }
+  GCP = gr;
+  for (int d = 0; d < N; d++) {
+    const int cl = h_cl[d];
+    croff_el[d] = croff[d] * KP;
+    arow[d] = gcrow[cl] + h_nlcl[cl] + 6 * (d - cl * P->nsub_cl);
+    abase[d] = croff[d] + h_nlc[d];
+  }
+  std::vector<int> lcmap;
+  d_lcmap.download(lcmap, st);
+  std::vector<int64_t> lcrow(off_lc[N]), lcpos(off_lc[N]);
+  for (int d = 0; d < N; d++) {
+    const int cl = h_cl[d];
+    for (int m = 0; m < h_nlc[d]; m++) {
+      const int64_t e = off_lc[d] + m;
+      lcrow[e] = lcmap[e] - h_la_off[cl] + gcrow[cl];
+      lcpos[e] = croff[d] + m;
+    }
+  }
+  b_obrow.upload(obrow, st);
+  b_orrow.upload(orrow, st);
+  b_xoff.upload(xoff, st);
+  b_roff.upload(roff, st);
+  b_croff.upload(croff_el, st);
+  b_crrow.upload(croff, st);
+  b_cro.upload(cro, st);
+  b_gcrow.upload(gcrow, st);
+  b_gcoff.upload(gcoff, st);
+  b_lcrow.upload(lcrow, st);
+  b_lcpos.upload(lcpos, st);
+  b_arow.upload(arow, st);
+  b_abase_cr.upload(abase, st);
+  b_v6off.upload(v6off, st);
+  b_v6row.upload(v6row, st);
+  b_crn.upload(crn, st);
+  // work blocks (zero: padded rows and unused columns must stay finite)
+  zero_alloc(b_xb, NBP * KP, st);
+  zero_alloc(b_yb, NBP * KP, st);
+  // CRQ = [C_i | R_b,i | Q_i | Yh_i]: n_s x (nlc + 18), column major, ld = ldb, with
+  // Q_i an orthonormal basis of span(R_b,i) and, for A = K+_bb, Y = A Q, W = Q^T Y,
+  // Yh = Y - Q W / 2, so that A = (I - QQ^T) A (I - QQ^T) + Q Yh^T + Yh Q^T.  The
+  // projected part is ~100x smaller than A (the regularised inverse is large along
+  // the rigid-body modes) and is the part stored in FP32.
+  zero_alloc(b_CR, co, st);
+  if (maxnlc > 0)
+    HC(launch_copy_cols(st, N, maxnlc, d_ns.p, d_nlc.p, d_C.p, d_off_C.p, d_ldb.p, b_CR.p, b_cro.p, d_ldb.p, 1.0));
+  {
+    std::vector<double> Rb, Q(off_Rb[N]), Y;
+    d_Rb.download(Rb, st);
+    for (int d = 0; d < N; d++) {  // modified Gram-Schmidt per subdomain
+      const int n = h_ns[d];
+      double* q = Q.data() + off_Rb[d];
+      std::copy(Rb.begin() + off_Rb[d], Rb.begin() + off_Rb[d + 1], q);
+      for (int a = 0; a < 6; a++) {
+        for (int b = 0; b < a; b++) {
+          double t = 0.0;
+          for (int p = 0; p < n; p++) t += q[p + (int64_t)a * n] * q[p + (int64_t)b * n];
+          for (int p = 0; p < n; p++) q[p + (int64_t)a * n] -= t * q[p + (int64_t)b * n];
+        }
+        double t = 0.0;
+        for (int p = 0; p < n; p++) t += q[p + (int64_t)a * n] * q[p + (int64_t)a * n];
+        if (!(t > 0.0)) throw std::runtime_error("interface kernel block is rank deficient");
+        t = 1.0 / std::sqrt(t);
+        for (int p = 0; p < n; p++) q[p + (int64_t)a * n] *= t;
+      }
+    }
+    DVec<double> dQ, dY;
+    dQ.upload(Q, st);
+    dY.alloc(off_Rb[N]);
+    HC(launch_kb_cols(st, N, maxns, 0, 6, KP, d_ns.p, b_obrow.p, dQ.p, d_off_Rb.p, d_ns.p, b_xb.p));
+    HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
+                        b_xoff.p, nullptr, KP, 1.0, nullptr));
+    HC(launch_kb_cols(st, N, maxns, 1, 6, KP, d_ns.p, b_obrow.p, dY.p, d_off_Rb.p, d_ns.p, b_yb.p));
+    dY.download(Y, st);
+    for (int d = 0; d < N; d++) {
+      const int n = h_ns[d];
+      const double* q = Q.data() + off_Rb[d];
+      double* y = Y.data() + off_Rb[d];
+      double W[36];
+      for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+          double t = 0.0;
+          for (int p = 0; p < n; p++) t += q[p + (int64_t)a * n] * y[p + (int64_t)b * n];
+          W[a * 6 + b] = t;
+        }
+      for (int b = 0; b < 6; b++)
+        for (int a = 0; a < 6; a++) {
+          const double w = 0.25 * (W[a * 6 + b] + W[b * 6 + a]);  // half of the symmetrised W
+          for (int p = 0; p < n; p++) y[p + (int64_t)b * n] -= q[p + (int64_t)a * n] * w;
+        }
+    }
+    dY.upload(Y, st);
+    DVec<int64_t> dcr, dcq, dcy;
+    dcr.upload(cro_rb, st);
+    dcq.upload(cro_q, st);
+    dcy.upload(cro_y, st);
+    HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_CR.p, dcr.p, d_ldb.p, 1.0));
+    HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, dQ.p, d_off_Rb.p, d_ns.p, b_CR.p, dcq.p, d_ldb.p, 1.0));
+    HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, dY.p, d_off_Rb.p, d_ns.p, b_CR.p, dcy.p, d_ldb.p, 1.0));
+    HC(hipMemsetAsync(b_xb.p, 0, b_xb.n * sizeof(double), st));
+    HC(hipMemsetAsync(b_yb.p, 0, b_yb.n * sizeof(double), st));
+    HC(hipStreamSynchronize(st));
+  }
+  // FP32 tiles: deflated K+_bb, plain S_i
+  b_Kt32.alloc(off_P[N]);
+  b_St32.alloc(off_P[N]);
+  HC(launch_kb_deflate_f32(st, N, maxtiles, d_ns.p, d_Kt.p, d_off_P.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, b_Kt32.p));
+  HC(launch_kb_to_f32(st, off_P[N], d_St.p, b_St32.p));
+  zero_alloc(b_xb32, NBP * KP, st);
+  zero_alloc(b_y32, NBP * KP, st);
+  zero_alloc(b_cr, CRP * KP, st);
+  zero_alloc(b_v6, (int64_t)64 * N * KP, st);
+  zero_alloc(b_gc, GCP * KP, st);
+  zero_alloc(b_la, GCP * KP, st);
+  zero_alloc(b_h, (int64_t)6 * N * KMAX, st);
+  zero_alloc(b_hg, (int64_t)6 * N * KMAX, st);
+  for (DVec<double>* v : {&b_lam, &b_g, &b_w, &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64}) zero_alloc(*v, nl * KMAX, st);
+  b_nparts = (int)((nl + 63) / 64);
+  zero_alloc(b_part, (int64_t)b_nparts * KMAX, st);
+  zero_alloc(b_Vr, NRP * KP, st);
+  zero_alloc(b_Xr, NRP * KP, st);
+  zero_alloc(b_V, np * KMAX, st);
+  zero_alloc(b_X, np * KMAX, st);
+  zero_alloc(b_U, ng * KMAX, st);
+  b_sc.alloc(1);
+  HC(hipStreamSynchronize(st));
+  blk_ready = true;
+}
+
+size_t Engine::block_bytes() const {
+  if (!blk_ready) return 0;
+  size_t s = 0;
+  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_y32.n) * sizeof(float);
+  for (const DVec<double>* v : {&b_CR, &b_xb, &b_yb, &b_cr, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+                                &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_X, &b_U})
+    s += v->n * sizeof(double);
+  return s;
+}
+
+// out = F win (- sub), with per-column partial dots against dotw; every launch is
+// skipped on the device while *gate == 0 (gate may be null)
+void Engine::bF(int k, int prec, const double* win, double* out, const double* sub, const double* dotw,
+                const int* gate) {
+  const int N = P->nsub, NC = P->ncl;
+  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
+                      prec == 32 ? b_xb32.p : nullptr, gate));
+  if (prec == 32) {
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+                        b_xoff.p, nullptr, KP, 1.0f, gate));
+    HC(launch_kb_widen(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, b_y32.p, b_yb.p, gate));
+  } else {
+    HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
+                        b_xoff.p, nullptr, KP, 1.0, gate));
+  }
+  // HTFETI correction (Cluster.cpp:1129-1186): [c; a] = [C_i | R_b,i]^T x_b,
+  // [lam_c; alpha] = Ac^-1 [c; -a], y_b += R_b alpha - C lam_c; in FP32 also the
+  // rank-12 FP64 part of K+_bb, Q (Yh^T x) + Yh (Q^T x)
+  HC(launch_tgemm_f64(st, N, maxTcr, 2, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_xb.p, b_xoff.p, b_cr.p,
+                      b_croff.p, nullptr, KP, 1.0, gate));
+  HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
+                          b_lcpos.p, b_cr.p, b_abase_cr.p, b_gc.p, b_gcrow.p, gate));
+  HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
+                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate));
+  HC(launch_kb_corr_apply(st, N, maxns, k, KP, d_ns.p, b_obrow.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, d_off_lc.p,
+                          b_lcrow.p, b_arow.p, b_la.p, 1, prec == 32 ? 1 : 0, b_cr.p, b_crrow.p, b_yb.p, gate));
+  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
+                       dotw, b_part.p, gate));
+}
+
+// out = P x = x - G (G^T G)^-1 G^T x (Cluster.cpp:1314-1329); hbuf = H alpha per subdomain
+void Engine::bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate) {
+  const int N = P->nsub;
+  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, nullptr, b_xb.p,
+                      nullptr, gate));
+  HC(launch_tgemm_f64(st, N, 1, 2, 1, d_ns.p, d_sixes.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_xb.p, b_xoff.p, b_v6.p,
+                      b_v6off.p, nullptr, KP, 1.0, gate));
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, hbuf, x, out, -1.0,
+                  dotv, b_part.p, gate));
+}
+
+// out = W Bf S Bf^T W x (Cluster.cpp:1043-1112)
+void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) {
+  const int N = P->nsub;
+  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
+                      prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
+  if (prec == 32) {
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+                        b_xoff.p, nullptr, KP, 1.0f, gate));
+    HC(launch_kb_widen(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, b_y32.p, b_yb.p, gate));
+  } else {
+    HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
+                        b_xoff.p, nullptr, KP, 1.0, gate));
+  }
+  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
+                       out, nullptr, nullptr, gate));
+}
+
+// From the local block b_V (FP64): b_Vr = v_r, b_xb = t_b = v_b - (K_rr^-1 K_rb)^T v_r,
+// b_yb = (K^+ (v - Bc^T lam_c))_b (+ R_b alpha_c if with_R), b_la = [lam_c; alpha_c],
+// b_v6 = R^T v (Cluster.cpp:967-1040).  With interior_too, b_X = K^+(v - Bc^T lam_c)
+// in local order (packed, column major).
+void Engine::b_kplus(int k, bool interior_too) {
+  const int N = P->nsub, NC = P->ncl;
+  const int64_t np = off_n[N];
+  HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, b_V.p, b_Vr.p, b_orrow.p,
+                       b_xb.p, b_obrow.p));
+  if (maxTr > 0)
+    HC(launch_tgemm_f64(st, N, maxTb, 2, 1, d_nr.p, d_ns.p, d_T.p, d_off_T.p, d_ldt.p, b_Vr.p, b_roff.p, b_xb.p,
+                        b_xoff.p, b_xb.p, KP, -1.0, nullptr));
+  HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
+                      b_xoff.p, nullptr, KP, 1.0, nullptr));
+  HC(launch_tgemm_f64(st, N, maxTcr, 2, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_xb.p, b_xoff.p, b_cr.p,
+                      b_croff.p, nullptr, KP, 1.0, nullptr));
+  HC(launch_kb_rtf(st, N, k, KP, np, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_V.p, b_v6.p));
+  HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
+                          b_lcpos.p, b_v6.p, b_v6row.p, b_gc.p, b_gcrow.p));
+  HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
+                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, nullptr));
+  HC(launch_kb_corr_apply(st, N, maxns, k, KP, d_ns.p, b_obrow.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, d_off_lc.p,
+                          b_lcrow.p, b_arow.p, b_la.p, interior_too ? 0 : 1, 0, b_cr.p, b_crrow.p, b_yb.p));
+  if (!interior_too) return;
+  // x_r = K_rr^-1 v_r - (K_rr^-1 K_rb) y_b
+  if (maxTr > 0) {
+    HC(launch_tgemm_f64(st, N, maxTr, 2, 0, d_nr.p, d_nr.p, d_Kinv.p, d_off_Kr.p, d_ldt.p, b_Vr.p, b_roff.p, b_Xr.p,
+                        b_roff.p, nullptr, KP, 1.0, nullptr));
+    HC(launch_tgemm_f64(st, N, maxTr, 2, 0, d_nr.p, d_ns.p, d_T.p, d_off_T.p, d_ldt.p, b_yb.p, b_xoff.p, b_Xr.p,
+                        b_roff.p, b_Xr.p, KP, -1.0, nullptr));
+  }
+  HC(launch_kb_permute(st, N, maxn, 1, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, b_X.p, b_Xr.p, b_orrow.p,
+                       b_yb.p, b_obrow.p));
+}
+
+void Engine::apply_block(int op, int k, int prec, const double* x_h, double* y_h) {
+  if (k < 1 || k > KMAX) throw std::runtime_error("block size must be 1..64");
+  if (prec != 32 && prec != 64) throw std::runtime_error("precision must be 32 or 64");
+  if (op < 0 || op > 2) throw std::runtime_error("operator must be 0 (F), 1 (M) or 2 (P)");
+  block_prepare();
+  const int64_t nl = P->nlam;
+  std::vector<double> t((size_t)nl * k);
+  for (int j = 0; j < k; j++)
+    for (int64_t r = 0; r < nl; r++) t[(size_t)r * k + j] = x_h[(size_t)j * nl + r];
+  HC(hipMemcpyAsync(b_w.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if (op == 0) bF(k, prec, b_w.p, b_Fw.p, nullptr, nullptr);
+  else if (op == 1) bM(k, prec, b_w.p, b_Fw.p);
+  else bP(k, b_w.p, b_Fw.p, b_h.p, nullptr);
+  HC(hipMemcpyAsync(t.data(), b_Fw.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  for (int j = 0; j < k; j++)
+    for (int64_t r = 0; r < nl; r++) y_h[(size_t)j * nl + r] = t[(size_t)r * k + j];
+}
+
+void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,
+                         int* iters, int* status, double* res, BlockInfo* info) {
+  if (k < 1 || k > KMAX) throw std::runtime_error("block size must be 1..64");
+  if (prec != 32 && prec != 64) throw std::runtime_error("precision must be 32 or 64");
+  block_prepare();
+  const int N = P->nsub;
+  const int64_t nl = P->nlam, np = off_n[N], ng = 3 * (int64_t)P->nPoints;
+  // loads, column major
+  if (f_h) {
+    HC(hipMemcpyAsync(b_V.p, f_h, (size_t)np * k * sizeof(double), hipMemcpyHostToDevice, st));
+  } else {
+    for (int j = 0; j < k; j++)
+      HC(hipMemcpyAsync(b_V.p + (size_t)j * np, d_f.p, np * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  BScalars s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.eps = eps;
+  s0.max_iter = max_iter;
+  s0.cert = prec == 32 ? 1 : 0;
+  s0.rtol = prec == 32 ? 1e-3 : 0.0;
+  for (int j = k; j < KMAX; j++) s0.done[j] = 4;
+  HC(hipMemcpyAsync(b_sc.p, &s0, sizeof(BScalars), hipMemcpyHostToDevice, st));
+  hipEvent_t ev[4];
+  for (auto& e : ev) HC(hipEventCreate(&e));
+  HC(hipEventRecord(ev[0], st));
+  // d = Bf K_c^+ f ; e = R^T f ; lam0 = G (G^T G)^-1 e (Cluster.cpp:1385-1394)
+  b_kplus(k, false);
+  HC(launch_kb_scatter(st, nl, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, nullptr, b_d.p,
+                       nullptr, nullptr));
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr, 1.0));
+  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_h.p, nullptr, b_lam.p,
+                  1.0, nullptr, nullptr));
+  // g0 = F lam0 - d in FP64 ; w0 = P M P g0
+  bF(k, 64, b_lam.p, b_g.p, b_d.p, nullptr);
+  bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr);
+  bM(k, prec, b_Pg.p, b_z.p);
+  bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p);
+  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 2));
+  HC(hipMemcpyAsync(b_w.p, b_Pz.p, (size_t)nl * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HC(hipEventRecord(ev[1], st));
+  const int chunk = 8;
+  const int* pending = &b_sc.p->pending;
+  int sweeps = 0;
+  BScalars sh;
+  for (;;) {
+    for (int c = 0; c < chunk; c++) {
+      bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p);
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0));
+      HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p));
+      bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr);
+      bM(k, prec, b_Pg.p, b_z.p);
+      bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p);
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1));
+      HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p));
+      if (prec == 32) {
+        // columns waiting at a refinement point: FP64 true residual g = F lam - d,
+        // certification or restart from it (skipped on the device when none waits)
+        bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
+        bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr, pending);
+        bM(k, prec, b_Pg.p, b_z.p, pending);
+        bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
+        HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
+        HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, pending));
+      }
+    }
+    sweeps += chunk;
+    HC(hipMemcpyAsync(&sh, b_sc.p, sizeof(BScalars), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    if (sh.err) break;
+    bool running = false;
+    for (int j = 0; j < k; j++)
+      if (sh.done[j] == 0 || sh.done[j] == 1) running = true;
+    if (!running) break;
+    if (sweeps > max_iter + 2 * chunk) break;
+  }
+  HC(hipEventRecord(ev[2], st));
+  // final FP64 true residual of every column; its projection gives alpha
+  bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
+  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
+  bM(k, prec, b_Pg.p, b_z.p);
+  bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
+  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
+  // recovery: u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678)
+  HC(launch_kb_sub_bt(st, N, maxns, k, np, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p,
+                      b_lam.p, b_V.p));
+  b_kplus(k, true);
+  HC(launch_kb_recover(st, N, maxn, k, KP, np, ng, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_arow.p, b_la.p, b_hg.p, b_X.p,
+                       d_gdof.p, d_owner.p, b_U.p));
+  HC(hipEventRecord(ev[3], st));
+  if (u_h) HC(hipMemcpyAsync(u_h, b_U.p, (size_t)ng * k * sizeof(double), hipMemcpyDeviceToHost, st));
+  std::vector<double> lt;
+  if (lam_h) {
+    lt.resize((size_t)nl * k);
+    HC(hipMemcpyAsync(lt.data(), b_lam.p, lt.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HC(hipMemcpyAsync(&sh, b_sc.p, sizeof(BScalars), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+  if (lam_h)
+    for (int j = 0; j < k; j++)
+      for (int64_t r = 0; r < nl; r++) lam_h[(size_t)j * nl + r] = lt[(size_t)r * k + j];
+  for (int j = 0; j < k; j++) {
+    if (iters) iters[j] = sh.it[j];
+    int s = sh.done[j];
+    if (prec == 64 && s == 1) s = 2;  // FP64 recursive residual: no certification step
+    if (s == 0 || s == 1) s = 3;      // stopped by the sweep limit
+    if (status) status[j] = s;
+    if (res) res[j] = sh.res[j];
+  }
+  if (info) {
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    info->solve_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    info->init_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    info->pcpg_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    info->recover_ms = ms;
+    info->sweeps = sweeps;
+    int nc = 0;
+    for (int j = 0; j < k; j++) nc += sh.ncert[j];
+    info->cert_steps = nc;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (sh.err) throw std::runtime_error("block PCPG produced a non-finite residual");
+}
+
+}  // namespace hfls

@@ -131,4 +131,74 @@ hipError_t launch_symv_tiles(hipStream_t st, int nsub, int maxT, const int* ns, 
 hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const int* ns, const int64_t* boff,
                               const double* part, const int64_t* partoff, double* y, const int* done);
 
+// ---------------------------------------------------------------- tile GEMM (tgemm.hip)
+// Y_d = alpha op(A_d) X_d (+ Z_d); A_d: mode 0 symmetric lower 64x64 tiles, mode 1
+// general tiles, mode 2 compact column major (lda); X, Y, Z row major, ld kp.
+hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
+                            const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff,
+                            float* Y, const int64_t* yoff, const float* Z, int kp, float alpha, const int* gate);
+hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
+                            const double* A, const int64_t* aoff, const int* lda, const double* X,
+                            const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
+                            double alpha, const int* gate);
+hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
+                           const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
+hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
+                           const double* A, const int64_t* aoff, const int* lda, double* P, const int64_t* poff);
+
+// ---------------------------------------------------------------- block PCPG (block.hip)
+constexpr int KMAX = 64;  // load cases per block
+struct BScalars {         // per-column PCPG scalars of a block solve
+  double rho[KMAX], gamma[KMAX], gPz[KMAX], gPz_prev[KMAX], wFw[KMAX], norm0[KMAX], res[KMAX], ref[KMAX];
+  int it[KMAX], done[KMAX], repl[KMAX], ncert[KMAX];
+  double eps, rtol;        // stopping tolerance; refinement factor (0: none)
+  int max_iter, err;
+  int cert;                // 1: stopping tests are confirmed on the FP64 true residual
+  int pending;             // some column waits for an FP64 true-residual evaluation
+};
+
+hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
+                            const int64_t* obrow, const int* gptr, const int* gidx, const double* v,
+                            const double* wgt, double* xb, float* xb32, const int* gate = nullptr);
+hipError_t launch_kb_widen(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
+                           const int64_t* obrow, const float* y32, double* yb, const int* gate = nullptr);
+hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b);
+hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
+                         const double* R, const int64_t* roff, const double* V, double* a);
+hipError_t launch_kb_cluster_gc(hipStream_t st, int ncl, int maxlen, int k, int kp, const int* gc_src,
+                                const int64_t* gc_off, const int* nlcl, const int* nrcl, const int* d0,
+                                const double* cr, const int64_t* lcpos, const double* a, const int64_t* abase,
+                                double* gc, const int64_t* gcrow, const int* gate = nullptr);
+hipError_t launch_kb_corr_apply(hipStream_t st, int nsub, int maxns, int k, int kp, const int* ns,
+                                const int64_t* obrow, const double* CRQ, const int64_t* coff, const int* ldc,
+                                const int* nlc, const int64_t* lcoff, const int64_t* lcrow, const int64_t* arow,
+                                const double* la, int with_R, int defl, const double* cr, const int64_t* prow,
+                                double* y, const int* gate = nullptr);
+hipError_t launch_kb_cols(hipStream_t st, int nsub, int maxns, int dir, int ncol, int kp, const int* ns,
+                          const int64_t* obrow, double* A, const int64_t* aoff, const int* lda, double* X);
+hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
+                                 const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
+                                 const int* nlc, float* P32);
+hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
+                             const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
+                             const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr);
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
+                        const int* rpb, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
+                        const double* x, double* out, double sgn, const double* dotv, double* part, const int* gate = nullptr);
+hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
+                                 const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate = nullptr);
+hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode);
+hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
+                            const double* e, const BScalars* sc, const int* gate = nullptr);
+hipError_t launch_kb_permute(hipStream_t st, int nsub, int maxn, int dir, int k, int kp, int64_t npacked,
+                             const int* n, const int* nr, const int* perm, const int64_t* noff, double* V, double* Vr,
+                             const int64_t* orrow, double* Vb, const int64_t* obrow);
+hipError_t launch_kb_sub_bt(hipStream_t st, int nsub, int maxns, int k, int64_t npacked, const int* nr, const int* ns,
+                            const int* perm, const int64_t* noff, const int64_t* boff, const int* gptr,
+                            const int* gidx, const double* lam, double* V);
+hipError_t launch_kb_recover(hipStream_t st, int nsub, int maxn, int k, int kp, int64_t npacked, int64_t nglob,
+                             const int* n, const int64_t* noff, const double* R, const int64_t* roff,
+                             const int64_t* arow, const double* la, const double* h, const double* X, const int* gdof,
+                             const unsigned char* owner, double* U);
+
 }  // namespace hfls

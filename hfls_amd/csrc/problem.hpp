@@ -72,4 +72,8 @@ struct Problem {
 uint64_t splitmix64(uint64_t& state);
 double uniform01(uint64_t& state);
 
+// k seeded random smoothed tractions on the x = 1 face as packed local loads
+// (column j at j * sum_d n_d) for load cases case0 .. case0 + k - 1
+void traction_loads(const Problem& P, uint64_t seed, int case0, int k, double* f);
+
 }  // namespace hfls

@@ -82,6 +82,38 @@ int hfls_kplus(hfls_mesh m, const double* f_local, double* x_local);
 int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f_local, double* u, double* lambda,
                hfls_stats* stats);
 
+/* ---- blocks of load cases (no single counterpart in the reference: its
+ *      Cluster::pcpg (Cluster.cpp:1363-1458) solves one load case; a block runs
+ *      that iteration per column, each with its own scalars and stopping test).
+ * Column-major blocks: load case j of f_local at j*packed_local_length, of u at
+ * j*3*nPoints, of lambda at j*nlam; 1 <= k <= 64.
+ * precision 64: local operators applied in FP64 (same method as hfls_solve);
+ * precision 32: F_i and S_i applied in FP32 on the matrix cores with FP32
+ * accumulation, everything else in FP64 (factorisation, kernel, coarse problem,
+ * dual vectors, dot products, projector, initialisation, recovery), and every
+ * column's stopping test is certified on its FP64 true residual F lambda - d
+ * (a column that fails continues from the true residual). */
+typedef struct {
+  int iterations;    /* F applications of this column's PCPG loop */
+  int status;        /* 2 converged (certified), 3 iteration limit */
+  double residual;   /* FP64 true residual sqrt(g'PMPg)/sqrt(g0'PMPg0) at exit */
+} hfls_case_stats;
+typedef struct {
+  double solve_ms, init_ms, pcpg_ms, recover_ms; /* device times of the block */
+  int sweeps;        /* block iterations executed */
+  int cert_steps;    /* FP64 true-residual evaluations inside the loop */
+} hfls_block_stats;
+int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, int precision, const double* f_local, double* u,
+                     double* lambda, hfls_case_stats* cases, hfls_block_stats* stats);
+/* block operators on k dual vectors (column major, ld nlam): op 0 mult_Ff
+ * (Cluster.cpp:1114-1188), 1 Preconditioning (1043-1112), 2 Projection
+ * (1314-1329); precision 32 or 64 selects the F_i / S_i arithmetic */
+int hfls_apply_block(hfls_mesh m, int op, int k, int precision, const double* x, double* y);
+/* k seeded random smoothed tractions on the x = 1 face (load cases case0..case0+k-1)
+ * as packed local loads, column major; the definition is in problem.cpp */
+int hfls_problem_traction(hfls_problem p, uint64_t seed, int case0, int k, double* f_local);
+int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f_local);
+
 /* Measurement hook (no counterpart in the reference): average device time of
  * the dominant kernel, the interface gather + dense GEMV that streams K+_bb
  * (which=0, inside mult_Ff) or S_i (which=1, inside Preconditioning), over reps

@@ -1,0 +1,118 @@
+"""GPU parity of the block (multi-load-case) PCPG through the C ABI.
+
+Each column of a block is one load case run through the reference iteration
+(Cluster.cpp:1363-1458) with its own scalars.  Tolerances:
+  * F applied to a block, FP64: 1e-9 relative to the oracle per column;
+    FP32 operator (precision 32): 2e-5 relative (FP32 storage and accumulation of
+    K+_bb; the cluster correction stays FP64)
+  * block FP64 solve: iterations within +-2 of the oracle, displacement within 1e-8
+    relative of the oracle's (same tolerance as the single-case path)
+  * block mixed-precision solve at eps 1e-8: every case converged with its FP64
+    true residual below eps, displacement within 1e-7 relative of the FP64 oracle
+    and error vs the FP64 global direct solve printed per case
+  * a case gives bit-identical results alone (k = 1) and inside a block
+"""
+import numpy as np
+import pytest
+
+import hfls_amd
+from oracle.mesh import opts
+from oracle.feti import HTFETI
+from oracle.direct import global_solve
+from oracle.loads import traction_loads
+
+pytestmark = pytest.mark.gpu
+
+CFG0 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform",
+            young_modulus="2.1e5", load="body", body_force="-1.0")
+MULTI = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, nx=3, ny=3, nz=4, bc="clamp_x0", material="lognormal",
+             young_modulus="2.1e5", seed="3", mat_sigma="0.5", load="body")
+CASES = {"cfg0": CFG0, "multi": MULTI}
+SEED = 20240611
+K = 5
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def pair(request):
+    kw = CASES[request.param]
+    print("case", request.param, kw, flush=True)
+    gpu = hfls_amd.Mesh(**kw)
+    ora = HTFETI(opts(**kw), kernel_mode="geometric")
+    F = gpu.traction(SEED, 0, K)
+    F[K - 1] = gpu.load_vector()  # last column: the body force of the options
+    yield request.param, gpu, ora, F
+    gpu.close()
+
+
+def split(ora, fpacked):
+    out, off = [], 0
+    for l2g in ora.subs.l2g:
+        out.append(fpacked[off:off + len(l2g)])
+        off += len(l2g)
+    return out
+
+
+def test_traction_matches_oracle(pair):
+    name, gpu, ora, F = pair
+    Fo = traction_loads(ora.mesh, ora.subs, SEED, range(K - 1))
+    for j in range(K - 1):
+        assert np.array_equal(F[j], Fo[j])
+
+
+def test_block_F_fp64_and_fp32(pair):
+    name, gpu, ora, F = pair
+    rng = np.random.default_rng(7)
+    W = rng.standard_normal((4, ora.nlam))
+    F64 = gpu.mult_Ff_block(W, precision=64)
+    F32 = gpu.mult_Ff_block(W, precision=32)
+    for j in range(4):
+        Fo = ora.mult_Ff(W[j])
+        e64 = np.linalg.norm(F64[j] - Fo) / np.linalg.norm(Fo)
+        e32 = np.linalg.norm(F32[j] - Fo) / np.linalg.norm(Fo)
+        print(name, "F block column", j, "fp64 %.2e fp32 %.2e" % (e64, e32), flush=True)
+        assert e64 <= 1e-9
+        assert e32 <= 2e-5
+
+
+def test_block_fp64_matches_oracle(pair):
+    name, gpu, ora, F = pair
+    r = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=64)
+    assert np.all(r["status"] == 2)
+    for j in range(K):
+        ro = ora.pcpg(f=split(ora, F[j]), eps=1e-8, max_iter=300)
+        print(name, "fp64 case", j, "iterations", r["iterations"][j], "oracle", ro["iterations"], flush=True)
+        assert abs(int(r["iterations"][j]) - ro["iterations"]) <= 2
+        assert np.linalg.norm(r["u"][j] - ro["u"]) <= 1e-8 * np.linalg.norm(ro["u"])
+
+
+def test_block_mixed_precision(pair):
+    name, gpu, ora, F = pair
+    eps = 1e-8
+    r = gpu.pcpg_block(F, eps=eps, max_iter=300, precision=32)
+    print(name, "mixed: sweeps", r["sweeps"], "certifications", r["cert_steps"], "iterations", r["iterations"],
+          "residuals", r["residual"], flush=True)
+    assert np.all(r["status"] == 2)
+    assert np.all(r["residual"] < eps)
+    for j in range(K):
+        ro = ora.pcpg(f=split(ora, F[j]), eps=eps, max_iter=300)
+        err_o = np.linalg.norm(r["u"][j] - ro["u"]) / np.linalg.norm(ro["u"])
+        u_ref, _, _ = global_solve(ora.mesh, split(ora, F[j]), ora.subs)
+        err_d = np.linalg.norm(r["u"][j] - u_ref) / np.linalg.norm(u_ref)
+        print(name, "mixed case", j, "iterations", r["iterations"][j], "oracle", ro["iterations"],
+              "err vs oracle %.2e vs direct %.2e" % (err_o, err_d), flush=True)
+        assert err_o <= 1e-7
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_same_alone_and_in_block(pair, precision):
+    name, gpu, ora, F = pair
+    blk = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=precision, want_lambda=True)
+    for j in (0, K - 1):
+        one = gpu.pcpg_block(F[j:j + 1], eps=1e-8, max_iter=300, precision=precision, want_lambda=True)
+        assert one["iterations"][0] == blk["iterations"][j]
+        assert np.array_equal(one["u"][0], blk["u"][j])
+        assert np.array_equal(one["lambda"][0], blk["lambda"][j])
+    # reordered block: case 0 moved to the last column
+    perm = list(range(1, K)) + [0]
+    rb = gpu.pcpg_block(F[perm], eps=1e-8, max_iter=300, precision=precision)
+    assert np.array_equal(rb["u"][K - 1], blk["u"][0])

@@ -85,3 +85,18 @@ def test_config1_integer_hashes():
     assert min(nlam_i) == 1191 and max(nlam_i) == 2598
     assert abs(np.mean(nlam_i) - 2254.8) < 0.1
     assert max(ns) == 1806
+
+
+@pytest.mark.parametrize("kw", [CASES[1], CASES[3], CASES[4]])
+def test_traction_loads_bit_exact(kw):
+    """The product's seeded face tractions equal the oracle's bit for bit (uneven
+    slabs and several clusters included)."""
+    from oracle.loads import traction_loads
+    o = oopts(**kw)
+    m = OMesh(o)
+    s = Subdomains(m)
+    p = hfls_amd.Problem(**kw)
+    F = p.traction(987654321, 2, 3)
+    Fo = traction_loads(m, s, 987654321, [2, 3, 4])
+    for j in range(3):
+        assert np.array_equal(F[j], Fo[j])
