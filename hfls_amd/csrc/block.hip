@@ -148,52 +148,38 @@ hipError_t launch_kb_cluster_gc(hipStream_t st, int ncl, int maxlen, int k, int 
   return hipGetLastError();
 }
 
-// y[(obrow[d] + p)*kp + j] += with_R * R_b[p,:] la[arow_d.., j] - C[p,:] la[lcrow.., j]
-//                              (+ Q[p,:] (Yh^T x)[:, j] + Yh[p,:] (Q^T x)[:, j] if defl)
-// C, R_b, Q, Yh are the columns 0.., nlc.., nlc+6.., nlc+12.. of the compact
-// n_s x (nlc + 18) block CRQ (ld ldc); the products CRQ^T x sit in rows
-// prow[d] + column of cr.
-__global__ __launch_bounds__(256) void kb_corr_apply(int k, int kp, const int* __restrict__ ns,
-                                                     const int64_t* __restrict__ obrow, const double* __restrict__ CRQ,
-                                                     const int64_t* __restrict__ coff, const int* __restrict__ ldc,
-                                                     const int* __restrict__ nlc, const int64_t* __restrict__ lcoff,
-                                                     const int64_t* __restrict__ lcrow,
-                                                     const int64_t* __restrict__ arow, const double* __restrict__ la,
-                                                     int with_R, int defl, const double* __restrict__ cr,
-                                                     const int64_t* __restrict__ prow, double* __restrict__ y, const int* __restrict__ gate) {
+// Coefficient block of the correction y_b += CRQ coef (a tile GEMM with the
+// compact n_s x (nlc + 18) block CRQ = [C | R_b | Q | Yh], ld ldc):
+//   coef[(crow[d] + m)*kp + j] = -la[lcrow_m][j]            m < nlc  (-C lam_c)
+//                              = with_R * la[arow_d + m'][j]         (R_b alpha)
+//                              = defl * (Yh^T x)[m'][j], defl * (Q^T x)[m'][j]
+// where (Q^T x, Yh^T x) are rows crow[d] + nlc + 6.. and + 12.. of cr
+__global__ __launch_bounds__(256) void kb_coef(int k, int kp, const int* __restrict__ nlc,
+                                               const int64_t* __restrict__ lcoff, const int64_t* __restrict__ lcrow,
+                                               const int64_t* __restrict__ arow, const double* __restrict__ la,
+                                               int with_R, int defl, const double* __restrict__ cr,
+                                               const int64_t* __restrict__ crow, double* __restrict__ coef,
+                                               const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int d = blockIdx.y;
-  const int n = ns[d];
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int p = (int)(t / k), j = (int)(t % k);
-  if (p >= n) return;
-  const double* Cd = CRQ + coff[d] + p;
-  const int64_t l = ldc[d];
   const int nc = nlc[d];
-  double s = 0.0;
-  if (with_R) {
-    const int64_t ar = arow[d];
-    for (int m = 0; m < 6; m++) s += Cd[(int64_t)(nc + m) * l] * la[(ar + m) * kp + j];
-  }
-  const int64_t* lr = lcrow + lcoff[d];
-  for (int m = 0; m < nc; m++) s -= Cd[(int64_t)m * l] * la[lr[m] * kp + j];
-  if (defl) {
-    const int64_t pr = prow[d] + nc + 6;
-    for (int m = 0; m < 6; m++)
-      s += Cd[(int64_t)(nc + 6 + m) * l] * cr[(pr + 6 + m) * kp + j] +
-           Cd[(int64_t)(nc + 12 + m) * l] * cr[(pr + m) * kp + j];
-  }
-  y[(obrow[d] + p) * kp + j] += s;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int m = (int)(t / k), j = (int)(t % k);
+  if (m >= nc + 18) return;
+  double v;
+  if (m < nc) v = -la[lcrow[lcoff[d] + m] * kp + j];
+  else if (m < nc + 6) v = with_R ? la[(arow[d] + (m - nc)) * kp + j] : 0.0;
+  else if (m < nc + 12) v = defl ? cr[(crow[d] + m + 6) * kp + j] : 0.0;
+  else v = defl ? cr[(crow[d] + m - 6) * kp + j] : 0.0;
+  coef[(crow[d] + m) * kp + j] = v;
 }
 
-hipError_t launch_kb_corr_apply(hipStream_t st, int nsub, int maxns, int k, int kp, const int* ns,
-                                const int64_t* obrow, const double* CRQ, const int64_t* coff, const int* ldc,
-                                const int* nlc, const int64_t* lcoff, const int64_t* lcrow, const int64_t* arow,
-                                const double* la, int with_R, int defl, const double* cr, const int64_t* prow,
-                                double* y, const int* gate) {
-  const int64_t n = (int64_t)maxns * k;
-  hipLaunchKernelGGL(kb_corr_apply, dim3((unsigned)((n + 255) / 256), nsub), dim3(256), 0, st, k, kp, ns, obrow, CRQ,
-                     coff, ldc, nlc, lcoff, lcrow, arow, la, with_R, defl, cr, prow, y, gate);
+hipError_t launch_kb_coef(hipStream_t st, int nsub, int maxrows, int k, int kp, const int* nlc, const int64_t* lcoff,
+                          const int64_t* lcrow, const int64_t* arow, const double* la, int with_R, int defl,
+                          const double* cr, const int64_t* crow, double* coef, const int* gate) {
+  const int64_t n = (int64_t)maxrows * k;
+  hipLaunchKernelGGL(kb_coef, dim3((unsigned)((n + 255) / 256), nsub), dim3(256), 0, st, k, kp, nlc, lcoff, lcrow, arow,
+                     la, with_R, defl, cr, crow, coef, gate);
   return hipGetLastError();
 }
 
@@ -408,19 +394,36 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
 //   mode 2: initial gPz, norm0
 //   mode 3: (only while pending) for status-1 columns, from the FP64 true residual:
 //           converged -> 2; limit -> 3; otherwise the iteration restarts from the
-//           true residual (repl = 1: g <- g_true, w <- P M P g_true); pending stays
-//           set for the gated launches that follow and is renewed by mode 1
+//           true residual (repl = 1: g <- g_true, w <- P M P g_true); pending is
+//           cleared (the restart update that follows is gated by active)
 //   mode 4: record the true residual of every column (res)
-__global__ __launch_bounds__(64) void kb_reduce(int nparts, int k, const double* __restrict__ part, BScalars* sc,
-                                                int mode) {
+// column sums of the partials: one workgroup per column, strided partial sums
+// then a fixed-shape tree (the order depends on nparts only, not on other columns)
+__global__ __launch_bounds__(256) void kb_colsum(int nparts, int k, const double* __restrict__ part, BScalars* sc,
+                                                 const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int j = blockIdx.x;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) s += part[(int64_t)b * k + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc->colsum[j] = red[0];
+}
+
+__global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate) {
   const int j = threadIdx.x;
-  __shared__ int any;
-  if (mode == 3 && sc->pending == 0) return;  // uniform over the workgroup
-  if (j == 0) any = 0;
+  __shared__ int any, run;
+  if (gate && *gate == 0) return;              // uniform over the workgroup
+  if (mode == 3 && sc->pending == 0) return;
+  if (j == 0) any = run = 0;
   __syncthreads();
   if (j < k) {
-    double s = 0.0;
-    for (int b = 0; b < nparts; b++) s += part[(int64_t)b * k + j];
+    const double s = sc->colsum[j];
     const int stt = sc->done[j];
     if (mode == 0) {
       if (!stt) {
@@ -473,17 +476,22 @@ __global__ __launch_bounds__(64) void kb_reduce(int nparts, int k, const double*
     } else {
       if (stt != 4) sc->res[j] = sc->norm0[j] > 0 ? sqrt(fabs(s)) / sc->norm0[j] : 0.0;
     }
+    if (sc->done[j] == 0 || sc->done[j] == 1) run = 1;
   }
   __syncthreads();
   if (j == 0) {
     if (mode == 1) sc->pending = any;
-    if (mode == 2) sc->pending = 0;
+    if (mode == 2 || mode == 3) sc->pending = 0;
+    if (mode >= 1 && mode <= 3) sc->active = run;
   }
 }
 
-hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode) {
+hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
+                            const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, nparts, k, part, sc, mode);
+  const int* g2 = mode == 3 ? &sc->pending : gate;
+  hipLaunchKernelGGL(kb_colsum, dim3(k), dim3(256), 0, st, nparts, k, part, sc, g2);
+  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, k, sc, mode, gate);
   return hipGetLastError();
 }
 

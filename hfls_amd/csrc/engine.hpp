@@ -86,6 +86,10 @@ class Engine {
   // (which=1)) with HIP events on the library stream; returns ms per launch and
   // the algorithmic bytes one launch moves.
   double time_stream(int which, int reps, double* bytes);
+  // Time the FP32 tile GEMM of the block solve over 64 columns (which = 0: the
+  // deflated K+_bb tiles, 1: S_i) with HIP events on the library stream; returns ms
+  // per launch and the algorithmic flops of one launch (2 * 64 * sum n_s^2).
+  double time_block_gemm(int which, int reps, double* flops);
 
   // Block of k load cases (column major: column j of f_local at j*packed_n(), of u
   // at j*3*nPoints, of lam at j*nlam).  precision 64: F_i and S_i applied in FP64;
@@ -155,7 +159,7 @@ class Engine {
   DVec<int> b_crn;
   DVec<double> b_CR;
   DVec<float> b_Kt32, b_St32, b_xb32, b_y32;
-  DVec<double> b_xb, b_yb, b_cr, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
+  DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_V, b_X, b_U;
   DVec<BScalars> b_sc;
   int b_nparts = 0;
@@ -164,6 +168,7 @@ class Engine {
           const int* gate = nullptr);
   void bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate = nullptr);
   void bM(int k, int prec, const double* x, double* out, const int* gate = nullptr);
+  void b_corr(int k, int with_R, int defl, const int* gate);
   void b_kplus(int k, bool interior_too);  // from b_V: yb = (K_c^+ v)_b-ish pieces; see engine_block.cpp
 
   void build_meta();

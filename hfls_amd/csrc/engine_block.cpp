@@ -185,6 +185,7 @@ void Engine::block_prepare() {
   zero_alloc(b_xb32, NBP * KP, st);
   zero_alloc(b_y32, NBP * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
+  zero_alloc(b_coef, CRP * KP, st);
   zero_alloc(b_v6, (int64_t)64 * N * KP, st);
   zero_alloc(b_gc, GCP * KP, st);
   zero_alloc(b_la, GCP * KP, st);
@@ -207,7 +208,7 @@ size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n + b_y32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_xb, &b_yb, &b_cr, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+  for (const DVec<double>* v : {&b_CR, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -237,10 +238,19 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
                           b_lcpos.p, b_cr.p, b_abase_cr.p, b_gc.p, b_gcrow.p, gate));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
                       b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate));
-  HC(launch_kb_corr_apply(st, N, maxns, k, KP, d_ns.p, b_obrow.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, d_off_lc.p,
-                          b_lcrow.p, b_arow.p, b_la.p, 1, prec == 32 ? 1 : 0, b_cr.p, b_crrow.p, b_yb.p, gate));
+  b_corr(k, 1, prec == 32 ? 1 : 0, gate);
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
                        dotw, b_part.p, gate));
+}
+
+// y_b += R_b alpha (with_R) - C lam_c (+ Q (Yh^T x) + Yh (Q^T x) if defl) as one
+// FP64 tile GEMM of CRQ with the coefficient block gathered from la and cr
+void Engine::b_corr(int k, int with_R, int defl, const int* gate) {
+  const int N = P->nsub;
+  HC(launch_kb_coef(st, N, maxnlc + 18, k, KP, d_nlc.p, d_off_lc.p, b_lcrow.p, b_arow.p, b_la.p, with_R, defl, b_cr.p,
+                    b_crrow.p, b_coef.p, gate));
+  HC(launch_tgemm_f64(st, N, maxTb, 2, 0, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_coef.p, b_croff.p, b_yb.p,
+                      b_xoff.p, b_yb.p, KP, 1.0, gate));
 }
 
 // out = P x = x - G (G^T G)^-1 G^T x (Cluster.cpp:1314-1329); hbuf = H alpha per subdomain
@@ -293,8 +303,7 @@ void Engine::b_kplus(int k, bool interior_too) {
                           b_lcpos.p, b_v6.p, b_v6row.p, b_gc.p, b_gcrow.p));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
                       b_la.p, b_gcoff.p, nullptr, KP, 1.0, nullptr));
-  HC(launch_kb_corr_apply(st, N, maxns, k, KP, d_ns.p, b_obrow.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, d_off_lc.p,
-                          b_lcrow.p, b_arow.p, b_la.p, interior_too ? 0 : 1, 0, b_cr.p, b_crrow.p, b_yb.p));
+  b_corr(k, interior_too ? 0 : 1, 0, nullptr);
   if (!interior_too) return;
   // x_r = K_rr^-1 v_r - (K_rr^-1 K_rb) y_b
   if (maxTr > 0) {
@@ -324,6 +333,34 @@ void Engine::apply_block(int op, int k, int prec, const double* x_h, double* y_h
   HC(hipStreamSynchronize(st));
   for (int j = 0; j < k; j++)
     for (int64_t r = 0; r < nl; r++) y_h[(size_t)j * nl + r] = t[(size_t)r * k + j];
+}
+
+double Engine::time_block_gemm(int which, int reps, double* flops) {
+  block_prepare();
+  const int N = P->nsub;
+  const float* A = which == 0 ? b_Kt32.p : b_St32.p;
+  auto launch = [&] {
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+                        b_xoff.p, nullptr, KP, 1.0f, nullptr));
+  };
+  launch();
+  hipEvent_t a, b;
+  HC(hipEventCreate(&a));
+  HC(hipEventCreate(&b));
+  HC(hipEventRecord(a, st));
+  for (int i = 0; i < reps; i++) launch();
+  HC(hipEventRecord(b, st));
+  HC(hipEventSynchronize(b));
+  float ms = 0;
+  HC(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (flops) {
+    double f = 0;
+    for (int d = 0; d < N; d++) f += 2.0 * KP * (double)h_ns[d] * h_ns[d];
+    *flops = f;
+  }
+  return ms / reps;
 }
 
 void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,
@@ -368,18 +405,21 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
   HC(hipEventRecord(ev[1], st));
   const int chunk = 8;
   const int* pending = &b_sc.p->pending;
+  const int* active = &b_sc.p->active;
   int sweeps = 0;
   BScalars sh;
   for (;;) {
     for (int c = 0; c < chunk; c++) {
-      bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0));
-      HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p));
-      bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr);
-      bM(k, prec, b_Pg.p, b_z.p);
-      bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1));
-      HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p));
+      // one PCPG iteration for every running column; the launches are skipped on
+      // the device once no column runs (active == 0)
+      bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p, active);
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0, active));
+      HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p, active));
+      bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr, active);
+      bM(k, prec, b_Pg.p, b_z.p, active);
+      bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p, active);
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1, active));
+      HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p, active));
       if (prec == 32) {
         // columns waiting at a refinement point: FP64 true residual g = F lam - d,
         // certification or restart from it (skipped on the device when none waits)
@@ -388,7 +428,7 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
         bM(k, prec, b_Pg.p, b_z.p, pending);
         bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
         HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
-        HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, pending));
+        HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, active));
       }
     }
     sweeps += chunk;

@@ -155,6 +155,8 @@ struct BScalars {         // per-column PCPG scalars of a block solve
   int max_iter, err;
   int cert;                // 1: stopping tests are confirmed on the FP64 true residual
   int pending;             // some column waits for an FP64 true-residual evaluation
+  int active;              // some column is still iterating
+  double colsum[KMAX];     // column sums of the last reduction
 };
 
 hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
@@ -169,11 +171,9 @@ hipError_t launch_kb_cluster_gc(hipStream_t st, int ncl, int maxlen, int k, int 
                                 const int64_t* gc_off, const int* nlcl, const int* nrcl, const int* d0,
                                 const double* cr, const int64_t* lcpos, const double* a, const int64_t* abase,
                                 double* gc, const int64_t* gcrow, const int* gate = nullptr);
-hipError_t launch_kb_corr_apply(hipStream_t st, int nsub, int maxns, int k, int kp, const int* ns,
-                                const int64_t* obrow, const double* CRQ, const int64_t* coff, const int* ldc,
-                                const int* nlc, const int64_t* lcoff, const int64_t* lcrow, const int64_t* arow,
-                                const double* la, int with_R, int defl, const double* cr, const int64_t* prow,
-                                double* y, const int* gate = nullptr);
+hipError_t launch_kb_coef(hipStream_t st, int nsub, int maxrows, int k, int kp, const int* nlc, const int64_t* lcoff,
+                          const int64_t* lcrow, const int64_t* arow, const double* la, int with_R, int defl,
+                          const double* cr, const int64_t* crow, double* coef, const int* gate = nullptr);
 hipError_t launch_kb_cols(hipStream_t st, int nsub, int maxns, int dir, int ncol, int kp, const int* ns,
                           const int64_t* obrow, double* A, const int64_t* aoff, const int* lda, double* X);
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
@@ -187,7 +187,8 @@ hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, con
                         const double* x, double* out, double sgn, const double* dotv, double* part, const int* gate = nullptr);
 hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
                                  const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate = nullptr);
-hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode);
+hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
+                            const int* gate = nullptr);
 hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
                             const double* e, const BScalars* sc, const int* gate = nullptr);
 hipError_t launch_kb_permute(hipStream_t st, int nsub, int maxn, int dir, int k, int kp, int64_t npacked,

@@ -115,10 +115,11 @@ int hfls_problem_traction(hfls_problem p, uint64_t seed, int case0, int k, doubl
 int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f_local);
 
 /* Measurement hook (no counterpart in the reference): average device time of
- * the dominant kernel, the interface gather + dense GEMV that streams K+_bb
- * (which=0, inside mult_Ff) or S_i (which=1, inside Preconditioning), over reps
- * back-to-back launches timed with HIP events on the library's stream; bytes =
- * algorithmic bytes of one launch. */
+ * the dominant kernel over reps back-to-back launches timed with HIP events on
+ * the library's stream.  which = 0 / 1: the packed-tile GEMV streaming K+_bb
+ * (inside mult_Ff) / S_i (inside Preconditioning), *bytes = algorithmic bytes of
+ * one launch; which = 2 / 3: the FP32 tile GEMM of the block solve over 64
+ * columns with K+_bb / S_i, *bytes = algorithmic flops of one launch. */
 int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
 
 #ifdef __cplusplus
