@@ -48,6 +48,11 @@ SAMPLE = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="un
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFS = 157.3
 SEED = 20240611
+# HBM bytes per launch of k_tgemm_f32 on the default workload, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
+# the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
+# 8.041e9 fetched + 2.222e8 written.  Valid for the default workload only.
+GEMM_TRAFFIC_BYTES = 8.263e9
 
 
 def cpu_sample():
@@ -106,6 +111,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", action="store_true")
     ap.add_argument("--small", action="store_true", help="2x2x2 x 5^3 smoke-sized workload")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="setup, then only the dominant kernel (FP32 tile GEMM, 4 launches) and a device copy of "
+                         "known size (4 launches; calibration of HBM traffic counters)")
     a = ap.parse_args()
     if a.cpu_sample:
         cpu_sample()
@@ -125,6 +133,12 @@ def main():
     setup_s = time.perf_counter() - t0
     log("setup %.2f s (factor %.2f s), nlam %d, device bytes %.2f GB" %
         (setup_s, mesh.factor_ms / 1e3, mesh.nlam, mesh.device_bytes / 1e9))
+    if a.kernel_only:
+        g_ms, g_flops = mesh.time_stream(2, reps=3)
+        c_ms, c_bytes = mesh.time_stream(4, reps=3)
+        print(json.dumps({"kernel_ms": g_ms, "flops": g_flops, "copy_ms": c_ms, "copy_bytes_each_way": c_bytes}))
+        mesh.close()
+        return 0
     nblk = a.warmup + a.steps
     F = mesh.traction(SEED, 0, a.k * max(1, nblk))
     log("loads: %d seeded face tractions" % F.shape[0])
@@ -199,7 +213,8 @@ def main():
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),
         "block_vs_plain_displacement_rel_diff": agree,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK_TFS, "traffic": None,
+                     "frac": achieved / FP32_MFMA_PEAK_TFS,
+                     "traffic": GEMM_TRAFFIC_BYTES if (a.elements == 10 and not a.small) else None,
                      "kernel": "k_tgemm_f32 (deflated K+_bb, packed symmetric 64x64 tiles, 64 columns)",
                      "kernel_ms": g_ms, "flops_per_launch": g_flops,
                      "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12},

@@ -74,6 +74,21 @@ hipError_t launch_kb_widen(hipStream_t st, int64_t nb, int k, int kp, const int*
   return hipGetLastError();
 }
 
+// plain copy of n doubles (16-byte accesses), the reference point for HBM traffic
+// counters: it reads and writes exactly 8 n bytes each
+__global__ __launch_bounds__(256) void kb_copy(int64_t n2, const double2* __restrict__ a, double2* __restrict__ b) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n2) b[t] = a[t];
+}
+
+hipError_t launch_kb_copy(hipStream_t st, int64_t n, const double* a, double* b) {
+  if (n % 2) return hipErrorInvalidValue;
+  const int64_t n2 = n / 2;
+  hipLaunchKernelGGL(kb_copy, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, n2, (const double2*)a,
+                     (double2*)b);
+  return hipGetLastError();
+}
+
 // elementwise FP64 -> FP32 copy (tile stores of the same layout)
 __global__ __launch_bounds__(256) void kb_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;

@@ -1033,7 +1033,7 @@ void Engine::kplus_local(const double* f_h, double* x_h) {
 }
 
 double Engine::time_stream(int which, int reps, double* bytes) {
-  if (which == 2 || which == 3) return time_block_gemm(which - 2, reps, bytes);
+  if (which >= 2 && which <= 4) return time_block_gemm(which - 2, reps, bytes);
   const int N = P->nsub;
   const int64_t nl = P->nlam;
   std::vector<double> ones(nl, 1.0);

@@ -89,6 +89,7 @@ class Engine {
   // Time the FP32 tile GEMM of the block solve over 64 columns (which = 0: the
   // deflated K+_bb tiles, 1: S_i) with HIP events on the library stream; returns ms
   // per launch and the algorithmic flops of one launch (2 * 64 * sum n_s^2).
+  // which = 2: a plain device copy of known size (*flops = bytes read = written).
   double time_block_gemm(int which, int reps, double* flops);
 
   // Block of k load cases (column major: column j of f_local at j*packed_n(), of u

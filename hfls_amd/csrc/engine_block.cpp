@@ -339,9 +339,13 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
   block_prepare();
   const int N = P->nsub;
   const float* A = which == 0 ? b_Kt32.p : b_St32.p;
+  const int64_t ncopy = b_xb.n & ~(int64_t)1;
   auto launch = [&] {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
-                        b_xoff.p, nullptr, KP, 1.0f, nullptr));
+    if (which == 2)
+      HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
+    else
+      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+                          b_xoff.p, nullptr, KP, 1.0f, nullptr));
   };
   launch();
   hipEvent_t a, b;
@@ -358,7 +362,7 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
   if (flops) {
     double f = 0;
     for (int d = 0; d < N; d++) f += 2.0 * KP * (double)h_ns[d] * h_ns[d];
-    *flops = f;
+    *flops = which == 2 ? 8.0 * (double)ncopy : f;  // the copy: bytes read (= bytes written)
   }
   return ms / reps;
 }

@@ -165,6 +165,7 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
 hipError_t launch_kb_widen(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
                            const int64_t* obrow, const float* y32, double* yb, const int* gate = nullptr);
 hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b);
+hipError_t launch_kb_copy(hipStream_t st, int64_t n, const double* a, double* b);
 hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
                          const double* R, const int64_t* roff, const double* V, double* a);
 hipError_t launch_kb_cluster_gc(hipStream_t st, int ncl, int maxlen, int k, int kp, const int* gc_src,

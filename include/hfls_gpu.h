@@ -119,7 +119,8 @@ int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f_l
  * the library's stream.  which = 0 / 1: the packed-tile GEMV streaming K+_bb
  * (inside mult_Ff) / S_i (inside Preconditioning), *bytes = algorithmic bytes of
  * one launch; which = 2 / 3: the FP32 tile GEMM of the block solve over 64
- * columns with K+_bb / S_i, *bytes = algorithmic flops of one launch. */
+ * columns with K+_bb / S_i, *bytes = algorithmic flops of one launch; which = 4:
+ * a plain device copy (reference for traffic counters), *bytes = bytes read. */
 int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
 
 #ifdef __cplusplus
