@@ -99,6 +99,69 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+FAMILIES = {64: dict(Cx=2, Cy=2, Cz=2, Nx=2, Ny=2, Nz=2), 216: dict(Cx=3, Cy=3, Cz=3, Nx=2, Ny=2, Nz=2),
+            512: dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4), 1000: dict(Cx=5, Cy=5, Cz=5, Nx=2, Ny=2, Nz=2)}
+
+
+def serve_mesh(nsub, e, seed):
+    """A cube family member with the uneven-batching materials: lognormal E per
+    subdomain (sigma 1.5, clipped to a 1e3 contrast), x=0 clamped."""
+    o = dict(FAMILIES[nsub], nx=e, ny=e, nz=e, bc="clamp_x0", material="lognormal", young_modulus="2.1e5",
+             mat_sigma="1.5", mat_contrast="1e3", seed=str(seed), poissons_ratio="0.3", load="body", typeBc="ker",
+             preconditioner="Dirichlet_explicit")
+    return o
+
+
+def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb):
+    """Request stream with Poisson arrivals: each request draws its mesh uniformly
+    from families x edges and 1..64 load cases log-uniformly (seeded); meshes are
+    set up once (cache warmed before the stream starts).  Returns latency
+    percentiles and served load-case throughput."""
+    rng = np.random.default_rng(seed)
+    meshes = [serve_mesh(f, e, 100 + i) for i, (f, e) in enumerate((f, e) for f in families for e in edges)]
+    srv = hfls_amd.Server(slots=64, precision=32, eps=eps, max_iter=1000, chunk=8, hbm_budget_gb=budget_gb)
+    t0 = time.perf_counter()
+    for m in meshes:  # warm the cache: one single-case request per mesh
+        rid = srv.submit(m, k=1, seed=1, case0=0)
+        while srv.step():
+            pass
+        srv.result(rid)
+    warm_s = time.perf_counter() - t0
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    picks = rng.integers(0, len(meshes), nreq)
+    ks = np.minimum(64, np.floor(np.exp(rng.uniform(0.0, np.log(65.0), nreq)))).astype(int)
+    ids = []
+    start = time.perf_counter()
+    nxt = 0
+    busy = False
+    while nxt < nreq or busy:
+        now = time.perf_counter() - start
+        while nxt < nreq and arrivals[nxt] <= now:
+            ids.append(srv.submit(meshes[picks[nxt]], k=int(ks[nxt]), seed=1000 + nxt, case0=0))
+            nxt += 1
+        busy = srv.step()
+        if not busy and nxt < nreq:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - start)))
+    wall = time.perf_counter() - start
+    lat, cases, iters, worst = [], 0, [], 0.0
+    for rid in ids:
+        r = srv.result(rid)
+        assert np.all(r["status"] == 2), "a served load case did not converge"
+        lat.append(r["latency_ms"])
+        cases += len(r["iterations"])
+        iters.extend(r["iterations"].tolist())
+        worst = max(worst, float(r["residual"].max()))
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    return {"requests": nreq, "load_cases": cases, "arrival_rate_per_s": rate, "wall_s": wall,
+            "solves_per_s": cases / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "meshes": ["%dx%d^3" % (f, e) for f in families for e in edges], "cache_warm_s": warm_s,
+            "iterations_min_max": [int(min(iters)), int(max(iters))], "max_true_residual": worst,
+            "slot_joins": int(st["joins"]), "mesh_setups": int(st["setups"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +174,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", action="store_true")
     ap.add_argument("--small", action="store_true", help="2x2x2 x 5^3 smoke-sized workload")
+    ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving run (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-families", default="64,216", help="subdomain counts of the served cube families")
+    ap.add_argument("--serve-edges", default="6,8", help="elements per subdomain edge of the served meshes")
     ap.add_argument("--kernel-only", action="store_true",
                     help="setup, then only the dominant kernel (FP32 tile GEMM, 4 launches) and a device copy of "
                          "known size (4 launches; calibration of HBM traffic counters)")
@@ -220,6 +287,11 @@ def main():
                      "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12},
     }
     mesh.close()
+    if a.serve_requests > 0:
+        fam = [int(x) for x in a.serve_families.split(",")]
+        edg = [int(x) for x in a.serve_edges.split(",")]
+        out["serving"] = run_serving(hfls_amd, fam, edg, a.serve_requests, a.serve_rate, SEED, a.eps, 200.0)
+        log("serving:", json.dumps(out["serving"]))
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(plain_it[-1], out["config"]["subdomains"])
     print(json.dumps(out))

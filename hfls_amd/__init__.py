@@ -20,7 +20,9 @@ EXPORTS = [
     "hfls_problem_corners", "hfls_mesh_create", "hfls_mesh_destroy", "hfls_mesh_sizes",
     "hfls_mesh_setup_times", "hfls_mesh_kernel", "hfls_mesh_load", "hfls_apply_F", "hfls_project",
     "hfls_precondition", "hfls_kplus", "hfls_solve", "hfls_time_stream", "hfls_solve_block",
-    "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction",
+    "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction", "hfls_server_create",
+    "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
+    "hfls_server_result", "hfls_server_stats",
 ]
 
 DEFAULTS = {
@@ -282,3 +284,75 @@ class Mesh:
                 "converged": np.array([c.status == 2 for c in cs]), "solve_ms": bs.solve_ms, "init_ms": bs.init_ms,
                 "pcpg_ms": bs.pcpg_ms, "recover_ms": bs.recover_ms, "sweeps": bs.sweeps,
                 "cert_steps": bs.cert_steps}
+
+
+class Server:
+    """Request loop with continuous batching over device-cached meshes (one GPU).
+
+    submit() queues k load cases for a mesh given by its options (packed local
+    loads of shape (k, npacked), or seeded face tractions); step() runs one
+    scheduling round (admission by HBM footprint, free slots take queued cases,
+    one chunk of PCPG iterations per busy mesh, stopped cases recovered);
+    result() returns the displacements and per-case stats of a finished request.
+    """
+
+    def __init__(self, device=0, hbm_budget_gb=200.0, slots=64, precision=32, eps=1e-8, max_iter=1000, chunk=8):
+        lib = load()
+        self._h = ctypes.c_void_p()
+        _check(lib.hfls_server_create(int(device), ctypes.c_double(hbm_budget_gb), int(slots), int(precision),
+                                      ctypes.c_double(eps), int(max_iter), int(chunk), ctypes.byref(self._h)))
+        self._k = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hfls_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def sizes(self, **opts):
+        """(npacked, 3*nPoints, estimated device bytes) of the mesh given by opts"""
+        n, k, v = _opts(opts)
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(load().hfls_server_sizes(self._h, n, k, v, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def submit(self, opts, F=None, k=None, seed=0, case0=0):
+        n, kk, v = _opts(opts)
+        if F is not None:
+            F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+            k = F.shape[0]
+        k = int(k or 1)
+        rid = ctypes.c_int64()
+        _check(load().hfls_server_submit(self._h, n, kk, v, k, _ptr(F), ctypes.c_uint64(seed), int(case0),
+                                         ctypes.byref(rid)))
+        self._k[rid.value] = (k, self.sizes(**opts)[1])
+        return rid.value
+
+    def step(self):
+        busy = ctypes.c_int()
+        _check(load().hfls_server_step(self._h, ctypes.byref(busy)))
+        return bool(busy.value)
+
+    def state(self, rid):
+        s = ctypes.c_int()
+        _check(load().hfls_server_state(self._h, ctypes.c_int64(rid), ctypes.byref(s)))
+        return s.value
+
+    def result(self, rid):
+        k, ng = self._k.pop(rid)
+        u = np.zeros((k, ng))
+        cs = (CaseStats * k)()
+        lat = np.zeros(2)
+        _check(load().hfls_server_result(self._h, ctypes.c_int64(rid), _ptr(u), cs, _ptr(lat)))
+        return {"u": u, "iterations": np.array([c.iterations for c in cs]),
+                "status": np.array([c.status for c in cs]), "residual": np.array([c.residual for c in cs]),
+                "latency_ms": float(lat[0]), "queue_ms": float(lat[1])}
+
+    def stats(self):
+        out = np.zeros(8)
+        _check(load().hfls_server_stats(self._h, _ptr(out)))
+        keys = ["meshes", "bytes", "setups", "evictions", "cases_done", "requests_done", "joins", "setup_ms"]
+        return dict(zip(keys, out.tolist()))
+

@@ -29,6 +29,7 @@ SOURCES = [
     ("problem.cpp", HOST_FLAGS + ["-ffp-contract=off"]),
     ("engine.cpp", HOST_FLAGS),
     ("engine_block.cpp", HOST_FLAGS),
+    ("serve.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
 ]
 

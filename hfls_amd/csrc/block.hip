@@ -400,13 +400,14 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
 
 // Per-column reductions of the partial dots (fixed order) and the PCPG scalar
 // logic (Cluster.cpp:1398-1448).  Column status: 0 running, 1 waiting for an FP64
-// true-residual evaluation, 2 converged, 3 iteration limit, 4 empty slot.
+// true-residual evaluation, 2 converged, 3 iteration limit, 4 empty slot,
+// 5 joining (initialised by mode 2).
 //   mode 0: wFw, rho = -gPz / wFw
 //   mode 1: gamma, iteration count; the stopping test sqrt(gPz) < eps*norm0 gives
 //           status 2 (status 1 when certification is on), a recursive residual
 //           below rtol times the last true one gives status 1 (refinement point),
 //           the iteration limit status 3; pending = some column has status 1
-//   mode 2: initial gPz, norm0
+//   mode 2: initial gPz, norm0 of the joining columns
 //   mode 3: (only while pending) for status-1 columns, from the FP64 true residual:
 //           converged -> 2; limit -> 3; otherwise the iteration restarts from the
 //           true residual (repl = 1: g <- g_true, w <- P M P g_true); pending is
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         if (sc->done[j] == 1) any = 1;
       }
     } else if (mode == 2) {
-      if (stt != 4) {
+      if (stt == 5) {
         sc->gPz[j] = s;
         sc->norm0[j] = sqrt(fabs(s));
         sc->ref[j] = sc->norm0[j];
@@ -512,6 +513,7 @@ hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* par
 
 // mode 0: lam += rho w ; g += rho Fw      mode 1: w = Pz + gamma w      (running columns)
 // mode 2: g = g_true, w = P M P g_true where repl[j]                     (restart)
+// mode 3: a = c for joining columns (status 5)                            (start)
 __global__ __launch_bounds__(256) void kb_update(int64_t nlam, int k, int mode, double* __restrict__ a,
                                                  double* __restrict__ b, const double* __restrict__ c,
                                                  const double* __restrict__ e, const BScalars* __restrict__ sc,
@@ -525,6 +527,10 @@ __global__ __launch_bounds__(256) void kb_update(int64_t nlam, int k, int mode, 
       a[t] = c[t];
       b[t] = e[t];
     }
+    return;
+  }
+  if (mode == 3) {
+    if (sc->done[j] == 5) a[t] = c[t];
     return;
   }
   if (sc->done[j]) return;

@@ -7,12 +7,16 @@
 #include "../../include/hfls_gpu.h"
 #include "engine.hpp"
 #include "problem.hpp"
+#include "serve.hpp"
 
 struct hfls_problem_s {
   hfls::Problem* p;
 };
 struct hfls_mesh_s {
   hfls::Engine* e;
+};
+struct hfls_server_s {
+  hfls::Server* s;
 };
 
 static thread_local std::string g_err;
@@ -293,5 +297,85 @@ extern "C" int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, 
   return guard([&] {
     if (k < 1 || case0 < 0) throw std::runtime_error("invalid load-case range");
     hfls::traction_loads(m->e->problem(), seed, case0, k, f);
+  });
+}
+
+// ------------------------------------------------------------------ server
+extern "C" int hfls_server_create(int device, double budget_gb, int slots, int precision, double eps, int max_iter,
+                                  int chunk, hfls_server* out) {
+  return guard([&] {
+    if (!(budget_gb > 0)) throw std::runtime_error("HBM budget must be positive");
+    auto* h = new hfls_server_s;
+    try {
+      h->s = new hfls::Server(device, (size_t)(budget_gb * 1e9), slots, precision, eps, max_iter, chunk);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+extern "C" int hfls_server_destroy(hfls_server s) {
+  return guard([&] {
+    if (s) {
+      delete s->s;
+      delete s;
+    }
+  });
+}
+
+extern "C" int hfls_server_sizes(hfls_server s, int nopt, const char* const* keys, const char* const* vals,
+                                 int64_t* npacked, int64_t* nglob, int64_t* bytes) {
+  return guard([&] { s->s->sizes(make_options(nopt, keys, vals), npacked, nglob, bytes); });
+}
+
+extern "C" int hfls_server_submit(hfls_server s, int nopt, const char* const* keys, const char* const* vals, int k,
+                                  const double* f, uint64_t seed, int case0, int64_t* id) {
+  return guard([&] { *id = s->s->submit(make_options(nopt, keys, vals), k, f, seed, case0); });
+}
+
+extern "C" int hfls_server_step(hfls_server s, int* busy) {
+  return guard([&] {
+    const bool b = s->s->step();
+    if (busy) *busy = b ? 1 : 0;
+  });
+}
+
+extern "C" int hfls_server_state(hfls_server s, int64_t id, int* state) {
+  return guard([&] { *state = s->s->state(id); });
+}
+
+extern "C" int hfls_server_result(hfls_server s, int64_t id, double* u, hfls_case_stats* cases, double* latency_ms) {
+  return guard([&] {
+    const int k = s->s->request_k(id);
+    std::vector<int> it(k), stt(k);
+    std::vector<double> res(k);
+    double lat[2] = {0, 0};
+    s->s->result(id, u, it.data(), stt.data(), res.data(), lat, nullptr);
+    if (cases)
+      for (int j = 0; j < k; j++) {
+        cases[j].iterations = it[j];
+        cases[j].status = stt[j];
+        cases[j].residual = res[j];
+      }
+    if (latency_ms) {
+      latency_ms[0] = lat[0];
+      latency_ms[1] = lat[1];
+    }
+  });
+}
+
+extern "C" int hfls_server_stats(hfls_server s, double* out) {
+  return guard([&] {
+    const hfls::ServeStats& t = s->s->stats();
+    out[0] = s->s->meshes();
+    out[1] = (double)s->s->bytes_in_use();
+    out[2] = (double)t.setups;
+    out[3] = (double)t.evictions;
+    out[4] = (double)t.cases_done;
+    out[5] = (double)t.requests_done;
+    out[6] = (double)t.joins;
+    out[7] = t.setup_ms;
   });
 }

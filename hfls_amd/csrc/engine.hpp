@@ -105,6 +105,20 @@ class Engine {
   void apply_block(int op, int k, int precision, const double* x, double* y);
   size_t block_bytes() const;
 
+  // Slot lifecycle of a block (continuous batching): blk_begin sets the block
+  // width k and empties every slot; blk_join starts load cases in free slots
+  // (f_local: one packed local load per slot, or null for the body force);
+  // blk_sweeps enqueues PCPG iterations; blk_sync downloads the per-slot scalars;
+  // blk_finish recovers the displacements of stopped slots (one 3*nPoints vector
+  // per slot into u) and frees them.
+  void blk_begin(int k, double eps, int max_iter, int precision);
+  void blk_join(const std::vector<int>& slots, const double* f_local);
+  void blk_sweeps(int n);
+  void blk_sync(BScalars* out);
+  void blk_finish(const std::vector<int>& slots, double* u, double* lam);
+  int blk_width() const { return blk_k; }
+  hipStream_t stream() const { return st; }
+
   // Accessors for tests: per-subdomain results of the setup (host copies).
   void get_R(std::vector<double>& R) const;       // packed local, n x 6 column major per subdomain
   void get_fix(std::vector<int>& fix) const;      // 6 local DOFs per subdomain
@@ -153,6 +167,8 @@ class Engine {
 
   // ---- block state (allocated on the first block call)
   bool blk_ready = false;
+  int blk_k = 0, blk_prec = 64;
+  void blk_set_status(const std::vector<int>& slots, int status);
   int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
@@ -161,7 +177,7 @@ class Engine {
   DVec<double> b_CR;
   DVec<float> b_Kt32, b_St32, b_xb32, b_y32;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
-      b_Vr, b_Xr, b_V, b_X, b_U;
+      b_Vr, b_Xr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;
   int b_nparts = 0;
   void block_prepare();
@@ -170,7 +186,7 @@ class Engine {
   void bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate = nullptr);
   void bM(int k, int prec, const double* x, double* out, const int* gate = nullptr);
   void b_corr(int k, int with_R, int defl, const int* gate);
-  void b_kplus(int k, bool interior_too);  // from b_V: yb = (K_c^+ v)_b-ish pieces; see engine_block.cpp
+  void b_kplus(int k, bool interior_too, double* V);  // K_c^+ pieces of the local block V
 
   void build_meta();
   void factor_subdomains();

@@ -197,6 +197,7 @@ void Engine::block_prepare() {
   zero_alloc(b_Vr, NRP * KP, st);
   zero_alloc(b_Xr, NRP * KP, st);
   zero_alloc(b_V, np * KMAX, st);
+  zero_alloc(b_V2, np * KMAX, st);
   zero_alloc(b_X, np * KMAX, st);
   zero_alloc(b_U, ng * KMAX, st);
   b_sc.alloc(1);
@@ -209,7 +210,7 @@ size_t Engine::block_bytes() const {
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n + b_y32.n) * sizeof(float);
   for (const DVec<double>* v : {&b_CR, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
-                                &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_X, &b_U})
+                                &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
 }
@@ -282,14 +283,14 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
                        out, nullptr, nullptr, gate));
 }
 
-// From the local block b_V (FP64): b_Vr = v_r, b_xb = t_b = v_b - (K_rr^-1 K_rb)^T v_r,
+// From the local block V (FP64, column major): b_Vr = v_r, b_xb = t_b = v_b - (K_rr^-1 K_rb)^T v_r,
 // b_yb = (K^+ (v - Bc^T lam_c))_b (+ R_b alpha_c if with_R), b_la = [lam_c; alpha_c],
 // b_v6 = R^T v (Cluster.cpp:967-1040).  With interior_too, b_X = K^+(v - Bc^T lam_c)
 // in local order (packed, column major).
-void Engine::b_kplus(int k, bool interior_too) {
+void Engine::b_kplus(int k, bool interior_too, double* V) {
   const int N = P->nsub, NC = P->ncl;
   const int64_t np = off_n[N];
-  HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, b_V.p, b_Vr.p, b_orrow.p,
+  HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, V, b_Vr.p, b_orrow.p,
                        b_xb.p, b_obrow.p));
   if (maxTr > 0)
     HC(launch_tgemm_f64(st, N, maxTb, 2, 1, d_nr.p, d_ns.p, d_T.p, d_off_T.p, d_ldt.p, b_Vr.p, b_roff.p, b_xb.p,
@@ -298,7 +299,7 @@ void Engine::b_kplus(int k, bool interior_too) {
                       b_xoff.p, nullptr, KP, 1.0, nullptr));
   HC(launch_tgemm_f64(st, N, maxTcr, 2, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_xb.p, b_xoff.p, b_cr.p,
                       b_croff.p, nullptr, KP, 1.0, nullptr));
-  HC(launch_kb_rtf(st, N, k, KP, np, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_V.p, b_v6.p));
+  HC(launch_kb_rtf(st, N, k, KP, np, d_n.p, d_off_n.p, d_R.p, d_off_R.p, V, b_v6.p));
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
                           b_lcpos.p, b_v6.p, b_v6row.p, b_gc.p, b_gcrow.p));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
@@ -367,77 +368,150 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
   return ms / reps;
 }
 
-void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,
-                         int* iters, int* status, double* res, BlockInfo* info) {
+// ------------------------------------------------------------ slot lifecycle
+// The block is a set of k slots (columns).  A load case joins a free slot
+// (blk_join: its d, lam0, g0, w0 and scalars are set up while the other columns
+// keep their state), all running slots advance together (blk_sweeps), and a slot
+// whose case has stopped is recovered and freed (blk_finish).  Joins and finishes
+// happen between sweeps, so a case's arithmetic does not depend on the others.
+void Engine::blk_begin(int k, double eps, int max_iter, int prec) {
   if (k < 1 || k > KMAX) throw std::runtime_error("block size must be 1..64");
   if (prec != 32 && prec != 64) throw std::runtime_error("precision must be 32 or 64");
   block_prepare();
-  const int N = P->nsub;
-  const int64_t nl = P->nlam, np = off_n[N], ng = 3 * (int64_t)P->nPoints;
-  // loads, column major
-  if (f_h) {
-    HC(hipMemcpyAsync(b_V.p, f_h, (size_t)np * k * sizeof(double), hipMemcpyHostToDevice, st));
-  } else {
-    for (int j = 0; j < k; j++)
-      HC(hipMemcpyAsync(b_V.p + (size_t)j * np, d_f.p, np * sizeof(double), hipMemcpyDeviceToDevice, st));
-  }
+  blk_k = k;
+  blk_prec = prec;
   BScalars s0;
   std::memset(&s0, 0, sizeof(s0));
   s0.eps = eps;
   s0.max_iter = max_iter;
   s0.cert = prec == 32 ? 1 : 0;
   s0.rtol = prec == 32 ? 1e-3 : 0.0;
-  for (int j = k; j < KMAX; j++) s0.done[j] = 4;
+  for (int j = 0; j < KMAX; j++) s0.done[j] = 4;
   HC(hipMemcpyAsync(b_sc.p, &s0, sizeof(BScalars), hipMemcpyHostToDevice, st));
+}
+
+void Engine::blk_set_status(const std::vector<int>& cols, int status) {
+  static const int codes[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  for (int j : cols) {
+    if (j < 0 || j >= blk_k) throw std::runtime_error("slot out of range");
+    HC(hipMemcpyAsync(&b_sc.p->done[j], &codes[status], sizeof(int), hipMemcpyHostToDevice, st));
+  }
+}
+
+void Engine::blk_join(const std::vector<int>& cols, const double* f_h) {
+  const int N = P->nsub, k = blk_k, prec = blk_prec;
+  const int64_t nl = P->nlam, np = off_n[N];
+  for (size_t c = 0; c < cols.size(); c++) {
+    double* dst = b_V.p + (size_t)cols[c] * np;
+    if (f_h)
+      HC(hipMemcpyAsync(dst, f_h + c * (size_t)np, np * sizeof(double), hipMemcpyHostToDevice, st));
+    else
+      HC(hipMemcpyAsync(dst, d_f.p, np * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  blk_set_status(cols, 5);
+  // d = Bf K_c^+ f ; e = R^T f ; lam0 = G (G^T G)^-1 e (Cluster.cpp:1385-1394); computed
+  // for every column into scratch, copied into the state of the joining ones
+  b_kplus(k, false, b_V.p);
+  HC(launch_kb_scatter(st, nl, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, nullptr, b_Fw.p,
+                       nullptr, nullptr));
+  HC(launch_kb_update(st, nl, k, 3, b_d.p, nullptr, b_Fw.p, nullptr, b_sc.p));
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr, 1.0));
+  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_h.p, nullptr, b_Pg.p,
+                  1.0, nullptr, nullptr));
+  HC(launch_kb_update(st, nl, k, 3, b_lam.p, nullptr, b_Pg.p, nullptr, b_sc.p));
+  // g0 = F lam0 - d in FP64 ; w0 = P M P g0
+  bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
+  HC(launch_kb_update(st, nl, k, 3, b_g.p, nullptr, b_g64.p, nullptr, b_sc.p));
+  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
+  bM(k, prec, b_Pg.p, b_z.p);
+  bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
+  HC(launch_kb_update(st, nl, k, 3, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p));
+  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 2));
+}
+
+void Engine::blk_sweeps(int n) {
+  const int k = blk_k, prec = blk_prec;
+  const int64_t nl = P->nlam;
+  const int* pending = &b_sc.p->pending;
+  const int* active = &b_sc.p->active;
+  for (int c = 0; c < n; c++) {
+    // one PCPG iteration for every running column; the launches are skipped on
+    // the device once no column runs (active == 0)
+    bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p, active);
+    HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0, active));
+    HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p, active));
+    bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr, active);
+    bM(k, prec, b_Pg.p, b_z.p, active);
+    bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p, active);
+    HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1, active));
+    HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p, active));
+    if (prec == 32) {
+      // columns waiting at a refinement point: FP64 true residual g = F lam - d,
+      // certification or restart from it (skipped on the device when none waits)
+      bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
+      bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr, pending);
+      bM(k, prec, b_Pg.p, b_z.p, pending);
+      bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
+      HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, active));
+    }
+  }
+}
+
+void Engine::blk_sync(BScalars* sh) {
+  HC(hipMemcpyAsync(sh, b_sc.p, sizeof(BScalars), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::blk_finish(const std::vector<int>& cols, double* u_h, double* lam_h) {
+  const int N = P->nsub, k = blk_k, prec = blk_prec;
+  const int64_t nl = P->nlam, np = off_n[N], ng = 3 * (int64_t)P->nPoints;
+  // final FP64 true residual; its projection gives alpha (h per subdomain)
+  bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
+  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
+  bM(k, prec, b_Pg.p, b_z.p);
+  bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
+  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
+  // recovery: u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678), on a
+  // copy of the loads (the running columns keep theirs)
+  HC(hipMemcpyAsync(b_V2.p, b_V.p, (size_t)np * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HC(launch_kb_sub_bt(st, N, maxns, k, np, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p,
+                      b_lam.p, b_V2.p));
+  b_kplus(k, true, b_V2.p);
+  HC(launch_kb_recover(st, N, maxn, k, KP, np, ng, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_arow.p, b_la.p, b_hg.p, b_X.p,
+                       d_gdof.p, d_owner.p, b_U.p));
+  for (size_t c = 0; c < cols.size(); c++)
+    if (u_h)
+      HC(hipMemcpyAsync(u_h + c * (size_t)ng, b_U.p + (size_t)cols[c] * ng, ng * sizeof(double),
+                        hipMemcpyDeviceToHost, st));
+  if (lam_h) {
+    std::vector<double> lt((size_t)nl * k);
+    HC(hipMemcpyAsync(lt.data(), b_lam.p, lt.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    for (size_t c = 0; c < cols.size(); c++)
+      for (int64_t r = 0; r < nl; r++) lam_h[c * (size_t)nl + r] = lt[(size_t)r * k + cols[c]];
+  }
+  blk_set_status(cols, 4);
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,
+                         int* iters, int* status, double* res, BlockInfo* info) {
+  blk_begin(k, eps, max_iter, prec);
+  std::vector<int> cols(k);
+  for (int j = 0; j < k; j++) cols[j] = j;
   hipEvent_t ev[4];
   for (auto& e : ev) HC(hipEventCreate(&e));
   HC(hipEventRecord(ev[0], st));
-  // d = Bf K_c^+ f ; e = R^T f ; lam0 = G (G^T G)^-1 e (Cluster.cpp:1385-1394)
-  b_kplus(k, false);
-  HC(launch_kb_scatter(st, nl, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, nullptr, b_d.p,
-                       nullptr, nullptr));
-  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr, 1.0));
-  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_h.p, nullptr, b_lam.p,
-                  1.0, nullptr, nullptr));
-  // g0 = F lam0 - d in FP64 ; w0 = P M P g0
-  bF(k, 64, b_lam.p, b_g.p, b_d.p, nullptr);
-  bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr);
-  bM(k, prec, b_Pg.p, b_z.p);
-  bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p);
-  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 2));
-  HC(hipMemcpyAsync(b_w.p, b_Pz.p, (size_t)nl * k * sizeof(double), hipMemcpyDeviceToDevice, st));
+  blk_join(cols, f_h);
   HC(hipEventRecord(ev[1], st));
   const int chunk = 8;
-  const int* pending = &b_sc.p->pending;
-  const int* active = &b_sc.p->active;
   int sweeps = 0;
   BScalars sh;
   for (;;) {
-    for (int c = 0; c < chunk; c++) {
-      // one PCPG iteration for every running column; the launches are skipped on
-      // the device once no column runs (active == 0)
-      bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p, active);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0, active));
-      HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p, active));
-      bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr, active);
-      bM(k, prec, b_Pg.p, b_z.p, active);
-      bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p, active);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1, active));
-      HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p, active));
-      if (prec == 32) {
-        // columns waiting at a refinement point: FP64 true residual g = F lam - d,
-        // certification or restart from it (skipped on the device when none waits)
-        bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
-        bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr, pending);
-        bM(k, prec, b_Pg.p, b_z.p, pending);
-        bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
-        HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
-        HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, active));
-      }
-    }
+    blk_sweeps(chunk);
     sweeps += chunk;
-    HC(hipMemcpyAsync(&sh, b_sc.p, sizeof(BScalars), hipMemcpyDeviceToHost, st));
-    HC(hipStreamSynchronize(st));
+    blk_sync(&sh);
     if (sh.err) break;
     bool running = false;
     for (int j = 0; j < k; j++)
@@ -446,35 +520,19 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
     if (sweeps > max_iter + 2 * chunk) break;
   }
   HC(hipEventRecord(ev[2], st));
-  // final FP64 true residual of every column; its projection gives alpha
-  bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
-  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
-  bM(k, prec, b_Pg.p, b_z.p);
-  bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
-  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
-  // recovery: u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678)
-  HC(launch_kb_sub_bt(st, N, maxns, k, np, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p,
-                      b_lam.p, b_V.p));
-  b_kplus(k, true);
-  HC(launch_kb_recover(st, N, maxn, k, KP, np, ng, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_arow.p, b_la.p, b_hg.p, b_X.p,
-                       d_gdof.p, d_owner.p, b_U.p));
-  HC(hipEventRecord(ev[3], st));
-  if (u_h) HC(hipMemcpyAsync(u_h, b_U.p, (size_t)ng * k * sizeof(double), hipMemcpyDeviceToHost, st));
-  std::vector<double> lt;
-  if (lam_h) {
-    lt.resize((size_t)nl * k);
-    HC(hipMemcpyAsync(lt.data(), b_lam.p, lt.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  }
-  HC(hipMemcpyAsync(&sh, b_sc.p, sizeof(BScalars), hipMemcpyDeviceToHost, st));
-  HC(hipStreamSynchronize(st));
-  if (lam_h)
-    for (int j = 0; j < k; j++)
-      for (int64_t r = 0; r < nl; r++) lam_h[(size_t)j * nl + r] = lt[(size_t)r * k + j];
+  // per-column results before the slots are released
+  std::vector<int> its(k), stt(k);
   for (int j = 0; j < k; j++) {
-    if (iters) iters[j] = sh.it[j];
-    int s = sh.done[j];
-    if (prec == 64 && s == 1) s = 2;  // FP64 recursive residual: no certification step
-    if (s == 0 || s == 1) s = 3;      // stopped by the sweep limit
+    its[j] = sh.it[j];
+    stt[j] = sh.done[j];
+  }
+  blk_finish(cols, u_h, lam_h);
+  HC(hipEventRecord(ev[3], st));
+  blk_sync(&sh);
+  for (int j = 0; j < k; j++) {
+    if (iters) iters[j] = its[j];
+    int s = stt[j];
+    if (s == 0 || s == 1) s = 3;  // stopped by the sweep limit
     if (status) status[j] = s;
     if (res) res[j] = sh.res[j];
   }

@@ -1,0 +1,275 @@
+// Request loop with continuous batching; see serve.hpp.
+#include "serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+
+namespace hfls {
+
+static int64_t round8l(int64_t x) { return (x + 7) / 8 * 8; }
+static int64_t align32l(int64_t x) { return (x + 31) / 32 * 32; }
+
+std::string options_key(const Options& o) {
+  Options full = default_options();
+  for (auto& kv : o) full[kv.first] = kv.second;
+  std::string k;
+  for (auto& kv : full) {
+    if (kv.first == "device" || kv.first == "eps_iter" || kv.first == "max_iter" || kv.first == "vtkWithinIter" ||
+        kv.first == "print_matrices" || kv.first == "precision" || kv.first == "nrhs")
+      continue;  // do not change the set-up mesh
+    k += kv.first + "=" + kv.second + ";";
+  }
+  return k;
+}
+
+Server::Server(int device, size_t budget, int slots, int prec, double eps, int max_iter, int chunk)
+    : device_(device), slots_(slots), prec_(prec), max_iter_(max_iter), chunk_(chunk), eps_(eps), budget_(budget) {
+  if (slots < 1 || slots > KMAX) throw std::runtime_error("slots per mesh must be 1..64");
+  if (prec != 32 && prec != 64) throw std::runtime_error("precision must be 32 or 64");
+  if (chunk < 1) throw std::runtime_error("chunk must be positive");
+  t0_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+Server::~Server() = default;
+
+double Server::now() const {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0_;
+}
+
+const Problem& Server::host_problem(const std::string& key, const Options& o) {
+  auto it = host_.find(key);
+  if (it != host_.end()) return *it->second;
+  std::unique_ptr<Problem> p(new Problem(o));
+  const Problem& ref = *p;
+  host_[key] = std::move(p);
+  return ref;
+}
+
+// device bytes a set-up mesh keeps (plain path + block buffers), from the host
+// problem; the transient factorisation scratch is sized by the free memory
+size_t Server::estimate(const Problem& p) const {
+  double b = 0.0;
+  const double nl = (double)p.nlam, ng = 3.0 * (double)p.nPoints;
+  double np = 0;
+  for (const Subdomain& s : p.subs) {
+    const int64_t ldb = round8l(s.n_s), ldt = round8l(s.n_r);
+    const int64_t T = (s.n_s + 63) / 64, tiles = T * (T + 1) / 2 * 4096;
+    np += s.n;
+    b += 8.0 * (align32l(ldt * s.n_s) + align32l(ldt * s.n_r));           // T, K_rr^-1
+    b += (8.0 + 8.0 + 4.0 + 4.0) * (double)tiles;                         // K+_bb, S (FP64, FP32)
+    b += 8.0 * (6.0 * s.n + 6.0 * s.n_s + 4.0 * s.n + 2.0 * s.n_s);       // R, R_b, vectors
+    b += 8.0 * (2.0 * align32l(ldb * 36) + 54.0 * ldb);                   // C, Bc^T, [C|R_b|Q|Yh]
+    b += 8.0 * (double)T * (T + 1) / 2 * 64;                              // tile-row partials
+    b += (8.0 + 8.0 + 4.0 + 4.0) * 64.0 * 64.0 * T;                       // b-blocks
+    b += 8.0 * 64.0 * (2.0 * ((s.n_r + 63) / 64 * 64) + 64.0 * 3);        // interior, cr/coef/v6 blocks
+  }
+  b += 8.0 * (8.0 * nl + 9.0 * 64.0 * nl);                                // dual vectors
+  b += 8.0 * 64.0 * (3.0 * np + ng) + 8.0 * (ng + 4.0 * np);               // local blocks, U
+  return (size_t)(1.1 * b);
+}
+
+size_t Server::bytes_in_use() const {
+  size_t s = 0;
+  for (auto& kv : mesh_) s += kv.second->bytes;
+  return s;
+}
+
+void Server::sizes(const Options& o, int64_t* npacked, int64_t* nglob, int64_t* bytes) {
+  const Problem& p = host_problem(options_key(o), o);
+  int64_t np = 0;
+  for (const Subdomain& s : p.subs) np += s.n;
+  if (npacked) *npacked = np;
+  if (nglob) *nglob = 3 * (int64_t)p.nPoints;
+  if (bytes) *bytes = (int64_t)estimate(p);
+}
+
+int64_t Server::submit(const Options& o, int k, const double* F, uint64_t seed, int case0) {
+  if (k < 1) throw std::runtime_error("a request needs at least one load case");
+  const std::string key = options_key(o);
+  const Problem& p = host_problem(key, o);
+  if (estimate(p) > budget_) throw std::runtime_error("mesh does not fit the HBM budget of the server");
+  int64_t np = 0;
+  for (const Subdomain& s : p.subs) np += s.n;
+  ServeRequest r;
+  r.id = next_id_++;
+  r.key = key;
+  r.k = k;
+  r.F.resize((size_t)k * np);
+  if (F)
+    std::memcpy(r.F.data(), F, r.F.size() * sizeof(double));
+  else
+    traction_loads(p, seed, case0, k, r.F.data());
+  r.t_submit = now();
+  r.remaining = k;
+  r.U.assign((size_t)k * 3 * p.nPoints, 0.0);
+  r.it.assign(k, 0);
+  r.status.assign(k, 0);
+  r.res.assign(k, 0.0);
+  const int64_t id = r.id;
+  req_[id] = std::move(r);
+  wait_.push_back(id);
+  return id;
+}
+
+bool Server::admit(ServeRequest& r) {
+  auto it = mesh_.find(r.key);
+  if (it == mesh_.end()) {
+    const Problem& p = *host_[r.key];
+    const size_t need = estimate(p);
+    // evict idle meshes, least recently used first, until the new one fits
+    while (bytes_in_use() + need > budget_) {
+      ServeMesh* lru = nullptr;
+      for (auto& kv : mesh_) {
+        ServeMesh* m = kv.second.get();
+        bool idle = m->queue.empty();
+        for (auto& s : m->slot) idle = idle && s.req < 0;
+        if (idle && (!lru || m->last_used < lru->last_used)) lru = m;
+      }
+      if (!lru) return false;  // everything cached is busy: wait
+      mesh_.erase(lru->key);
+      st_.evictions++;
+    }
+    std::unique_ptr<ServeMesh> m(new ServeMesh);
+    m->key = r.key;
+    m->opts = p.o;
+    HC(hipSetDevice(device_));
+    size_t f0 = 0, f1 = 0, tot = 0;
+    HC(hipMemGetInfo(&f0, &tot));
+    const double ts = now();
+    m->e.reset(new Engine(p.o, device_));
+    m->e->setup();
+    m->e->blk_begin(slots_, eps_, max_iter_, prec_);
+    HC(hipStreamSynchronize(m->e->stream()));
+    HC(hipMemGetInfo(&f1, &tot));
+    st_.setup_ms += now() - ts;
+    st_.setups++;
+    m->bytes = std::max(need, f0 > f1 ? f0 - f1 : (size_t)0);
+    m->np = m->e->packed_n();
+    m->ng = 3 * (int64_t)p.nPoints;
+    m->slot.assign(slots_, ServeSlot());
+    it = mesh_.emplace(r.key, std::move(m)).first;
+  }
+  ServeMesh& m = *it->second;
+  for (int c = 0; c < r.k; c++) m.queue.push_back(ServeSlot{r.id, c});
+  m.last_used = now();
+  r.t_admit = now();
+  return true;
+}
+
+bool Server::step() {
+  HC(hipSetDevice(device_));
+  // 1. admission in arrival order; a request whose mesh does not fit yet waits
+  //    without blocking the requests behind it
+  for (auto it = wait_.begin(); it != wait_.end();) {
+    if (admit(req_[*it])) it = wait_.erase(it);
+    else ++it;
+  }
+  // 2. free slots take queued cases (all meshes, asynchronously)
+  for (auto& kv : mesh_) {
+    ServeMesh& m = *kv.second;
+    std::vector<int> cols;
+    std::vector<double> F;
+    for (int j = 0; j < slots_ && !m.queue.empty(); j++) {
+      if (m.slot[j].req >= 0) continue;
+      ServeSlot c = m.queue.front();
+      m.queue.pop_front();
+      m.slot[j] = c;
+      ServeRequest& r = req_[c.req];
+      cols.push_back(j);
+      F.insert(F.end(), r.F.begin() + (size_t)c.idx * m.np, r.F.begin() + (size_t)(c.idx + 1) * m.np);
+      r.started++;
+    }
+    if (!cols.empty()) {
+      m.e->blk_join(cols, F.data());
+      st_.joins += (int64_t)cols.size();
+    }
+  }
+  // 3. one chunk of sweeps for every mesh with work (different streams overlap)
+  for (auto& kv : mesh_) {
+    ServeMesh& m = *kv.second;
+    m.swept = false;
+    for (auto& s : m.slot)
+      if (s.req >= 0) m.swept = true;
+    if (m.swept) {
+      m.e->blk_sweeps(chunk_);
+      m.last_used = now();
+    }
+  }
+  // 4. stopped slots: recover, hand the displacements to their requests, release
+  for (auto& kv : mesh_) {
+    ServeMesh& m = *kv.second;
+    if (!m.swept) continue;
+    BScalars sh;
+    m.e->blk_sync(&sh);
+    if (sh.err) throw std::runtime_error("PCPG produced a non-finite residual in a served block");
+    std::vector<int> cols;
+    for (int j = 0; j < slots_; j++)
+      if (m.slot[j].req >= 0 && (sh.done[j] == 2 || sh.done[j] == 3)) cols.push_back(j);
+    if (cols.empty()) continue;
+    std::vector<double> U(cols.size() * (size_t)m.ng);
+    m.e->blk_finish(cols, U.data(), nullptr);
+    BScalars s2;
+    m.e->blk_sync(&s2);
+    const double t = now();
+    for (size_t c = 0; c < cols.size(); c++) {
+      const int j = cols[c];
+      ServeRequest& r = req_[m.slot[j].req];
+      const int i = m.slot[j].idx;
+      std::copy(U.begin() + c * m.ng, U.begin() + (c + 1) * m.ng, r.U.begin() + (size_t)i * m.ng);
+      r.it[i] = sh.it[j];
+      r.status[i] = sh.done[j];
+      r.res[i] = s2.res[j];
+      if (--r.remaining == 0) {
+        r.t_done = t;
+        st_.requests_done++;
+      }
+      st_.cases_done++;
+      m.slot[j] = ServeSlot();
+    }
+    st_.finishes += (int64_t)cols.size();
+  }
+  bool busy = !wait_.empty();
+  for (auto& kv : mesh_) {
+    if (!kv.second->queue.empty()) busy = true;
+    for (auto& s : kv.second->slot)
+      if (s.req >= 0) busy = true;
+  }
+  return busy;
+}
+
+int Server::state(int64_t id) const {
+  auto it = req_.find(id);
+  if (it == req_.end()) return -1;
+  if (it->second.t_done >= 0) return 2;
+  if (it->second.t_admit >= 0) return 1;
+  return 0;
+}
+
+int Server::request_k(int64_t id) const {
+  auto it = req_.find(id);
+  if (it == req_.end()) throw std::runtime_error("unknown request");
+  return it->second.k;
+}
+
+void Server::result(int64_t id, double* u, int* it, int* status, double* res, double* latency_ms, int* k) {
+  auto f = req_.find(id);
+  if (f == req_.end()) throw std::runtime_error("unknown request");
+  ServeRequest& r = f->second;
+  if (r.t_done < 0) throw std::runtime_error("request not finished");
+  if (k) *k = r.k;
+  if (u) std::memcpy(u, r.U.data(), r.U.size() * sizeof(double));
+  for (int i = 0; i < r.k; i++) {
+    if (it) it[i] = r.it[i];
+    if (status) status[i] = r.status[i];
+    if (res) res[i] = r.res[i];
+  }
+  if (latency_ms) {
+    latency_ms[0] = r.t_done - r.t_submit;
+    latency_ms[1] = r.t_admit - r.t_submit;
+  }
+  req_.erase(f);
+}
+
+}  // namespace hfls

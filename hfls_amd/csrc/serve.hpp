@@ -1,0 +1,94 @@
+// Request loop with continuous batching (one GPU).
+//
+// Requests carry k load cases for a mesh given by its options.  Set-up meshes are
+// cached on the device (factorisations, explicit local operators, coarse
+// problems, block buffers) and admitted by their HBM footprint; idle meshes are
+// evicted least recently used when a new one does not fit.  Every cached mesh owns
+// a block of slots: queued load cases of any request for that mesh take free
+// slots at the next chunk boundary, all running slots advance together, and a
+// slot whose case has stopped is recovered and released.  Meshes run on their own
+// streams, so the work of different meshes overlaps on the device.  A case's
+// arithmetic is independent of the other slots, so its result is the same alone
+// and in the mix.
+#pragma once
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace hfls {
+
+struct ServeRequest {
+  int64_t id = 0;
+  std::string key;
+  int k = 0;
+  std::vector<double> F;               // k packed local loads
+  double t_submit = 0, t_admit = -1, t_done = -1;  // ms since the server started
+  int started = 0, remaining = 0;
+  std::vector<double> U;               // k x 3*nPoints
+  std::vector<int> it, status;
+  std::vector<double> res;
+};
+
+struct ServeSlot {
+  int64_t req = -1;
+  int idx = 0;
+};
+
+struct ServeMesh {
+  std::string key;
+  Options opts;
+  std::unique_ptr<Engine> e;
+  size_t bytes = 0;
+  int64_t np = 0, ng = 0;
+  std::vector<ServeSlot> slot;
+  std::deque<ServeSlot> queue;         // cases admitted, waiting for a slot
+  bool swept = false;
+  double last_used = 0;
+};
+
+struct ServeStats {
+  int64_t cases_done = 0, requests_done = 0, setups = 0, evictions = 0, joins = 0, finishes = 0;
+  double setup_ms = 0;
+};
+
+class Server {
+ public:
+  Server(int device, size_t budget_bytes, int slots, int precision, double eps, int max_iter, int chunk);
+  ~Server();
+  // packed local length and global DOF count of a mesh (host-side problem)
+  void sizes(const Options& o, int64_t* npacked, int64_t* nglob, int64_t* bytes);
+  // F: k packed local loads, or null for k seeded face tractions (seed, case0..)
+  int64_t submit(const Options& o, int k, const double* F, uint64_t seed, int case0);
+  bool step();                                   // one scheduling round; false when idle
+  int state(int64_t id) const;                   // 0 queued, 1 running, 2 done, -1 unknown
+  int request_k(int64_t id) const;
+  void result(int64_t id, double* u, int* it, int* status, double* res, double* latency_ms, int* k);
+  const ServeStats& stats() const { return st_; }
+  size_t bytes_in_use() const;
+  int meshes() const { return (int)mesh_.size(); }
+
+ private:
+  int device_, slots_, prec_, max_iter_, chunk_;
+  double eps_;
+  size_t budget_;
+  int64_t next_id_ = 1;
+  double t0_;
+  std::map<int64_t, ServeRequest> req_;
+  std::deque<int64_t> wait_;                     // submitted, mesh not admitted yet
+  std::map<std::string, std::unique_ptr<ServeMesh>> mesh_;
+  std::map<std::string, std::unique_ptr<Problem>> host_;  // host problems (sizes, loads)
+  ServeStats st_;
+  double now() const;
+  const Problem& host_problem(const std::string& key, const Options& o);
+  size_t estimate(const Problem& p) const;
+  bool admit(ServeRequest& r);
+};
+
+std::string options_key(const Options& o);
+
+}  // namespace hfls

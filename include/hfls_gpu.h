@@ -114,6 +114,39 @@ int hfls_apply_block(hfls_mesh m, int op, int k, int precision, const double* x,
 int hfls_problem_traction(hfls_problem p, uint64_t seed, int case0, int k, double* f_local);
 int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f_local);
 
+/* ---- request loop with continuous batching (no counterpart in the reference,
+ *      whose Driver solves one mesh and one load case per run, Driver.cpp:20-38).
+ * A server caches set-up meshes on the device (admitted by HBM footprint, idle
+ * ones evicted least recently used), gives each mesh `slots` (<= 64) block
+ * columns, starts queued load cases of any request in free slots between chunks
+ * of `chunk` PCPG iterations, runs different meshes on their own streams, and
+ * recovers and releases a slot as soon as its case stops.  A case's result does
+ * not depend on the other cases in flight.  Driven by the caller: step() runs
+ * one scheduling round. */
+typedef struct hfls_server_s* hfls_server;
+int hfls_server_create(int device, double hbm_budget_gb, int slots, int precision, double eps, int max_iter,
+                       int chunk, hfls_server* out);
+int hfls_server_destroy(hfls_server s);
+/* packed local length, 3*nPoints and the estimated device footprint (bytes, the
+ * admission figure) of a mesh given by its options */
+int hfls_server_sizes(hfls_server s, int nopt, const char* const* keys, const char* const* vals, int64_t* npacked,
+                      int64_t* nglob, int64_t* bytes);
+/* k load cases (column major packed local loads, or NULL for k seeded face
+ * tractions seed, case0..case0+k-1) on the mesh given by the options */
+int hfls_server_submit(hfls_server s, int nopt, const char* const* keys, const char* const* vals, int k,
+                       const double* f_local, uint64_t seed, int case0, int64_t* id);
+/* one scheduling round (admission, joins, one chunk per busy mesh, recoveries);
+ * *busy = 0 when nothing is queued or running */
+int hfls_server_step(hfls_server s, int* busy);
+/* 0 queued, 1 admitted, 2 done */
+int hfls_server_state(hfls_server s, int64_t id, int* state);
+/* displacements (k x 3*nPoints), per-case stats, latency_ms[0] = submit -> done,
+ * latency_ms[1] = submit -> admitted; releases the request */
+int hfls_server_result(hfls_server s, int64_t id, double* u, hfls_case_stats* cases, double* latency_ms);
+/* [0] cached meshes, [1] device bytes held by them, [2] mesh setups, [3] evictions,
+ * [4] cases done, [5] requests done, [6] slot joins, [7] summed setup ms */
+int hfls_server_stats(hfls_server s, double* out);
+
 /* Measurement hook (no counterpart in the reference): average device time of
  * the dominant kernel over reps back-to-back launches timed with HIP events on
  * the library's stream.  which = 0 / 1: the packed-tile GEMV streaming K+_bb
