@@ -116,6 +116,11 @@ class Engine {
   void blk_sweeps(int n);
   void blk_sync(BScalars* out);
   void blk_finish(const std::vector<int>& slots, double* u, double* lam);
+  // the same in parts: loads to HBM / start, and recovery / results to the host
+  void blk_load(const std::vector<int>& slots, const double* f_local);
+  void blk_init(const std::vector<int>& slots);
+  void blk_recover();
+  void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
   hipStream_t stream() const { return st; }
 

@@ -17,7 +17,7 @@ std::string options_key(const Options& o) {
   std::string k;
   for (auto& kv : full) {
     if (kv.first == "device" || kv.first == "eps_iter" || kv.first == "max_iter" || kv.first == "vtkWithinIter" ||
-        kv.first == "print_matrices" || kv.first == "precision" || kv.first == "nrhs")
+        kv.first == "print_matrices" || kv.first == "precision" || kv.first == "nrhs" || kv.first == "refine_tol")
       continue;  // do not change the set-up mesh
     k += kv.first + "=" + kv.second + ";";
   }

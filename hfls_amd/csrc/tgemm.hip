@@ -13,7 +13,8 @@
 // One workgroup computes one 64-row x 64-column output block; the inner
 // dimension is streamed in chunks of 32 through LDS (A chunk stored [k][row] with
 // a padded row so both the transposed tile loads and the operand reads are free of
-// bank conflicts).  FP32: v_mfma_f32_32x32x2f32, wave w owns the 32x32 block
+// bank conflicts), with 16-byte global loads of the next chunk issued before the
+// matrix-core work on the current one.  FP32 takes tiles only (modes 0, 1).  FP32: v_mfma_f32_32x32x2f32, wave w owns the 32x32 block
 // (w/2, w%2).  FP64: v_mfma_f64_16x16x4f64, wave w owns rows 16w..16w+15 and all
 // four 16-column blocks.  Operand/result lane layouts (checked on gfx950):
 //   f32 32x32x2: A[l%32][l/32], B[l/32][l%32], D row 8(r/4)+4(l/32)+r%4, col l%32
@@ -47,74 +48,141 @@ static __device__ __forceinline__ int64_t tile_of(int mode, int op, int Ra, int 
   return (int64_t)i * Ra + j;  // stored tile (j, i)
 }
 
-template <typename T>
-struct TG;
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
-template <>
-struct TG<float> {
-  static constexpr int LDA = 65;
+// Per-thread staging of one K chunk (A: 32 x 64 of op(A)^T, X: 32 x 64): the next
+// chunk is fetched into registers while the matrix cores work on the current one
+// from LDS.  kind: 0 tile, 1 transposed tile, 2 compact op 0, 3 compact op 1.
+struct StageF {
+  float a[8], x[8];
 };
-template <>
-struct TG<double> {
-  static constexpr int LDA = 65;
+struct StageD {
+  double a[8], x[8];
 };
 
-template <typename T>
-static __device__ __forceinline__ void load_chunk(const T* __restrict__ tile, bool tr, int k0, T* As,
-                                                  const T* __restrict__ Xrow, int kp, T* Xs) {
+static __device__ __forceinline__ void fetch_f32(const float* __restrict__ tile, int kind, int k0,
+                                                 const float* __restrict__ Xrow, int kp, StageF& s) {
   const int t = threadIdx.x;
-  if (!tr) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) {
-      const int e = t + 256 * m;  // 32 x 64
-      const int kk = e >> 6, row = e & 63;
-      As[kk * 65 + row] = tile[(k0 + kk) * 64 + row];
-    }
-  } else {
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-      const int e = t + 256 * m;
-      const int row = e >> 5, kk = e & 31;
-      As[kk * 65 + row] = tile[row * 64 + k0 + kk];
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 8; m++) {
-    const int e = t + 256 * m;
-    const int kk = e >> 6, col = e & 63;
-    Xs[kk * 64 + col] = Xrow[(int64_t)(k0 + kk) * kp + col];
+  for (int q = 0; q < 2; q++) {
+    const int f = t + 256 * q;
+    f4 v;
+    if (kind == 0) v = *(const f4*)(tile + (k0 + (f >> 4)) * 64 + (f & 15) * 4);
+    else v = *(const f4*)(tile + (f >> 3) * 64 + k0 + (f & 7) * 4);
+    s.a[4 * q + 0] = v.x;
+    s.a[4 * q + 1] = v.y;
+    s.a[4 * q + 2] = v.z;
+    s.a[4 * q + 3] = v.w;
+    const f4 xv = *(const f4*)(Xrow + (int64_t)(k0 + (f >> 4)) * kp + (f & 15) * 4);
+    s.x[4 * q + 0] = xv.x;
+    s.x[4 * q + 1] = xv.y;
+    s.x[4 * q + 2] = xv.z;
+    s.x[4 * q + 3] = xv.w;
   }
 }
 
-// chunk loader for a compact column-major A (m x n, lda): op(A) rows 64i.., inner 64j+k0..
-template <typename T>
-static __device__ __forceinline__ void load_chunk_cm(const T* __restrict__ A, int64_t lda, int m, int n, int op,
-                                                     int i, int j, int k0, T* As, const T* __restrict__ Xrow, int kp,
-                                                     T* Xs) {
+static __device__ __forceinline__ void stash_f32(int kind, const StageF& s, float* As, float* Xs) {
   const int t = threadIdx.x;
-  if (op == 0) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = t + 256 * q;
-      const int kk = e >> 6, row = e & 63;
-      const int r = 64 * i + row, c = 64 * j + k0 + kk;
-      As[kk * 65 + row] = (r < m && c < n) ? A[r + (int64_t)c * lda] : (T)0;
+  for (int q = 0; q < 2; q++) {
+    const int f = t + 256 * q;
+    if (kind == 0) {
+      const int kk = f >> 4, r4 = (f & 15) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; u++) As[kk * 65 + r4 + u] = s.a[4 * q + u];
+    } else {
+      const int row = f >> 3, k4 = (f & 7) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; u++) As[(k4 + u) * 65 + row] = s.a[4 * q + u];
+    }
+    f4 xv;
+    xv.x = s.x[4 * q + 0];
+    xv.y = s.x[4 * q + 1];
+    xv.z = s.x[4 * q + 2];
+    xv.w = s.x[4 * q + 3];
+    *(f4*)(Xs + (f >> 4) * 64 + (f & 15) * 4) = xv;
+  }
+}
+
+template <bool CM>
+static __device__ __forceinline__ void fetch_f64(const double* __restrict__ tile, const double* __restrict__ Ac,
+                                                 int64_t lda, int m, int n, int i, int j, int kind, int k0,
+                                                 const double* __restrict__ Xrow, int kp, StageD& s) {
+  const int t = threadIdx.x;
+  if (!CM) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int f = t + 256 * q;
+      d2 v;
+      if (kind == 0) v = *(const d2*)(tile + (k0 + (f >> 5)) * 64 + (f & 31) * 2);
+      else v = *(const d2*)(tile + (f >> 4) * 64 + k0 + (f & 15) * 2);
+      s.a[2 * q + 0] = v.x;
+      s.a[2 * q + 1] = v.y;
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int e = t + 256 * q;
-      const int row = e >> 5, kk = e & 31;
-      const int r = 64 * i + row, c = 64 * j + k0 + kk;  // op(A)(r, c) = A(c, r)
-      As[kk * 65 + row] = (c < m && r < n) ? A[c + (int64_t)r * lda] : (T)0;
+      if (kind == 2) {
+        const int r = 64 * i + (e & 63), c = 64 * j + k0 + (e >> 6);
+        s.a[q] = (r < m && c < n) ? Ac[r + (int64_t)c * lda] : 0.0;
+      } else {
+        const int r = 64 * i + (e >> 5), c = 64 * j + k0 + (e & 31);  // op(A)(r, c) = A(c, r)
+        s.a[q] = (c < m && r < n) ? Ac[c + (int64_t)r * lda] : 0.0;
+      }
     }
   }
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int e = t + 256 * q;
-    const int kk = e >> 6, col = e & 63;
-    Xs[kk * 64 + col] = Xrow[(int64_t)(k0 + kk) * kp + col];
+  for (int q = 0; q < 4; q++) {
+    const int f = t + 256 * q;
+    const d2 xv = *(const d2*)(Xrow + (int64_t)(k0 + (f >> 5)) * kp + (f & 31) * 2);
+    s.x[2 * q + 0] = xv.x;
+    s.x[2 * q + 1] = xv.y;
   }
+}
+
+template <bool CM>
+static __device__ __forceinline__ void stash_f64(int kind, const StageD& s, double* As, double* Xs) {
+  const int t = threadIdx.x;
+  if (!CM) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int f = t + 256 * q;
+      if (kind == 0) {
+        const int kk = f >> 5, r2 = (f & 31) * 2;
+        As[kk * 65 + r2] = s.a[2 * q];
+        As[kk * 65 + r2 + 1] = s.a[2 * q + 1];
+      } else {
+        const int row = f >> 4, k2 = (f & 15) * 2;
+        As[k2 * 65 + row] = s.a[2 * q];
+        As[(k2 + 1) * 65 + row] = s.a[2 * q + 1];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = t + 256 * q;
+      if (kind == 2) As[(e >> 6) * 65 + (e & 63)] = s.a[q];
+      else As[(e & 31) * 65 + (e >> 5)] = s.a[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int f = t + 256 * q;
+    d2 xv;
+    xv.x = s.x[2 * q];
+    xv.y = s.x[2 * q + 1];
+    *(d2*)(Xs + (f >> 5) * 64 + (f & 31) * 2) = xv;
+  }
+}
+
+// source of K chunk c (tile j = c/2, rows k0 = 32*(c%2) of it)
+static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i, int c, int64_t* ti) {
+  bool tr = false;
+  *ti = mode == 2 ? 0 : tile_of(mode, op, Ra, i, c >> 1, &tr);
+  if (mode == 2) return op == 0 ? 2 : 3;
+  return tr ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* __restrict__ mrows,
@@ -141,24 +209,32 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
   for (int r = 0; r < 16; r++) acc[r] = 0.0f;
   const float* Ad = A + aoff[d];
   const float* Xd = X + xoff[d] + c0;
-  for (int j = 0; j < Ko; j++) {
-    bool tr = false;
-    const int64_t ti = mode == 2 ? 0 : tile_of(mode, op, Ra, i, j, &tr);
-    const float* tile = Ad + ti * 4096;
-    for (int k0 = 0; k0 < 64; k0 += 32) {
-      if (mode == 2)
-        load_chunk_cm<float>(Ad, lda[d], mrows[d], mcols[d], op, i, j, k0, As, Xd + (int64_t)(j * 64) * kp, kp, Xs);
-      else
-        load_chunk<float>(tile, tr, k0, As, Xd + (int64_t)(j * 64) * kp, kp, Xs);
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < 32; kk += 2) {
-        const float a = As[(kk + (lane >> 5)) * 65 + rb + (lane & 31)];
-        const float b = Xs[(kk + (lane >> 5)) * 64 + cb + (lane & 31)];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-      }
-      __syncthreads();
+  const int nch = 2 * Ko;
+  StageF s;
+  if (nch > 0) {
+    int64_t ti;
+    const int kind = chunk_kind(mode, op, Ra, i, 0, &ti);
+    fetch_f32(Ad + ti * 4096, kind, 0, Xd, kp, s);
+    stash_f32(kind, s, As, Xs);
+  }
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const bool more = c + 1 < nch;
+    int kind_n = 0;
+    if (more) {
+      int64_t ti;
+      kind_n = chunk_kind(mode, op, Ra, i, c + 1, &ti);
+      fetch_f32(Ad + ti * 4096, kind_n, ((c + 1) & 1) * 32, Xd + (int64_t)(((c + 1) >> 1) * 64) * kp, kp, s);
     }
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 2) {
+      const float a = As[(kk + (lane >> 5)) * 65 + rb + (lane & 31)];
+      const float b = Xs[(kk + (lane >> 5)) * 64 + cb + (lane & 31)];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+    if (more) stash_f32(kind_n, s, As, Xs);
+    __syncthreads();
   }
   float* Yd = Y + yoff[d] + c0;
   const float* Zd = Z ? Z + yoff[d] + c0 : nullptr;
@@ -172,6 +248,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
   }
 }
 
+template <bool CM>  // compact column-major A (mode 2) or tiles
 __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const double* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const int* __restrict__ lda,
@@ -181,7 +258,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
                                                    int kp, double alpha, const int* __restrict__ gate) {
   const int d = blockIdx.y;
   if (gate && *gate == 0) return;
-  const int Ra = (mrows[d] + 63) / 64, Ca = (mcols[d] + 63) / 64;
+  const int m = mrows[d], n = mcols[d];
+  const int Ra = (m + 63) / 64, Ca = (n + 63) / 64;
   const int Ro = (mode != 0 && op == 1) ? Ca : Ra;
   const int Ko = (mode != 0 && op == 1) ? Ra : Ca;
   const int i = blockIdx.x;
@@ -197,38 +275,48 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
   const double* Ad = A + aoff[d];
+  const int64_t la = mode == 2 ? (int64_t)lda[d] : 0;
   const double* Xd = X + xoff[d] + c0;
-  for (int j = 0; j < Ko; j++) {
-    bool tr = false;
-    const int64_t ti = mode == 2 ? 0 : tile_of(mode, op, Ra, i, j, &tr);
-    const double* tile = Ad + ti * 4096;
-    for (int k0 = 0; k0 < 64; k0 += 32) {
-      if (mode == 2)
-        load_chunk_cm<double>(Ad, lda[d], mrows[d], mcols[d], op, i, j, k0, As, Xd + (int64_t)(j * 64) * kp, kp, Xs);
-      else
-        load_chunk<double>(tile, tr, k0, As, Xd + (int64_t)(j * 64) * kp, kp, Xs);
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < 32; kk += 4) {
-        const double a = As[(kk + (lane >> 4)) * 65 + rb + (lane & 15)];
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-          const double b = Xs[(kk + (lane >> 4)) * 64 + 16 * c + (lane & 15)];
-          acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
-        }
-      }
-      __syncthreads();
+  const int nch = 2 * Ko;
+  StageD s;
+  if (nch > 0) {
+    int64_t ti;
+    const int kind = chunk_kind(mode, op, Ra, i, 0, &ti);
+    fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, 0, kind, 0, Xd, kp, s);
+    stash_f64<CM>(kind, s, As, Xs);
+  }
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const bool more = c + 1 < nch;
+    int kind_n = 0;
+    if (more) {
+      int64_t ti;
+      kind_n = chunk_kind(mode, op, Ra, i, c + 1, &ti);
+      fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, (c + 1) >> 1, kind_n, ((c + 1) & 1) * 32,
+                    Xd + (int64_t)(((c + 1) >> 1) * 64) * kp, kp, s);
     }
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 4) {
+      const double a = As[(kk + (lane >> 4)) * 65 + rb + (lane & 15)];
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) {
+        const double b = Xs[(kk + (lane >> 4)) * 64 + 16 * cc + (lane & 15)];
+        acc[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[cc], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    if (more) stash_f64<CM>(kind_n, s, As, Xs);
+    __syncthreads();
   }
   double* Yd = Y + yoff[d] + c0;
   const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
 #pragma unroll
-  for (int c = 0; c < 4; c++)
+  for (int cc = 0; cc < 4; cc++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int row = i * 64 + rb + 4 * r + (lane >> 4);
-      const int col = 16 * c + (lane & 15);
-      double v = alpha * acc[c][r];
+      const int col = 16 * cc + (lane & 15);
+      double v = alpha * acc[cc][r];
       if (Zd) v += Zd[(int64_t)row * kp + col];
       Yd[(int64_t)row * kp + col] = v;
     }
@@ -237,8 +325,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, float* Y,
                             const int64_t* yoff, const float* Z, int kp, float alpha, const int* gate) {
-  if (kp % 64) return hipErrorInvalidValue;
-  if (mode == 2 && !lda) return hipErrorInvalidValue;
+  if (kp % 64 || mode == 2) return hipErrorInvalidValue;  // FP32: tiles only
   hipLaunchKernelGGL(k_tgemm_f32, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff, lda, X,
                      xoff, Y, yoff, Z, kp, alpha, gate);
   return hipGetLastError();
@@ -249,8 +336,12 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
                             const int64_t* yoff, const double* Z, int kp, double alpha, const int* gate) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_tgemm_f64, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff, lda, X,
-                     xoff, Y, yoff, Z, kp, alpha, gate);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_tgemm_f64<true>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
+                       lda, X, xoff, Y, yoff, Z, kp, alpha, gate);
+  else
+    hipLaunchKernelGGL(k_tgemm_f64<false>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A,
+                       aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate);
   return hipGetLastError();
 }
 
