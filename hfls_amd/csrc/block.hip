@@ -8,9 +8,15 @@
 // A column whose status is not 0 is frozen: its scalars and dual iterates are not
 // updated, so every column's arithmetic is the same whether it is solved alone or
 // inside a block.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace hfls {
+
+// workgroups of the grid-stride elementwise kernels: enough to cover every CU
+// several times, few enough that a launch skipped through its gate costs ~us
+static constexpr int64_t kStrideGrid = 4096;
 
 static __device__ __forceinline__ double wsum64(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -24,31 +30,32 @@ __global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, cons
                                                  const double* __restrict__ v, const double* __restrict__ wgt,
                                                  double* __restrict__ xb, float* __restrict__ xb32, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t q = t / k;
-  const int j = (int)(t - q * k);
-  if (q >= nb) return;
-  const int d = bsub[q];
-  double s = 0.0;
-  const int e1 = gptr[q + 1];
-  for (int e = gptr[q]; e < e1; e++) {
-    const int code = gidx[e];
-    const int r = code >= 0 ? code : ~code;
-    double val = v[(int64_t)r * k + j];
-    if (wgt) val *= wgt[r];
-    s += code >= 0 ? val : -val;
+  const int64_t n = nb * k;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int64_t q = t / k;
+    const int j = (int)(t - q * k);
+    const int d = bsub[q];
+    double s = 0.0;
+    const int e1 = gptr[q + 1];
+    for (int e = gptr[q]; e < e1; e++) {
+      const int code = gidx[e];
+      const int r = code >= 0 ? code : ~code;
+      double val = v[(int64_t)r * k + j];
+      if (wgt) val *= wgt[r];
+      s += code >= 0 ? val : -val;
+    }
+    const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
+    if (xb) xb[o] = s;
+    if (xb32) xb32[o] = (float)s;
   }
-  const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
-  if (xb) xb[o] = s;
-  if (xb32) xb32[o] = (float)s;
 }
 
 hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
                             const int64_t* obrow, const int* gptr, const int* gidx, const double* v,
                             const double* wgt, double* xb, float* xb32, const int* gate) {
   const int64_t n = nb * k;
-  hipLaunchKernelGGL(kb_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nb, k, kp, bsub, boff, obrow,
-                     gptr, gidx, v, wgt, xb, xb32, gate);
+  hipLaunchKernelGGL(kb_gather, dim3((unsigned)std::min<int64_t>((n + 255) / 256, kStrideGrid)), dim3(256), 0, st, nb,
+                     k, kp, bsub, boff, obrow, gptr, gidx, v, wgt, xb, xb32, gate);
   return hipGetLastError();
 }
 
@@ -303,14 +310,15 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
   return hipGetLastError();
 }
 
-// out[r*k+j] = x[r*k+j] + sgn * (R_b,a[pos_a,:] h_a[:,j] - R_b,b[pos_b,:] h_b[:,j]); partial dots
+// out[r*k+j] = x[r*k+j] + sgn * (R_b,a[pos_a,:] h_a[:,j] - R_b,b[pos_b,:] h_b[:,j]); partial dots.
+// R_b rows are read from the row-major copy Rbr[(boff[d] + p)*6 + m] (one line per side)
 __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __restrict__ rsa,
                                              const int* __restrict__ rsb, const int* __restrict__ rpa,
-                                             const int* __restrict__ rpb, const double* __restrict__ Rb,
-                                             const int64_t* __restrict__ rboff, const int* __restrict__ ns,
-                                             const double* __restrict__ h, const double* __restrict__ x,
-                                             double* __restrict__ out, double sgn, const double* __restrict__ dotv,
-                                             double* __restrict__ part, const int* __restrict__ gate) {
+                                             const int* __restrict__ rpb, const double* __restrict__ Rbr,
+                                             const int64_t* __restrict__ boff, const double* __restrict__ h,
+                                             const double* __restrict__ x, double* __restrict__ out, double sgn,
+                                             const double* __restrict__ dotv, double* __restrict__ part,
+                                             const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double red[4 * 64];
   const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -320,16 +328,16 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
       const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
       if (r >= nlam) break;
       const int a = rsa[r];
-      const int na = ns[a];
-      const double* Ra = Rb + rboff[a] + rpa[r];
+      const double* Ra = Rbr + (boff[a] + rpa[r]) * 6;
+      const double* ha = h + (int64_t)6 * a * k + j;
       double v = 0.0;
-      for (int m = 0; m < 6; m++) v += Ra[(int64_t)m * na] * h[(int64_t)(6 * a + m) * k + j];
+      for (int m = 0; m < 6; m++) v += Ra[m] * ha[(int64_t)m * k];
       const int b = rsb[r];
       if (b >= 0) {
-        const int nb = ns[b];
-        const double* Rbb = Rb + rboff[b] + rpb[r];
+        const double* Rb2 = Rbr + (boff[b] + rpb[r]) * 6;
+        const double* hb = h + (int64_t)6 * b * k + j;
         double vb = 0.0;
-        for (int m = 0; m < 6; m++) vb += Rbb[(int64_t)m * nb] * h[(int64_t)(6 * b + m) * k + j];
+        for (int m = 0; m < 6; m++) vb += Rb2[m] * hb[(int64_t)m * k];
         v -= vb;
       }
       const double o = (x ? x[r * k + j] : 0.0) + sgn * v;
@@ -345,11 +353,11 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
 }
 
 hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
-                        const int* rpb, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
-                        const double* x, double* out, double sgn, const double* dotv, double* part, const int* gate) {
+                        const int* rpb, const double* Rbr, const int64_t* boff, const double* h, const double* x,
+                        double* out, double sgn, const double* dotv, double* part, const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, rpa, rpb, Rb,
-                     rboff, ns, h, x, out, sgn, dotv, part, gate);
+  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, rpa, rpb, Rbr,
+                     boff, h, x, out, sgn, dotv, part, gate);
   return hipGetLastError();
 }
 
@@ -411,7 +419,7 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
 //   mode 3: (only while pending) for status-1 columns, from the FP64 true residual:
 //           converged -> 2; limit -> 3; otherwise the iteration restarts from the
 //           true residual (repl = 1: g <- g_true, w <- P M P g_true); pending is
-//           cleared (the restart update that follows is gated by active)
+//           cleared (the restart update that follows is gated by `restart`)
 //   mode 4: record the true residual of every column (res)
 // column sums of the partials: one workgroup per column, strided partial sums
 // then a fixed-shape tree (the order depends on nparts only, not on other columns)
@@ -433,10 +441,10 @@ __global__ __launch_bounds__(256) void kb_colsum(int nparts, int k, const double
 
 __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate) {
   const int j = threadIdx.x;
-  __shared__ int any, run;
+  __shared__ int any, run, rst;
   if (gate && *gate == 0) return;              // uniform over the workgroup
   if (mode == 3 && sc->pending == 0) return;
-  if (j == 0) any = run = 0;
+  if (j == 0) any = run = rst = 0;
   __syncthreads();
   if (j < k) {
     const double s = sc->colsum[j];
@@ -487,6 +495,7 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
           sc->gPz[j] = s;
           sc->ref[j] = r;
           sc->done[j] = 0;
+          rst = 1;
         }
       }
     } else {
@@ -499,6 +508,8 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
     if (mode == 1) sc->pending = any;
     if (mode == 2 || mode == 3) sc->pending = 0;
     if (mode >= 1 && mode <= 3) sc->active = run;
+    if (mode == 1) sc->restart = 0;
+    if (mode == 3) sc->restart = rst;
   }
 }
 
@@ -519,35 +530,33 @@ __global__ __launch_bounds__(256) void kb_update(int64_t nlam, int k, int mode, 
                                                  const double* __restrict__ e, const BScalars* __restrict__ sc,
                                                  const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= nlam * k) return;
-  const int j = (int)(t % k);
-  if (mode == 2) {
-    if (sc->repl[j]) {
-      a[t] = c[t];
-      b[t] = e[t];
+  const int64_t n = nlam * k;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int j = (int)(t % k);
+    if (mode == 2) {
+      if (sc->repl[j]) {
+        a[t] = c[t];
+        b[t] = e[t];
+      }
+    } else if (mode == 3) {
+      if (sc->done[j] == 5) a[t] = c[t];
+    } else if (!sc->done[j]) {
+      if (mode == 0) {
+        const double rho = sc->rho[j];
+        a[t] += rho * c[t];
+        b[t] += rho * e[t];
+      } else {
+        a[t] = c[t] + sc->gamma[j] * a[t];
+      }
     }
-    return;
-  }
-  if (mode == 3) {
-    if (sc->done[j] == 5) a[t] = c[t];
-    return;
-  }
-  if (sc->done[j]) return;
-  if (mode == 0) {
-    const double rho = sc->rho[j];
-    a[t] += rho * c[t];
-    b[t] += rho * e[t];
-  } else {
-    a[t] = c[t] + sc->gamma[j] * a[t];
   }
 }
 
 hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
                             const double* e, const BScalars* sc, const int* gate) {
   const int64_t n = nlam * k;
-  hipLaunchKernelGGL(kb_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nlam, k, mode, a, b, c, e, sc,
-                     gate);
+  hipLaunchKernelGGL(kb_update, dim3((unsigned)std::min<int64_t>((n + 255) / 256, kStrideGrid)), dim3(256), 0, st, nlam,
+                     k, mode, a, b, c, e, sc, gate);
   return hipGetLastError();
 }
 

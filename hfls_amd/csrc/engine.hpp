@@ -179,8 +179,8 @@ class Engine {
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
       b_v6off, b_v6row, b_crrow;
   DVec<int> b_crn;
-  DVec<double> b_CR;
-  DVec<float> b_Kt32, b_St32, b_xb32, b_y32;
+  DVec<double> b_CR, b_Rbr;
+  DVec<float> b_Kt32, b_St32, b_xb32;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;

@@ -123,6 +123,13 @@ void Engine::block_prepare() {
   {
     std::vector<double> Rb, Q(off_Rb[N]), Y;
     d_Rb.download(Rb, st);
+    {  // row-major copy of R_b for the projector's per-row reads
+      std::vector<double> Rr((size_t)off_b[N] * 6);
+      for (int d = 0; d < N; d++)
+        for (int p = 0; p < h_ns[d]; p++)
+          for (int m = 0; m < 6; m++) Rr[(size_t)(off_b[d] + p) * 6 + m] = Rb[off_Rb[d] + p + (int64_t)m * h_ns[d]];
+      b_Rbr.upload(Rr, st);
+    }
     for (int d = 0; d < N; d++) {  // modified Gram-Schmidt per subdomain
       const int n = h_ns[d];
       double* q = Q.data() + off_Rb[d];
@@ -183,7 +190,6 @@ void Engine::block_prepare() {
   HC(launch_kb_deflate_f32(st, N, maxtiles, d_ns.p, d_Kt.p, d_off_P.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, b_Kt32.p));
   HC(launch_kb_to_f32(st, off_P[N], d_St.p, b_St32.p));
   zero_alloc(b_xb32, NBP * KP, st);
-  zero_alloc(b_y32, NBP * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
   zero_alloc(b_v6, (int64_t)64 * N * KP, st);
@@ -208,8 +214,8 @@ void Engine::block_prepare() {
 size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
-  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_y32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+  s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
+  for (const DVec<double>* v : {&b_CR, &b_Rbr, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -223,9 +229,8 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
                       prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0f, gate));
-    HC(launch_kb_widen(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, b_y32.p, b_yb.p, gate));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
@@ -262,8 +267,8 @@ void Engine::bP(int k, const double* x, double* out, double* hbuf, const double*
   HC(launch_tgemm_f64(st, N, 1, 2, 1, d_ns.p, d_sixes.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_xb.p, b_xoff.p, b_v6.p,
                       b_v6off.p, nullptr, KP, 1.0, gate));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0, gate));
-  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, hbuf, x, out, -1.0,
-                  dotv, b_part.p, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_Rbr.p, d_off_b.p, hbuf, x, out, -1.0, dotv,
+                  b_part.p, gate));
 }
 
 // out = W Bf S Bf^T W x (Cluster.cpp:1043-1112)
@@ -272,9 +277,8 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
   HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0f, gate));
-    HC(launch_kb_widen(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, b_y32.p, b_yb.p, gate));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
@@ -345,7 +349,7 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
     if (which == 2)
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
     else
-      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_y32.p,
+      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
                           b_xoff.p, nullptr, KP, 1.0f, nullptr));
   };
   launch();
@@ -427,7 +431,7 @@ void Engine::blk_init(const std::vector<int>& cols) {
                        nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_d.p, nullptr, b_Fw.p, nullptr, b_sc.p));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr, 1.0));
-  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_h.p, nullptr, b_Pg.p,
+  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_Rbr.p, d_off_b.p, b_h.p, nullptr, b_Pg.p,
                   1.0, nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_lam.p, nullptr, b_Pg.p, nullptr, b_sc.p));
   // g0 = F lam0 - d in FP64 ; w0 = P M P g0
@@ -464,7 +468,7 @@ void Engine::blk_sweeps(int n) {
       bM(k, prec, b_Pg.p, b_z.p, pending);
       bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
       HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
-      HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, active));
+      HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, &b_sc.p->restart));
     }
   }
 }

@@ -136,7 +136,7 @@ hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const
 // general tiles, mode 2 compact column major (lda); X, Y, Z row major, ld kp.
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff,
-                            float* Y, const int64_t* yoff, const float* Z, int kp, float alpha, const int* gate);
+                            double* Y, const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate);
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
@@ -156,6 +156,7 @@ struct BScalars {         // per-column PCPG scalars of a block solve
   int cert;                // 1: stopping tests are confirmed on the FP64 true residual
   int pending;             // some column waits for an FP64 true-residual evaluation
   int active;              // some column is still iterating
+  int restart;             // some column restarts from its FP64 true residual
   double colsum[KMAX];     // column sums of the last reduction
 };
 
@@ -184,8 +185,8 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
                              const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
                              const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr);
 hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
-                        const int* rpb, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
-                        const double* x, double* out, double sgn, const double* dotv, double* part, const int* gate = nullptr);
+                        const int* rpb, const double* Rbr, const int64_t* boff, const double* h, const double* x,
+                        double* out, double sgn, const double* dotv, double* part, const int* gate = nullptr);
 hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
                                  const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate = nullptr);
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,

@@ -62,7 +62,7 @@ size_t Server::estimate(const Problem& p) const {
     b += 8.0 * (6.0 * s.n + 6.0 * s.n_s + 4.0 * s.n + 2.0 * s.n_s);       // R, R_b, vectors
     b += 8.0 * (2.0 * align32l(ldb * 36) + 54.0 * ldb);                   // C, Bc^T, [C|R_b|Q|Yh]
     b += 8.0 * (double)T * (T + 1) / 2 * 64;                              // tile-row partials
-    b += (8.0 + 8.0 + 4.0 + 4.0) * 64.0 * 64.0 * T;                       // b-blocks
+    b += (8.0 + 8.0 + 4.0) * 64.0 * 64.0 * T;                             // b-blocks
     b += 8.0 * 64.0 * (2.0 * ((s.n_r + 63) / 64 * 64) + 64.0 * 3);        // interior, cr/coef/v6 blocks
   }
   b += 8.0 * (8.0 * nl + 9.0 * 64.0 * nl);                                // dual vectors

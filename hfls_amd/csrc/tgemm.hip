@@ -1,5 +1,6 @@
 // Batched tile GEMM on the matrix cores (gfx950 MFMA) for blocks of right-hand
-// sides: Y_d = alpha * op(A_d) X_d (+ Z_d), one matrix per subdomain.
+// sides: Y_d = alpha * op(A_d) X_d (+ Z_d), one matrix per subdomain.  The FP32
+// kernel accumulates in FP32 and stores FP64 (Y, Z double).
 //
 // A_d is stored as 64x64 tiles, column major inside a tile, zero padded:
 //   mode 0 symmetric: lower tiles only, tile (i,j), i>=j, at j*T - j*(j-1)/2 + (i-j);
@@ -189,8 +190,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
                                                    const int* __restrict__ mcols, const float* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const int* __restrict__ lda,
                                                    const float* __restrict__ X,
-                                                   const int64_t* __restrict__ xoff, float* __restrict__ Y,
-                                                   const int64_t* __restrict__ yoff, const float* __restrict__ Z,
+                                                   const int64_t* __restrict__ xoff, double* __restrict__ Y,
+                                                   const int64_t* __restrict__ yoff, const double* __restrict__ Z,
                                                    int kp, float alpha, const int* __restrict__ gate) {
   const int d = blockIdx.y;
   if (gate && *gate == 0) return;
@@ -236,13 +237,13 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
     if (more) stash_f32(kind_n, s, As, Xs);
     __syncthreads();
   }
-  float* Yd = Y + yoff[d] + c0;
-  const float* Zd = Z ? Z + yoff[d] + c0 : nullptr;
+  double* Yd = Y + yoff[d] + c0;
+  const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int row = i * 64 + rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     const int col = cb + (lane & 31);
-    float v = alpha * acc[r];
+    double v = (double)(alpha * acc[r]);
     if (Zd) v += Zd[(int64_t)row * kp + col];
     Yd[(int64_t)row * kp + col] = v;
   }
@@ -323,8 +324,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
 }
 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
-                            const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, float* Y,
-                            const int64_t* yoff, const float* Z, int kp, float alpha, const int* gate) {
+                            const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, double* Y,
+                            const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate) {
   if (kp % 64 || mode == 2) return hipErrorInvalidValue;  // FP32: tiles only
   hipLaunchKernelGGL(k_tgemm_f32, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff, lda, X,
                      xoff, Y, yoff, Z, kp, alpha, gate);
