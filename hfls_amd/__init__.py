@@ -22,7 +22,7 @@ EXPORTS = [
     "hfls_precondition", "hfls_kplus", "hfls_solve", "hfls_time_stream", "hfls_solve_block",
     "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction", "hfls_server_create",
     "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
-    "hfls_server_result", "hfls_server_stats",
+    "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
 ]
 
 DEFAULTS = {
@@ -35,7 +35,7 @@ DEFAULTS = {
     "eps_iter": "1e-6", "max_iter": "200", "Ac_extended_by_kerGc": "true",
     "slabs_x": "", "slabs_y": "", "slabs_z": "",
     "linear_solver": "hip", "vtkWithinIter": "false", "print_matrices": "0",
-    "precision": "fp64", "nrhs": "1", "device": "0", "refine_tol": "1e-3",
+    "precision": "fp64", "nrhs": "1", "device": "0", "refine_tol": "1e-4",
 }
 
 
@@ -191,6 +191,13 @@ class Mesh:
 
     def __del__(self):
         self.close()
+
+    def configure(self, **settings):
+        """Runtime settings: hip_graph=True/False, refine_tol=..., poll_chunk=..."""
+        for k, v in settings.items():
+            if isinstance(v, bool):
+                v = "1" if v else "0"
+            _check(load().hfls_mesh_configure(self._h, k.encode(), str(v).encode()))
 
     def kernel(self):
         R = np.zeros(6 * self.npacked)

@@ -379,3 +379,10 @@ extern "C" int hfls_server_stats(hfls_server s, double* out) {
     out[7] = t.setup_ms;
   });
 }
+
+extern "C" int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value) {
+  return guard([&] {
+    if (!key || !value) throw std::runtime_error("null setting");
+    m->e->configure(key, value);
+  });
+}

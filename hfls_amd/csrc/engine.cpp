@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <stdexcept>
 
@@ -231,7 +232,55 @@ Engine::Engine(const Options& o, int dev) : device(dev) {
 }
 
 Engine::~Engine() {
+  if (gx_plain) (void)hipGraphExecDestroy(gx_plain);
+  if (gx_blk) (void)hipGraphExecDestroy(gx_blk);
   if (st) (void)hipStreamDestroy(st);
+}
+
+// Enqueue work on the library stream, through a captured HIP graph when graphs
+// are on: the first call records the launches into a graph, later calls replay
+// it.  Captured work contains kernel launches only (no memset or copy nodes) and
+// every kernel reads its state from device memory, so a replay is valid in any
+// state of the solve.
+void Engine::run_graph(hipGraphExec_t* exec, const std::function<void()>& enqueue) {
+  if (!use_graph) {
+    enqueue();
+    return;
+  }
+  if (!*exec) {
+    hipGraph_t g = nullptr;
+    HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue();
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(st, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    HC(hipStreamEndCapture(st, &g));
+    HC(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    HC(hipGraphDestroy(g));
+  }
+  HC(hipGraphLaunch(*exec, st));
+}
+
+void Engine::configure(const std::string& key, const std::string& value) {
+  if (key == "hip_graph") {
+    use_graph = value == "1" || value == "true";
+  } else if (key == "refine_tol") {
+    P->o["refine_tol"] = value;
+  } else if (key == "poll_chunk") {
+    const int c = std::atoi(value.c_str());
+    if (c < 1 || c > 64) throw std::runtime_error("poll_chunk must be 1..64");
+    poll_chunk = c;
+  } else {
+    throw std::runtime_error("unknown runtime setting " + key);
+  }
+  if (gx_plain) (void)hipGraphExecDestroy(gx_plain);
+  if (gx_blk) (void)hipGraphExecDestroy(gx_blk);
+  gx_plain = gx_blk = nullptr;
+  gx_blk_key = -1;
 }
 
 void Engine::build_meta() {
@@ -1120,19 +1169,26 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   HC(hipMemcpyAsync(w.p, Pz.p, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
   HC(hipEventRecord(e1, st));
   Scalars sh;
-  const int chunk = 8;
+  const int chunk = poll_chunk;
   int launched = 0;
+  if (gx_plain && gx_plain_n != chunk) {
+    (void)hipGraphExecDestroy(gx_plain);
+    gx_plain = nullptr;
+  }
+  gx_plain_n = chunk;
   for (;;) {
-    for (int k = 0; k < chunk; k++) {
-      F_apply(w.p, Fw.p, nullptr, w.p, done);
-      HC(launch_reduce(st, nparts, part.p, sc.p, 0));
-      HC(launch_axpy2(st, nl, lam.p, g.p, w.p, Fw.p, sc.p));
-      proj(g.p, Pg.p, hg.p, alpha.p, nullptr, done);
-      M_apply(Pg.p, z.p, done);
-      proj(z.p, Pz.p, h.p, nullptr, g.p, done);
-      HC(launch_reduce(st, nparts, part.p, sc.p, 1));
-      HC(launch_update_w(st, nl, w.p, Pz.p, sc.p));
-    }
+    run_graph(&gx_plain, [&] {
+      for (int k = 0; k < chunk; k++) {
+        F_apply(w.p, Fw.p, nullptr, w.p, done);
+        HC(launch_reduce(st, nparts, part.p, sc.p, 0));
+        HC(launch_axpy2(st, nl, lam.p, g.p, w.p, Fw.p, sc.p));
+        proj(g.p, Pg.p, hg.p, alpha.p, nullptr, done);
+        M_apply(Pg.p, z.p, done);
+        proj(z.p, Pz.p, h.p, nullptr, g.p, done);
+        HC(launch_reduce(st, nparts, part.p, sc.p, 1));
+        HC(launch_update_w(st, nl, w.p, Pz.p, sc.p));
+      }
+    });
     launched += chunk;
     HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
     HC(hipStreamSynchronize(st));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -122,6 +123,10 @@ class Engine {
   void blk_recover();
   void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
+  // runtime settings: hip_graph (1/0: capture and replay the iteration chunks),
+  // refine_tol (mixed-precision refinement factor), poll_chunk (iterations
+  // between host polls)
+  void configure(const std::string& key, const std::string& value);
   hipStream_t stream() const { return st; }
 
   // Accessors for tests: per-subdomain results of the setup (host copies).
@@ -171,9 +176,16 @@ class Engine {
   std::vector<int> h_fix, h_def;
 
   // ---- block state (allocated on the first block call)
+  bool use_graph = true;
+  int poll_chunk = 8;
+  hipGraphExec_t gx_plain = nullptr, gx_blk = nullptr;
+  int gx_plain_n = -1;
+  int64_t gx_blk_key = -1;
+  void run_graph(hipGraphExec_t* exec, const std::function<void()>& enqueue);
   bool blk_ready = false;
   int blk_k = 0, blk_prec = 64;
   void blk_set_status(const std::vector<int>& slots, int status);
+  void blk_enqueue_sweeps(int n);
   int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,

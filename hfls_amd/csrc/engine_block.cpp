@@ -446,6 +446,17 @@ void Engine::blk_init(const std::vector<int>& cols) {
 
 void Engine::blk_sweeps(int n) {
   const int k = blk_k, prec = blk_prec;
+  const int64_t key = ((int64_t)n * 128 + k) * 2 + (prec == 32 ? 1 : 0);
+  if (gx_blk && gx_blk_key != key) {
+    (void)hipGraphExecDestroy(gx_blk);
+    gx_blk = nullptr;
+  }
+  gx_blk_key = key;
+  run_graph(&gx_blk, [&] { blk_enqueue_sweeps(n); });
+}
+
+void Engine::blk_enqueue_sweeps(int n) {
+  const int k = blk_k, prec = blk_prec;
   const int64_t nl = P->nlam;
   const int* pending = &b_sc.p->pending;
   const int* active = &b_sc.p->active;
@@ -531,7 +542,7 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
   HC(hipEventRecord(ev[0], st));
   blk_init(cols);
   HC(hipEventRecord(ev[1], st));
-  const int chunk = 8;
+  const int chunk = poll_chunk;
   int sweeps = 0;
   BScalars sh;
   for (;;) {

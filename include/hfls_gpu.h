@@ -69,6 +69,12 @@ int hfls_mesh_setup_times(hfls_mesh m, double* t);
 int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect);
 int hfls_mesh_load(hfls_mesh m, double* f);
 
+/* runtime settings of a set-up mesh (no counterpart in the reference):
+ * "hip_graph" "1"/"0": replay the PCPG iteration chunks from captured HIP graphs
+ * (default on); "refine_tol": mixed-precision refinement factor (default 1e-4);
+ * "poll_chunk": iterations between host polls of the stopping flags (default 8) */
+int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value);
+
 /* operators: Cluster::mult_Ff (Cluster.cpp:1114-1188), Projection (1314-1329,
  * alpha as returned there), Preconditioning (1043-1112), mult_Kplus_f (967-1040) */
 int hfls_apply_F(hfls_mesh m, const double* w, double* Fw);

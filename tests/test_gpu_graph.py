@@ -1,0 +1,46 @@
+"""Captured HIP graphs of the PCPG iteration chunks (plain and block paths).
+
+Every replay must reproduce the eager launches bit for bit: each solve below
+replays the captured chunk several times, with the eager initialisation,
+recovery and host synchronisations between the replays, and is compared with the
+same solve run without graphs.  The block results are also independent of the
+host poll interval (a stopped column is frozen on the device).
+"""
+import numpy as np
+import pytest
+
+import hfls_amd
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "cfg0": dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+                 load="body"),
+    "uneven": dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slabs_x="3,5,4,6", slabs_y="4,3", slabs_z="5,3",
+                   bc="clamp_x0", material="lognormal", young_modulus="2.1e5", seed="9", mat_sigma="1.5",
+                   mat_contrast="1e3", load="body"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_graph_replays_match_eager(name):
+    kw = CASES[name]
+    print("graph case", name, flush=True)
+    m = hfls_amd.Mesh(**kw)
+    F = m.traction(77, 0, 5)
+    m.configure(hip_graph=False)
+    p0 = m.pcpg(eps=1e-8, max_iter=500)
+    b0 = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32, want_lambda=True)
+    assert p0["converged"] and np.all(b0["status"] == 2)
+    m.configure(hip_graph=True)
+    for rep in range(3):
+        p = m.pcpg(eps=1e-8, max_iter=500)
+        assert p["iterations"] == p0["iterations"]
+        assert np.array_equal(p["u"], p0["u"]) and np.array_equal(p["lambda"], p0["lambda"])
+        b = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32, want_lambda=True)
+        assert np.array_equal(b["iterations"], b0["iterations"])
+        assert np.array_equal(b["u"], b0["u"]) and np.array_equal(b["lambda"], b0["lambda"])
+    m.configure(poll_chunk=3)
+    b = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    assert np.array_equal(b["u"], b0["u"]) and np.array_equal(b["iterations"], b0["iterations"])
+    m.close()
