@@ -24,7 +24,7 @@ Options default_options() {
       {"eps_iter", "1e-6"}, {"max_iter", "200"}, {"Ac_extended_by_kerGc", "true"},
       {"slabs_x", ""}, {"slabs_y", ""}, {"slabs_z", ""},
       {"linear_solver", "hip"}, {"vtkWithinIter", "false"}, {"print_matrices", "0"},
-      {"precision", "fp64"}, {"nrhs", "1"}, {"device", "0"}, {"refine_tol", "1e-4"}};
+      {"precision", "fp64"}, {"nrhs", "1"}, {"device", "0"}, {"refine_tol", "1e-4"}, {"slab_choices", ""}};
 }
 
 std::string opt(const Options& o, const std::string& k) {
@@ -55,11 +55,27 @@ static std::vector<int> parse_slabs(const Options& o, char axis) {
   int C = opti(o, std::string("C") + a), N = opti(o, std::string("N") + a);
   std::string s = opt(o, "slabs_" + a);
   std::vector<int> v;
+  const std::string choices = opt(o, "slab_choices");
   if (!s.empty()) {
     std::stringstream ss(s);
     std::string t;
     while (std::getline(ss, t, ',')) v.push_back(std::atoi(t.c_str()));
     if ((int)v.size() != C * N) throw std::runtime_error("slabs_" + a + " needs C*N entries");
+  } else if (!choices.empty()) {
+    // seeded draw of every slab's element count from the list (uneven batching):
+    // slab i of axis a takes choices[floor(u * #choices)], u from a splitmix64
+    // stream seeded with seed ^ (0xC2B2AE3D27D4EB4F * (a + 1)), a = 0, 1, 2 for x, y, z
+    std::vector<int> ch;
+    std::stringstream ss(choices);
+    std::string t;
+    while (std::getline(ss, t, ',')) ch.push_back(std::atoi(t.c_str()));
+    if (ch.empty()) throw std::runtime_error("slab_choices is empty");
+    const uint64_t ax = (uint64_t)(axis - 'x') + 1;
+    uint64_t st = (uint64_t)std::strtoull(opt(o, "seed").c_str(), nullptr, 10) ^ (0xC2B2AE3D27D4EB4FULL * ax);
+    for (int i = 0; i < C * N; i++) {
+      const int idx = (int)(uniform01(st) * (double)ch.size());
+      v.push_back(ch[std::min(idx, (int)ch.size() - 1)]);
+    }
   } else {
     v.assign(C * N, opti(o, std::string("n") + a));
   }

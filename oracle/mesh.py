@@ -26,7 +26,7 @@ DEFAULTS = {
     "bc": "symmetry", "load": "reference", "body_force": "-1.0", "density": "2.0",
     "typeBc": "ker", "Bc_fullRank": "true", "preconditioner": "Dirichlet_explicit",
     "eps_iter": "1e-6", "max_iter": "200", "Ac_extended_by_kerGc": "true",
-    "slabs_x": "", "slabs_y": "", "slabs_z": "",
+    "slabs_x": "", "slabs_y": "", "slabs_z": "", "slab_choices": "",
 }
 
 
@@ -49,6 +49,14 @@ def _slabs(o, axis):
         v = [int(t) for t in s.split(",")]
         assert len(v) == C * N, "slabs_%s needs %d entries" % (axis, C * N)
         return v
+    ch = o.get("slab_choices", "")
+    if ch:
+        # seeded draw from the list (uneven batching): choices[floor(u * len)], u from a
+        # splitmix64 stream seeded with seed ^ (0xC2B2AE3D27D4EB4F * (axis index + 1))
+        ch = [int(t) for t in ch.split(",")]
+        ax = "xyz".index(axis) + 1
+        u = uniforms((int(o["seed"]) ^ ((0xC2B2AE3D27D4EB4F * ax) & MASK64)) & MASK64, C * N)
+        return [ch[min(int(x * len(ch)), len(ch) - 1)] for x in u]
     return [int(o["n" + axis])] * (C * N)
 
 

@@ -27,7 +27,9 @@ CFG0 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform
             young_modulus="2.1e5", load="body", body_force="-1.0")
 MULTI = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, nx=3, ny=3, nz=4, bc="clamp_x0", material="lognormal",
              young_modulus="2.1e5", seed="3", mat_sigma="0.5", load="body")
-CASES = {"cfg0": CFG0, "multi": MULTI}
+UNEVEN = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", bc="clamp_x0", material="lognormal",
+              young_modulus="2.1e5", seed="5", mat_sigma="1.5", mat_contrast="1e3", load="body")
+CASES = {"cfg0": CFG0, "multi": MULTI, "uneven": UNEVEN}
 SEED = 20240611
 K = 5
 

@@ -39,6 +39,9 @@ CASES = [
     dict(Cx=2, Cy=1, Cz=2, Nx=2, Ny=2, Nz=1, nx=3, ny=2, nz=2, bc="clamp_x0"),
     dict(Cx=1, Cy=2, Cz=1, Nx=2, Ny=1, Nz=2, slabs_x="2,4", slabs_y="3,1", slabs_z="2,3",
          material="lognormal", young_modulus="2.1e5", seed="7", bc="clamp_x0"),
+    # uneven batching: per-axis slab sizes drawn from a list with the seed
+    dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", material="lognormal", young_modulus="2.1e5",
+         seed="5", mat_sigma="1.5", bc="clamp_x0"),
 ]
 
 
@@ -100,3 +103,21 @@ def test_traction_loads_bit_exact(kw):
     Fo = traction_loads(m, s, 987654321, [2, 3, 4])
     for j in range(3):
         assert np.array_equal(F[j], Fo[j])
+
+
+def test_slab_draw_uses_only_the_choices_and_all_of_them():
+    """The seeded slab draw (uneven batching) picks every listed size and nothing
+    else, roughly uniformly, and the product draws exactly the oracle's sizes."""
+    from oracle.mesh import _slabs
+    o = oopts(Cx=8, Cy=8, Cz=8, Nx=8, Ny=8, Nz=8, slab_choices="6,8,10,12,14", seed="20240611")
+    for ax in "xyz":
+        v = _slabs(o, ax)
+        assert len(v) == 64 and set(v) == {6, 8, 10, 12, 14}
+        counts = np.bincount(v)[[6, 8, 10, 12, 14]]
+        assert counts.min() >= 4 and counts.max() <= 24
+    kw = dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4, slab_choices="6,8,10,12,14", seed="20240611")
+    m = OMesh(oopts(**kw))
+    p = hfls_amd.Problem(**kw)
+    assert p.nPoints == m.nPoints
+    for d in range(0, m.nsub, 37):
+        assert p.sub_info(d)["n"] == 3 * len(np.unique(m.elements[m.part == d]))
