@@ -251,6 +251,7 @@ def main():
     # roofline of the dominant kernel: the FP32 tile GEMM over 64 columns
     g_ms, g_flops = mesh.time_stream(2, reps=10)
     s_ms, s_flops = mesh.time_stream(3, reps=10)
+    tune = mesh.tuning()
     achieved = g_flops / (g_ms * 1e-3) / 1e12
     lat = np.array(blk_ms)
     out = {
@@ -284,7 +285,9 @@ def main():
                      "traffic": GEMM_TRAFFIC_BYTES if (a.elements == 10 and not a.small) else None,
                      "kernel": "k_tgemm_f32 (deflated K+_bb, packed symmetric 64x64 tiles, 64 columns)",
                      "kernel_ms": g_ms, "flops_per_launch": g_flops,
-                     "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12},
+                     "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12,
+                     "autotune": {"row_tiles_per_workgroup": tune["row_tiles"],
+                                  "candidate_ms": [tune["ms"][1], tune["ms"][2]]}},
     }
     mesh.close()
     if a.serve_requests > 0:

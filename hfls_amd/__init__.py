@@ -23,6 +23,7 @@ EXPORTS = [
     "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction", "hfls_server_create",
     "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
+    "hfls_mesh_tuning",
 ]
 
 DEFAULTS = {
@@ -198,6 +199,13 @@ class Mesh:
             if isinstance(v, bool):
                 v = "1" if v else "0"
             _check(load().hfls_mesh_configure(self._h, k.encode(), str(v).encode()))
+
+    def tuning(self):
+        """Autotuner result of the block GEMM: chosen row tiles per workgroup and the
+        measured ms of the candidates (after the first block call)."""
+        out = np.zeros(3)
+        _check(load().hfls_mesh_tuning(self._h, _ptr(out)))
+        return {"row_tiles": int(out[0]), "ms": {1: float(out[1]), 2: float(out[2])}}
 
     def kernel(self):
         R = np.zeros(6 * self.npacked)

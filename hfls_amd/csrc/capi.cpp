@@ -386,3 +386,11 @@ extern "C" int hfls_mesh_configure(hfls_mesh m, const char* key, const char* val
     m->e->configure(key, value);
   });
 }
+
+extern "C" int hfls_mesh_tuning(hfls_mesh m, double* out) {
+  return guard([&] {
+    out[0] = m->e->tuned_rows();
+    out[1] = m->e->tuned_ms(1);
+    out[2] = m->e->tuned_ms(2);
+  });
+}

@@ -270,6 +270,15 @@ void Engine::configure(const std::string& key, const std::string& value) {
     use_graph = value == "1" || value == "true";
   } else if (key == "refine_tol") {
     P->o["refine_tol"] = value;
+  } else if (key == "gemm_rows") {  // override the autotuner: 1, 2 or auto
+    if (value == "auto") {
+      gemm_rows_forced = 0;
+    } else {
+      const int r = std::atoi(value.c_str());
+      if (r != 1 && r != 2) throw std::runtime_error("gemm_rows must be 1, 2 or auto");
+      gemm_rows_forced = r;
+    }
+    if (blk_ready) tune_rt32 = gemm_rows_forced ? gemm_rows_forced : (tune_ms32[1] < tune_ms32[0] ? 2 : 1);
   } else if (key == "poll_chunk") {
     const int c = std::atoi(value.c_str());
     if (c < 1 || c > 64) throw std::runtime_error("poll_chunk must be 1..64");

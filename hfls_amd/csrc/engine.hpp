@@ -123,9 +123,13 @@ class Engine {
   void blk_recover();
   void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
+  // autotuner result: output row tiles per workgroup of the FP32 block GEMM and
+  // the measured ms of each candidate (1, 2)
+  int tuned_rows() const { return tune_rt32; }
+  double tuned_ms(int rt) const { return tune_ms32[rt - 1]; }
   // runtime settings: hip_graph (1/0: capture and replay the iteration chunks),
   // refine_tol (mixed-precision refinement factor), poll_chunk (iterations
-  // between host polls)
+  // between host polls), gemm_rows (1, 2 or auto: block GEMM tile shape)
   void configure(const std::string& key, const std::string& value);
   hipStream_t stream() const { return st; }
 
@@ -177,6 +181,8 @@ class Engine {
 
   // ---- block state (allocated on the first block call)
   bool use_graph = true;
+  int tune_rt32 = 1, gemm_rows_forced = 0;
+  double tune_ms32[2] = {0, 0};
   int poll_chunk = 8;
   hipGraphExec_t gx_plain = nullptr, gx_blk = nullptr;
   int gx_plain_n = -1;
@@ -186,6 +192,7 @@ class Engine {
   int blk_k = 0, blk_prec = 64;
   void blk_set_status(const std::vector<int>& slots, int status);
   void blk_enqueue_sweeps(int n);
+  void autotune();
   int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,

@@ -208,7 +208,35 @@ void Engine::block_prepare() {
   zero_alloc(b_U, ng * KMAX, st);
   b_sc.alloc(1);
   HC(hipStreamSynchronize(st));
+  autotune();
   blk_ready = true;
+}
+
+// Autotuner of the block GEMM tile shape: every candidate (64 or 128 output rows
+// per workgroup) is timed on this mesh's own FP32 K+_bb tiles over a 64-column
+// block with HIP events, and the fastest is used for K+_bb and S_i from then on.
+void Engine::autotune() {
+  const int N = P->nsub;
+  hipEvent_t a, b;
+  HC(hipEventCreate(&a));
+  HC(hipEventCreate(&b));
+  for (int rt = 1; rt <= 2; rt++) {
+    auto launch = [&] {
+      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p,
+                          b_yb.p, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt));
+    };
+    launch();
+    HC(hipEventRecord(a, st));
+    for (int r = 0; r < 3; r++) launch();
+    HC(hipEventRecord(b, st));
+    HC(hipEventSynchronize(b));
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, a, b));
+    tune_ms32[rt - 1] = ms / 3;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  tune_rt32 = gemm_rows_forced ? gemm_rows_forced : (tune_ms32[1] < tune_ms32[0] ? 2 : 1);
 }
 
 size_t Engine::block_bytes() const {
@@ -230,7 +258,7 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
                       prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0f, gate));
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
@@ -278,7 +306,7 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0f, gate));
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
@@ -350,7 +378,7 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
     else
       HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                          b_xoff.p, nullptr, KP, 1.0f, nullptr));
+                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32));
   };
   launch();
   hipEvent_t a, b;

@@ -136,7 +136,8 @@ hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const
 // general tiles, mode 2 compact column major (lda); X, Y, Z row major, ld kp.
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff,
-                            double* Y, const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate);
+                            double* Y, const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate,
+                            int rt = 1);  // rt: 64-row output tiles per workgroup (1 or 2)
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,

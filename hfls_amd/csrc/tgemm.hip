@@ -186,66 +186,88 @@ static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i
   return tr ? 1 : 0;
 }
 
+// RT row tiles (64 rows each) per workgroup share the X chunk in LDS: RT = 2
+// halves the X staging and LDS reads per matrix-core operation (chosen per mesh
+// by the autotuner of the block solve)
+template <int RT>
 __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const float* __restrict__ A,
-                                                   const int64_t* __restrict__ aoff, const int* __restrict__ lda,
-                                                   const float* __restrict__ X,
+                                                   const int64_t* __restrict__ aoff, const float* __restrict__ X,
                                                    const int64_t* __restrict__ xoff, double* __restrict__ Y,
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
                                                    int kp, float alpha, const int* __restrict__ gate) {
   const int d = blockIdx.y;
   if (gate && *gate == 0) return;
   const int Ra = (mrows[d] + 63) / 64, Ca = (mcols[d] + 63) / 64;
-  const int Ro = (mode != 0 && op == 1) ? Ca : Ra;
-  const int Ko = (mode != 0 && op == 1) ? Ra : Ca;
-  const int i = blockIdx.x;
-  if (i >= Ro) return;
+  const int Ro = op == 1 && mode != 0 ? Ca : Ra;
+  const int Ko = op == 1 && mode != 0 ? Ra : Ca;
+  const int i0 = blockIdx.x * RT;
+  if (i0 >= Ro) return;
+  const int nrt = min(RT, Ro - i0);
   const int c0 = blockIdx.z * 64;
-  __shared__ float As[32 * 65];
+  __shared__ float As[RT][32 * 65];
   __shared__ float Xs[32 * 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int rb = (w >> 1) * 32, cb = (w & 1) * 32;
-  floatx16 acc;
+  floatx16 acc[RT];
 #pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+  for (int q = 0; q < RT; q++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[q][r] = 0.0f;
   const float* Ad = A + aoff[d];
   const float* Xd = X + xoff[d] + c0;
   const int nch = 2 * Ko;
-  StageF s;
+  StageF s[RT];
+  int kind[RT];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < RT; q++) {
+      if (q < nrt) {
+        int64_t ti;
+        kind[q] = chunk_kind(mode, op, Ra, i0 + q, c, &ti);
+        fetch_f32(Ad + ti * 4096, kind[q], (c & 1) * 32, Xd + (int64_t)((c >> 1) * 64) * kp, kp, s[q]);
+      }
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < RT; q++)
+      if (q < nrt) stash_f32(kind[q], s[q], As[q], Xs);
+  };
   if (nch > 0) {
-    int64_t ti;
-    const int kind = chunk_kind(mode, op, Ra, i, 0, &ti);
-    fetch_f32(Ad + ti * 4096, kind, 0, Xd, kp, s);
-    stash_f32(kind, s, As, Xs);
+    fetch(0);
+    stash();
   }
   __syncthreads();
   for (int c = 0; c < nch; c++) {
     const bool more = c + 1 < nch;
-    int kind_n = 0;
-    if (more) {
-      int64_t ti;
-      kind_n = chunk_kind(mode, op, Ra, i, c + 1, &ti);
-      fetch_f32(Ad + ti * 4096, kind_n, ((c + 1) & 1) * 32, Xd + (int64_t)(((c + 1) >> 1) * 64) * kp, kp, s);
-    }
+    if (more) fetch(c + 1);
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 2) {
-      const float a = As[(kk + (lane >> 5)) * 65 + rb + (lane & 31)];
       const float b = Xs[(kk + (lane >> 5)) * 64 + cb + (lane & 31)];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < RT; q++) {
+        const float a = As[q][(kk + (lane >> 5)) * 65 + rb + (lane & 31)];
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+      }
     }
     __syncthreads();
-    if (more) stash_f32(kind_n, s, As, Xs);
+    if (more) stash();
     __syncthreads();
   }
   double* Yd = Y + yoff[d] + c0;
   const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int row = i * 64 + rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const int col = cb + (lane & 31);
-    double v = (double)(alpha * acc[r]);
-    if (Zd) v += Zd[(int64_t)row * kp + col];
-    Yd[(int64_t)row * kp + col] = v;
+  for (int q = 0; q < RT; q++) {
+    if (q >= nrt) break;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int row = (i0 + q) * 64 + rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int col = cb + (lane & 31);
+      double v = (double)(alpha * acc[q][r]);
+      if (Zd) v += Zd[(int64_t)row * kp + col];
+      Yd[(int64_t)row * kp + col] = v;
+    }
   }
 }
 
@@ -325,10 +347,15 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, double* Y,
-                            const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate) {
+                            const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt) {
+  (void)lda;
   if (kp % 64 || mode == 2) return hipErrorInvalidValue;  // FP32: tiles only
-  hipLaunchKernelGGL(k_tgemm_f32, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff, lda, X,
-                     xoff, Y, yoff, Z, kp, alpha, gate);
+  if (rt == 2)
+    hipLaunchKernelGGL(k_tgemm_f32<2>, dim3((maxRo + 1) / 2, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols,
+                       A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
+  else
+    hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
+                       X, xoff, Y, yoff, Z, kp, alpha, gate);
   return hipGetLastError();
 }
 

@@ -72,8 +72,14 @@ int hfls_mesh_load(hfls_mesh m, double* f);
 /* runtime settings of a set-up mesh (no counterpart in the reference):
  * "hip_graph" "1"/"0": replay the PCPG iteration chunks from captured HIP graphs
  * (default on); "refine_tol": mixed-precision refinement factor (default 1e-4);
- * "poll_chunk": iterations between host polls of the stopping flags (default 8) */
+ * "poll_chunk": iterations between host polls of the stopping flags (default 8);
+ * "gemm_rows": "1", "2" or "auto" (default): output row tiles per workgroup of the
+ * FP32 block GEMM, overriding the autotuner */
 int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value);
+/* autotuner result (after the first block call): [0] output row tiles per
+ * workgroup chosen for the FP32 block GEMM, [1], [2] measured ms per launch of
+ * the candidates with 1 and 2 row tiles */
+int hfls_mesh_tuning(hfls_mesh m, double* out);
 
 /* operators: Cluster::mult_Ff (Cluster.cpp:1114-1188), Projection (1314-1329,
  * alpha as returned there), Preconditioning (1043-1112), mult_Kplus_f (967-1040) */

@@ -44,3 +44,21 @@ def test_graph_replays_match_eager(name):
     b = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     assert np.array_equal(b["u"], b0["u"]) and np.array_equal(b["iterations"], b0["iterations"])
     m.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_autotuned_gemm_shapes_agree(name):
+    """The autotuner times both block-GEMM tile shapes on the mesh and picks one;
+    either shape gives bit-identical solves."""
+    m = hfls_amd.Mesh(**CASES[name])
+    F = m.traction(78, 0, 3)
+    r_auto = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    t = m.tuning()
+    print(name, "autotune", t, flush=True)
+    assert t["row_tiles"] in (1, 2) and t["ms"][1] > 0 and t["ms"][2] > 0
+    out = {}
+    for rows in (1, 2):
+        m.configure(gemm_rows=rows)
+        out[rows] = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    assert np.array_equal(out[1]["u"], out[2]["u"]) and np.array_equal(out[1]["u"], r_auto["u"])
+    m.close()
