@@ -5,7 +5,7 @@
 //                interior-first matrix, lower triangle, 8-colour element order so
 //                no two waves touch one entry (deterministic, no atomics)
 //   k_panel      Cholesky of a <=64 wide diagonal block + TRSM of the rows below
-//   k_syrk       trailing update of a right-looking partial Cholesky
+//   k_syrk       trailing update of a right-looking partial Cholesky (matrix cores)
 //   k_detect     kernel detection on the Schur complement of the s-set: cyclic
 //                Jacobi, eigen-jump test (Matrix.cpp:2352-2368)
 //   k_coldot / k_bwd / k_fwd   triangular solves with a few right-hand sides
@@ -260,6 +260,13 @@ hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* 
 }
 
 // ============================================================ SYRK trailing update
+// One workgroup per 64x64 lower tile (ti, tj) of the trailing block: C -= P_i P_j^T
+// with the panel rows P (64 x w, w <= 64) staged in LDS ([k][row], padded rows)
+// and the product on the matrix cores (v_mfma_f64_16x16x4f64; wave v owns rows
+// 16v..16v+15 and the four 16-column blocks; lane layouts A[l%16][l/16],
+// B[l/16][l%16], D row 4r+l/16, col l%16).
+typedef double syrk_d4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_syrk(const PanelDesc* __restrict__ desc, const int64_t* __restrict__ moff,
                                               const int* __restrict__ ld, double* __restrict__ M) {
   const int mi = blockIdx.y;
@@ -278,39 +285,38 @@ __global__ __launch_bounds__(256) void k_syrk(const PanelDesc* __restrict__ desc
   const int ri = t0 + 64 * ti, cj = t0 + 64 * tj;
   double* A = M + moff[mi];
   const int64_t lda = ld[mi];
-  __shared__ double As[64 * 64];
-  __shared__ double Bs[64 * 64];
-  for (int idx = threadIdx.x; idx < 64 * w; idx += 256) {
+  __shared__ double As[64 * 65];
+  __shared__ double Bs[64 * 65];
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
     const int k = idx >> 6, r = idx & 63;
-    As[idx] = (ri + r < end) ? A[(ri + r) + (int64_t)(k0 + k) * lda] : 0.0;
-    Bs[idx] = (cj + r < end) ? A[(cj + r) + (int64_t)(k0 + k) * lda] : 0.0;
+    const bool kin = k < w;
+    As[k * 65 + r] = (kin && ri + r < end) ? A[(ri + r) + (int64_t)(k0 + k) * lda] : 0.0;
+    Bs[k * 65 + r] = (kin && cj + r < end) ? A[(cj + r) + (int64_t)(k0 + k) * lda] : 0.0;
   }
   __syncthreads();
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  const int rb = 16 * v;
+  syrk_d4 acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int c = 0; c < 4; c++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
-  for (int k = 0; k < w; k++) {
-    double av[4], bv[4];
+    for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
+  const int kw = (w + 3) & ~3;
+  for (int kk = 0; kk < kw; kk += 4) {
+    const double a = As[(kk + (lane >> 4)) * 65 + rb + (lane & 15)];
 #pragma unroll
-    for (int a = 0; a < 4; a++) av[a] = As[k * 64 + tx + 16 * a];
-#pragma unroll
-    for (int b = 0; b < 4; b++) bv[b] = Bs[k * 64 + ty + 16 * b];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) acc[a][b] += av[a] * bv[b];
+    for (int c = 0; c < 4; c++) {
+      const double b = Bs[(kk + (lane >> 4)) * 65 + 16 * c + (lane & 15)];
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+    }
   }
 #pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const int col = cj + ty + 16 * b;
-    if (col >= end) continue;
+  for (int c = 0; c < 4; c++) {
+    const int col = cj + 16 * c + (lane & 15);
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int row = ri + tx + 16 * a;
-      if (row < end && row >= col) A[row + (int64_t)col * lda] -= acc[a][b];
+    for (int r = 0; r < 4; r++) {
+      const int row = ri + rb + 4 * r + (lane >> 4);
+      if (row < end && col < end && row >= col) A[row + (int64_t)col * lda] -= acc[c][r];
     }
   }
 }
