@@ -23,39 +23,58 @@ static __device__ __forceinline__ double wsum64(double v) {
   return v;
 }
 
-// xb[(obrow[d] + p)*kp + j] = sum_e sign_e * wgt[r_e] * v[r_e*k + j]; optional FP32 copy
+// xb[(obrow[d] + p)*kp + j] = sum_e sign_e * wgt[r_e] * v[r_e*k + j]; optional FP32 copy.
+// A wave takes GQ consecutive b positions per step (lanes = columns), issuing all
+// their loads before any store; the grid is capped so a gated launch is cheap.
+constexpr int GQ = 4;
 __global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, const int* __restrict__ bsub,
                                                  const int64_t* __restrict__ boff, const int64_t* __restrict__ obrow,
                                                  const int* __restrict__ gptr, const int* __restrict__ gidx,
                                                  const double* __restrict__ v, const double* __restrict__ wgt,
-                                                 double* __restrict__ xb, float* __restrict__ xb32, const int* __restrict__ gate) {
+                                                 double* __restrict__ xb, float* __restrict__ xb32,
+                                                 const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t n = nb * k;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-    const int64_t q = t / k;
-    const int j = (int)(t - q * k);
-    const int d = bsub[q];
-    double s = 0.0;
-    const int e1 = gptr[q + 1];
-    for (int e = gptr[q]; e < e1; e++) {
-      const int code = gidx[e];
-      const int r = code >= 0 ? code : ~code;
-      double val = v[(int64_t)r * k + j];
-      if (wgt) val *= wgt[r];
-      s += code >= 0 ? val : -val;
+  const int j = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t ngroups = (nb + GQ - 1) / GQ;
+  if (j >= k) return;
+  for (int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); grp < ngroups; grp += nwaves) {
+    double s[GQ];
+    int64_t o[GQ];
+#pragma unroll
+    for (int u = 0; u < GQ; u++) {
+      const int64_t q = grp * GQ + u;
+      s[u] = 0.0;
+      o[u] = -1;
+      if (q < nb) {
+        const int d = bsub[q];
+        o[u] = (obrow[d] + (q - boff[d])) * kp + j;
+        const int e1 = gptr[q + 1];
+        for (int e = gptr[q]; e < e1; e++) {
+          const int code = gidx[e];
+          const int r = code >= 0 ? code : ~code;
+          double val = v[(int64_t)r * k + j];
+          if (wgt) val *= wgt[r];
+          s[u] += code >= 0 ? val : -val;
+        }
+      }
     }
-    const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
-    if (xb) xb[o] = s;
-    if (xb32) xb32[o] = (float)s;
+#pragma unroll
+    for (int u = 0; u < GQ; u++) {
+      if (o[u] < 0) continue;
+      if (xb) xb[o[u]] = s[u];
+      if (xb32) xb32[o[u]] = (float)s[u];
+    }
   }
 }
 
 hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
                             const int64_t* obrow, const int* gptr, const int* gidx, const double* v,
                             const double* wgt, double* xb, float* xb32, const int* gate) {
-  const int64_t n = nb * k;
-  hipLaunchKernelGGL(kb_gather, dim3((unsigned)std::min<int64_t>((n + 255) / 256, kStrideGrid)), dim3(256), 0, st, nb,
-                     k, kp, bsub, boff, obrow, gptr, gidx, v, wgt, xb, xb32, gate);
+  if (k > 64) return hipErrorInvalidValue;
+  const int64_t groups = (nb + GQ - 1) / GQ;
+  hipLaunchKernelGGL(kb_gather, dim3((unsigned)std::min<int64_t>((groups + 3) / 4, kStrideGrid)), dim3(256), 0, st,
+                     nb, k, kp, bsub, boff, obrow, gptr, gidx, v, wgt, xb, xb32, gate);
   return hipGetLastError();
 }
 
@@ -310,35 +329,33 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
   return hipGetLastError();
 }
 
-// out[r*k+j] = x[r*k+j] + sgn * (R_b,a[pos_a,:] h_a[:,j] - R_b,b[pos_b,:] h_b[:,j]); partial dots.
-// R_b rows are read from the row-major copy Rbr[(boff[d] + p)*6 + m] (one line per side)
+// out[r*k+j] = x[r*k+j] + sgn * (G_r,a . h_a[:,j] - G_r,b . h_b[:,j]); partial dots, where
+// Gr[r*12 + m] = R_b,a[pos_a, m] and Gr[r*12 + 6 + m] = R_b,b[pos_b, m] (0 for Dirichlet
+// rows) are the rows of G_f restricted to the two subdomains of dual row r
 __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __restrict__ rsa,
-                                             const int* __restrict__ rsb, const int* __restrict__ rpa,
-                                             const int* __restrict__ rpb, const double* __restrict__ Rbr,
-                                             const int64_t* __restrict__ boff, const double* __restrict__ h,
-                                             const double* __restrict__ x, double* __restrict__ out, double sgn,
-                                             const double* __restrict__ dotv, double* __restrict__ part,
-                                             const int* __restrict__ gate) {
+                                             const int* __restrict__ rsb, const double* __restrict__ Gr,
+                                             const double* __restrict__ h, const double* __restrict__ x,
+                                             double* __restrict__ out, double sgn, const double* __restrict__ dotv,
+                                             double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double red[4 * 64];
   const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
   double s = 0.0;
   if (j < k) {
+#pragma unroll 4
     for (int q = 0; q < 16; q++) {
       const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
       if (r >= nlam) break;
-      const int a = rsa[r];
-      const double* Ra = Rbr + (boff[a] + rpa[r]) * 6;
+      const int a = rsa[r], b = rsb[r];
+      const double* G = Gr + r * 12;
       const double* ha = h + (int64_t)6 * a * k + j;
       double v = 0.0;
-      for (int m = 0; m < 6; m++) v += Ra[m] * ha[(int64_t)m * k];
-      const int b = rsb[r];
+#pragma unroll
+      for (int m = 0; m < 6; m++) v += G[m] * ha[(int64_t)m * k];
       if (b >= 0) {
-        const double* Rb2 = Rbr + (boff[b] + rpb[r]) * 6;
         const double* hb = h + (int64_t)6 * b * k + j;
-        double vb = 0.0;
-        for (int m = 0; m < 6; m++) vb += Rb2[m] * hb[(int64_t)m * k];
-        v -= vb;
+#pragma unroll
+        for (int m = 0; m < 6; m++) v -= G[6 + m] * hb[(int64_t)m * k];
       }
       const double o = (x ? x[r * k + j] : 0.0) + sgn * v;
       out[r * k + j] = o;
@@ -352,21 +369,22 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
   }
 }
 
-hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
-                        const int* rpb, const double* Rbr, const int64_t* boff, const double* h, const double* x,
-                        double* out, double sgn, const double* dotv, double* part, const int* gate) {
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const double* Gr,
+                        const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
+                        const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, rpa, rpb, Rbr,
-                     boff, h, x, out, sgn, dotv, part, gate);
+  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, Gr, h, x, out, sgn,
+                     dotv, part, gate);
   return hipGetLastError();
 }
 
 // per column j: y = sum_{i in cl} H_i^T v_i ; alpha = invGtG y ; h_i = sgn * H_i alpha_cl(i)
-// v_i rows at v[(64 i + a)*kp + j]; one workgroup per column
+// v_i rows at v[(vrow[i] + a)*kp + j]; one workgroup per column
 __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k, int kp, const double* __restrict__ v,
-                                                      const double* __restrict__ H, const int* __restrict__ cl,
-                                                      const double* __restrict__ invGtG, double* __restrict__ h,
-                                                      double* __restrict__ alpha, double sgn, const int* __restrict__ gate) {
+                                                      const int64_t* __restrict__ vrow, const double* __restrict__ H,
+                                                      const int* __restrict__ cl, const double* __restrict__ invGtG,
+                                                      double* __restrict__ h, double* __restrict__ alpha, double sgn,
+                                                      const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int j = blockIdx.x;
   __shared__ double y[1024];
@@ -377,7 +395,7 @@ __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k,
     double s = 0.0;
     for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
       const double* Hd = H + 36 * d;
-      for (int a = 0; a < 6; a++) s += Hd[a * 6 + kk] * v[((int64_t)64 * d + a) * kp + j];
+      for (int a = 0; a < 6; a++) s += Hd[a * 6 + kk] * v[(vrow[d] + a) * kp + j];
     }
     y[c] = s;
   }
@@ -399,10 +417,12 @@ __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k,
   }
 }
 
-hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
-                                 const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate) {
+hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v,
+                                 const int64_t* vrow, const double* H, const int* cl, const double* invGtG, double* h,
+                                 double* alpha, double sgn, const int* gate) {
   if (ncol > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_proj_coarse, dim3(k), dim3(256), 0, st, nsub, ncol, k, kp, v, H, cl, invGtG, h, alpha, sgn, gate);
+  hipLaunchKernelGGL(kb_proj_coarse, dim3(k), dim3(256), 0, st, nsub, ncol, k, kp, v, vrow, H, cl, invGtG, h, alpha,
+                     sgn, gate);
   return hipGetLastError();
 }
 

@@ -123,12 +123,18 @@ void Engine::block_prepare() {
   {
     std::vector<double> Rb, Q(off_Rb[N]), Y;
     d_Rb.download(Rb, st);
-    {  // row-major copy of R_b for the projector's per-row reads
-      std::vector<double> Rr((size_t)off_b[N] * 6);
-      for (int d = 0; d < N; d++)
-        for (int p = 0; p < h_ns[d]; p++)
-          for (int m = 0; m < 6; m++) Rr[(size_t)(off_b[d] + p) * 6 + m] = Rb[off_Rb[d] + p + (int64_t)m * h_ns[d]];
-      b_Rbr.upload(Rr, st);
+    {  // per dual row, the rows of R_b of its two subdomains (the projector's G rows)
+      std::vector<double> Gr((size_t)P->nlam * 12, 0.0);
+      for (int d = 0; d < N; d++) {
+        const Subdomain& sd = P->subs[d];
+        for (size_t e = 0; e < sd.bf_row.size(); e++) {
+          const int64_t r = sd.bf_row[e];
+          const int p = sd.bpos[sd.bf_dof[e]];
+          const int side = sd.bf_sign[e] > 0 ? 0 : 6;
+          for (int m = 0; m < 6; m++) Gr[(size_t)r * 12 + side + m] = Rb[off_Rb[d] + p + (int64_t)m * h_ns[d]];
+        }
+      }
+      b_Gr.upload(Gr, st);
     }
     for (int d = 0; d < N; d++) {  // modified Gram-Schmidt per subdomain
       const int n = h_ns[d];
@@ -180,6 +186,19 @@ void Engine::block_prepare() {
     HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_CR.p, dcr.p, d_ldb.p, 1.0));
     HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, dQ.p, d_off_Rb.p, d_ns.p, b_CR.p, dcq.p, d_ldb.p, 1.0));
     HC(launch_copy_cols(st, N, 6, d_ns.p, d_sixes.p, dY.p, d_off_Rb.p, d_ns.p, b_CR.p, dcy.p, d_ldb.p, 1.0));
+    // the same block as general 64x64 tiles for the tile GEMMs
+    std::vector<int64_t> crto(N);
+    int64_t ct = 0;
+    int maxt = 0;
+    for (int d = 0; d < N; d++) {
+      crto[d] = ct;
+      const int t = ((h_ns[d] + 63) / 64) * ((crn[d] + 63) / 64);
+      ct += (int64_t)t * 4096;
+      maxt = std::max(maxt, t);
+    }
+    b_crto.upload(crto, st);
+    b_CRt.alloc(std::max<int64_t>(ct, 1));
+    HC(launch_pack_f64(st, N, maxt, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_CRt.p, b_crto.p));
     HC(hipMemsetAsync(b_xb.p, 0, b_xb.n * sizeof(double), st));
     HC(hipMemsetAsync(b_yb.p, 0, b_yb.n * sizeof(double), st));
     HC(hipStreamSynchronize(st));
@@ -192,6 +211,7 @@ void Engine::block_prepare() {
   zero_alloc(b_xb32, NBP * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
+  zero_alloc(b_kpart, (int64_t)N * 8 * 64 * KP, st);
   zero_alloc(b_v6, (int64_t)64 * N * KP, st);
   zero_alloc(b_gc, GCP * KP, st);
   zero_alloc(b_la, GCP * KP, st);
@@ -243,7 +263,7 @@ size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_Rbr, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -266,8 +286,7 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   // HTFETI correction (Cluster.cpp:1129-1186): [c; a] = [C_i | R_b,i]^T x_b,
   // [lam_c; alpha] = Ac^-1 [c; -a], y_b += R_b alpha - C lam_c; in FP32 also the
   // rank-12 FP64 part of K+_bb, Q (Yh^T x) + Yh (Q^T x)
-  HC(launch_tgemm_f64(st, N, maxTcr, 2, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_xb.p, b_xoff.p, b_cr.p,
-                      b_croff.p, nullptr, KP, 1.0, gate));
+  crq_products(gate);
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
                           b_lcpos.p, b_cr.p, b_abase_cr.p, b_gc.p, b_gcrow.p, gate));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
@@ -283,8 +302,17 @@ void Engine::b_corr(int k, int with_R, int defl, const int* gate) {
   const int N = P->nsub;
   HC(launch_kb_coef(st, N, maxnlc + 18, k, KP, d_nlc.p, d_off_lc.p, b_lcrow.p, b_arow.p, b_la.p, with_R, defl, b_cr.p,
                     b_crrow.p, b_coef.p, gate));
-  HC(launch_tgemm_f64(st, N, maxTb, 2, 0, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_coef.p, b_croff.p, b_yb.p,
+  HC(launch_tgemm_f64(st, N, maxTb, 1, 0, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_coef.p, b_croff.p, b_yb.p,
                       b_xoff.p, b_yb.p, KP, 1.0, gate));
+}
+
+// cr = CRQ^T x_b per subdomain (c = C^T x, a = R_b^T x, Q^T x, Yh^T x), from the
+// CRQ tiles; split over K when the product is a single 64-row block
+void Engine::crq_products(const int* gate) {
+  const int N = P->nsub;
+  const int S = maxTcr == 1 ? 8 : 1;
+  HC(launch_tgemm_f64(st, N, maxTcr, 1, 1, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_xb.p, b_xoff.p, b_cr.p,
+                      b_croff.p, nullptr, KP, 1.0, gate, S, b_kpart.p));
 }
 
 // out = P x = x - G (G^T G)^-1 G^T x (Cluster.cpp:1314-1329); hbuf = H alpha per subdomain
@@ -292,11 +320,10 @@ void Engine::bP(int k, const double* x, double* out, double* hbuf, const double*
   const int N = P->nsub;
   HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, nullptr, b_xb.p,
                       nullptr, gate));
-  HC(launch_tgemm_f64(st, N, 1, 2, 1, d_ns.p, d_sixes.p, d_Rb.p, d_off_Rb.p, d_ns.p, b_xb.p, b_xoff.p, b_v6.p,
-                      b_v6off.p, nullptr, KP, 1.0, gate));
-  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0, gate));
-  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_Rbr.p, d_off_b.p, hbuf, x, out, -1.0, dotv,
-                  b_part.p, gate));
+  crq_products(gate);  // rows nlc..nlc+5 of each subdomain: R_b^T x_b
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_cr.p, b_abase_cr.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr,
+                           1.0, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, out, -1.0, dotv, b_part.p, gate));
 }
 
 // out = W Bf S Bf^T W x (Cluster.cpp:1043-1112)
@@ -329,8 +356,7 @@ void Engine::b_kplus(int k, bool interior_too, double* V) {
                         b_xoff.p, b_xb.p, KP, -1.0, nullptr));
   HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                       b_xoff.p, nullptr, KP, 1.0, nullptr));
-  HC(launch_tgemm_f64(st, N, maxTcr, 2, 1, d_ns.p, b_crn.p, b_CR.p, b_cro.p, d_ldb.p, b_xb.p, b_xoff.p, b_cr.p,
-                      b_croff.p, nullptr, KP, 1.0, nullptr));
+  crq_products(nullptr);
   HC(launch_kb_rtf(st, N, k, KP, np, d_n.p, d_off_n.p, d_R.p, d_off_R.p, V, b_v6.p));
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
                           b_lcpos.p, b_v6.p, b_v6row.p, b_gc.p, b_gcrow.p));
@@ -458,8 +484,9 @@ void Engine::blk_init(const std::vector<int>& cols) {
   HC(launch_kb_scatter(st, nl, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, nullptr, b_Fw.p,
                        nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_d.p, nullptr, b_Fw.p, nullptr, b_sc.p));
-  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr, 1.0));
-  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_Rbr.p, d_off_b.p, b_h.p, nullptr, b_Pg.p,
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, b_v6row.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr,
+                           1.0));
+  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, b_Gr.p, b_h.p, nullptr, b_Pg.p,
                   1.0, nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_lam.p, nullptr, b_Pg.p, nullptr, b_sc.p));
   // g0 = F lam0 - d in FP64 ; w0 = P M P g0

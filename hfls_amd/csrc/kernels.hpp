@@ -141,7 +141,7 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
-                            double alpha, const int* gate);
+                            double alpha, const int* gate, int S = 1, double* Ypart = nullptr);  // S: split K (maxRo 1)
 hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
                            const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
 hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
@@ -185,11 +185,12 @@ hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const i
 hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
                              const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
                              const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr);
-hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* rpa,
-                        const int* rpb, const double* Rbr, const int64_t* boff, const double* h, const double* x,
-                        double* out, double sgn, const double* dotv, double* part, const int* gate = nullptr);
-hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v, const double* H,
-                                 const int* cl, const double* invGtG, double* h, double* alpha, double sgn, const int* gate = nullptr);
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const double* Gr,
+                        const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
+                        const int* gate = nullptr);
+hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v,
+                                 const int64_t* vrow, const double* H, const int* cl, const double* invGtG, double* h,
+                                 double* alpha, double sgn, const int* gate = nullptr);
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
                             const int* gate = nullptr);
 hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,

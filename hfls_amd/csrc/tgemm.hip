@@ -271,6 +271,9 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
   }
 }
 
+// S > 1 (split K, one output row block): workgroup s of a subdomain takes K chunks
+// [s*nch/S, (s+1)*nch/S) and writes its partial 64 x kp block to Ypart at
+// (d*S + s)*64*kp; k_splitk_sum adds the partials in a fixed order
 template <bool CM>  // compact column-major A (mode 2) or tiles
 __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const double* __restrict__ A,
@@ -278,14 +281,16 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
                                                    const double* __restrict__ X,
                                                    const int64_t* __restrict__ xoff, double* __restrict__ Y,
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
-                                                   int kp, double alpha, const int* __restrict__ gate) {
+                                                   int kp, double alpha, const int* __restrict__ gate, int S,
+                                                   double* __restrict__ Ypart) {
   const int d = blockIdx.y;
   if (gate && *gate == 0) return;
   const int m = mrows[d], n = mcols[d];
   const int Ra = (m + 63) / 64, Ca = (n + 63) / 64;
   const int Ro = (mode != 0 && op == 1) ? Ca : Ra;
   const int Ko = (mode != 0 && op == 1) ? Ra : Ca;
-  const int i = blockIdx.x;
+  const int i = S > 1 ? 0 : blockIdx.x;
+  const int sp = S > 1 ? blockIdx.x : 0;
   if (i >= Ro) return;
   const int c0 = blockIdx.z * 64;
   __shared__ double As[32 * 65];
@@ -300,17 +305,19 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
   const double* Ad = A + aoff[d];
   const int64_t la = mode == 2 ? (int64_t)lda[d] : 0;
   const double* Xd = X + xoff[d] + c0;
-  const int nch = 2 * Ko;
+  const int nall = 2 * Ko;
+  const int cbeg = (int)((int64_t)sp * nall / S), cend = (int)((int64_t)(sp + 1) * nall / S);
   StageD s;
-  if (nch > 0) {
+  if (cend > cbeg) {
     int64_t ti;
-    const int kind = chunk_kind(mode, op, Ra, i, 0, &ti);
-    fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, 0, kind, 0, Xd, kp, s);
+    const int kind = chunk_kind(mode, op, Ra, i, cbeg, &ti);
+    fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, cbeg >> 1, kind, (cbeg & 1) * 32,
+                  Xd + (int64_t)((cbeg >> 1) * 64) * kp, kp, s);
     stash_f64<CM>(kind, s, As, Xs);
   }
   __syncthreads();
-  for (int c = 0; c < nch; c++) {
-    const bool more = c + 1 < nch;
+  for (int c = cbeg; c < cend; c++) {
+    const bool more = c + 1 < cend;
     int kind_n = 0;
     if (more) {
       int64_t ti;
@@ -331,6 +338,15 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
     if (more) stash_f64<CM>(kind_n, s, As, Xs);
     __syncthreads();
   }
+  if (S > 1) {
+    double* Pd = Ypart + ((int64_t)d * S + sp) * 64 * kp + c0;
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Pd[(int64_t)(rb + 4 * r + (lane >> 4)) * kp + 16 * cc + (lane & 15)] = alpha * acc[cc][r];
+    return;
+  }
   double* Yd = Y + yoff[d] + c0;
   const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
 #pragma unroll
@@ -343,6 +359,21 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
       if (Zd) v += Zd[(int64_t)row * kp + col];
       Yd[(int64_t)row * kp + col] = v;
     }
+}
+
+// Y[yoff[d] + r*kp + j] = sum_{s < S} Ypart[((d*S + s)*64 + r)*kp + j] (+ Z), fixed order
+__global__ __launch_bounds__(256) void k_splitk_sum(int S, int kp, const double* __restrict__ Ypart,
+                                                    double* __restrict__ Y, const int64_t* __restrict__ yoff,
+                                                    const double* __restrict__ Z, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int d = blockIdx.y;
+  const int e = blockIdx.x * 256 + threadIdx.x;  // 64 * kp entries
+  if (e >= 64 * kp) return;
+  const double* p = Ypart + (int64_t)d * S * 64 * kp + e;
+  double v = 0.0;
+  for (int s = 0; s < S; s++) v += p[(int64_t)s * 64 * kp];
+  if (Z) v += Z[yoff[d] + e];
+  Y[yoff[d] + e] = v;
 }
 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
@@ -361,15 +392,21 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
 
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X, const int64_t* xoff, double* Y,
-                            const int64_t* yoff, const double* Z, int kp, double alpha, const int* gate) {
+                            const int64_t* yoff, const double* Z, int kp, double alpha, const int* gate, int S,
+                            double* Ypart) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
+  if (S > 1 && (maxRo != 1 || !Ypart)) return hipErrorInvalidValue;
+  const int gx = S > 1 ? S : maxRo;
   if (mode == 2)
-    hipLaunchKernelGGL(k_tgemm_f64<true>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
-                       lda, X, xoff, Y, yoff, Z, kp, alpha, gate);
+    hipLaunchKernelGGL(k_tgemm_f64<true>, dim3(gx, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
+                       lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
   else
-    hipLaunchKernelGGL(k_tgemm_f64<false>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A,
-                       aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate);
+    hipLaunchKernelGGL(k_tgemm_f64<false>, dim3(gx, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A,
+                       aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
+  if (S > 1)
+    hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub), dim3(256), 0, st, S, kp, Ypart, Y, yoff, Z,
+                       gate);
   return hipGetLastError();
 }
 
