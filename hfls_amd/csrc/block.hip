@@ -23,13 +23,47 @@ static __device__ __forceinline__ double wsum64(double v) {
   return v;
 }
 
-// xb[(obrow[d] + p)*kp + j] = sum_e sign_e * wgt[r_e] * v[r_e*k + j]; optional FP32 copy.
-// A wave takes GQ consecutive b positions per step (lanes = columns), issuing all
-// their loads before any store; the grid is capped so a gated launch is cheap.
-constexpr int GQ = 4;
-__global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, const int* __restrict__ bsub,
-                                                 const int64_t* __restrict__ boff, const int64_t* __restrict__ obrow,
-                                                 const int* __restrict__ gptr, const int* __restrict__ gidx,
+// Signed row sums over the Bf^T lists of NQ consecutive b positions q0..q0+nq-1
+// (entries gptr[q]..gptr[q+1]-1, code >= 0 for +1 at dual row code, ~code for -1):
+// s[u] = sum_e sign_e f(r_e), added in entry order.  q0, nq and the entry indices
+// are wave-uniform (scalar loads); the entries of the group are walked as one flat
+// range, EB per step with every value load issued before any is used, so a wave
+// keeps EB loads in flight instead of one per row.  gidx carries EB - 1 readable
+// entries past its end (every r it holds is a dual row, so f(r) stays in bounds).
+template <int NQ, int EB, class F>
+__device__ __forceinline__ void bt_row_sums(int64_t q0, int nq, const int* __restrict__ gptr,
+                                            const int* __restrict__ gidx, F f, double* s) {
+  int eb[NQ + 1];
+#pragma unroll
+  for (int u = 0; u <= NQ; u++) eb[u] = gptr[q0 + (u < nq ? u : nq)];
+#pragma unroll
+  for (int u = 0; u < NQ; u++) s[u] = 0.0;
+  const int e1 = eb[NQ];
+  for (int e = eb[0]; e < e1; e += EB) {
+    double val[EB];
+#pragma unroll
+    for (int i = 0; i < EB; i++) {
+      const int code = gidx[e + i];  // past e1: a padding or next-group entry, loaded and dropped
+      const double v = f(code >= 0 ? code : ~code);
+      val[i] = code >= 0 ? v : -v;
+    }
+#pragma unroll
+    for (int i = 0; i < EB; i++) {
+      const int ei = e + i;
+#pragma unroll
+      for (int u = 0; u < NQ; u++)
+        if (ei >= eb[u] && ei < eb[u + 1]) s[u] += val[i];
+    }
+  }
+}
+
+// xb[qrow[q]*kp + j] = sum_e sign_e * wgt[r_e] * v[r_e*k + j] (qrow[q] = obrow[d] + p for
+// b position p of subdomain d); optional FP32 copy.
+// A wave takes GQ consecutive b positions per step (lanes = columns); the grid is
+// capped so a gated launch is cheap.
+constexpr int GQ = 8;
+template <bool W>
+__global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, const int* __restrict__ qrow,                                                  const int* __restrict__ gptr, const int* __restrict__ gidx,
                                                  const double* __restrict__ v, const double* __restrict__ wgt,
                                                  double* __restrict__ xb, float* __restrict__ xb32,
                                                  const int* __restrict__ gate) {
@@ -38,65 +72,150 @@ __global__ __launch_bounds__(256) void kb_gather(int64_t nb, int k, int kp, cons
   const int64_t nwaves = (int64_t)gridDim.x * 4;
   const int64_t ngroups = (nb + GQ - 1) / GQ;
   if (j >= k) return;
-  for (int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); grp < ngroups; grp += nwaves) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: index loads go scalar
+  for (int64_t grp = (int64_t)blockIdx.x * 4 + wv; grp < ngroups; grp += nwaves) {
+    const int64_t q0 = grp * GQ;
+    const int nq = (int)std::min<int64_t>(GQ, nb - q0);
     double s[GQ];
-    int64_t o[GQ];
+    bt_row_sums<GQ, 8>(q0, nq, gptr, gidx,
+                       [&](int r) {
+                         const double val = v[(int64_t)r * k + j];
+                         return W ? val * wgt[r] : val;
+                       },
+                       s);
 #pragma unroll
     for (int u = 0; u < GQ; u++) {
-      const int64_t q = grp * GQ + u;
-      s[u] = 0.0;
-      o[u] = -1;
-      if (q < nb) {
-        const int d = bsub[q];
-        o[u] = (obrow[d] + (q - boff[d])) * kp + j;
-        const int e1 = gptr[q + 1];
-        for (int e = gptr[q]; e < e1; e++) {
-          const int code = gidx[e];
-          const int r = code >= 0 ? code : ~code;
-          double val = v[(int64_t)r * k + j];
-          if (wgt) val *= wgt[r];
-          s[u] += code >= 0 ? val : -val;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < GQ; u++) {
-      if (o[u] < 0) continue;
-      if (xb) xb[o[u]] = s[u];
-      if (xb32) xb32[o[u]] = (float)s[u];
+      if (u >= nq) break;
+      const int64_t o = (int64_t)qrow[q0 + u] * kp + j;
+      if (xb) xb[o] = s[u];
+      if (xb32) xb32[o] = (float)s[u];
     }
   }
 }
 
-hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
-                            const int64_t* obrow, const int* gptr, const int* gidx, const double* v,
+hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* qrow, const int* gptr, const int* gidx, const double* v,
                             const double* wgt, double* xb, float* xb32, const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
   const int64_t groups = (nb + GQ - 1) / GQ;
-  hipLaunchKernelGGL(kb_gather, dim3((unsigned)std::min<int64_t>((groups + 3) / 4, kStrideGrid)), dim3(256), 0, st,
-                     nb, k, kp, bsub, boff, obrow, gptr, gidx, v, wgt, xb, xb32, gate);
+  const dim3 grid((unsigned)std::min<int64_t>((groups + 3) / 4, kStrideGrid));
+  if (wgt)
+    hipLaunchKernelGGL(kb_gather<true>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, v, wgt, xb,
+                       xb32, gate);
+  else
+    hipLaunchKernelGGL(kb_gather<false>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, v, wgt, xb,
+                       xb32, gate);
   return hipGetLastError();
 }
 
-// yb = (double) y32 on the real rows and columns of a b-block
-__global__ __launch_bounds__(256) void kb_widen(int64_t nb, int k, int kp, const int* __restrict__ bsub,
-                                                const int64_t* __restrict__ boff, const int64_t* __restrict__ obrow,
-                                                const float* __restrict__ y32, double* __restrict__ yb, const int* __restrict__ gate) {
+// G^T x of the projector without an interface block in HBM: one workgroup per
+// subdomain, out[(6d + m)*kp + j] = sum_p R_b[p][m] * (sum_e sign_e x[r_e][j]) over
+// the b positions p of d (R_b column major at Rb[offRb[d] + p + m*n_s]).  Each wave
+// takes GQ positions per step; the 16 wave sums are added in a fixed order, so a
+// column's value does not depend on the other columns.
+constexpr int GTW = 16;
+__global__ __launch_bounds__(64 * GTW) void kb_gtx(int k, int kp, const int* __restrict__ ns,
+                                                   const int64_t* __restrict__ boff, const int* __restrict__ gptr,
+                                                   const int* __restrict__ gidx, const double* __restrict__ Rb,
+                                                   const int64_t* __restrict__ offRb, const double* __restrict__ x,
+                                                   double* __restrict__ out, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t q = t / k;
-  const int j = (int)(t - q * k);
-  if (q >= nb) return;
-  const int d = bsub[q];
-  const int64_t o = (obrow[d] + (q - boff[d])) * kp + j;
-  yb[o] = (double)y32[o];
+  __shared__ double red[GTW][6][64];
+  const int d = blockIdx.x, j = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n = ns[d];
+  const double* R = Rb + offRb[d];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (j < k) {
+    for (int p0 = w * GQ; p0 < n; p0 += GTW * GQ) {
+      const int nq = std::min(GQ, n - p0);
+      double xq[GQ];
+      bt_row_sums<GQ, 8>(boff[d] + p0, nq, gptr, gidx, [&](int r) { return x[(int64_t)r * k + j]; }, xq);
+#pragma unroll
+      for (int u = 0; u < GQ; u++) {
+        if (u >= nq) break;
+#pragma unroll
+        for (int m = 0; m < 6; m++) acc[m] += R[p0 + u + (int64_t)m * n] * xq[u];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 6; m++) red[w][m][j] = acc[m];
+  __syncthreads();
+  for (int t = threadIdx.x; t < 6 * 64; t += 64 * GTW) {
+    const int m = t >> 6, jj = t & 63;
+    if (jj >= k) continue;
+    double sum = 0.0;
+    for (int ww = 0; ww < GTW; ww++) sum += red[ww][m][jj];
+    out[((int64_t)6 * d + m) * kp + jj] = sum;
+  }
 }
 
-hipError_t launch_kb_widen(hipStream_t st, int64_t nb, int k, int kp, const int* bsub, const int64_t* boff,
-                           const int64_t* obrow, const float* y32, double* yb, const int* gate) {
-  const int64_t n = nb * k;
-  hipLaunchKernelGGL(kb_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nb, k, kp, bsub, boff, obrow, y32,
-                     yb, gate);
+hipError_t launch_kb_gtx(hipStream_t st, int nsub, int k, int kp, const int* ns, const int64_t* boff, const int* gptr,
+                         const int* gidx, const double* Rb, const int64_t* offRb, const double* x, double* out,
+                         const int* gate) {
+  if (k > 64 || k > kp) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_gtx, dim3(nsub), dim3(64 * GTW), 0, st, k, kp, ns, boff, gptr, gidx, Rb, offRb, x, out, gate);
+  return hipGetLastError();
+}
+
+// The preconditioner's gather of P x without P x in HBM:
+// xb[q][j] = sum_e sign_e wgt[r_e] (x[r_e][j] - c[r_e][j]), c[r] = G_r,a . h_a - G_r,b . h_b
+// (the same arithmetic as kb_gh followed by kb_gather)
+template <bool W>
+__global__ __launch_bounds__(256) void kb_gather_px(int64_t nb, int k, int kp, const int* __restrict__ qrow,                                                     const int* __restrict__ gptr, const int* __restrict__ gidx,
+                                                    const double* __restrict__ x, const double* __restrict__ wgt,
+                                                    const int* __restrict__ rsa, const int* __restrict__ rsb,
+                                                    const double* __restrict__ Gr, const double* __restrict__ h,
+                                                    double* __restrict__ xb, float* __restrict__ xb32,
+                                                    const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int j = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t ngroups = (nb + GQ - 1) / GQ;
+  if (j >= k) return;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t grp = (int64_t)blockIdx.x * 4 + wv; grp < ngroups; grp += nwaves) {
+    const int64_t q0 = grp * GQ;
+    const int nq = (int)std::min<int64_t>(GQ, nb - q0);
+    double s[GQ];
+    // a dual row with one subdomain reads h_a twice against the zero half of its G row
+    bt_row_sums<GQ, 4>(q0, nq, gptr, gidx,
+                       [&](int r) {
+                         const int a = rsa[r], b0 = rsb[r];
+                         const int b = b0 >= 0 ? b0 : a;
+                         const double* G = Gr + (int64_t)r * 12;
+                         const double* ha = h + (int64_t)6 * a * k + j;
+                         const double* hb = h + (int64_t)6 * b * k + j;
+                         double c = 0.0;
+#pragma unroll
+                         for (int m = 0; m < 6; m++) c += G[m] * ha[(int64_t)m * k];
+#pragma unroll
+                         for (int m = 0; m < 6; m++) c -= G[6 + m] * hb[(int64_t)m * k];
+                         const double val = x[(int64_t)r * k + j] - c;
+                         return W ? val * wgt[r] : val;
+                       },
+                       s);
+#pragma unroll
+    for (int u = 0; u < GQ; u++) {
+      if (u >= nq) break;
+      const int64_t o = (int64_t)qrow[q0 + u] * kp + j;
+      if (xb) xb[o] = s[u];
+      if (xb32) xb32[o] = (float)s[u];
+    }
+  }
+}
+
+hipError_t launch_kb_gather_px(hipStream_t st, int64_t nb, int k, int kp, const int* qrow, const int* gptr, const int* gidx, const double* x,
+                               const double* wgt, const int* rsa, const int* rsb, const double* Gr, const double* h,
+                               double* xb, float* xb32, const int* gate) {
+  if (k > 64) return hipErrorInvalidValue;
+  const int64_t groups = (nb + GQ - 1) / GQ;
+  const dim3 grid((unsigned)std::min<int64_t>((groups + 3) / 4, kStrideGrid));
+  if (wgt)
+    hipLaunchKernelGGL(kb_gather_px<true>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, x, wgt,
+                       rsa, rsb, Gr, h, xb, xb32, gate);
+  else
+    hipLaunchKernelGGL(kb_gather_px<false>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, x, wgt,
+                       rsa, rsb, Gr, h, xb, xb32, gate);
   return hipGetLastError();
 }
 
@@ -298,7 +417,7 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
                                                   const double* __restrict__ dotw, double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double red[4 * 64];
-  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double s = 0.0;
   if (j < k) {
     for (int q = 0; q < 16; q++) {
@@ -339,27 +458,40 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
                                              double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double red[4 * 64];
-  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double s = 0.0;
   if (j < k) {
-#pragma unroll 4
-    for (int q = 0; q < 16; q++) {
-      const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
-      if (r >= nlam) break;
-      const int a = rsa[r], b = rsb[r];
-      const double* G = Gr + r * 12;
-      const double* ha = h + (int64_t)6 * a * k + j;
-      double v = 0.0;
+    // GH rows per step with every load issued before the arithmetic; a row with a
+    // single subdomain reads h_a twice against the zero half of its G row
+    constexpr int GH = 4;
+    for (int q0 = 0; q0 < 16; q0 += GH) {
+      double c[GH], xv[GH], dv[GH];
 #pragma unroll
-      for (int m = 0; m < 6; m++) v += G[m] * ha[(int64_t)m * k];
-      if (b >= 0) {
+      for (int u = 0; u < GH; u++) {
+        const int64_t r0 = (int64_t)blockIdx.x * 64 + g + 4 * (q0 + u);
+        const int64_t r = r0 < nlam ? r0 : nlam - 1;
+        const int a = rsa[r], b0 = rsb[r];
+        const int b = b0 >= 0 ? b0 : a;
+        const double* G = Gr + r * 12;
+        const double* ha = h + (int64_t)6 * a * k + j;
         const double* hb = h + (int64_t)6 * b * k + j;
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; m++) v += G[m] * ha[(int64_t)m * k];
 #pragma unroll
         for (int m = 0; m < 6; m++) v -= G[6 + m] * hb[(int64_t)m * k];
+        c[u] = v;
+        xv[u] = x ? x[r * k + j] : 0.0;
+        dv[u] = dotv ? dotv[r * k + j] : 0.0;
       }
-      const double o = (x ? x[r * k + j] : 0.0) + sgn * v;
-      out[r * k + j] = o;
-      if (dotv) s += o * dotv[r * k + j];
+#pragma unroll
+      for (int u = 0; u < GH; u++) {
+        const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * (q0 + u);
+        if (r >= nlam) continue;
+        const double o = xv[u] + sgn * c[u];
+        out[r * k + j] = o;
+        s += o * dv[u];
+      }
     }
   }
   if (dotv) {
@@ -379,7 +511,8 @@ hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, con
 }
 
 // per column j: y = sum_{i in cl} H_i^T v_i ; alpha = invGtG y ; h_i = sgn * H_i alpha_cl(i)
-// v_i rows at v[(vrow[i] + a)*kp + j]; one workgroup per column
+// v_i rows at v[(vrow[i] + a)*kp + j] (vrow[i] = 6 i when vrow is null); one
+// workgroup per column
 __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k, int kp, const double* __restrict__ v,
                                                       const int64_t* __restrict__ vrow, const double* __restrict__ H,
                                                       const int* __restrict__ cl, const double* __restrict__ invGtG,
@@ -395,7 +528,10 @@ __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k,
     double s = 0.0;
     for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
       const double* Hd = H + 36 * d;
-      for (int a = 0; a < 6; a++) s += Hd[a * 6 + kk] * v[(vrow[d] + a) * kp + j];
+      for (int a = 0; a < 6; a++) {
+        const int64_t row = vrow ? vrow[d] + a : (int64_t)6 * d + a;
+        s += Hd[a * 6 + kk] * v[row * kp + j];
+      }
     }
     y[c] = s;
   }

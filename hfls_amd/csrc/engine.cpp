@@ -384,6 +384,7 @@ void Engine::build_meta() {
     gptr[q + 1] = gptr[q] + (int)lists[q].size();
     for (int c : lists[q]) gidx.push_back(c);
   }
+  gidx.resize(gidx.size() + 8, 0);  // the batched Bf^T walks read up to 7 entries past a group's end
   d_gptr.upload(gptr, st); d_gidx.upload(gidx, st);
   d_ra.upload(ra, st); d_rb.upload(rb, st); d_rsa.upload(rsa, st); d_rsb.upload(rsb, st);
   d_rpa.upload(rpa, st); d_rpb.upload(rpb, st);

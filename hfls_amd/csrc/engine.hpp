@@ -197,8 +197,8 @@ class Engine {
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
       b_v6off, b_v6row, b_crrow;
-  DVec<int> b_crn;
-  DVec<double> b_CR, b_CRt, b_Gr, b_kpart;
+  DVec<int> b_crn, b_qrow;
+  DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
@@ -210,6 +210,8 @@ class Engine {
           const int* gate = nullptr);
   void bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate = nullptr);
   void bM(int k, int prec, const double* x, double* out, const int* gate = nullptr);
+  void bPM(int k, int prec, const double* x, double* out, double* hbuf, const int* gate = nullptr);
+  void b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate);
   void b_corr(int k, int with_R, int defl, const int* gate);
   void crq_products(const int* gate);
   void b_kplus(int k, bool interior_too, double* V);  // K_c^+ pieces of the local block V

@@ -94,6 +94,12 @@ void Engine::block_prepare() {
     }
   }
   b_obrow.upload(obrow, st);
+  {  // b-block row of every b position
+    std::vector<int> qrow(NB);
+    for (int d = 0; d < N; d++)
+      for (int p = 0; p < h_ns[d]; p++) qrow[off_b[d] + p] = (int)(obrow[d] + p);
+    b_qrow.upload(qrow, st);
+  }
   b_orrow.upload(orrow, st);
   b_xoff.upload(xoff, st);
   b_roff.upload(roff, st);
@@ -220,6 +226,7 @@ void Engine::block_prepare() {
   for (DVec<double>* v : {&b_lam, &b_g, &b_w, &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64}) zero_alloc(*v, nl * KMAX, st);
   b_nparts = (int)((nl + 63) / 64);
   zero_alloc(b_part, (int64_t)b_nparts * KMAX, st);
+  zero_alloc(b_gt, (int64_t)6 * N * KP, st);
   zero_alloc(b_Vr, NRP * KP, st);
   zero_alloc(b_Xr, NRP * KP, st);
   zero_alloc(b_V, np * KMAX, st);
@@ -263,7 +270,7 @@ size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -274,7 +281,7 @@ size_t Engine::block_bytes() const {
 void Engine::bF(int k, int prec, const double* win, double* out, const double* sub, const double* dotw,
                 const int* gate) {
   const int N = P->nsub, NC = P->ncl;
-  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
+  HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
                       prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
@@ -315,21 +322,43 @@ void Engine::crq_products(const int* gate) {
                       b_croff.p, nullptr, KP, 1.0, gate, S, b_kpart.p));
 }
 
+// hbuf = H (G^T G)^-1 G^T x per subdomain, with G^T x = R_b^T (Bf^T x) taken
+// straight from the dual block (Cluster.cpp:1314-1329)
+void Engine::b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate) {
+  const int N = P->nsub;
+  HC(launch_kb_gtx(st, N, k, KP, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, d_Rb.p, d_off_Rb.p, x, b_gt.p, gate));
+  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_gt.p, nullptr, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0,
+                           gate));
+}
+
 // out = P x = x - G (G^T G)^-1 G^T x (Cluster.cpp:1314-1329); hbuf = H alpha per subdomain
 void Engine::bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate) {
-  const int N = P->nsub;
-  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, nullptr, b_xb.p,
-                      nullptr, gate));
-  crq_products(gate);  // rows nlc..nlc+5 of each subdomain: R_b^T x_b
-  HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_cr.p, b_abase_cr.p, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr,
-                           1.0, gate));
+  b_gtx_coarse(k, x, hbuf, gate);
   HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, out, -1.0, dotv, b_part.p, gate));
+}
+
+// out = M P x (Cluster.cpp:1043-1112 applied to the projection of x): P x is formed
+// inside the preconditioner's gather and never stored
+void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, const int* gate) {
+  const int N = P->nsub;
+  b_gtx_coarse(k, x, hbuf, gate);
+  HC(launch_kb_gather_px(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p, d_rsa.p,
+                         d_rsb.p, b_Gr.p, hbuf, prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
+  if (prec == 32) {
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
+  } else {
+    HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
+                        b_xoff.p, nullptr, KP, 1.0, gate));
+  }
+  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
+                       out, nullptr, nullptr, gate));
 }
 
 // out = W Bf S Bf^T W x (Cluster.cpp:1043-1112)
 void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) {
   const int N = P->nsub;
-  HC(launch_kb_gather(st, NB, k, KP, d_bsub.p, d_off_b.p, b_obrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
+  HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
@@ -492,8 +521,7 @@ void Engine::blk_init(const std::vector<int>& cols) {
   // g0 = F lam0 - d in FP64 ; w0 = P M P g0
   bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
   HC(launch_kb_update(st, nl, k, 3, b_g.p, nullptr, b_g64.p, nullptr, b_sc.p));
-  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
-  bM(k, prec, b_Pg.p, b_z.p);
+  bPM(k, prec, b_g64.p, b_z.p, b_hg.p);
   bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
   HC(launch_kb_update(st, nl, k, 3, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p));
   HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 2));
@@ -521,8 +549,7 @@ void Engine::blk_enqueue_sweeps(int n) {
     bF(k, prec, b_w.p, b_Fw.p, nullptr, b_w.p, active);
     HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 0, active));
     HC(launch_kb_update(st, nl, k, 0, b_lam.p, b_g.p, b_w.p, b_Fw.p, b_sc.p, active));
-    bP(k, b_g.p, b_Pg.p, b_hg.p, nullptr, active);
-    bM(k, prec, b_Pg.p, b_z.p, active);
+    bPM(k, prec, b_g.p, b_z.p, b_hg.p, active);
     bP(k, b_z.p, b_Pz.p, b_h.p, b_g.p, active);
     HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 1, active));
     HC(launch_kb_update(st, nl, k, 1, b_w.p, nullptr, b_Pz.p, nullptr, b_sc.p, active));
@@ -530,8 +557,7 @@ void Engine::blk_enqueue_sweeps(int n) {
       // columns waiting at a refinement point: FP64 true residual g = F lam - d,
       // certification or restart from it (skipped on the device when none waits)
       bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
-      bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr, pending);
-      bM(k, prec, b_Pg.p, b_z.p, pending);
+      bPM(k, prec, b_g64.p, b_z.p, b_hg.p, pending);
       bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
       HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
       HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, &b_sc.p->restart));
@@ -554,8 +580,7 @@ void Engine::blk_recover() {
   const int64_t np = off_n[N], ng = 3 * (int64_t)P->nPoints;
   // final FP64 true residual; its projection gives alpha (h per subdomain)
   bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
-  bP(k, b_g64.p, b_Pg.p, b_hg.p, nullptr);
-  bM(k, prec, b_Pg.p, b_z.p);
+  bPM(k, prec, b_g64.p, b_z.p, b_hg.p);
   bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
   HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
   // recovery: u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678), on a
