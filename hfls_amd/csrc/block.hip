@@ -520,22 +520,35 @@ __global__ __launch_bounds__(256) void kb_proj_coarse(int nsub, int ncol, int k,
                                                       const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   const int j = blockIdx.x;
+  constexpr int CH = 256;  // subdomains per chunk of the first product
   __shared__ double y[1024];
   __shared__ double al[1024];
+  __shared__ double z[CH * 6];
   const int ncl = ncol / 6, nsc = nsub / ncl;
-  for (int c = threadIdx.x; c < ncol; c += 256) {
-    const int ccl = c / 6, kk = c % 6;
-    double s = 0.0;
-    for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
+  for (int c = threadIdx.x; c < ncol; c += 256) y[c] = 0.0;
+  // y = H^T v per cluster, in chunks: z_d = H_d^T v_d for CH subdomains in
+  // parallel, then each y entry adds its cluster's z_d of the chunk in order
+  for (int d0 = 0; d0 < nsub; d0 += CH) {
+    for (int t = threadIdx.x; t < CH * 6; t += 256) {
+      const int d = d0 + t / 6, kk = t % 6;
+      if (d >= nsub) break;
       const double* Hd = H + 36 * d;
-      for (int a = 0; a < 6; a++) {
-        const int64_t row = vrow ? vrow[d] + a : (int64_t)6 * d + a;
-        s += Hd[a * 6 + kk] * v[row * kp + j];
-      }
+      const int64_t row = vrow ? vrow[d] : (int64_t)6 * d;
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 6; a++) s += Hd[a * 6 + kk] * v[(row + a) * kp + j];
+      z[t] = s;
     }
-    y[c] = s;
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncol; c += 256) {
+      const int ccl = c / 6, kk = c % 6;
+      const int da = max(d0, ccl * nsc), db = min(min(d0 + CH, nsub), (ccl + 1) * nsc);
+      double s = 0.0;
+      for (int d = da; d < db; d++) s += z[(d - d0) * 6 + kk];
+      y[c] += s;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int c = threadIdx.x; c < ncol; c += 256) {
     double s = 0.0;
     for (int q = 0; q < ncol; q++) s += invGtG[c + (int64_t)q * ncol] * y[q];
