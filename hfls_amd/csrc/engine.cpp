@@ -394,32 +394,47 @@ void Engine::build_meta() {
 void Engine::partial_cholesky(double* M, const std::vector<int64_t>& moff, const std::vector<int>& ld,
                               const std::vector<int>& start, const std::vector<int>& ncols,
                               const std::vector<int>& end) {
+  // Right-looking, two levels: 64-wide panels (diagonal Cholesky + TRSM of the rows
+  // below), grouped into super-panels of kSuper columns.  After a panel only the
+  // remaining columns of its super-panel are updated; the trailing matrix beyond
+  // the super-panel takes one update with all its columns (K = kSuper), so it is
+  // read and written once per super-panel instead of once per panel.
+  constexpr int kSuper = 256;
   const int nmat = (int)moff.size();
   int nsteps = 0;
   for (int i = 0; i < nmat; i++) nsteps = std::max(nsteps, (ncols[i] + 63) / 64);
   if (nsteps == 0) return;
   std::vector<PanelDesc> desc((size_t)nsteps * nmat);
-  std::vector<int> maxrows(nsteps, 0), maxpairs(nsteps, 0);
+  std::vector<SyrkDesc> sdesc((size_t)nsteps * nmat);
+  std::vector<int> maxrows(nsteps, 0);
+  std::vector<int64_t> maxtiles(nsteps, 0);
   for (int p = 0; p < nsteps; p++)
     for (int i = 0; i < nmat; i++) {
       PanelDesc& pd = desc[(size_t)p * nmat + i];
-      int k0 = start[i] + 64 * p, w = std::min(64, start[i] + ncols[i] - k0);
+      SyrkDesc& sd = sdesc[(size_t)p * nmat + i];
+      const int stop = start[i] + ncols[i];
+      const int k0 = start[i] + 64 * p, w = std::min(64, stop - k0);
       pd.k0 = k0; pd.w = w > 0 ? w : 0; pd.end = end[i]; pd.active = w > 0;
-      if (w > 0) {
-        int m = end[i] - k0 - w;
-        maxrows[p] = std::max(maxrows[p], m);
-        int T = (m + 63) / 64;
-        maxpairs[p] = std::max(maxpairs[p], T * (T + 1) / 2);
-      }
+      sd = SyrkDesc{0, 0, 0, 0, end[i], 0};
+      if (w <= 0) continue;
+      maxrows[p] = std::max(maxrows[p], end[i] - k0 - w);
+      const int s0 = start[i] + (64 * p) / kSuper * kSuper, s1 = std::min(stop, s0 + kSuper);
+      if (k0 + w < s1) sd = SyrkDesc{k0, w, k0 + w, s1, end[i], 1};  // rest of the super-panel
+      else sd = SyrkDesc{s0, s1 - s0, s1, end[i], end[i], 1};       // trailing matrix
+      const int64_t T = (end[i] - sd.t0 + 63) / 64, Tc = std::min<int64_t>(T, (sd.cend - sd.t0 + 63) / 64);
+      if (T > 0 && Tc > 0) maxtiles[p] = std::max(maxtiles[p], Tc * T - Tc * (Tc - 1) / 2);
     }
   DVec<PanelDesc> dd;
   dd.alloc(desc.size());
   HC(hipMemcpyAsync(dd.p, desc.data(), desc.size() * sizeof(PanelDesc), hipMemcpyHostToDevice, st));
+  DVec<SyrkDesc> ds;
+  ds.alloc(sdesc.size());
+  HC(hipMemcpyAsync(ds.p, sdesc.data(), sdesc.size() * sizeof(SyrkDesc), hipMemcpyHostToDevice, st));
   DVec<int64_t> dm; dm.upload(moff, st);
   DVec<int> dl; dl.upload(ld, st);
   for (int p = 0; p < nsteps; p++) {
     HC(launch_panel(st, nmat, maxrows[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M, d_err.p));
-    HC(launch_syrk(st, nmat, maxpairs[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M));
+    HC(launch_syrk(st, nmat, maxtiles[p], ds.p + (size_t)p * nmat, dm.p, dl.p, M));
   }
   HC(hipStreamSynchronize(st));
 }

@@ -260,71 +260,83 @@ hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* 
 }
 
 // ============================================================ SYRK trailing update
-// One workgroup per 64x64 lower tile (ti, tj) of the trailing block: C -= P_i P_j^T
-// with the panel rows P (64 x w, w <= 64) staged in LDS ([k][row], padded rows)
-// and the product on the matrix cores (v_mfma_f64_16x16x4f64; wave v owns rows
-// 16v..16v+15 and the four 16-column blocks; lane layouts A[l%16][l/16],
-// B[l/16][l%16], D row 4r+l/16, col l%16).
+// C[r][c] -= sum_{k in [k0, k0+w)} L[r][k] L[c][k] for t0 <= c <= r, r < end, c < cend:
+// the lower trapezoid of a right-looking partial Cholesky step (cend = end: the
+// whole trailing triangle; cend < end: only the columns of the current super-panel).
+// One workgroup per 64x64 tile (ti, tj), tj < ceil((cend - t0)/64), ti >= tj,
+// enumerated column by column.  K is streamed in chunks of 32 through LDS
+// ([k][row], padded rows); the product runs on the matrix cores
+// (v_mfma_f64_16x16x4f64, lane layouts A[l%16][l/16], B[l/16][l%16], D row 4r+l/16,
+// col l%16) as C^T blocks, so that a D column is a matrix row and the
+// read-modify-write of C is 16 consecutive doubles per lane group.  Wave v owns
+// the matrix columns 16v..16v+15 of the tile and all 64 rows.
 typedef double syrk_d4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_syrk(const PanelDesc* __restrict__ desc, const int64_t* __restrict__ moff,
+__global__ __launch_bounds__(256) void k_syrk(const SyrkDesc* __restrict__ desc, const int64_t* __restrict__ moff,
                                               const int* __restrict__ ld, double* __restrict__ M) {
   const int mi = blockIdx.y;
-  const PanelDesc pd = desc[mi];
-  if (!pd.active || pd.w <= 0) return;
-  const int k0 = pd.k0, w = pd.w, t0 = pd.k0 + pd.w, end = pd.end;
+  const SyrkDesc sd = desc[mi];
+  if (!sd.active || sd.w <= 0) return;
+  const int k0 = sd.k0, w = sd.w, t0 = sd.t0, cend = sd.cend, end = sd.end;
   const int m = end - t0;
-  if (m <= 0) return;
-  const int T = (m + 63) / 64;
-  const int q = blockIdx.x;
-  if (q >= T * (T + 1) / 2) return;
-  int ti = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
-  while (ti * (ti + 1) / 2 > q) ti--;
-  while ((ti + 1) * (ti + 2) / 2 <= q) ti++;
-  const int tj = q - ti * (ti + 1) / 2;
+  if (m <= 0 || cend <= t0) return;
+  const int T = (m + 63) / 64, Tc = min(T, (cend - t0 + 63) / 64);
+  const int64_t q = blockIdx.x;
+  auto count = [&](int64_t x) { return x * T - x * (x - 1) / 2; };  // tiles in columns < x
+  if (q >= count(Tc)) return;
+  const double tb = T + 0.5;
+  int tj = (int)(tb - sqrt(tb * tb - 2.0 * (double)q));
+  tj = max(0, min(tj, Tc - 1));
+  while (tj > 0 && count(tj) > q) tj--;
+  while (tj + 1 < Tc && count(tj + 1) <= q) tj++;
+  const int ti = tj + (int)(q - count(tj));
   const int ri = t0 + 64 * ti, cj = t0 + 64 * tj;
   double* A = M + moff[mi];
   const int64_t lda = ld[mi];
-  __shared__ double As[64 * 65];
-  __shared__ double Bs[64 * 65];
-  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-    const int k = idx >> 6, r = idx & 63;
-    const bool kin = k < w;
-    As[k * 65 + r] = (kin && ri + r < end) ? A[(ri + r) + (int64_t)(k0 + k) * lda] : 0.0;
-    Bs[k * 65 + r] = (kin && cj + r < end) ? A[(cj + r) + (int64_t)(k0 + k) * lda] : 0.0;
-  }
-  __syncthreads();
+  __shared__ double As[32 * 65];  // rows of the tile's row block
+  __shared__ double Bs[32 * 65];  // rows of the tile's column block
   const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-  const int rb = 16 * v;
   syrk_d4 acc[4];
 #pragma unroll
   for (int c = 0; c < 4; c++)
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
-  const int kw = (w + 3) & ~3;
-  for (int kk = 0; kk < kw; kk += 4) {
-    const double a = As[(kk + (lane >> 4)) * 65 + rb + (lane & 15)];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const double b = Bs[(kk + (lane >> 4)) * 65 + 16 * c + (lane & 15)];
-      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+  for (int kc = 0; kc < w; kc += 32) {
+    for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
+      const int k = idx >> 6, r = idx & 63;
+      const bool kin = kc + k < w;
+      const int64_t col = k0 + kc + k;
+      As[k * 65 + r] = (kin && ri + r < end) ? A[(ri + r) + col * lda] : 0.0;
+      Bs[k * 65 + r] = (kin && cj + r < end) ? A[(cj + r) + col * lda] : 0.0;
     }
+    __syncthreads();
+    const int kw = min(32, (w - kc + 3) & ~3);
+    for (int kk = 0; kk < kw; kk += 4) {
+      const double a = Bs[(kk + (lane >> 4)) * 65 + 16 * v + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double b = As[(kk + (lane >> 4)) * 65 + 16 * c + (lane & 15)];
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+      }
+    }
+    __syncthreads();
   }
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    const int col = cj + 16 * c + (lane & 15);
+    const int row = ri + 16 * c + (lane & 15);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int row = ri + rb + 4 * r + (lane >> 4);
-      if (row < end && col < end && row >= col) A[row + (int64_t)col * lda] -= acc[c][r];
+      const int col = cj + 16 * v + 4 * r + (lane >> 4);
+      if (row < end && col < cend && row >= col) A[row + (int64_t)col * lda] -= acc[c][r];
     }
   }
 }
 
-hipError_t launch_syrk(hipStream_t st, int nmat, int maxpairs, const PanelDesc* desc, const int64_t* moff,
+hipError_t launch_syrk(hipStream_t st, int nmat, int64_t maxtiles, const SyrkDesc* desc, const int64_t* moff,
                        const int* ld, double* M) {
-  if (maxpairs < 1) return hipSuccess;
-  hipLaunchKernelGGL(k_syrk, dim3(maxpairs, nmat), dim3(256), 0, st, desc, moff, ld, M);
+  if (maxtiles < 1) return hipSuccess;
+  if (maxtiles > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_syrk, dim3((unsigned)maxtiles, nmat), dim3(256), 0, st, desc, moff, ld, M);
   return hipGetLastError();
 }
 

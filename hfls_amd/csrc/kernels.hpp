@@ -11,6 +11,10 @@ struct PanelDesc {
   int k0, w, end, active;  // panel columns [k0,k0+w), trailing rows/cols up to end
 };
 
+struct SyrkDesc {  // update by columns [k0,k0+w) of rows/cols t0 <= c <= r < end, c < cend
+  int k0, w, t0, cend, end, active;
+};
+
 struct AsmSub {  // box subdomain for element assembly
   int e0[3], ne[3], n, nr;  // nr > 0: also write the identity coupling M[n+i][i] = 1, i < nr
   double E;
@@ -28,7 +32,7 @@ hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const i
                            double az);
 hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* desc, const int64_t* moff,
                         const int* ld, double* M, int* err);
-hipError_t launch_syrk(hipStream_t st, int nmat, int maxpairs, const PanelDesc* desc, const int64_t* moff,
+hipError_t launch_syrk(hipStream_t st, int nmat, int64_t maxtiles, const SyrkDesc* desc, const int64_t* moff,
                        const int* ld, double* M);
 hipError_t launch_copy_block(hipStream_t st, int nmat, int maxn, const double* src, const int64_t* soff,
                              const int* sld, const int* srow0, const int* n, double* dst, const int64_t* doff,
