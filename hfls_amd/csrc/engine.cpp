@@ -432,8 +432,10 @@ void Engine::partial_cholesky(double* M, const std::vector<int64_t>& moff, const
   HC(hipMemcpyAsync(ds.p, sdesc.data(), sdesc.size() * sizeof(SyrkDesc), hipMemcpyHostToDevice, st));
   DVec<int64_t> dm; dm.upload(moff, st);
   DVec<int> dl; dl.upload(ld, st);
+  DVec<double> dw;  // inverse of each matrix's current 64 x 64 diagonal factor
+  dw.alloc((size_t)nmat * 4096);
   for (int p = 0; p < nsteps; p++) {
-    HC(launch_panel(st, nmat, maxrows[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M, d_err.p));
+    HC(launch_panel(st, nmat, maxrows[p], dd.p + (size_t)p * nmat, dm.p, dl.p, M, dw.p, d_err.p));
     HC(launch_syrk(st, nmat, maxtiles[p], ds.p + (size_t)p * nmat, dm.p, dl.p, M));
   }
   HC(hipStreamSynchronize(st));

@@ -4,7 +4,7 @@
 //   k_assemble   hex8 stiffness (Data.cpp:297-490) straight into a dense
 //                interior-first matrix, lower triangle, 8-colour element order so
 //                no two waves touch one entry (deterministic, no atomics)
-//   k_panel      Cholesky of a <=64 wide diagonal block + TRSM of the rows below
+//   k_panel_*    Cholesky of a <=64 wide diagonal block (+ its inverse) and TRSM of the rows below
 //   k_syrk       trailing update of a right-looking partial Cholesky (matrix cores)
 //   k_detect     kernel detection on the Schur complement of the s-set: cyclic
 //                Jacobi, eigen-jump test (Matrix.cpp:2352-2368)
@@ -194,68 +194,117 @@ hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const i
 }
 
 // ============================================================ Cholesky panel
-// mode 0: one workgroup per matrix factors the <=64 wide diagonal block in LDS
-//         and writes it back;
-// mode 1: each workgroup solves 64 rows below the (already factored) block,
-//         X L^T = A_rows  (TRSM).  Two launches so no workgroup can read a
-//         half-written diagonal block.
-// L[c*64 + r] holds the diagonal block (column major), X[c*64 + t] the rows.
-__global__ __launch_bounds__(64) void k_panel(const PanelDesc* __restrict__ desc, const int64_t* __restrict__ moff,
-                                              const int* __restrict__ ld, double* __restrict__ M,
-                                              int* __restrict__ err, int mode) {
+// k_panel_diag: one workgroup (64 threads) per matrix factors the <=64 wide
+// diagonal block in LDS, writes it back, and writes the inverse of the factor,
+// W = L^-1 (lower, column major, zero outside w x w), for the TRSM below.
+// L[c*64 + r] holds the diagonal block (column major).
+__global__ __launch_bounds__(64) void k_panel_diag(const PanelDesc* __restrict__ desc,
+                                                   const int64_t* __restrict__ moff, const int* __restrict__ ld,
+                                                   double* __restrict__ M, double* __restrict__ Winv,
+                                                   int* __restrict__ err) {
+  const int mi = blockIdx.y;
+  const PanelDesc pd = desc[mi];
+  double* W = Winv + (int64_t)mi * 4096;
+  const int t = threadIdx.x;
+  if (!pd.active || pd.w <= 0) return;
+  const int k0 = pd.k0, w = pd.w;
+  double* A = M + moff[mi];
+  const int64_t lda = ld[mi];
+  __shared__ double L[64 * 64];
+  __shared__ double Y[64 * 65];
+  for (int c = 0; c < w; c++) L[c * 64 + t] = (t < w && t >= c) ? A[(k0 + t) + (int64_t)(k0 + c) * lda] : 0.0;
+  __syncthreads();
+  for (int j = 0; j < w; j++) {
+    const double djj = L[j * 64 + j];
+    __syncthreads();
+    double dj = 1.0;
+    if (djj > 0.0) {
+      dj = sqrt(djj);
+    } else if (t == 0) {
+      atomicOr(err, 1);
+    }
+    if (t == j) L[j * 64 + j] = dj;
+    if (t > j && t < w) L[j * 64 + t] /= dj;
+    __syncthreads();
+    if (t > j && t < w) {
+      const double ltj = L[j * 64 + t];
+      for (int k = j + 1; k <= t; k++) L[k * 64 + t] -= ltj * L[j * 64 + k];
+    }
+    __syncthreads();
+  }
+  if (t < w)
+    for (int c = 0; c <= t; c++) A[(k0 + t) + (int64_t)(k0 + c) * lda] = L[c * 64 + t];
+  // column t of L^-1 by forward substitution (thread t), kept in Y[t*65 + i]
+  for (int i = 0; i < 64; i++) {
+    double y = 0.0;
+    if (t < w && i < w && i >= t) {
+      double s = i == t ? 1.0 : 0.0;
+      for (int k = t; k < i; k++) s -= L[k * 64 + i] * Y[t * 65 + k];
+      y = s / L[i * 64 + i];
+    }
+    Y[t * 65 + i] = y;
+  }
+  __syncthreads();
+  for (int c = 0; c < 64; c++) W[c * 64 + t] = Y[c * 65 + t];
+}
+
+// k_panel_trsm: the rows below the diagonal block, X L^T = A_rows, as X = A_rows W^T
+// on the matrix cores: one workgroup (256 threads) per 64 rows, the product laid out
+// as in k_syrk (D columns are matrix rows, so the stores are row-contiguous).
+typedef double trsm_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_panel_trsm(const PanelDesc* __restrict__ desc,
+                                                    const int64_t* __restrict__ moff, const int* __restrict__ ld,
+                                                    double* __restrict__ M, const double* __restrict__ Winv) {
   const int mi = blockIdx.y;
   const PanelDesc pd = desc[mi];
   if (!pd.active || pd.w <= 0) return;
   const int k0 = pd.k0, w = pd.w, end = pd.end;
-  const int row0 = k0 + w + blockIdx.x * 64;
-  if (mode == 1 && row0 >= end) return;
+  const int ri = k0 + w + blockIdx.x * 64;
+  if (ri >= end) return;
   double* A = M + moff[mi];
   const int64_t lda = ld[mi];
-  __shared__ double L[64 * 64];
-  __shared__ double X[64 * 64];
-  const int t = threadIdx.x;
-  for (int c = 0; c < w; c++) L[c * 64 + t] = (t < w && t >= c) ? A[(k0 + t) + (int64_t)(k0 + c) * lda] : 0.0;
+  const double* W = Winv + (int64_t)mi * 4096;
+  __shared__ double As[64 * 65];  // [k][row] of the rows being solved
+  __shared__ double Bs[64 * 65];  // [k][c] = W[c][k]
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int k = idx >> 6, r = idx & 63;
+    As[k * 65 + r] = (k < w && ri + r < end) ? A[(ri + r) + (int64_t)(k0 + k) * lda] : 0.0;
+    Bs[k * 65 + r] = W[k * 64 + r];
+  }
   __syncthreads();
-  if (mode == 0) {
-    for (int j = 0; j < w; j++) {
-      const double djj = L[j * 64 + j];
-      __syncthreads();
-      double dj = 1.0;
-      if (djj > 0.0) {
-        dj = sqrt(djj);
-      } else if (t == 0) {
-        atomicOr(err, 1);
-      }
-      if (t == j) L[j * 64 + j] = dj;
-      if (t > j && t < w) L[j * 64 + t] /= dj;
-      __syncthreads();
-      if (t > j && t < w) {
-        const double ltj = L[j * 64 + t];
-        for (int k = j + 1; k <= t; k++) L[k * 64 + t] -= ltj * L[j * 64 + k];
-      }
-      __syncthreads();
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  trsm_d4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
+  const int kw = (w + 3) & ~3;
+  for (int kk = 0; kk < kw; kk += 4) {
+    const double a = Bs[(kk + (lane >> 4)) * 65 + 16 * v + (lane & 15)];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const double b = As[(kk + (lane >> 4)) * 65 + 16 * c + (lane & 15)];
+      acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
     }
-    if (t < w)
-      for (int c = 0; c <= t; c++) A[(k0 + t) + (int64_t)(k0 + c) * lda] = L[c * 64 + t];
-    return;
   }
-  const int row = row0 + t;
-  if (row >= end) return;
-  for (int c = 0; c < w; c++) X[c * 64 + t] = A[row + (int64_t)(k0 + c) * lda];
-  for (int j = 0; j < w; j++) {
-    double v = X[j * 64 + t];
-    for (int l = 0; l < j; l++) v -= X[l * 64 + t] * L[l * 64 + j];
-    X[j * 64 + t] = v / L[j * 64 + j];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int row = ri + 16 * c + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = 16 * v + 4 * r + (lane >> 4);
+      if (row < end && col < w) A[row + (int64_t)(k0 + col) * lda] = acc[c][r];
+    }
   }
-  for (int c = 0; c < w; c++) A[row + (int64_t)(k0 + c) * lda] = X[c * 64 + t];
 }
 
 hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* desc, const int64_t* moff,
-                        const int* ld, double* M, int* err) {
-  hipLaunchKernelGGL(k_panel, dim3(1, nmat), dim3(64), 0, st, desc, moff, ld, M, err, 0);
+                        const int* ld, double* M, double* Winv, int* err) {
+  hipLaunchKernelGGL(k_panel_diag, dim3(1, nmat), dim3(64), 0, st, desc, moff, ld, M, Winv, err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || maxrows <= 0) return e;
-  hipLaunchKernelGGL(k_panel, dim3((maxrows + 63) / 64, nmat), dim3(64), 0, st, desc, moff, ld, M, err, 1);
+  hipLaunchKernelGGL(k_panel_trsm, dim3((maxrows + 63) / 64, nmat), dim3(256), 0, st, desc, moff, ld, M, Winv);
   return hipGetLastError();
 }
 
@@ -301,15 +350,33 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkDesc* __restrict__ desc,
   for (int c = 0; c < 4; c++)
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
-  for (int kc = 0; kc < w; kc += 32) {
-    for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
-      const int k = idx >> 6, r = idx & 63;
+  // the next K chunk is loaded into registers while the matrix cores work on the
+  // current one from LDS
+  double pa[8], pb[8];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int idx = threadIdx.x + 256 * t, k = idx >> 6, r = idx & 63;
       const bool kin = kc + k < w;
       const int64_t col = k0 + kc + k;
-      As[k * 65 + r] = (kin && ri + r < end) ? A[(ri + r) + col * lda] : 0.0;
-      Bs[k * 65 + r] = (kin && cj + r < end) ? A[(cj + r) + col * lda] : 0.0;
+      pa[t] = (kin && ri + r < end) ? A[(ri + r) + col * lda] : 0.0;
+      pb[t] = (kin && cj + r < end) ? A[(cj + r) + col * lda] : 0.0;
     }
-    __syncthreads();
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int idx = threadIdx.x + 256 * t, k = idx >> 6, r = idx & 63;
+      As[k * 65 + r] = pa[t];
+      Bs[k * 65 + r] = pb[t];
+    }
+  };
+  fetch(0);
+  stash();
+  __syncthreads();
+  for (int kc = 0; kc < w; kc += 32) {
+    const bool more = kc + 32 < w;
+    if (more) fetch(kc + 32);
     const int kw = min(32, (w - kc + 3) & ~3);
     for (int kk = 0; kk < kw; kk += 4) {
       const double a = Bs[(kk + (lane >> 4)) * 65 + 16 * v + (lane & 15)];
@@ -320,6 +387,10 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkDesc* __restrict__ desc,
       }
     }
     __syncthreads();
+    if (more) {
+      stash();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int c = 0; c < 4; c++) {

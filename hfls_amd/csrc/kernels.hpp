@@ -31,7 +31,7 @@ hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const i
                            double* kdiag, double dx, double dy, double dz, double nu, double ax, double ay,
                            double az);
 hipError_t launch_panel(hipStream_t st, int nmat, int maxrows, const PanelDesc* desc, const int64_t* moff,
-                        const int* ld, double* M, int* err);
+                        const int* ld, double* M, double* Winv, int* err);
 hipError_t launch_syrk(hipStream_t st, int nmat, int64_t maxtiles, const SyrkDesc* desc, const int64_t* moff,
                        const int* ld, double* M);
 hipError_t launch_copy_block(hipStream_t st, int nmat, int maxn, const double* src, const int64_t* soff,
