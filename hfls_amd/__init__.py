@@ -181,9 +181,11 @@ class Mesh:
         s = np.zeros(6, dtype=np.int64)
         _check(lib.hfls_mesh_sizes(self._h, _ptr(s)))
         self.nsub, self.nlam, self.ncol, self.npacked, self.nPoints, self.device_bytes = (int(x) for x in s)
-        t = np.zeros(2)
+        t = np.zeros(5)
         _check(lib.hfls_mesh_setup_times(self._h, _ptr(t)))
         self.setup_ms, self.factor_ms = float(t[0]), float(t[1])
+        self.setup_phases_ms = {"index": float(t[2]), "assemble": float(t[3]), "factor": float(t[1]),
+                                "coarse": float(t[4])}
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -190,8 +190,12 @@ extern "C" int hfls_mesh_sizes(hfls_mesh m, int64_t* s) {
 
 extern "C" int hfls_mesh_setup_times(hfls_mesh m, double* t) {
   return guard([&] {
-    t[0] = m->e->setup_info().setup_ms;
-    t[1] = m->e->setup_info().factor_ms;
+    const hfls::SetupInfo& i = m->e->setup_info();
+    t[0] = i.setup_ms;
+    t[1] = i.factor_ms;
+    t[2] = i.meta_ms;
+    t[3] = i.assemble_ms;
+    t[4] = i.coarse_ms;
   });
 }
 

@@ -464,17 +464,31 @@ void Engine::factor_subdomains() {
   size_t freeb = 0, totb = 0;
   HC(hipMemGetInfo(&freeb, &totb));
   size_t budget = std::min<size_t>((size_t)48 << 30, freeb / 2);
-  int c0 = 0;
-  while (c0 < N) {
+  // chunks of subdomains whose scratch fits the budget; the scratch is allocated
+  // once at the largest chunk's size (allocating tens of GB takes ~0.5 s)
+  std::vector<int> cuts{0};
+  int64_t cap1 = 0, cap2 = 0, cap3 = 0;
+  while (cuts.back() < N) {
+    const int c0 = cuts.back();
     int c1 = c0;
     size_t bytes = 0;
+    int64_t s1 = 0, s2 = 0, s3 = 0;
     while (c1 < N) {
       const int64_t l1 = h_n[c1] + h_nr[c1];
-      size_t b = 8 * (align32(l1 * l1) + align32(4 * (int64_t)h_ns[c1] * h_ns[c1]) + 6 * h_n[c1]);
+      const int64_t b1 = align32(l1 * l1), b2 = align32(4 * (int64_t)h_ns[c1] * h_ns[c1]), b3 = 6 * (int64_t)h_n[c1];
+      const size_t b = 8 * (size_t)(b1 + b2 + b3);
       if (c1 > c0 && bytes + b > budget) break;
       bytes += b;
+      s1 += b1; s2 += b2; s3 += b3;
       c1++;
     }
+    cap1 = std::max(cap1, s1); cap2 = std::max(cap2, s2); cap3 = std::max(cap3, s3);
+    cuts.push_back(c1);
+  }
+  DVec<double> M1, M2, R1;
+  M1.alloc(cap1); M2.alloc(cap2); R1.alloc(cap3);
+  for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
+    const int c0 = cuts[ci], c1 = cuts[ci + 1];
     const int nm = c1 - c0;
     std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), toff(nm), kroff(nm), koff(nm), roff(nm), rboff(nm),
         t_src(nm);
@@ -522,8 +536,8 @@ void Engine::factor_subdomains() {
       span("R", off_R[d], s.n, s.n, 6, off_R[N]);
       span("Rb", off_Rb[d], s.n_s, s.n_s, 6, off_Rb[N]);
     }
-    DVec<double> M1, M2, R1;
-    M1.alloc(o1); M2.alloc(o2); R1.alloc(o3);
+    const auto ta = std::chrono::steady_clock::now();
+    if (o1 > cap1 || o2 > cap2 || o3 > cap3) throw std::runtime_error("factorisation scratch too small");
     HC(hipMemsetAsync(M1.p, 0, o1 * sizeof(double), st));
     HC(hipMemsetAsync(M2.p, 0, o2 * sizeof(double), st));
     DVec<int64_t> dm1, dm2, dr1, dnoff, dto, dkro, dk, dr, drb, dtsrc;
@@ -555,7 +569,9 @@ void Engine::factor_subdomains() {
                        P->dx[0], P->dx[1], P->dx[2], nu, acc[0], acc[1], acc[2]));
     if (lumped)  // lumped preconditioner: K_bb (Cluster.cpp:1106-1107)
       HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
+    HC(hipStreamSynchronize(st));
     auto t0 = std::chrono::steady_clock::now();
+    sinfo.assemble_ms += std::chrono::duration<double, std::milli>(t0 - ta).count();
     // M1 = [K_rr K_rb I; K_br K_bb 0; I 0 0]: eliminate the interior; the trailing
     // block is [S_bb, -(K_rr^-1 K_rb)^T ; -(K_rr^-1 K_rb), -K_rr^-1]
     {
@@ -603,7 +619,6 @@ void Engine::factor_subdomains() {
     HC(launch_copy_block(st, nm, maxns, M2.p, dm2.p, dld2.p, dns.p, dns.p, d_K.p, dk.p, dldb.p, -1.0, 0));
     HC(hipStreamSynchronize(st));
     sinfo.factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    c0 = c1;
   }
   int err = 0;
   HC(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -954,9 +969,17 @@ void Engine::setup() {
   auto t0 = std::chrono::steady_clock::now();
   d_err.alloc(1);
   HC(hipMemsetAsync(d_err.p, 0, sizeof(int), st));
+  auto ms_since = [](std::chrono::steady_clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+  };
   build_meta();
+  HC(hipStreamSynchronize(st));
+  sinfo.meta_ms = ms_since(t0);
   factor_subdomains();
+  auto t1 = std::chrono::steady_clock::now();
   build_bc_and_coarse();
+  HC(hipStreamSynchronize(st));
+  sinfo.coarse_ms = ms_since(t1);
   // work vectors
   const int64_t nl = P->nlam;
   const int N = P->nsub;

@@ -37,7 +37,10 @@ struct SolveInfo {
 
 struct SetupInfo {
   double setup_ms = 0.0;
-  double factor_ms = 0.0;
+  double factor_ms = 0.0;    // partial Cholesky, kernel detection, K+_bb (host wall, synchronised)
+  double meta_ms = 0.0;      // index structures
+  double assemble_ms = 0.0;  // scratch allocation and element assembly
+  double coarse_ms = 0.0;    // Bc, cluster and coarse problems
   size_t device_bytes = 0;
   int min_defect = 0, max_defect = 0;
 };

@@ -62,7 +62,9 @@ int hfls_mesh_create(int device, int nopt, const char* const* keys, const char* 
 int hfls_mesh_destroy(hfls_mesh m);
 /* sizes[0..5] = nsub, nlam, ncol_G, packed_local_length, nPoints, device_bytes */
 int hfls_mesh_sizes(hfls_mesh m, int64_t* sizes);
-/* setup timings: [0] setup_ms, [1] factor_ms */
+/* setup timings (ms, t holds 5): [0] whole setup, [1] factorisation and kernel
+ * detection, [2] index structures, [3] scratch allocation and element assembly,
+ * [4] Bc, cluster and coarse problems */
 int hfls_mesh_setup_times(hfls_mesh m, double* t);
 /* kernel R (packed local, n x 6 column major per subdomain), fixing DOFs (6 local
  * DOFs per subdomain), detected defect per subdomain, body-force loads (packed local) */
