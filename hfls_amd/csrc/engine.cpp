@@ -294,22 +294,25 @@ void Engine::configure(const std::string& key, const std::string& value) {
 
 void Engine::build_meta() {
   const int N = P->nsub;
-  h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N); h_ldt.resize(N);
+  h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N);
   h_cl.resize(N);
   off_n.assign(N + 1, 0); off_b.assign(N + 1, 0);
-  off_K.assign(N + 1, 0); off_T.assign(N + 1, 0); off_Kr.assign(N + 1, 0); off_R.assign(N + 1, 0);
+  off_K.assign(N + 1, 0); off_L.assign(N + 1, 0); off_r.assign(N + 1, 0); off_R.assign(N + 1, 0);
   off_Rb.assign(N + 1, 0);
   off_P.assign(N + 1, 0);
   off_part.assign(N + 1, 0);
   for (int d = 0; d < N; d++) {
     const Subdomain& s = P->subs[d];
     h_n[d] = s.n; h_nr[d] = s.n_r; h_ns[d] = s.n_s; h_nbs[d] = s.n_bs; h_nss[d] = s.n_ss;
-    h_ldb[d] = round8(s.n_s); h_ldt[d] = round8(s.n_r); h_cl[d] = s.cluster;
+    h_ldb[d] = round8(s.n_s); h_cl[d] = s.cluster;
     off_n[d + 1] = off_n[d] + s.n;
     off_b[d + 1] = off_b[d] + s.n_s;
     off_K[d + 1] = off_K[d] + align32((int64_t)h_ldb[d] * s.n_s);
-    off_T[d + 1] = off_T[d] + align32((int64_t)h_ldt[d] * s.n_s);
-    off_Kr[d + 1] = off_Kr[d] + align32((int64_t)h_ldt[d] * s.n_r);
+    const int TL = (s.n_r + 63) / 64;
+    off_L[d + 1] = off_L[d] + (int64_t)TL * (TL + 1) / 2 * 4096;
+    off_r[d + 1] = off_r[d] + s.n_r;
+    maxTL = std::max(maxTL, TL);
+    maxLtiles = std::max(maxLtiles, TL * (TL + 1) / 2);
     off_R[d + 1] = off_R[d] + 6 * (int64_t)s.n;
     off_Rb[d + 1] = off_Rb[d] + 6 * (int64_t)s.n_s;
     const int T = (s.n_s + 63) / 64;
@@ -322,13 +325,12 @@ void Engine::build_meta() {
   if (maxns > 8000) throw std::runtime_error("interface too large for the LDS-staged gather (n_s > 8000)");
   NB = off_b[N];
   d_n.upload(h_n, st); d_nr.upload(h_nr, st); d_ns.upload(h_ns, st); d_nbs.upload(h_nbs, st);
-  d_ldb.upload(h_ldb, st); d_ldt.upload(h_ldt, st); d_cl.upload(h_cl, st);
+  d_ldb.upload(h_ldb, st); d_cl.upload(h_cl, st);
   d_ldn.upload(h_n, st);
   d_off_n.upload(std::vector<int64_t>(off_n.begin(), off_n.end() - 1), st);
   d_off_b.upload(std::vector<int64_t>(off_b.begin(), off_b.end() - 1), st);
   d_off_K.upload(std::vector<int64_t>(off_K.begin(), off_K.end() - 1), st);
-  d_off_T.upload(std::vector<int64_t>(off_T.begin(), off_T.end() - 1), st);
-  d_off_Kr.upload(std::vector<int64_t>(off_Kr.begin(), off_Kr.end() - 1), st);
+  d_off_L.upload(std::vector<int64_t>(off_L.begin(), off_L.end() - 1), st);
   d_off_R.upload(std::vector<int64_t>(off_R.begin(), off_R.end() - 1), st);
   d_off_Rb.upload(std::vector<int64_t>(off_Rb.begin(), off_Rb.end() - 1), st);
   d_off_P.upload(std::vector<int64_t>(off_P.begin(), off_P.end() - 1), st);
@@ -366,6 +368,53 @@ void Engine::build_meta() {
   d_perm1.upload(perm, st);
   d_gdof.upload(gdof, st);
   d_owner.upload(owner, st);
+  // sparse coupling between interior and b DOFs: the hex8 stencil (nodes within one
+  // step along every axis inside the subdomain box), columns ascending
+  NR = off_r[N];
+  {
+    std::vector<int64_t> brp(NB + 1, 0), rbp(NR + 1, 0);
+    std::vector<int> brc, rbc, bloc(NB), rsub(NR), rloc(NR);
+    std::vector<int> cols;
+    for (int d = 0; d < N; d++) {
+      const Subdomain& s = P->subs[d];
+      std::vector<int> rpos(s.n, -1);
+      for (int q = 0; q < s.n_r; q++) rpos[s.rdofs[q]] = q;
+      auto neighbours = [&](int l, const std::vector<int>& pos) {
+        cols.clear();
+        const int ln = l / 3;
+        const int i = ln % s.nn[0], j = (ln / s.nn[0]) % s.nn[1], kk = ln / (s.nn[0] * s.nn[1]);
+        for (int dk = -1; dk <= 1; dk++)
+          for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+              const int a = i + di, b = j + dj, c = kk + dk;
+              if (a < 0 || b < 0 || c < 0 || a >= s.nn[0] || b >= s.nn[1] || c >= s.nn[2]) continue;
+              const int node = a + b * s.nn[0] + c * s.nn[0] * s.nn[1];
+              for (int comp = 0; comp < 3; comp++) {
+                const int p = pos[3 * node + comp];
+                if (p >= 0) cols.push_back(p);
+              }
+            }
+        std::sort(cols.begin(), cols.end());
+      };
+      for (int p = 0; p < s.n_s; p++) {
+        neighbours(s.border[p], rpos);
+        brc.insert(brc.end(), cols.begin(), cols.end());
+        brp[off_b[d] + p + 1] = (int64_t)brc.size();
+        bloc[off_b[d] + p] = p;
+      }
+      for (int q = 0; q < s.n_r; q++) {
+        neighbours(s.rdofs[q], s.bpos);
+        rbc.insert(rbc.end(), cols.begin(), cols.end());
+        rbp[off_r[d] + q + 1] = (int64_t)rbc.size();
+        rsub[off_r[d] + q] = d;
+        rloc[off_r[d] + q] = q;
+      }
+    }
+    d_brp.upload(brp, st); d_brc.upload(brc, st); d_bloc.upload(bloc, st);
+    d_rbp.upload(rbp, st); d_rbc.upload(rbc, st); d_rsub.upload(rsub, st); d_rloc.upload(rloc, st);
+    d_brv.alloc(std::max<size_t>(brc.size(), 1));
+    d_rbv.alloc(std::max<size_t>(rbc.size(), 1));
+  }
   // gather lists over packed b positions and dual-row entries
   std::vector<std::vector<int>> lists(NB);
   const int64_t nl = P->nlam;
@@ -447,8 +496,7 @@ void Engine::factor_subdomains() {
   d_kdiag.alloc(off_n[N]);
   d_f.alloc(off_n[N]);
   HC(hipMemsetAsync(d_f.p, 0, off_n[N] * sizeof(double), st));
-  d_T.alloc(off_T[N]);
-  d_Kinv.alloc(off_Kr[N]);
+  d_Lt.alloc(std::max<int64_t>(off_L[N], 1));
   d_K.alloc(off_K[N]);
   d_S.alloc(off_K[N]);
   d_R.alloc(off_R[N]);
@@ -467,14 +515,14 @@ void Engine::factor_subdomains() {
   // chunks of subdomains whose scratch fits the budget; the scratch is allocated
   // once at the largest chunk's size (allocating tens of GB takes ~0.5 s)
   std::vector<int> cuts{0};
-  int64_t cap1 = 0, cap2 = 0, cap3 = 0;
+  int64_t cap1 = 0, cap2 = 0, cap3 = 0, capL = 0;
   while (cuts.back() < N) {
     const int c0 = cuts.back();
     int c1 = c0;
     size_t bytes = 0;
     int64_t s1 = 0, s2 = 0, s3 = 0;
     while (c1 < N) {
-      const int64_t l1 = h_n[c1] + h_nr[c1];
+      const int64_t l1 = h_n[c1];
       const int64_t b1 = align32(l1 * l1), b2 = align32(4 * (int64_t)h_ns[c1] * h_ns[c1]), b3 = 6 * (int64_t)h_n[c1];
       const size_t b = 8 * (size_t)(b1 + b2 + b3);
       if (c1 > c0 && bytes + b > budget) break;
@@ -483,16 +531,16 @@ void Engine::factor_subdomains() {
       c1++;
     }
     cap1 = std::max(cap1, s1); cap2 = std::max(cap2, s2); cap3 = std::max(cap3, s3);
+    capL = std::max(capL, off_L[c1] - off_L[c0]);
     cuts.push_back(c1);
   }
-  DVec<double> M1, M2, R1;
-  M1.alloc(cap1); M2.alloc(cap2); R1.alloc(cap3);
+  DVec<double> M1, M2, R1, R1T, LS;  // LS: the chunk's L_rr tiles (W = L^-1 is kept)
+  M1.alloc(cap1); M2.alloc(cap2); R1.alloc(cap3); R1T.alloc(cap3); LS.alloc(std::max<int64_t>(capL, 1));
   for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
     const int c0 = cuts[ci], c1 = cuts[ci + 1];
     const int nm = c1 - c0;
-    std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), toff(nm), kroff(nm), koff(nm), roff(nm), rboff(nm),
-        t_src(nm);
-    std::vector<int> ld1(nm), ld2(nm), nrv(nm), nsv(nm), nbsv(nm), nssv(nm), nv(nm), ldb(nm), ldt(nm), end1(nm),
+    std::vector<int64_t> m1off(nm), m2off(nm), r1off(nm), noff(nm), loff(nm), koff(nm), roff(nm), rboff(nm);
+    std::vector<int> ld1(nm), ld2(nm), nrv(nm), nsv(nm), nbsv(nm), nssv(nm), nv(nm), ldb(nm), end1(nm),
         zero(nm, 0), ones(nm, 1), six(nm, 6), ldn(nm);
     std::vector<int> start_bs(nm), prow0(nm), r0s(nm);
     std::vector<AsmSub> asub(nm);
@@ -500,19 +548,18 @@ void Engine::factor_subdomains() {
     for (int i = 0; i < nm; i++) {
       int d = c0 + i;
       const Subdomain& s = P->subs[d];
-      const int64_t l1 = s.n + s.n_r;
+      const int64_t l1 = s.n;
       m1off[i] = o1; o1 += align32(l1 * l1);
       m2off[i] = o2; o2 += align32(4 * (int64_t)s.n_s * s.n_s);
       r1off[i] = o3; o3 += 6 * (int64_t)s.n;
-      noff[i] = off_n[d]; toff[i] = off_T[d]; kroff[i] = off_Kr[d]; koff[i] = off_K[d]; roff[i] = off_R[d];
+      noff[i] = off_n[d]; loff[i] = off_L[d]; koff[i] = off_K[d]; roff[i] = off_R[d];
       rboff[i] = off_Rb[d];
-      t_src[i] = m1off[i] + s.n + (int64_t)s.n_r * l1;  // block (n.., n_r..) = -K_rr^-1 K_rb
       ld1[i] = (int)l1; ld2[i] = 2 * s.n_s; nrv[i] = s.n_r; nsv[i] = s.n_s; nbsv[i] = s.n_bs; nssv[i] = s.n_ss;
-      nv[i] = s.n; ldb[i] = h_ldb[d]; ldt[i] = h_ldt[d]; end1[i] = (int)l1; ldn[i] = s.n;
+      nv[i] = s.n; ldb[i] = h_ldb[d]; end1[i] = (int)l1; ldn[i] = s.n;
       prow0[i] = s.n_r + s.n_bs;
       r0s[i] = s.n_r;
       for (int a = 0; a < 3; a++) { asub[i].e0[a] = s.e0[a]; asub[i].ne[a] = s.ne[a]; }
-      asub[i].n = s.n; asub[i].nr = s.n_r; asub[i].E = s.E;
+      asub[i].n = s.n; asub[i].nr = 0; asub[i].E = s.E;  // plain K (no identity coupling)
     }
     // host-side extent checks of every block a kernel of this chunk touches:
     // (offset, ld, rows, cols) must stay inside its buffer
@@ -529,9 +576,6 @@ void Engine::factor_subdomains() {
       span("M2", m2off[i], ld2[i], ld2[i], ld2[i], o2);
       span("R1", r1off[i], s.n, s.n, 6, o3);
       span("R1 s-rows", r1off[i] + prow0[i], s.n, s.n_ss, 6, o3);
-      span("T source", t_src[i], l1, s.n_r, s.n_s, o1);
-      span("T", off_T[d], h_ldt[d], s.n_r, s.n_s, off_T[N]);
-      span("Kinv", off_Kr[d], h_ldt[d], s.n_r, s.n_r, off_Kr[N]);
       span("K+", off_K[d], h_ldb[d], s.n_s, s.n_s, off_K[N]);
       span("R", off_R[d], s.n, s.n, 6, off_R[N]);
       span("Rb", off_Rb[d], s.n_s, s.n_s, 6, off_Rb[N]);
@@ -540,14 +584,13 @@ void Engine::factor_subdomains() {
     if (o1 > cap1 || o2 > cap2 || o3 > cap3) throw std::runtime_error("factorisation scratch too small");
     HC(hipMemsetAsync(M1.p, 0, o1 * sizeof(double), st));
     HC(hipMemsetAsync(M2.p, 0, o2 * sizeof(double), st));
-    DVec<int64_t> dm1, dm2, dr1, dnoff, dto, dkro, dk, dr, drb, dtsrc;
-    DVec<int> dld1, dld2, dnr, dns, dnbs, dnss, dn, dldb, dldt, dprow0, dr0s, dpinv, dsix, dldn, dnrow0;
+    DVec<int64_t> dm1, dm2, dr1, dnoff, dlo, dk, dr, drb;
+    DVec<int> dld1, dld2, dnr, dns, dnbs, dnss, dn, dldb, dprow0, dr0s, dpinv, dsix, dldn, dnrow0;
     DVec<AsmSub> das;
     dm1.upload(m1off, st); dm2.upload(m2off, st); dr1.upload(r1off, st); dnoff.upload(noff, st);
-    dto.upload(toff, st); dkro.upload(kroff, st); dk.upload(koff, st); dr.upload(roff, st); drb.upload(rboff, st);
-    dtsrc.upload(t_src, st);
+    dlo.upload(loff, st); dk.upload(koff, st); dr.upload(roff, st); drb.upload(rboff, st);
     dld1.upload(ld1, st); dld2.upload(ld2, st); dnr.upload(nrv, st); dns.upload(nsv, st); dnbs.upload(nbsv, st);
-    dnss.upload(nssv, st); dn.upload(nv, st); dldb.upload(ldb, st); dldt.upload(ldt, st); dprow0.upload(prow0, st);
+    dnss.upload(nssv, st); dn.upload(nv, st); dldb.upload(ldb, st); dprow0.upload(prow0, st);
     dr0s.upload(r0s, st); dsix.upload(six, st); dldn.upload(ldn, st); dnrow0.upload(nv, st);
     das.alloc(nm);
     HC(hipMemcpyAsync(das.p, asub.data(), nm * sizeof(AsmSub), hipMemcpyHostToDevice, st));
@@ -569,19 +612,33 @@ void Engine::factor_subdomains() {
                        P->dx[0], P->dx[1], P->dx[2], nu, acc[0], acc[1], acc[2]));
     if (lumped)  // lumped preconditioner: K_bb (Cluster.cpp:1106-1107)
       HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
+    // the coupling K_br / K_rb of this chunk's subdomains, before the elimination
+    HC(launch_csr_extract(st, off_b[c0], off_b[c1], c0, 0, d_brp.p, d_brc.p, d_bsub.p, d_bloc.p, dnr.p, M1.p, dm1.p,
+                          dld1.p, d_brv.p));
+    HC(launch_csr_extract(st, off_r[c0], off_r[c1], c0, 1, d_rbp.p, d_rbc.p, d_rsub.p, d_rloc.p, dnr.p, M1.p, dm1.p,
+                          dld1.p, d_rbv.p));
     HC(hipStreamSynchronize(st));
     auto t0 = std::chrono::steady_clock::now();
     sinfo.assemble_ms += std::chrono::duration<double, std::milli>(t0 - ta).count();
-    // M1 = [K_rr K_rb I; K_br K_bb 0; I 0 0]: eliminate the interior; the trailing
-    // block is [S_bb, -(K_rr^-1 K_rb)^T ; -(K_rr^-1 K_rb), -K_rr^-1]
+    // M1 = [K_rr K_rb; K_br K_bb]: eliminate the interior; the first n_r columns hold
+    // [L_rr; L_br], the trailing block is S_bb
     {
       std::vector<int> st0(nm, 0);
       partial_cholesky(M1.p, m1off, ld1, st0, nrv, end1);
     }
+    // L_rr as packed lower tiles (chunk scratch), diagonal tiles inverted, then the
+    // kept inverse W = L_rr^-1 in the same layout
+    {
+      std::vector<int64_t> lso(nm);
+      for (int i = 0; i < nm; i++) lso[i] = off_L[c0 + i] - off_L[c0];
+      DVec<int64_t> dlso; dlso.upload(lso, st);
+      HC(launch_pack_f64(st, nm, maxLtiles, 0, dnr.p, dnr.p, M1.p, dm1.p, dld1.p, LS.p, dlso.p));
+      HC(launch_lrr_diag_inv(st, nm, maxTL, dnr.p, LS.p, dlso.p));
+      HC(launch_lrr_trtri(st, nm, maxTL, dnr.p, LS.p, dlso.p, d_Lt.p, dlo.p));
+      HC(hipStreamSynchronize(st));
+    }
     if (!lumped) HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, d_S.p, dk.p, dldb.p, 1.0, 0));
     HC(launch_copy_block(st, nm, maxns, M1.p, dm1.p, dld1.p, dr0s.p, dns.p, M2.p, dm2.p, dld2.p, 1.0, 1));
-    HC(launch_copy_cols(st, nm, maxns, dnr.p, dns.p, M1.p, dtsrc.p, dld1.p, d_T.p, dto.p, dldt.p, -1.0));
-    HC(launch_copy_block(st, nm, maxnr, M1.p, dm1.p, dld1.p, dnrow0.p, dnr.p, d_Kinv.p, dkro.p, dldt.p, -1.0, 0));
     // M2: eliminate b \ s
     {
       std::vector<int> st0(nm, 0), end2(nm);
@@ -603,9 +660,11 @@ void Engine::factor_subdomains() {
       HC(launch_coldot(st, nm, maxns, M2.p, dm2.p, dld2.p, dnbs.p, dnss.p, dnbs.p, R1.p, drs.p, dldn.p, R1.p, drbo.p,
                        dldn.p, 6, -1.0, 0.0));
       HC(launch_bwd(st, nm, M2.p, dm2.p, dld2.p, dnbs.p, R1.p, drbo.p, dldn.p, 6));
-      // R_r = -(K_rr^-1 K_rb) R_b  (Matrix.cpp:2393-2421)
-      HC(launch_gemm_nn(st, nm, maxnr, 6, dnr.p, dns.p, dsix.p, d_T.p, dto.p, dldt.p, R1.p, drbo.p, dldn.p, R1.p,
-                        dr1.p, dldn.p, -1.0));
+      // R_r = -K_rr^-1 K_rb R_b  (Matrix.cpp:2393-2421): sparse product, then the two
+      // triangular solves with L_rr
+      HC(launch_csr_apply(st, off_r[c0], off_r[c1], c0, 6, d_rbp.p, d_rbc.p, d_rbv.p, d_rsub.p, d_rloc.p, R1.p,
+                          drbo.p, 1, 0, R1.p, dr1.p, 1, 0, dldn.p, -1.0, 0.0));
+      rr_solve(nm, c0, 6, R1.p, dr1.p, 1, 0, dldn.p, R1T.p, dr1.p);
       HC(launch_gs_pivots(st, nm, d_R.p, dr.p, dn.p, d_perm1.p, dnoff.p, dnr.p, dns.p, dnbs.p, d_Rb.p, drb.p, R1.p,
                           dr1.p, d_kdiag.p, dnoff.p, M2.p, dm2.p, dld2.p, d_fixpos.p + 6 * c0));
       HC(hipStreamSynchronize(st));
@@ -997,6 +1056,7 @@ void Engine::setup() {
   part.alloc(nparts);
   X1.alloc(off_n[N]);
   X2.alloc(off_n[N]);
+  rtmp.alloc(off_n[N]);
   partbuf.alloc(off_part[N]);
   xb.alloc(NB);
   vloc.alloc(off_n[N]);
@@ -1007,7 +1067,8 @@ void Engine::setup() {
   sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   size_t freeb = 0, totb = 0;
   HC(hipMemGetInfo(&freeb, &totb));
-  sinfo.device_bytes = (off_T[N] + off_Kr[N] + 2 * off_P[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double);
+  sinfo.device_bytes = (off_L[N] + 2 * off_P[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double) +
+                       (d_brv.n + d_rbv.n) * (sizeof(double) + sizeof(int));
 }
 
 // ------------------------------------------------------------ solve pieces
@@ -1041,12 +1102,19 @@ void Engine::M_apply(const double* x, double* out, const int* done) {
   HC(launch_scatter(st, P->nlam, d_ra.p, d_rb.p, yb.p, d_wgt.p, nullptr, out, nullptr, nullptr, done));
 }
 
+void Engine::rr_solve(int nsub, int sub0, int k, double* X, const int64_t* xbase, int64_t rs, int64_t cs,
+                      const int* csv, double* tmp, const int64_t* tbase) {
+  HC(launch_trmv(st, nsub, maxTL, 0, k, d_nr.p + sub0, d_Lt.p, d_off_L.p + sub0, X, xbase, rs, cs, tmp, tbase, csv));
+  HC(launch_trmv(st, nsub, maxTL, 1, k, d_nr.p + sub0, d_Lt.p, d_off_L.p + sub0, tmp, tbase, rs, cs, X, xbase, csv));
+}
+
 void Engine::prep_t(const double* vin) {
   const int N = P->nsub;
   HC(launch_permute(st, N, maxn, 0, d_n.p, d_perm1.p, d_off_n.p, vin, d_off_n.p, X1.p, d_off_n.p, 1, 0, 0));
-  // X1[nr:n] = v_b - (K_rr^-1 K_rb)^T v_r
-  HC(launch_coldot(st, N, maxns, d_T.p, d_off_T.p, d_ldt.p, d_zeros.p, d_nr.p, d_ns.p, X1.p, d_off_n.p, d_ldn.p, X1.p,
-                   d_xboff.p, d_ldn.p, 1, -1.0, 1.0));
+  // X1[0:nr] = K_rr^-1 v_r ; X1[nr:n] = v_b - K_br K_rr^-1 v_r
+  rr_solve(N, 0, 1, X1.p, d_off_n.p, 1, 0, nullptr, rtmp.p, d_off_n.p);
+  HC(launch_csr_apply(st, 0, NB, 0, 1, d_brp.p, d_brc.p, d_brv.p, d_bsub.p, d_bloc.p, X1.p, d_off_n.p, 1, 0, X1.p,
+                      d_xboff.p, 1, 0, nullptr, -1.0, 1.0));
 }
 
 void Engine::symv(const double* tiles, const double* x, const int64_t* xoff, const int* done) {
@@ -1070,9 +1138,11 @@ void Engine::cluster_solve_t(const double* vin) {
 
 void Engine::interior(double* xout) {
   const int N = P->nsub;
-  // X2[0:nr] = K_rr^-1 v_r - (K_rr^-1 K_rb) yb ; X2[nr:n] = yb
-  HC(launch_gemv_two(st, N, maxnr, std::max(maxnr, maxns), d_nr.p, d_nr.p, d_ns.p, d_Kinv.p, d_off_Kr.p, d_ldt.p,
-                     d_T.p, d_off_T.p, d_ldt.p, X1.p, d_off_n.p, yb.p, d_off_b.p, -1.0, X2.p, d_off_n.p));
+  // X2[0:nr] = K_rr^-1 v_r - K_rr^-1 K_rb yb (X1[0:nr] holds K_rr^-1 v_r) ; X2[nr:n] = yb
+  HC(launch_csr_apply(st, 0, NR, 0, 1, d_rbp.p, d_rbc.p, d_rbv.p, d_rsub.p, d_rloc.p, yb.p, d_off_b.p, 1, 0, X2.p,
+                      d_off_n.p, 1, 0, nullptr, 1.0, 0.0));
+  rr_solve(N, 0, 1, X2.p, d_off_n.p, 1, 0, nullptr, rtmp.p, d_off_n.p);
+  HC(launch_rr_axpby(st, N, maxnr, 1, d_nr.p, X1.p, d_off_n.p, X2.p, d_off_n.p, 1, 0, 1.0, -1.0));
   HC(launch_copy_cols(st, N, 1, d_ns.p, d_ones.p, yb.p, d_off_b.p, d_ns.p, X2.p, d_xboff.p, d_ns.p, 1.0));
   HC(launch_permute(st, N, maxn, 1, d_n.p, d_perm1.p, d_off_n.p, X2.p, d_off_n.p, xout, d_off_n.p, 1, 0, 0));
 }

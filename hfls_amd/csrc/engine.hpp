@@ -152,18 +152,25 @@ class Engine {
   int maxn = 0, maxns = 0, maxnr = 0, maxnlc = 0;
   int64_t NB = 0;
   // per-subdomain meta (host copies)
-  std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_ldt, h_cl, h_nlc;
-  std::vector<int64_t> off_K, off_T, off_Kr, off_R, off_Rb, off_C, off_lc, off_P, off_part;
+  std::vector<int> h_n, h_nr, h_ns, h_nbs, h_nss, h_ldb, h_cl, h_nlc;
+  std::vector<int64_t> off_K, off_L, off_R, off_Rb, off_C, off_lc, off_P, off_part, off_r;
+  int64_t NR = 0;       // interior rows, all subdomains
+  int maxTL = 0, maxLtiles = 0;  // tiles per side / per subdomain of L_rr
   // device meta
-  DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_ldt, d_cl, d_nlc, d_ldn, d_ones, d_sixes, d_zeros;
+  DVec<int> d_n, d_nr, d_ns, d_nbs, d_ldb, d_cl, d_nlc, d_ldn, d_ones, d_sixes, d_zeros;
   DVec<int64_t> d_off_P, d_off_part;
-  DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_T, d_off_Kr, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
+  DVec<int64_t> d_off_n, d_off_b, d_off_K, d_off_L, d_off_R, d_off_Rb, d_off_C, d_off_lc, d_alpha_idx, d_xboff,
       d_v6off;
+  // interior factor L_rr (packed lower tiles, inverted diagonal tiles) and the sparse
+  // coupling K_br (rows: b positions) / K_rb (rows: interior positions)
+  DVec<double> d_Lt, d_brv, d_rbv;
+  DVec<int64_t> d_brp, d_rbp;
+  DVec<int> d_brc, d_rbc, d_bloc, d_rsub, d_rloc;
   DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap, d_bsub;
   DVec<double> d_Kt, d_St, partbuf, xb;  // packed lower tiles of K+_bb and S_i, tile partials
   int maxT = 0, maxtiles = 0;
   DVec<unsigned char> d_owner;
-  DVec<double> d_kdiag, d_f, d_T, d_Kinv, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
+  DVec<double> d_kdiag, d_f, d_K, d_S, d_R, d_Rb, d_C, d_BcT, d_H, d_invGtG;
   DVec<int> d_fixpos, d_defect, d_err;
   // dual rows
   DVec<int> d_ra, d_rb, d_rsa, d_rsb, d_rpa, d_rpb;
@@ -176,7 +183,8 @@ class Engine {
   DVec<double> d_Ainv;
   int max_cl_len = 0;
   // work vectors
-  DVec<double> lam, g, w, Fw, Pg, z, Pz, dvec, yb, cvec, avec, gc, la, v6, h, hg, alpha, part, X1, X2, vloc, xloc, ug;
+  DVec<double> lam, g, w, Fw, Pg, z, Pz, dvec, yb, cvec, avec, gc, la, v6, h, hg, alpha, part, X1, X2, rtmp, vloc, xloc,
+      ug;
   DVec<Scalars> sc;
   int nparts = 0;
   bool lumped = false;
@@ -205,7 +213,7 @@ class Engine {
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
-      b_Vr, b_Xr, b_V, b_V2, b_X, b_U;
+      b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;
   int b_nparts = 0;
   void block_prepare();
@@ -218,6 +226,7 @@ class Engine {
   void b_corr(int k, int with_R, int defl, const int* gate);
   void crq_products(const int* gate);
   void b_kplus(int k, bool interior_too, double* V);  // K_c^+ pieces of the local block V
+  void b_rr_solve(double* X);
 
   void build_meta();
   void factor_subdomains();
@@ -232,7 +241,11 @@ class Engine {
   void prep_t(const double* vloc_in);        // X1 = [v_r; t_b], t_b = v_b - K_br K_rr^-1 v_r
   void kplus_b_from_t();                     // yb = K+_bb t_b = (K^+ v)_b
   void cluster_solve_t(const double* vloc_in);  // la = Ac^-1 [C^T t; -R^T v]
-  void interior(double* xout);               // xout = [K_rr^-1 v_r - (K_rr^-1 K_rb) yb ; yb] in local order
+  void interior(double* xout);               // xout = [K_rr^-1 v_r - K_rr^-1 K_rb yb ; yb] in local order
+  // X <- K_rr^-1 X = W^T W X on the interior rows of every subdomain (element (i, j)
+  // of d at X[xbase[d] + i*rs + j*cs]); xbase holds the subdomains' bases from sub0 on
+  void rr_solve(int nsub, int sub0, int k, double* X, const int64_t* xbase, int64_t rs, int64_t cs, const int* csv,
+                double* tmp, const int64_t* tbase);  // tmp: same layout as X
 };
 
 }  // namespace hfls

@@ -229,6 +229,7 @@ void Engine::block_prepare() {
   zero_alloc(b_gt, (int64_t)6 * N * KP, st);
   zero_alloc(b_Vr, NRP * KP, st);
   zero_alloc(b_Xr, NRP * KP, st);
+  zero_alloc(b_Tr, NRP * KP, st);
   zero_alloc(b_V, np * KMAX, st);
   zero_alloc(b_V2, np * KMAX, st);
   zero_alloc(b_X, np * KMAX, st);
@@ -271,7 +272,7 @@ size_t Engine::block_bytes() const {
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
-                                &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_V, &b_V2, &b_X, &b_U})
+                                &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
 }
@@ -371,6 +372,16 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
                        out, nullptr, nullptr, gate));
 }
 
+// X <- K_rr^-1 X = W^T (W X) on an interior block (rows orrow, ld KP), as two
+// lower-triangular tile GEMMs with the kept W = L_rr^-1
+void Engine::b_rr_solve(double* X) {
+  const int N = P->nsub;
+  HC(launch_tgemm_f64(st, N, maxTr, 3, 0, d_nr.p, d_nr.p, d_Lt.p, d_off_L.p, nullptr, X, b_roff.p, b_Tr.p, b_roff.p,
+                      nullptr, KP, 1.0, nullptr));
+  HC(launch_tgemm_f64(st, N, maxTr, 3, 1, d_nr.p, d_nr.p, d_Lt.p, d_off_L.p, nullptr, b_Tr.p, b_roff.p, X, b_roff.p,
+                      nullptr, KP, 1.0, nullptr));
+}
+
 // From the local block V (FP64, column major): b_Vr = v_r, b_xb = t_b = v_b - (K_rr^-1 K_rb)^T v_r,
 // b_yb = (K^+ (v - Bc^T lam_c))_b (+ R_b alpha_c if with_R), b_la = [lam_c; alpha_c],
 // b_v6 = R^T v (Cluster.cpp:967-1040).  With interior_too, b_X = K^+(v - Bc^T lam_c)
@@ -380,9 +391,12 @@ void Engine::b_kplus(int k, bool interior_too, double* V) {
   const int64_t np = off_n[N];
   HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, V, b_Vr.p, b_orrow.p,
                        b_xb.p, b_obrow.p));
-  if (maxTr > 0)
-    HC(launch_tgemm_f64(st, N, maxTb, 2, 1, d_nr.p, d_ns.p, d_T.p, d_off_T.p, d_ldt.p, b_Vr.p, b_roff.p, b_xb.p,
-                        b_xoff.p, b_xb.p, KP, -1.0, nullptr));
+  // b_Vr = K_rr^-1 v_r = W^T (W v_r) ; t_b = v_b - K_br K_rr^-1 v_r
+  if (maxTr > 0) {
+    b_rr_solve(b_Vr.p);
+    HC(launch_csr_apply(st, 0, NB, 0, k, d_brp.p, d_brc.p, d_brv.p, d_bsub.p, d_bloc.p, b_Vr.p, b_roff.p, KP, 1,
+                        b_xb.p, b_xoff.p, KP, 1, nullptr, -1.0, 1.0));
+  }
   HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                       b_xoff.p, nullptr, KP, 1.0, nullptr));
   crq_products(nullptr);
@@ -393,12 +407,12 @@ void Engine::b_kplus(int k, bool interior_too, double* V) {
                       b_la.p, b_gcoff.p, nullptr, KP, 1.0, nullptr));
   b_corr(k, interior_too ? 0 : 1, 0, nullptr);
   if (!interior_too) return;
-  // x_r = K_rr^-1 v_r - (K_rr^-1 K_rb) y_b
+  // x_r = K_rr^-1 v_r - K_rr^-1 K_rb y_b
   if (maxTr > 0) {
-    HC(launch_tgemm_f64(st, N, maxTr, 2, 0, d_nr.p, d_nr.p, d_Kinv.p, d_off_Kr.p, d_ldt.p, b_Vr.p, b_roff.p, b_Xr.p,
-                        b_roff.p, nullptr, KP, 1.0, nullptr));
-    HC(launch_tgemm_f64(st, N, maxTr, 2, 0, d_nr.p, d_ns.p, d_T.p, d_off_T.p, d_ldt.p, b_yb.p, b_xoff.p, b_Xr.p,
-                        b_roff.p, b_Xr.p, KP, -1.0, nullptr));
+    HC(launch_csr_apply(st, 0, NR, 0, k, d_rbp.p, d_rbc.p, d_rbv.p, d_rsub.p, d_rloc.p, b_yb.p, b_xoff.p, KP, 1,
+                        b_Xr.p, b_roff.p, KP, 1, nullptr, 1.0, 0.0));
+    b_rr_solve(b_Xr.p);
+    HC(launch_rr_axpby(st, N, maxnr, k, d_nr.p, b_Vr.p, b_roff.p, b_Xr.p, b_roff.p, KP, 1, 1.0, -1.0));
   }
   HC(launch_kb_permute(st, N, maxn, 1, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, b_X.p, b_Xr.p, b_orrow.p,
                        b_yb.p, b_obrow.p));

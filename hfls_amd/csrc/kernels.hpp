@@ -25,6 +25,24 @@ struct Scalars {  // PCPG scalars kept on the device
   int it, done, max_iter, err;
 };
 
+// ---------------------------------------------------------------- interior solves (lrr.hip)
+hipError_t launch_lrr_diag_inv(hipStream_t st, int nsub, int maxT, const int* nr, double* Lt, const int64_t* loff);
+hipError_t launch_lrr_trtri(hipStream_t st, int nsub, int maxT, const int* nr, const double* Lt, const int64_t* loff,
+                            double* Wt, const int64_t* woff);
+hipError_t launch_trmv(hipStream_t st, int nsub, int maxT, int op, int k, const int* nr, const double* Wt,
+                       const int64_t* woff, const double* X, const int64_t* xbase, int64_t rs, int64_t cs,
+                       double* Y, const int64_t* ybase, const int* csv = nullptr);
+hipError_t launch_csr_apply(hipStream_t st, int64_t row0, int64_t row1, int sub0, int k, const int64_t* ptr,
+                            const int* col, const double* val, const int* rsub, const int* rloc, const double* X,
+                            const int64_t* xbase, int64_t xrs, int64_t xcs, double* Y, const int64_t* ybase,
+                            int64_t yrs, int64_t ycs, const int* csv, double alpha, double beta);
+hipError_t launch_csr_extract(hipStream_t st, int64_t row0, int64_t row1, int sub0, int trans, const int64_t* ptr,
+                              const int* col, const int* rsub, const int* rloc, const int* nr, const double* M,
+                              const int64_t* moff, const int* ld, double* val);
+hipError_t launch_rr_axpby(hipStream_t st, int nsub, int maxnr, int k, const int* nr, const double* A,
+                           const int64_t* abase, double* B, const int64_t* bbase, int64_t rs, int64_t cs,
+                           double alpha, double beta);
+
 // ---------------------------------------------------------------- setup
 hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const int64_t* moff, const int* ld,
                            const int* pinv, const int64_t* pinv_off, double* M, double* f, const int64_t* f_off,

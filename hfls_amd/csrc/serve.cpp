@@ -54,10 +54,12 @@ size_t Server::estimate(const Problem& p) const {
   const double nl = (double)p.nlam, ng = 3.0 * (double)p.nPoints;
   double np = 0;
   for (const Subdomain& s : p.subs) {
-    const int64_t ldb = round8l(s.n_s), ldt = round8l(s.n_r);
+    const int64_t ldb = round8l(s.n_s);
     const int64_t T = (s.n_s + 63) / 64, tiles = T * (T + 1) / 2 * 4096;
+    const int64_t TL = (s.n_r + 63) / 64;
     np += s.n;
-    b += 8.0 * (align32l(ldt * s.n_s) + align32l(ldt * s.n_r));           // T, K_rr^-1
+    b += 8.0 * (double)(TL * (TL + 1) / 2 * 4096);                        // L_rr tiles
+    b += 2.0 * 12.0 * 81.0 * (double)s.n_s;                               // K_br, K_rb (stencil bound)
     b += (8.0 + 8.0 + 4.0 + 4.0) * (double)tiles;                         // K+_bb, S (FP64, FP32)
     b += 8.0 * (6.0 * s.n + 6.0 * s.n_s + 4.0 * s.n + 2.0 * s.n_s);       // R, R_b, vectors
     b += 8.0 * (2.0 * align32l(ldb * 36) + 54.0 * ldb);                   // C, Bc^T, [C|R_b|Q|Yh]

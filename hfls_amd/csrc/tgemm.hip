@@ -7,6 +7,7 @@
 //          op(A)(i,j) = tile(i,j) for i>=j, tile(j,i)^T otherwise
 //   mode 1 general, tile (i,j) at j*Ra + i; op 0: A, op 1: A^T
 //   mode 2 compact column major with leading dimension lda[d] (no tiles); op 0/1
+//   mode 3 lower triangular, stored like mode 0: op 0: A, op 1: A^T (FP64 only)
 // X_d, Y_d, Z_d are row major (row = DOF, column = right-hand side) with leading
 // dimension kp (a multiple of 64) and rows padded to whole tiles.  With a gate
 // pointer the launch does nothing while *gate == 0 (device-side condition).
@@ -33,6 +34,10 @@ static __device__ __forceinline__ int64_t tidx_sym(int T, int i, int j) {
 
 // pointer to stored tile and whether it must be transposed, for op(A)(i,j)
 static __device__ __forceinline__ int64_t tile_of(int mode, int op, int Ra, int i, int j, bool* tr) {
+  if (mode == 3) {  // lower triangular: op 0 uses tile (i, j), j <= i; op 1 tile (j, i)^T, j >= i
+    *tr = op == 1;
+    return op == 0 ? tidx_sym(Ra, i, j) : tidx_sym(Ra, j, i);
+  }
   if (mode == 0) {
     if (i >= j) {
       *tr = false;
@@ -306,7 +311,11 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
   const int64_t la = mode == 2 ? (int64_t)lda[d] : 0;
   const double* Xd = X + xoff[d] + c0;
   const int nall = 2 * Ko;
-  const int cbeg = (int)((int64_t)sp * nall / S), cend = (int)((int64_t)(sp + 1) * nall / S);
+  int cbeg = (int)((int64_t)sp * nall / S), cend = (int)((int64_t)(sp + 1) * nall / S);
+  if (mode == 3) {  // only the tiles of the triangle (S == 1)
+    cbeg = op == 0 ? 0 : 2 * i;
+    cend = op == 0 ? 2 * (i + 1) : nall;
+  }
   StageD s;
   if (cend > cbeg) {
     int64_t ti;
@@ -380,7 +389,7 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, double* Y,
                             const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt) {
   (void)lda;
-  if (kp % 64 || mode == 2) return hipErrorInvalidValue;  // FP32: tiles only
+  if (kp % 64 || mode == 2 || mode == 3) return hipErrorInvalidValue;  // FP32: modes 0, 1
   if (rt == 2)
     hipLaunchKernelGGL(k_tgemm_f32<2>, dim3((maxRo + 1) / 2, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols,
                        A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
@@ -396,7 +405,7 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
                             double* Ypart) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
-  if (S > 1 && (maxRo != 1 || !Ypart)) return hipErrorInvalidValue;
+  if (S > 1 && (maxRo != 1 || !Ypart || mode == 3)) return hipErrorInvalidValue;
   const int gx = S > 1 ? S : maxRo;
   if (mode == 2)
     hipLaunchKernelGGL(k_tgemm_f64<true>, dim3(gx, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
