@@ -112,21 +112,25 @@ def serve_mesh(nsub, e, seed):
     return o
 
 
-def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb):
+def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb, warm):
     """Request stream with Poisson arrivals: each request draws its mesh uniformly
-    from families x edges and 1..64 load cases log-uniformly (seeded); meshes are
-    set up once (cache warmed before the stream starts).  Returns latency
-    percentiles and served load-case throughput."""
+    from families x edges and 1..64 load cases log-uniformly (seeded).  Meshes are
+    set up on first use and kept on the device while they fit the HBM budget
+    (least recently used idle meshes are evicted); with warm, one single-case
+    request per mesh runs before the stream starts.  Latency is submission to the
+    last case recovered, set-up time of cold meshes included."""
     rng = np.random.default_rng(seed)
     meshes = [serve_mesh(f, e, 100 + i) for i, (f, e) in enumerate((f, e) for f in families for e in edges)]
     srv = hfls_amd.Server(slots=64, precision=32, eps=eps, max_iter=1000, chunk=8, hbm_budget_gb=budget_gb)
     t0 = time.perf_counter()
-    for m in meshes:  # warm the cache: one single-case request per mesh
-        rid = srv.submit(m, k=1, seed=1, case0=0)
-        while srv.step():
-            pass
-        srv.result(rid)
+    if warm:
+        for m in meshes:
+            rid = srv.submit(m, k=1, seed=1, case0=0)
+            while srv.step():
+                pass
+            srv.result(rid)
     warm_s = time.perf_counter() - t0
+    st0 = srv.stats()
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     picks = rng.integers(0, len(meshes), nreq)
     ks = np.minimum(64, np.floor(np.exp(rng.uniform(0.0, np.log(65.0), nreq)))).astype(int)
@@ -157,9 +161,13 @@ def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb):
     return {"requests": nreq, "load_cases": cases, "arrival_rate_per_s": rate, "wall_s": wall,
             "solves_per_s": cases / wall, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-            "meshes": ["%dx%d^3" % (f, e) for f in families for e in edges], "cache_warm_s": warm_s,
+            "meshes": ["%dx%d^3" % (f, e) for f in families for e in edges], "hbm_budget_gb": budget_gb,
+            "cache": "warmed before the stream" if warm else "cold start (set-ups inside the latencies)",
+            "cache_warm_s": warm_s, "setups_in_stream": int(st["setups"] - st0["setups"]),
+            "setup_s_in_stream": (st["setup_ms"] - st0["setup_ms"]) / 1e3,
+            "evictions_in_stream": int(st["evictions"] - st0["evictions"]),
             "iterations_min_max": [int(min(iters)), int(max(iters))], "max_true_residual": worst,
-            "slot_joins": int(st["joins"]), "mesh_setups": int(st["setups"])}
+            "slot_joins": int(st["joins"])}
 
 
 def main():
@@ -174,10 +182,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", action="store_true")
     ap.add_argument("--small", action="store_true", help="2x2x2 x 5^3 smoke-sized workload")
-    ap.add_argument("--serve-requests", type=int, default=40, help="requests in the serving run (0: none)")
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate of requests per second")
-    ap.add_argument("--serve-families", default="64,216", help="subdomain counts of the served cube families")
-    ap.add_argument("--serve-edges", default="6,8", help="elements per subdomain edge of the served meshes")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests in the serving run (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-families", default="64,216,512,1000", help="subdomain counts of the served cube families")
+    ap.add_argument("--serve-edges", default="8,9,10,11,12", help="elements per subdomain edge of the served meshes")
+    ap.add_argument("--serve-budget-gb", type=float, default=250.0, help="HBM budget of the server")
+    ap.add_argument("--serve-warm", action="store_true", help="set every mesh up before the stream starts")
     ap.add_argument("--kernel-only", action="store_true",
                     help="setup, then only the dominant kernel (FP32 tile GEMM, 4 launches) and a device copy of "
                          "known size (4 launches; calibration of HBM traffic counters)")
@@ -293,7 +303,8 @@ def main():
     if a.serve_requests > 0:
         fam = [int(x) for x in a.serve_families.split(",")]
         edg = [int(x) for x in a.serve_edges.split(",")]
-        out["serving"] = run_serving(hfls_amd, fam, edg, a.serve_requests, a.serve_rate, SEED, a.eps, 200.0)
+        out["serving"] = run_serving(hfls_amd, fam, edg, a.serve_requests, a.serve_rate, SEED, a.eps,
+                                     a.serve_budget_gb, a.serve_warm)
         log("serving:", json.dumps(out["serving"]))
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(plain_it[-1], out["config"]["subdomains"])
