@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--cpu-sample", action="store_true")
     ap.add_argument("--small", action="store_true", help="2x2x2 x 5^3 smoke-sized workload")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests in the serving run (0: none)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of requests per second")
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate of requests per second")
     ap.add_argument("--serve-families", default="64,216,512,1000", help="subdomain counts of the served cube families")
     ap.add_argument("--serve-edges", default="8,9,10,11,12", help="elements per subdomain edge of the served meshes")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0, help="HBM budget of the server")

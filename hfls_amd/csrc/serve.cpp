@@ -32,7 +32,9 @@ Server::Server(int device, size_t budget, int slots, int prec, double eps, int m
   t0_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-Server::~Server() = default;
+Server::~Server() {
+  if (pend_ && pend_->fut.valid()) pend_->fut.wait();
+}
 
 double Server::now() const {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0_;
@@ -73,9 +75,30 @@ size_t Server::estimate(const Problem& p) const {
 }
 
 size_t Server::bytes_in_use() const {
-  size_t s = 0;
+  size_t s = pend_ ? pend_->need : 0;
   for (auto& kv : mesh_) s += kv.second->bytes;
   return s;
+}
+
+bool Server::any_work() const {
+  for (auto& kv : mesh_) {
+    if (!kv.second->queue.empty()) return true;
+    for (auto& sl : kv.second->slot)
+      if (sl.req >= 0) return true;
+  }
+  return false;
+}
+
+bool Server::finish_setup(bool wait) {
+  if (!pend_) return false;
+  if (!wait && pend_->fut.wait_for(std::chrono::seconds(0)) != std::future_status::ready) return false;
+  std::unique_ptr<ServeMesh> m = pend_->fut.get();  // rethrows a failed set-up
+  st_.setup_ms += now() - pend_->t0;
+  st_.setups++;
+  m->last_used = now();
+  mesh_.emplace(pend_->key, std::move(m));
+  pend_.reset();
+  return true;
 }
 
 void Server::sizes(const Options& o, int64_t* npacked, int64_t* nglob, int64_t* bytes) {
@@ -118,6 +141,7 @@ int64_t Server::submit(const Options& o, int k, const double* F, uint64_t seed, 
 bool Server::admit(ServeRequest& r) {
   auto it = mesh_.find(r.key);
   if (it == mesh_.end()) {
+    if (pend_) return false;  // one set-up at a time (possibly this very mesh's)
     const Problem& p = *host_[r.key];
     const size_t need = estimate(p);
     // evict idle meshes, least recently used first, until the new one fits
@@ -133,25 +157,35 @@ bool Server::admit(ServeRequest& r) {
       mesh_.erase(lru->key);
       st_.evictions++;
     }
-    std::unique_ptr<ServeMesh> m(new ServeMesh);
-    m->key = r.key;
-    m->opts = p.o;
-    HC(hipSetDevice(device_));
-    size_t f0 = 0, f1 = 0, tot = 0;
-    HC(hipMemGetInfo(&f0, &tot));
-    const double ts = now();
-    m->e.reset(new Engine(p.o, device_));
-    m->e->setup();
-    m->e->blk_begin(slots_, eps_, max_iter_, prec_);
-    HC(hipStreamSynchronize(m->e->stream()));
-    HC(hipMemGetInfo(&f1, &tot));
-    st_.setup_ms += now() - ts;
-    st_.setups++;
-    m->bytes = std::max(need, f0 > f1 ? f0 - f1 : (size_t)0);
-    m->np = m->e->packed_n();
-    m->ng = 3 * (int64_t)p.nPoints;
-    m->slot.assign(slots_, ServeSlot());
-    it = mesh_.emplace(r.key, std::move(m)).first;
+    // set the mesh up on the worker thread; the request is admitted once it is cached
+    pend_.reset(new PendingSetup);
+    pend_->key = r.key;
+    pend_->need = need;
+    pend_->t0 = now();
+    const Options o = p.o;
+    const int64_t ng = 3 * (int64_t)p.nPoints;
+    const int dev = device_, slots = slots_, mi = max_iter_, prec = prec_;
+    const double eps = eps_;
+    const std::string key = r.key;
+    pend_->fut = std::async(std::launch::async, [o, ng, dev, slots, mi, prec, eps, key, need]() {
+      HC(hipSetDevice(dev));
+      std::unique_ptr<ServeMesh> m(new ServeMesh);
+      m->key = key;
+      m->opts = o;
+      size_t f0 = 0, f1 = 0, tot = 0;
+      HC(hipMemGetInfo(&f0, &tot));
+      m->e.reset(new Engine(o, dev));
+      m->e->setup();
+      m->e->blk_begin(slots, eps, mi, prec);
+      HC(hipStreamSynchronize(m->e->stream()));
+      HC(hipMemGetInfo(&f1, &tot));
+      m->bytes = std::max(need, f0 > f1 ? f0 - f1 : (size_t)0);
+      m->np = m->e->packed_n();
+      m->ng = ng;
+      m->slot.assign(slots, ServeSlot());
+      return m;
+    });
+    return false;
   }
   ServeMesh& m = *it->second;
   for (int c = 0; c < r.k; c++) m.queue.push_back(ServeSlot{r.id, c});
@@ -163,10 +197,19 @@ bool Server::admit(ServeRequest& r) {
 bool Server::step() {
   HC(hipSetDevice(device_));
   // 1. admission in arrival order; a request whose mesh does not fit yet waits
-  //    without blocking the requests behind it
-  for (auto it = wait_.begin(); it != wait_.end();) {
-    if (admit(req_[*it])) it = wait_.erase(it);
-    else ++it;
+  //    without blocking the requests behind it.  A finished set-up is installed
+  //    first; with nothing else to run, the loop waits for the running set-up.
+  finish_setup(false);
+  auto admission = [&] {
+    for (auto it = wait_.begin(); it != wait_.end();) {
+      if (admit(req_[*it])) it = wait_.erase(it);
+      else ++it;
+    }
+  };
+  admission();
+  if (pend_ && !any_work()) {
+    finish_setup(true);
+    admission();
   }
   // 2. free slots take queued cases (all meshes, asynchronously)
   for (auto& kv : mesh_) {
@@ -232,7 +275,7 @@ bool Server::step() {
     }
     st_.finishes += (int64_t)cols.size();
   }
-  bool busy = !wait_.empty();
+  bool busy = !wait_.empty() || pend_ != nullptr;
   for (auto& kv : mesh_) {
     if (!kv.second->queue.empty()) busy = true;
     for (auto& s : kv.second->slot)

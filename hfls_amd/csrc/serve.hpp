@@ -9,10 +9,12 @@
 // slot whose case has stopped is recovered and released.  Meshes run on their own
 // streams, so the work of different meshes overlaps on the device.  A case's
 // arithmetic is independent of the other slots, so its result is the same alone
-// and in the mix.
+// and in the mix.  A cold mesh is set up on a worker thread (its own stream) while
+// the cached meshes keep iterating; one set-up runs at a time.
 #pragma once
 #include <cstdint>
 #include <deque>
+#include <future>
 #include <map>
 #include <memory>
 #include <string>
@@ -83,6 +85,15 @@ class Server {
   std::map<std::string, std::unique_ptr<ServeMesh>> mesh_;
   std::map<std::string, std::unique_ptr<Problem>> host_;  // host problems (sizes, loads)
   ServeStats st_;
+  struct PendingSetup {  // a mesh being set up on the worker thread
+    std::string key;
+    size_t need = 0;
+    double t0 = 0;
+    std::future<std::unique_ptr<ServeMesh>> fut;
+  };
+  std::unique_ptr<PendingSetup> pend_;
+  bool finish_setup(bool wait);                  // install a finished set-up; true if one was installed
+  bool any_work() const;                         // a cached mesh has running or queued cases
   double now() const;
   const Problem& host_problem(const std::string& key, const Options& o);
   size_t estimate(const Problem& p) const;
