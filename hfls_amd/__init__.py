@@ -23,7 +23,7 @@ EXPORTS = [
     "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction", "hfls_server_create",
     "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
-    "hfls_mesh_tuning",
+    "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk",
 ]
 
 DEFAULTS = {
@@ -114,6 +114,14 @@ class Problem:
         info = np.zeros(8, dtype=np.int32)
         _check(load().hfls_problem_sub_info(self._h, d, _ptr(info)))
         return dict(zip(["n", "n_r", "n_s", "n_bs", "n_ss", "cluster", "nbf"], (int(x) for x in info[:7])))
+
+    def write_vtk(self, u, folder, iter=-1):
+        """Legacy ASCII VTK of the displacements u (3 * nPoints) as <folder>/box.vtk
+        (iter < 0) or <folder>/box<iter>.vtk (Mesh::SaveVTK, Mesh.cpp:292-351)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != 3 * self.nPoints:
+            raise ValueError("u must hold 3 * nPoints values")
+        _check(load().hfls_problem_write_vtk(self._h, _ptr(u), str(folder).encode(), int(iter)))
 
     def sub_bf(self, d):
         m = self.sub_info(d)["nbf"]
@@ -208,6 +216,13 @@ class Mesh:
         out = np.zeros(3)
         _check(load().hfls_mesh_tuning(self._h, _ptr(out)))
         return {"row_tiles": int(out[0]), "ms": {1: float(out[1]), 2: float(out[2])}}
+
+    def write_vtk(self, u, folder, iter=-1):
+        """Legacy ASCII VTK of the displacements u (3 * nPoints), as Problem.write_vtk."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != 3 * self.nPoints:
+            raise ValueError("u must hold 3 * nPoints values")
+        _check(load().hfls_mesh_write_vtk(self._h, _ptr(u), str(folder).encode(), int(iter)))
 
     def kernel(self):
         R = np.zeros(6 * self.npacked)

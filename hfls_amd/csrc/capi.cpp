@@ -8,6 +8,7 @@
 #include "engine.hpp"
 #include "problem.hpp"
 #include "serve.hpp"
+#include "vtk.hpp"
 
 struct hfls_problem_s {
   hfls::Problem* p;
@@ -127,6 +128,13 @@ extern "C" int hfls_problem_sub_young(hfls_problem h, int d, double* E) {
   return guard([&] { *E = sub(h, d).E; });
 }
 
+extern "C" int hfls_problem_write_vtk(hfls_problem h, const double* u, const char* folder, int iter) {
+  return guard([&] {
+    if (!u || !folder) throw std::runtime_error("null argument");
+    hfls::write_vtk(*h->p, u, folder, iter);
+  });
+}
+
 extern "C" int hfls_problem_rows(hfls_problem h, int* a, int* b, int64_t* dof, double* w) {
   return guard([&] {
     const hfls::Problem& p = *h->p;
@@ -196,6 +204,13 @@ extern "C" int hfls_mesh_setup_times(hfls_mesh m, double* t) {
     t[2] = i.meta_ms;
     t[3] = i.assemble_ms;
     t[4] = i.coarse_ms;
+  });
+}
+
+extern "C" int hfls_mesh_write_vtk(hfls_mesh m, const double* u, const char* folder, int iter) {
+  return guard([&] {
+    if (!u || !folder) throw std::runtime_error("null argument");
+    hfls::write_vtk(m->e->problem(), u, folder, iter);
   });
 }
 

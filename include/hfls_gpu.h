@@ -50,6 +50,12 @@ int hfls_problem_sub_bdofs(hfls_problem p, int d, int* bdofs);
 int hfls_problem_sub_sset(hfls_problem p, int d, int* sset);
 int hfls_problem_sub_l2g(hfls_problem p, int d, int64_t* l2g);
 int hfls_problem_sub_young(hfls_problem p, int d, double* E);
+/* legacy ASCII VTK of a displacement field u (3 * nPoints, global DOF order):
+ * points, hex8 cells, displacements, decomposition and material id per cell;
+ * file <folder>/box.vtk (iter < 0) or <folder>/box<iter>.vtk.  Replaces
+ * Mesh::SaveVTK (Mesh.cpp:292-351) as called from Cluster::printVTK
+ * (Cluster.cpp:1648-1678). */
+int hfls_problem_write_vtk(hfls_problem p, const double* u, const char* folder, int iter);
 /* dual rows: subdomain of the +1 entry, of the -1 entry (-1 for Dirichlet rows),
  * global DOF, multiplicity weight (Cluster.cpp:893-895,918) */
 int hfls_problem_rows(hfls_problem p, int* sub_a, int* sub_b, int64_t* dof, double* weight);
@@ -69,6 +75,8 @@ int hfls_mesh_setup_times(hfls_mesh m, double* t);
 /* kernel R (packed local, n x 6 column major per subdomain), fixing DOFs (6 local
  * DOFs per subdomain), detected defect per subdomain, body-force loads (packed local) */
 int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect);
+/* the same VTK output for a set-up mesh (e.g. the u of hfls_solve) */
+int hfls_mesh_write_vtk(hfls_mesh m, const double* u, const char* folder, int iter);
 int hfls_mesh_load(hfls_mesh m, double* f);
 
 /* runtime settings of a set-up mesh (no counterpart in the reference):
