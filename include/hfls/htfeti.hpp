@@ -152,6 +152,13 @@ class Cluster {
     solution.assign(3 * s[4], 0.0);
     lambda.assign(nLam_f, 0.0);
     check(hfls_solve(m, eps, max_iter, nullptr, solution.data(), lambda.data(), &stats));
+    // final solution to <path2data>/box.vtk (Cluster.cpp:1455-1456)
+    if (options2.count("path2data") && !options2["path2data"].empty()) printVTK(-1);
+  }
+  // Cluster::printVTK (Cluster.cpp:1648-1678): the recovered displacements as VTK
+  // (<path2data>/box.vtk for it < 0, box<it>.vtk otherwise)
+  void printVTK(int it) {
+    check(hfls_mesh_write_vtk(m, solution.data(), options2["path2data"].c_str(), it));
   }
 
   OptionMap options2;
