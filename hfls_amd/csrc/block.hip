@@ -608,12 +608,17 @@ __global__ __launch_bounds__(256) void kb_colsum(int nparts, int k, const double
   if (threadIdx.x == 0) sc->colsum[j] = red[0];
 }
 
-__global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate) {
+// mode 3 with hg: H alpha (hg[q*k + j], q < n6) of every column certified here is
+// saved into hgf, so the recovery can reuse it instead of recomputing the true residual
+__global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate, int n6,
+                                                const double* __restrict__ hg, double* __restrict__ hgf) {
   const int j = threadIdx.x;
   __shared__ int any, run, rst;
+  __shared__ int certnow[64];
   if (gate && *gate == 0) return;              // uniform over the workgroup
   if (mode == 3 && sc->pending == 0) return;
   if (j == 0) any = run = rst = 0;
+  certnow[j] = 0;
   __syncthreads();
   if (j < k) {
     const double s = sc->colsum[j];
@@ -646,6 +651,7 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->res[j] = 1.0;
         sc->repl[j] = 0;
         sc->ncert[j] = 0;
+        sc->hgok[j] = 0;
         if (!(s == s)) sc->err = 1;
         sc->done[j] = !(s > 0.0) ? 2 : (sc->max_iter <= 0 ? 3 : 0);
       }
@@ -657,6 +663,8 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->ncert[j] += 1;
         if (r < sc->eps * sc->norm0[j]) {
           sc->done[j] = 2;
+          certnow[j] = 1;
+          sc->hgok[j] = hgf != nullptr;
         } else if (sc->it[j] >= sc->max_iter) {
           sc->done[j] = 3;
         } else {
@@ -673,6 +681,9 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
     if (sc->done[j] == 0 || sc->done[j] == 1) run = 1;
   }
   __syncthreads();
+  if (mode == 3 && hgf)
+    for (int idx = j; idx < n6 * k; idx += 64)
+      if (certnow[idx % k]) hgf[idx] = hg[idx];
   if (j == 0) {
     if (mode == 1) sc->pending = any;
     if (mode == 2 || mode == 3) sc->pending = 0;
@@ -683,11 +694,11 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
 }
 
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
-                            const int* gate) {
+                            const int* gate, int n6, const double* hg, double* hgf) {
   if (k > 64) return hipErrorInvalidValue;
   const int* g2 = mode == 3 ? &sc->pending : gate;
   hipLaunchKernelGGL(kb_colsum, dim3(k), dim3(256), 0, st, nparts, k, part, sc, g2);
-  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, k, sc, mode, gate);
+  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, k, sc, mode, gate, n6, hg, hgf);
   return hipGetLastError();
 }
 

@@ -119,11 +119,11 @@ class Engine {
   void blk_join(const std::vector<int>& slots, const double* f_local);
   void blk_sweeps(int n);
   void blk_sync(BScalars* out);
-  void blk_finish(const std::vector<int>& slots, double* u, double* lam);
+  void blk_finish(const std::vector<int>& slots, double* u, double* lam, const BScalars* state = nullptr);
   // the same in parts: loads to HBM / start, and recovery / results to the host
   void blk_load(const std::vector<int>& slots, const double* f_local);
   void blk_init(const std::vector<int>& slots);
-  void blk_recover();
+  void blk_recover(const BScalars* state = nullptr);  // state: the scalars of the last blk_sync
   void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
   // autotuner result: output row tiles per workgroup of the FP32 block GEMM and
@@ -212,7 +212,7 @@ class Engine {
   DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32;
-  DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
+  DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_hgf, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;
   int b_nparts = 0;

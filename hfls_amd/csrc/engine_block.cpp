@@ -223,6 +223,7 @@ void Engine::block_prepare() {
   zero_alloc(b_la, GCP * KP, st);
   zero_alloc(b_h, (int64_t)6 * N * KMAX, st);
   zero_alloc(b_hg, (int64_t)6 * N * KMAX, st);
+  zero_alloc(b_hgf, (int64_t)6 * N * KMAX, st);
   for (DVec<double>* v : {&b_lam, &b_g, &b_w, &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64}) zero_alloc(*v, nl * KMAX, st);
   b_nparts = (int)((nl + 63) / 64);
   zero_alloc(b_part, (int64_t)b_nparts * KMAX, st);
@@ -271,7 +272,7 @@ size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_lam, &b_g, &b_w,
+  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -573,7 +574,7 @@ void Engine::blk_enqueue_sweeps(int n) {
       bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
       bPM(k, prec, b_g64.p, b_z.p, b_hg.p, pending);
       bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3, nullptr, 6 * P->nsub, b_hg.p, b_hgf.p));
       HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, &b_sc.p->restart));
     }
   }
@@ -584,26 +585,35 @@ void Engine::blk_sync(BScalars* sh) {
   HC(hipStreamSynchronize(st));
 }
 
-void Engine::blk_finish(const std::vector<int>& cols, double* u_h, double* lam_h) {
-  blk_recover();
+void Engine::blk_finish(const std::vector<int>& cols, double* u_h, double* lam_h, const BScalars* sh) {
+  blk_recover(sh);
   blk_collect(cols, u_h, lam_h);
 }
 
-void Engine::blk_recover() {
+void Engine::blk_recover(const BScalars* sh) {
   const int N = P->nsub, k = blk_k, prec = blk_prec;
   const int64_t np = off_n[N], ng = 3 * (int64_t)P->nPoints;
-  // final FP64 true residual; its projection gives alpha (h per subdomain)
-  bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
-  bPM(k, prec, b_g64.p, b_z.p, b_hg.p);
-  bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
-  HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
+  // alpha (H alpha per subdomain) from the final FP64 true residual's projection.
+  // Columns certified on their FP64 true residual saved it at certification; it is
+  // recomputed (the same arithmetic on the same frozen lambda) unless every stopped
+  // column has it
+  bool saved = sh != nullptr && prec == 32;
+  for (int j = 0; saved && j < k; j++)
+    if (sh->done[j] == 3 || (sh->done[j] == 2 && !sh->hgok[j])) saved = false;
+  if (!saved) {
+    bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
+    bPM(k, prec, b_g64.p, b_z.p, b_hg.p);
+    bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p);
+    HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 4));
+  }
+  const double* hal = saved ? b_hgf.p : b_hg.p;
   // recovery: u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678), on a
   // copy of the loads (the running columns keep theirs)
   HC(hipMemcpyAsync(b_V2.p, b_V.p, (size_t)np * k * sizeof(double), hipMemcpyDeviceToDevice, st));
   HC(launch_kb_sub_bt(st, N, maxns, k, np, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p,
                       b_lam.p, b_V2.p));
   b_kplus(k, true, b_V2.p);
-  HC(launch_kb_recover(st, N, maxn, k, KP, np, ng, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_arow.p, b_la.p, b_hg.p, b_X.p,
+  HC(launch_kb_recover(st, N, maxn, k, KP, np, ng, d_n.p, d_off_n.p, d_R.p, d_off_R.p, b_arow.p, b_la.p, hal, b_X.p,
                        d_gdof.p, d_owner.p, b_U.p));
 }
 
@@ -657,7 +667,7 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
     its[j] = sh.it[j];
     stt[j] = sh.done[j];
   }
-  blk_recover();
+  blk_recover(&sh);
   HC(hipEventRecord(ev[3], st));
   blk_collect(cols, u_h, lam_h);  // displacements to the host: outside the timed solve
   blk_sync(&sh);

@@ -174,6 +174,7 @@ constexpr int KMAX = 64;  // load cases per block
 struct BScalars {         // per-column PCPG scalars of a block solve
   double rho[KMAX], gamma[KMAX], gPz[KMAX], gPz_prev[KMAX], wFw[KMAX], norm0[KMAX], res[KMAX], ref[KMAX];
   int it[KMAX], done[KMAX], repl[KMAX], ncert[KMAX];
+  int hgok[KMAX];          // the saved H alpha of the column's certified true residual is valid
   double eps, rtol;        // stopping tolerance; refinement factor (0: none)
   int max_iter, err;
   int cert;                // 1: stopping tests are confirmed on the FP64 true residual
@@ -217,7 +218,7 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
                                  const int64_t* vrow, const double* H, const int* cl, const double* invGtG, double* h,
                                  double* alpha, double sgn, const int* gate = nullptr);
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
-                            const int* gate = nullptr);
+                            const int* gate = nullptr, int n6 = 0, const double* hg = nullptr, double* hgf = nullptr);
 hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
                             const double* e, const BScalars* sc, const int* gate = nullptr);
 hipError_t launch_kb_permute(hipStream_t st, int nsub, int maxn, int dir, int k, int kp, int64_t npacked,

@@ -254,7 +254,7 @@ bool Server::step() {
       if (m.slot[j].req >= 0 && (sh.done[j] == 2 || sh.done[j] == 3)) cols.push_back(j);
     if (cols.empty()) continue;
     std::vector<double> U(cols.size() * (size_t)m.ng);
-    m.e->blk_finish(cols, U.data(), nullptr);
+    m.e->blk_finish(cols, U.data(), nullptr, &sh);
     BScalars s2;
     m.e->blk_sync(&s2);
     const double t = now();
