@@ -176,10 +176,12 @@ hipError_t launch_lrr_trtri(hipStream_t st, int nsub, int maxT, const int* nr, c
 // k columns of an interior block (element (i, j) of subdomain d at
 // X[xbase[d] + i*rs + j*cs]); op 0: Y = W X, op 1: Y = W^T X.  One 256-thread
 // workgroup per (row block I, subdomain); every tile is read once per column.
-// op 0: lane = row, wave w takes the tile columns w, w+4, ..; the 4 wave sums are
-// added in LDS.  op 1 (Y_I = sum_{J>=I} W_JI^T X_J): wave w takes rows 16w..16w+15
-// of the result, lane = tile row c; each wave sums its 16 values over the lanes at
-// the end.  Fixed order throughout.
+// op 0 (Y_I = sum_{J<=I} W_IJ X_J): lane = row; wave w takes the tiles J = w, w+4,
+// .. whole (64 columns each), two at a time, so each wave streams its own tiles
+// with 32 loads in flight; the 4 wave sums are added in LDS.  op 1 (Y_I = sum_{J>=I}
+// W_JI^T X_J): wave w takes rows 16w..16w+15 of the result, lane = tile row c, two
+// tiles per step; each wave sums its 16 values over the lanes at the end.  Fixed
+// order throughout.
 __global__ __launch_bounds__(256) void k_trmv(int op, int k, const int* __restrict__ nr, const double* __restrict__ Wt,
                                               const int64_t* __restrict__ woff, const double* __restrict__ X,
                                               const int64_t* __restrict__ xbase, int64_t rs, int64_t cs0,
@@ -192,18 +194,41 @@ __global__ __launch_bounds__(256) void k_trmv(int op, int k, const int* __restri
   const double* W = Wt + woff[d];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   __shared__ double red[4][64];
+  __shared__ double xs[64 * 64];  // op 0: the x blocks of the row block's tiles, 64 at a time
   for (int j = 0; j < k; j++) {
     const double* x = X + xbase[d] + j * cs;
     double* y = Y + ybase[d] + j * cs;
     if (op == 0) {
       double acc = 0.0;
-      for (int J = 0; J <= I; J++) {
-        const double* tile = W + ltile(T, I, J) * 4096;
-#pragma unroll 4
-        for (int m = 0; m < 16; m++) {
-          const int c = w + 4 * m, gc = 64 * J + c;
-          const double xc = gc < n ? x[gc * rs] : 0.0;
-          acc += tile[c * 64 + lane] * xc;
+      for (int J0 = 0; J0 <= I; J0 += 64) {
+        const int nJ = min(64, I + 1 - J0);
+        __syncthreads();
+        for (int e = t; e < nJ * 64; e += 256) {
+          const int gc = 64 * J0 + e;
+          xs[e] = gc < n ? x[gc * rs] : 0.0;
+        }
+        __syncthreads();
+        int q = w;
+        for (; q + 4 < nJ; q += 8) {
+          const double* t0 = W + ltile(T, I, J0 + q) * 4096 + lane;
+          const double* t1 = W + ltile(T, I, J0 + q + 4) * 4096 + lane;
+          double a0[64], a1[64];
+#pragma unroll
+          for (int c = 0; c < 64; c++) a0[c] = t0[c * 64];
+#pragma unroll
+          for (int c = 0; c < 64; c++) a1[c] = t1[c * 64];
+#pragma unroll
+          for (int c = 0; c < 64; c++) acc += a0[c] * xs[q * 64 + c];
+#pragma unroll
+          for (int c = 0; c < 64; c++) acc += a1[c] * xs[(q + 4) * 64 + c];
+        }
+        for (; q < nJ; q += 4) {
+          const double* t0 = W + ltile(T, I, J0 + q) * 4096 + lane;
+          double a0[64];
+#pragma unroll
+          for (int c = 0; c < 64; c++) a0[c] = t0[c * 64];
+#pragma unroll
+          for (int c = 0; c < 64; c++) acc += a0[c] * xs[q * 64 + c];
         }
       }
       red[w][lane] = acc;
@@ -215,12 +240,28 @@ __global__ __launch_bounds__(256) void k_trmv(int op, int k, const int* __restri
       double acc[16];
 #pragma unroll
       for (int m = 0; m < 16; m++) acc[m] = 0.0;
-      for (int J = I; J < T; J++) {
-        const double* tile = W + ltile(T, J, I) * 4096;
-        const int gc = 64 * J + lane;
-        const double xc = gc < n ? x[gc * rs] : 0.0;
+      int J = I;
+      for (; J + 1 < T; J += 2) {
+        const double* t0 = W + ltile(T, J, I) * 4096 + 16 * w * 64 + lane;
+        const double* t1 = W + ltile(T, J + 1, I) * 4096 + 16 * w * 64 + lane;
+        const int g0 = 64 * J + lane, g1 = g0 + 64;
+        const double x0 = g0 < n ? x[g0 * rs] : 0.0, x1 = g1 < n ? x[g1 * rs] : 0.0;
+        double a0[16], a1[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++) acc[m] += tile[(16 * w + m) * 64 + lane] * xc;
+        for (int m = 0; m < 16; m++) a0[m] = t0[m * 64];
+#pragma unroll
+        for (int m = 0; m < 16; m++) a1[m] = t1[m * 64];
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc[m] += a0[m] * x0;
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc[m] += a1[m] * x1;
+      }
+      if (J < T) {
+        const double* t0 = W + ltile(T, J, I) * 4096 + 16 * w * 64 + lane;
+        const int g0 = 64 * J + lane;
+        const double x0 = g0 < n ? x[g0 * rs] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc[m] += t0[m * 64] * x0;
       }
 #pragma unroll
       for (int m = 0; m < 16; m++) {
@@ -254,6 +295,24 @@ hipError_t launch_trmv(hipStream_t st, int nsub, int maxT, int op, int k, const 
 // Y = beta*Y + alpha * K X over CSR rows [row0, row1): row r belongs to subdomain
 // rsub[r] (index into the base arrays after subtracting sub0), local row rloc[r];
 // entry e couples it to local column col[e].  One wave per row, lanes = columns.
+// k == 1: one thread per row
+__global__ __launch_bounds__(256) void k_csr_apply1(int64_t row0, int64_t row1, int sub0,
+                                                    const int64_t* __restrict__ ptr, const int* __restrict__ col,
+                                                    const double* __restrict__ val, const int* __restrict__ rsub,
+                                                    const int* __restrict__ rloc, const double* __restrict__ X,
+                                                    const int64_t* __restrict__ xbase, int64_t xrs,
+                                                    double* __restrict__ Y, const int64_t* __restrict__ ybase,
+                                                    int64_t yrs, double alpha, double beta) {
+  const int64_t r = row0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= row1) return;
+  const int s = rsub[r] - sub0;
+  const double* Xs = X + xbase[s];
+  double acc = 0.0;
+  for (int64_t e = ptr[r]; e < ptr[r + 1]; e++) acc += val[e] * Xs[(int64_t)col[e] * xrs];
+  double* y = Y + ybase[s] + (int64_t)rloc[r] * yrs;
+  *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * acc;
+}
+
 __global__ __launch_bounds__(256) void k_csr_apply(int64_t row0, int64_t row1, int sub0, int k,
                                                    const int64_t* __restrict__ ptr, const int* __restrict__ col,
                                                    const double* __restrict__ val, const int* __restrict__ rsub,
@@ -285,6 +344,11 @@ hipError_t launch_csr_apply(hipStream_t st, int64_t row0, int64_t row1, int sub0
                             int64_t yrs, int64_t ycs, const int* csv, double alpha, double beta) {
   if (k < 1 || k > 64) return hipErrorInvalidValue;
   if (row1 <= row0) return hipSuccess;
+  if (k == 1) {
+    hipLaunchKernelGGL(k_csr_apply1, dim3((unsigned)((row1 - row0 + 255) / 256)), dim3(256), 0, st, row0, row1, sub0,
+                       ptr, col, val, rsub, rloc, X, xbase, xrs, Y, ybase, yrs, alpha, beta);
+    return hipGetLastError();
+  }
   const int64_t g = std::min<int64_t>((row1 - row0 + 3) / 4, 4096);
   hipLaunchKernelGGL(k_csr_apply, dim3((unsigned)g), dim3(256), 0, st, row0, row1, sub0, k, ptr, col, val, rsub, rloc,
                      X, xbase, xrs, xcs, Y, ybase, yrs, ycs, csv, alpha, beta);
