@@ -608,17 +608,12 @@ __global__ __launch_bounds__(256) void kb_colsum(int nparts, int k, const double
   if (threadIdx.x == 0) sc->colsum[j] = red[0];
 }
 
-// mode 3 with hg: H alpha (hg[q*k + j], q < n6) of every column certified here is
-// saved into hgf, so the recovery can reuse it instead of recomputing the true residual
-__global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate, int n6,
-                                                const double* __restrict__ hg, double* __restrict__ hgf) {
+__global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate) {
   const int j = threadIdx.x;
-  __shared__ int any, run, rst;
-  __shared__ int certnow[64];
+  __shared__ int any, run, rst, cert;
   if (gate && *gate == 0) return;              // uniform over the workgroup
   if (mode == 3 && sc->pending == 0) return;
-  if (j == 0) any = run = rst = 0;
-  certnow[j] = 0;
+  if (j == 0) any = run = rst = cert = 0;
   __syncthreads();
   if (j < k) {
     const double s = sc->colsum[j];
@@ -657,14 +652,16 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
       }
     } else if (mode == 3) {
       sc->repl[j] = 0;
+      sc->certnow[j] = 0;
       if (stt == 1) {
         const double r = sqrt(fabs(s));
         sc->res[j] = sc->norm0[j] > 0 ? r / sc->norm0[j] : 0.0;
         sc->ncert[j] += 1;
         if (r < sc->eps * sc->norm0[j]) {
           sc->done[j] = 2;
-          certnow[j] = 1;
-          sc->hgok[j] = hgf != nullptr;
+          sc->certnow[j] = 1;
+          sc->hgok[j] = 1;
+          cert = 1;
         } else if (sc->it[j] >= sc->max_iter) {
           sc->done[j] = 3;
         } else {
@@ -681,11 +678,10 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
     if (sc->done[j] == 0 || sc->done[j] == 1) run = 1;
   }
   __syncthreads();
-  if (mode == 3 && hgf)
-    for (int idx = j; idx < n6 * k; idx += 64)
-      if (certnow[idx % k]) hgf[idx] = hg[idx];
   if (j == 0) {
     if (mode == 1) sc->pending = any;
+    if (mode == 1) sc->certified = 0;
+    if (mode == 3) sc->certified = cert;
     if (mode == 2 || mode == 3) sc->pending = 0;
     if (mode >= 1 && mode <= 3) sc->active = run;
     if (mode == 1) sc->restart = 0;
@@ -694,11 +690,29 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
 }
 
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
-                            const int* gate, int n6, const double* hg, double* hgf) {
+                            const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
   const int* g2 = mode == 3 ? &sc->pending : gate;
   hipLaunchKernelGGL(kb_colsum, dim3(k), dim3(256), 0, st, nparts, k, part, sc, g2);
-  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, k, sc, mode, gate, n6, hg, hgf);
+  hipLaunchKernelGGL(kb_reduce, dim3(1), dim3(64), 0, st, k, sc, mode, gate);
+  return hipGetLastError();
+}
+
+// H alpha of the columns certified by the last certification reduction (saved for
+// the recovery); skipped on the device unless one was certified
+__global__ __launch_bounds__(256) void kb_save_hg(int64_t n, int k, const BScalars* sc, const double* __restrict__ hg,
+                                                  double* __restrict__ hgf) {
+  if (sc->certified == 0) return;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += stride)
+    if (sc->certnow[e % k]) hgf[e] = hg[e];
+}
+
+hipError_t launch_kb_save_hg(hipStream_t st, int n6, int k, BScalars* sc, const double* hg, double* hgf) {
+  if (k > 64) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)n6 * k;
+  hipLaunchKernelGGL(kb_save_hg, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, st, n, k, sc,
+                     hg, hgf);
   return hipGetLastError();
 }
 

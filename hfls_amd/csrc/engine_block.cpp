@@ -574,7 +574,8 @@ void Engine::blk_enqueue_sweeps(int n) {
       bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr, pending);
       bPM(k, prec, b_g64.p, b_z.p, b_hg.p, pending);
       bP(k, b_z.p, b_Pz.p, b_h.p, b_g64.p, pending);
-      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3, nullptr, 6 * P->nsub, b_hg.p, b_hgf.p));
+      HC(launch_kb_reduce(st, b_nparts, k, b_part.p, b_sc.p, 3));
+      HC(launch_kb_save_hg(st, 6 * P->nsub, k, b_sc.p, b_hg.p, b_hgf.p));
       HC(launch_kb_update(st, nl, k, 2, b_g.p, b_w.p, b_g64.p, b_Pz.p, b_sc.p, &b_sc.p->restart));
     }
   }

@@ -175,6 +175,8 @@ struct BScalars {         // per-column PCPG scalars of a block solve
   double rho[KMAX], gamma[KMAX], gPz[KMAX], gPz_prev[KMAX], wFw[KMAX], norm0[KMAX], res[KMAX], ref[KMAX];
   int it[KMAX], done[KMAX], repl[KMAX], ncert[KMAX];
   int hgok[KMAX];          // the saved H alpha of the column's certified true residual is valid
+  int certnow[KMAX];       // certified by the last certification reduction
+  int certified;           // some column was certified since the last gamma reduction
   double eps, rtol;        // stopping tolerance; refinement factor (0: none)
   int max_iter, err;
   int cert;                // 1: stopping tests are confirmed on the FP64 true residual
@@ -218,7 +220,10 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
                                  const int64_t* vrow, const double* H, const int* cl, const double* invGtG, double* h,
                                  double* alpha, double sgn, const int* gate = nullptr);
 hipError_t launch_kb_reduce(hipStream_t st, int nparts, int k, const double* part, BScalars* sc, int mode,
-                            const int* gate = nullptr, int n6 = 0, const double* hg = nullptr, double* hgf = nullptr);
+                            const int* gate = nullptr);
+// hgf[q*k + j] = hg[q*k + j] (q < n6) for the columns certified by the last
+// certification reduction (skipped on the device when none was)
+hipError_t launch_kb_save_hg(hipStream_t st, int n6, int k, BScalars* sc, const double* hg, double* hgf);
 hipError_t launch_kb_update(hipStream_t st, int64_t nlam, int k, int mode, double* a, double* b, const double* c,
                             const double* e, const BScalars* sc, const int* gate = nullptr);
 hipError_t launch_kb_permute(hipStream_t st, int nsub, int maxn, int dir, int k, int kp, int64_t npacked,
