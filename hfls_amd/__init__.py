@@ -23,7 +23,7 @@ EXPORTS = [
     "hfls_apply_block", "hfls_problem_traction", "hfls_mesh_traction", "hfls_server_create",
     "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
-    "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk",
+    "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk", "hfls_solve_blocks",
 ]
 
 DEFAULTS = {
@@ -316,6 +316,24 @@ class Mesh:
                 "converged": np.array([c.status == 2 for c in cs]), "solve_ms": bs.solve_ms, "init_ms": bs.init_ms,
                 "pcpg_ms": bs.pcpg_ms, "recover_ms": bs.recover_ms, "sweeps": bs.sweeps,
                 "cert_steps": bs.cert_steps}
+
+    def pcpg_blocks(self, F, k=64, eps=1e-8, max_iter=200, precision=32):
+        """F (nblocks*k, npacked): nblocks blocks of k load cases solved concurrently,
+        one block context (own stream) each.  solve_ms is the device time span of all
+        blocks; per-case results in F's order."""
+        F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+        if F.shape[0] % k:
+            raise ValueError("the number of load cases must be a multiple of k")
+        nb = F.shape[0] // k
+        u = np.zeros((nb * k, 3 * self.nPoints))
+        cs = (CaseStats * (nb * k))()
+        bs = BlockStats()
+        _check(load().hfls_solve_blocks(self._h, ctypes.c_double(eps), int(max_iter), nb, int(k), int(precision),
+                                        _ptr(F), _ptr(u), cs, ctypes.byref(bs)))
+        return {"u": u, "iterations": np.array([c.iterations for c in cs]),
+                "status": np.array([c.status for c in cs]), "residual": np.array([c.residual for c in cs]),
+                "solve_ms": bs.solve_ms, "init_ms": bs.init_ms, "pcpg_ms": bs.pcpg_ms, "recover_ms": bs.recover_ms,
+                "sweeps": bs.sweeps, "cert_steps": bs.cert_steps, "blocks": nb}
 
 
 class Server:

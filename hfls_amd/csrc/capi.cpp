@@ -301,6 +301,34 @@ extern "C" int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, in
   });
 }
 
+extern "C" int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblocks, int k, int precision,
+                                 const double* f, double* u, hfls_case_stats* cases, hfls_block_stats* stats) {
+  return guard([&] {
+    if (k < 1 || k > hfls::KMAX) throw std::runtime_error("k must be in 1..64");
+    if (nblocks < 1 || nblocks > 8) throw std::runtime_error("nblocks must be in 1..8");
+    if (!f || !u) throw std::runtime_error("null argument");
+    const size_t kt = (size_t)nblocks * k;
+    std::vector<int> it(kt), stt(kt);
+    std::vector<double> res(kt);
+    hfls::BlockInfo info;
+    m->e->solve_blocks(nblocks, eps, max_iter, k, precision, f, u, it.data(), stt.data(), res.data(), &info);
+    if (cases)
+      for (size_t j = 0; j < kt; j++) {
+        cases[j].iterations = it[j];
+        cases[j].status = stt[j];
+        cases[j].residual = res[j];
+      }
+    if (stats) {
+      stats->solve_ms = info.solve_ms;
+      stats->init_ms = info.init_ms;
+      stats->pcpg_ms = info.pcpg_ms;
+      stats->recover_ms = info.recover_ms;
+      stats->sweeps = info.sweeps;
+      stats->cert_steps = info.cert_steps;
+    }
+  });
+}
+
 extern "C" int hfls_apply_block(hfls_mesh m, int op, int k, int precision, const double* x, double* y) {
   return guard([&] { m->e->apply_block(op, k, precision, x, y); });
 }

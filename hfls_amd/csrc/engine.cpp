@@ -232,6 +232,12 @@ Engine::Engine(const Options& o, int dev) : device(dev) {
 }
 
 Engine::~Engine() {
+  if (!ctxs_.empty() && cur_ctx_ != 0) use_ctx(0);
+  for (size_t i = 1; i < ctxs_.size(); i++) {  // the other block contexts' graphs and streams
+    if (ctxs_[i]->gx) (void)hipGraphExecDestroy(ctxs_[i]->gx);
+    if (ctxs_[i]->st) (void)hipStreamSynchronize(ctxs_[i]->st);
+    if (ctxs_[i]->st) (void)hipStreamDestroy(ctxs_[i]->st);
+  }
   if (gx_plain) (void)hipGraphExecDestroy(gx_plain);
   if (gx_blk) (void)hipGraphExecDestroy(gx_blk);
   if (st) (void)hipStreamDestroy(st);

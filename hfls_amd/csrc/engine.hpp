@@ -64,6 +64,25 @@ struct DVec {
   void alloc(size_t count);
   void upload(const std::vector<T>& v, hipStream_t st);
   void download(std::vector<T>& v, hipStream_t st) const;
+  void swap(DVec& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+// The state of one block solve: the k columns' buffers and scalars, its stream and
+// its captured sweep graph.  Several contexts of one mesh run blocks concurrently
+// on their own streams and share the mesh's operators (tiles, factors, index
+// arrays); a context is brought into the engine with Engine::use_ctx.
+struct BlockCtx {
+  hipStream_t st = nullptr;
+  hipGraphExec_t gx = nullptr;
+  int64_t gx_key = -1;
+  int k = 0, prec = 64;
+  DVec<float> xb32;
+  DVec<double> xb, yb, cr, coef, kpart, v6, gc, la, h, hg, hgf, lam, g, w, Fw, Pg, z, Pz, d, g64, part, gt, Vr, Xr, Tr,
+      V, V2, X, U;
+  DVec<BScalars> sc;
 };
 
 class Engine {
@@ -104,6 +123,13 @@ class Engine {
   // final FP64 residual sqrt(g'PMPg)/sqrt(g0'PMPg0).
   void solve_block(double eps, int max_iter, int k, int precision, const double* f_local, double* u, double* lam,
                    int* iters, int* status, double* res, BlockInfo* info);
+  // nb blocks of k load cases solved concurrently, one block context each (f_local:
+  // nb*k packed local loads; u: nb*k displacements); info->solve_ms is the device
+  // time span of all of them, the other times those of the first block
+  void solve_blocks(int nb, double eps, int max_iter, int k, int precision, const double* f_local, double* u,
+                    int* iters, int* status, double* res, BlockInfo* info);
+  int blk_contexts(int n);  // make n block contexts available (context 0 is the engine's own)
+  void use_ctx(int c);      // bring block context c into the engine
   // op 0 F, 1 M (preconditioner), 2 P (projector) applied to k dual vectors
   // (column major, ld nlam); precision selects the F_i / S_i arithmetic
   void apply_block(int op, int k, int precision, const double* x, double* y);
@@ -227,6 +253,10 @@ class Engine {
   void crq_products(const int* gate);
   void b_kplus(int k, bool interior_too, double* V);  // K_c^+ pieces of the local block V
   void b_rr_solve(double* X);
+  std::vector<std::unique_ptr<BlockCtx>> ctxs_;  // ctxs_[i]: context i while it is not the current one
+  int cur_ctx_ = 0;
+  void swap_ctx(BlockCtx& c);
+  void alloc_ctx_buffers();
 
   void build_meta();
   void factor_subdomains();

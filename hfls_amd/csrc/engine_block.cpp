@@ -115,9 +115,10 @@ void Engine::block_prepare() {
   b_v6off.upload(v6off, st);
   b_v6row.upload(v6row, st);
   b_crn.upload(crn, st);
-  // work blocks (zero: padded rows and unused columns must stay finite)
-  zero_alloc(b_xb, NBP * KP, st);
-  zero_alloc(b_yb, NBP * KP, st);
+  // work blocks of block context 0 (zero: padded rows and unused columns must stay finite)
+  b_nparts = (int)((nl + 63) / 64);
+  if (ctxs_.empty()) ctxs_.emplace_back(new BlockCtx);  // placeholder of the current context 0
+  alloc_ctx_buffers();
   // CRQ = [C_i | R_b,i | Q_i | Yh_i]: n_s x (nlc + 18), column major, ld = ldb, with
   // Q_i an orthonormal basis of span(R_b,i) and, for A = K+_bb, Y = A Q, W = Q^T Y,
   // Yh = Y - Q W / 2, so that A = (I - QQ^T) A (I - QQ^T) + Q Yh^T + Yh Q^T.  The
@@ -214,6 +215,17 @@ void Engine::block_prepare() {
   b_St32.alloc(off_P[N]);
   HC(launch_kb_deflate_f32(st, N, maxtiles, d_ns.p, d_Kt.p, d_off_P.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, b_Kt32.p));
   HC(launch_kb_to_f32(st, off_P[N], d_St.p, b_St32.p));
+  HC(hipStreamSynchronize(st));
+  autotune();
+  blk_ready = true;
+}
+
+// the buffers of the current block context
+void Engine::alloc_ctx_buffers() {
+  const int N = P->nsub;
+  const int64_t nl = P->nlam, np = off_n[N], ng = 3 * (int64_t)P->nPoints;
+  zero_alloc(b_xb, NBP * KP, st);
+  zero_alloc(b_yb, NBP * KP, st);
   zero_alloc(b_xb32, NBP * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
@@ -225,7 +237,6 @@ void Engine::block_prepare() {
   zero_alloc(b_hg, (int64_t)6 * N * KMAX, st);
   zero_alloc(b_hgf, (int64_t)6 * N * KMAX, st);
   for (DVec<double>* v : {&b_lam, &b_g, &b_w, &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64}) zero_alloc(*v, nl * KMAX, st);
-  b_nparts = (int)((nl + 63) / 64);
   zero_alloc(b_part, (int64_t)b_nparts * KMAX, st);
   zero_alloc(b_gt, (int64_t)6 * N * KP, st);
   zero_alloc(b_Vr, NRP * KP, st);
@@ -236,9 +247,45 @@ void Engine::block_prepare() {
   zero_alloc(b_X, np * KMAX, st);
   zero_alloc(b_U, ng * KMAX, st);
   b_sc.alloc(1);
-  HC(hipStreamSynchronize(st));
-  autotune();
-  blk_ready = true;
+}
+
+void Engine::swap_ctx(BlockCtx& c) {
+  std::swap(st, c.st);
+  std::swap(gx_blk, c.gx);
+  std::swap(gx_blk_key, c.gx_key);
+  std::swap(blk_k, c.k);
+  std::swap(blk_prec, c.prec);
+  b_xb32.swap(c.xb32);
+  b_xb.swap(c.xb); b_yb.swap(c.yb); b_cr.swap(c.cr); b_coef.swap(c.coef); b_kpart.swap(c.kpart); b_v6.swap(c.v6);
+  b_gc.swap(c.gc); b_la.swap(c.la); b_h.swap(c.h); b_hg.swap(c.hg); b_hgf.swap(c.hgf); b_lam.swap(c.lam);
+  b_g.swap(c.g); b_w.swap(c.w); b_Fw.swap(c.Fw); b_Pg.swap(c.Pg); b_z.swap(c.z); b_Pz.swap(c.Pz); b_d.swap(c.d);
+  b_g64.swap(c.g64); b_part.swap(c.part); b_gt.swap(c.gt); b_Vr.swap(c.Vr); b_Xr.swap(c.Xr); b_Tr.swap(c.Tr);
+  b_V.swap(c.V); b_V2.swap(c.V2); b_X.swap(c.X); b_U.swap(c.U);
+  b_sc.swap(c.sc);
+}
+
+void Engine::use_ctx(int c) {
+  if (c == cur_ctx_) return;
+  if (c < 0 || c >= (int)ctxs_.size()) throw std::runtime_error("no such block context");
+  swap_ctx(*ctxs_[cur_ctx_]);  // the current context into its slot, its placeholder into the engine
+  swap_ctx(*ctxs_[c]);         // context c into the engine
+  cur_ctx_ = c;
+}
+
+int Engine::blk_contexts(int n) {
+  if (n < 1 || n > 8) throw std::runtime_error("block contexts must be 1..8");
+  use_ctx(0);
+  block_prepare();
+  while ((int)ctxs_.size() < n) {
+    std::unique_ptr<BlockCtx> c(new BlockCtx);
+    HC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    ctxs_.push_back(std::move(c));
+    use_ctx((int)ctxs_.size() - 1);
+    alloc_ctx_buffers();
+    HC(hipStreamSynchronize(st));
+    use_ctx(0);
+  }
+  return (int)ctxs_.size();
 }
 
 // Autotuner of the block GEMM tile shape: every candidate (64 or 128 output rows
@@ -634,6 +681,106 @@ void Engine::blk_collect(const std::vector<int>& cols, double* u_h, double* lam_
   }
   blk_set_status(cols, 4);
   HC(hipStreamSynchronize(st));
+}
+
+void Engine::solve_blocks(int nb, double eps, int max_iter, int k, int prec, const double* f_h, double* u_h,
+                          int* iters, int* status, double* res, BlockInfo* info) {
+  if (nb < 1) throw std::runtime_error("at least one block");
+  blk_contexts(nb);
+  const int64_t np = off_n[P->nsub], ng = 3 * (int64_t)P->nPoints;
+  std::vector<int> cols(k);
+  for (int j = 0; j < k; j++) cols[j] = j;
+  for (int c = 0; c < nb; c++) {  // loads to HBM: outside the timed span
+    use_ctx(c);
+    blk_begin(k, eps, max_iter, prec);
+    blk_load(cols, f_h + (size_t)c * k * np);
+    HC(hipStreamSynchronize(st));
+  }
+  std::vector<hipEvent_t> ev(nb + 4);
+  for (auto& e : ev) HC(hipEventCreate(&e));
+  // ev[0]: start (context 0), ev[1]: init done (context 0), ev[2]: loop done
+  // (context 0), ev[3]: span end; ev[4 + c]: end of context c
+  use_ctx(0);
+  HC(hipEventRecord(ev[0], st));
+  for (int c = 1; c < nb; c++) {
+    use_ctx(c);
+    HC(hipStreamWaitEvent(st, ev[0], 0));
+  }
+  for (int c = 0; c < nb; c++) {
+    use_ctx(c);
+    blk_init(cols);
+    if (c == 0) HC(hipEventRecord(ev[1], st));
+  }
+  std::vector<BScalars> sh(nb);
+  std::vector<int> sweeps(nb, 0), run(nb, 1);
+  const int chunk = poll_chunk;
+  for (bool any = true; any;) {
+    any = false;
+    for (int c = 0; c < nb; c++)  // one chunk per running block, all streams busy at once
+      if (run[c]) {
+        use_ctx(c);
+        blk_sweeps(chunk);
+        sweeps[c] += chunk;
+      }
+    for (int c = 0; c < nb; c++)
+      if (run[c]) {
+        use_ctx(c);
+        blk_sync(&sh[c]);
+        bool r = false;
+        for (int j = 0; j < k; j++)
+          if (sh[c].done[j] == 0 || sh[c].done[j] == 1) r = true;
+        if (sh[c].err || !r || sweeps[c] > max_iter + 2 * chunk) run[c] = 0;
+        else any = true;
+      }
+  }
+  use_ctx(0);
+  HC(hipEventRecord(ev[2], st));
+  for (int c = 0; c < nb; c++) {
+    use_ctx(c);
+    blk_recover(&sh[c]);
+    if (c > 0) HC(hipEventRecord(ev[4 + c], st));
+  }
+  use_ctx(0);
+  for (int c = 1; c < nb; c++) HC(hipStreamWaitEvent(st, ev[4 + c], 0));
+  HC(hipEventRecord(ev[3], st));
+  HC(hipEventSynchronize(ev[3]));
+  for (int c = 0; c < nb; c++) {
+    use_ctx(c);
+    std::vector<int> its(k), stt(k);
+    for (int j = 0; j < k; j++) {
+      its[j] = sh[c].it[j];
+      stt[j] = sh[c].done[j];
+    }
+    blk_collect(cols, u_h + (size_t)c * k * ng, nullptr);
+    BScalars s2;
+    blk_sync(&s2);
+    for (int j = 0; j < k; j++) {
+      const size_t o = (size_t)c * k + j;
+      if (iters) iters[o] = its[j];
+      int sj = stt[j];
+      if (sj == 0 || sj == 1) sj = 3;
+      if (status) status[o] = sj;
+      if (res) res[o] = s2.res[j];
+    }
+  }
+  use_ctx(0);
+  if (info) {
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    info->solve_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    info->init_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    info->pcpg_ms = ms;
+    HC(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    info->recover_ms = ms;
+    info->sweeps = *std::max_element(sweeps.begin(), sweeps.end());
+    int cs = 0;
+    for (int c = 0; c < nb; c++)
+      for (int j = 0; j < k; j++) cs += sh[c].ncert[j];
+    info->cert_steps = cs;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,

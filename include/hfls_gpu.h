@@ -127,6 +127,13 @@ typedef struct {
 } hfls_block_stats;
 int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, int precision, const double* f_local, double* u,
                      double* lambda, hfls_case_stats* cases, hfls_block_stats* stats);
+/* nblocks (1..8) blocks of k load cases solved concurrently on the mesh, one block
+ * context (own buffers, scalars and HIP stream, shared mesh operators) per block:
+ * f_local holds nblocks*k load cases, u and cases nblocks*k results; the block
+ * stats report the device time span of all blocks (no counterpart in the
+ * reference: a throughput form of hfls_solve_block). */
+int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblocks, int k, int precision, const double* f_local,
+                      double* u, hfls_case_stats* cases, hfls_block_stats* stats);
 /* block operators on k dual vectors (column major, ld nlam): op 0 mult_Ff
  * (Cluster.cpp:1114-1188), 1 Preconditioning (1043-1112), 2 Projection
  * (1314-1329); precision 32 or 64 selects the F_i / S_i arithmetic */

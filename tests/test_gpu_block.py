@@ -118,3 +118,22 @@ def test_same_alone_and_in_block(pair, precision):
     perm = list(range(1, K)) + [0]
     rb = gpu.pcpg_block(F[perm], eps=1e-8, max_iter=300, precision=precision)
     assert np.array_equal(rb["u"][K - 1], blk["u"][0])
+
+
+@pytest.mark.parametrize("nblocks", [2, 3])
+def test_concurrent_blocks_same_as_single(pair, nblocks):
+    """Blocks solved concurrently in separate block contexts (own buffers and
+    streams, shared mesh operators) give bit-identical results to one block at a
+    time."""
+    name, gpu, ora, F = pair
+    one = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=32)
+    order = [F, F[::-1], F[1:].tolist() + [F[0]]][:nblocks]
+    Fs = np.concatenate([np.asarray(f) for f in order])
+    r = gpu.pcpg_blocks(Fs, k=K, eps=1e-8, max_iter=300, precision=32)
+    print(name, "blocks", nblocks, "span %.1f ms" % r["solve_ms"], flush=True)
+    assert np.all(r["status"] == 2)
+    perms = [list(range(K)), list(range(K))[::-1], list(range(1, K)) + [0]][:nblocks]
+    for b, p in enumerate(perms):
+        for j, src in enumerate(p):
+            assert np.array_equal(r["u"][b * K + j], one["u"][src])
+            assert r["iterations"][b * K + j] == one["iterations"][src]
