@@ -253,6 +253,15 @@ def main():
     total_ms = float(np.sum(blk_ms))
     value = a.k * len(blk_ms) / (total_ms / 1e3)
     its = np.concatenate(its)
+    # two blocks at once in two block contexts (own streams, shared mesh operators)
+    conc = None
+    if nblk >= 2 and not a.small:
+        F2 = F[:2 * a.k]
+        mesh.pcpg_blocks(F2, k=a.k, eps=a.eps, max_iter=1000, precision=32)  # graphs of the second context
+        rc = mesh.pcpg_blocks(F2, k=a.k, eps=a.eps, max_iter=1000, precision=32)
+        assert np.all(rc["status"] == 2), "a load case did not converge"
+        conc = {"blocks": 2, "span_ms": rc["solve_ms"], "solves_per_s": 2 * a.k / (rc["solve_ms"] / 1e3)}
+        log("2 concurrent blocks: span %.1f ms -> %.1f solves/s" % (rc["solve_ms"], conc["solves_per_s"]))
     # plain vs block on the same case (first timed case): displacement agreement
     r1 = mesh.pcpg(eps=a.eps, max_iter=1000, f_packed=F[a.warmup * a.k], want_lambda=False)
     rb = mesh.pcpg_block(F[a.warmup * a.k:a.warmup * a.k + 1], eps=a.eps, max_iter=1000, precision=32)
@@ -288,6 +297,7 @@ def main():
         "plain": {"value": plain_rate, "unit": "solves/s", "ms_per_solve": float(np.mean(plain_ms)),
                   "iterations": plain_it[-1], "dtype": "f64"},
         "speedup_vs_plain": value / plain_rate,
+        "concurrent_blocks": conc,
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),
         "block_vs_plain_displacement_rel_diff": agree,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
