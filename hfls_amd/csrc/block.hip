@@ -610,10 +610,10 @@ __global__ __launch_bounds__(256) void kb_colsum(int nparts, int k, const double
 
 __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, const int* __restrict__ gate) {
   const int j = threadIdx.x;
-  __shared__ int any, run, rst, cert;
+  __shared__ int any, run, rst, cert, nwait, nrun, maxw;
   if (gate && *gate == 0) return;              // uniform over the workgroup
   if (mode == 3 && sc->pending == 0) return;
-  if (j == 0) any = run = rst = cert = 0;
+  if (j == 0) any = run = rst = cert = nwait = nrun = maxw = 0;
   __syncthreads();
   if (j < k) {
     const double s = sc->colsum[j];
@@ -635,7 +635,14 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         if (!(r >= sc->eps * sc->norm0[j])) sc->done[j] = sc->cert ? 1 : 2;
         else if (r < sc->rtol * sc->ref[j]) sc->done[j] = 1;
         else if (sc->it[j] >= sc->max_iter) sc->done[j] = 3;
-        if (sc->done[j] == 1) any = 1;
+      }
+      if (sc->done[j] == 1) {  // waiting (newly or still) for the FP64 evaluation
+        sc->waitn[j] += 1;
+        any = 1;
+        atomicAdd(&nwait, 1);
+        atomicMax(&maxw, sc->waitn[j]);
+      } else if (sc->done[j] == 0) {
+        atomicAdd(&nrun, 1);
       }
     } else if (mode == 2) {
       if (stt == 5) {
@@ -647,6 +654,7 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->repl[j] = 0;
         sc->ncert[j] = 0;
         sc->hgok[j] = 0;
+        sc->waitn[j] = 0;
         if (!(s == s)) sc->err = 1;
         sc->done[j] = !(s > 0.0) ? 2 : (sc->max_iter <= 0 ? 3 : 0);
       }
@@ -657,6 +665,7 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         const double r = sqrt(fabs(s));
         sc->res[j] = sc->norm0[j] > 0 ? r / sc->norm0[j] : 0.0;
         sc->ncert[j] += 1;
+        sc->waitn[j] = 0;
         if (r < sc->eps * sc->norm0[j]) {
           sc->done[j] = 2;
           sc->certnow[j] = 1;
@@ -679,7 +688,10 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
   }
   __syncthreads();
   if (j == 0) {
-    if (mode == 1) sc->pending = any;
+    // FP64 evaluations are batched: they run once a quarter of the live columns
+    // wait, when no column is left iterating, or when a column has waited 3 sweeps
+    // (a waiting column is frozen, so its results do not depend on when it runs)
+    if (mode == 1) sc->pending = any && (nrun == 0 || 4 * nwait >= nwait + nrun || maxw >= 3);
     if (mode == 1) sc->certified = 0;
     if (mode == 3) sc->certified = cert;
     if (mode == 2 || mode == 3) sc->pending = 0;

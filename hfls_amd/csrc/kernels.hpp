@@ -176,6 +176,7 @@ struct BScalars {         // per-column PCPG scalars of a block solve
   int it[KMAX], done[KMAX], repl[KMAX], ncert[KMAX];
   int hgok[KMAX];          // the saved H alpha of the column's certified true residual is valid
   int certnow[KMAX];       // certified by the last certification reduction
+  int waitn[KMAX];         // gamma reductions the column has spent waiting for its FP64 evaluation
   int certified;           // some column was certified since the last gamma reduction
   double eps, rtol;        // stopping tolerance; refinement factor (0: none)
   int max_iter, err;
