@@ -51,8 +51,8 @@ SEED = 20240611
 # HBM bytes per launch of k_tgemm_f32 on the default workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
 # the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
-# 9.844e9 fetched + 4.445e8 written.  Valid for the default workload only.
-GEMM_TRAFFIC_BYTES = 10.29e9
+# 9.640e9 fetched + 4.445e8 written.  Valid for the default workload only.
+GEMM_TRAFFIC_BYTES = 10.08e9
 
 
 def cpu_sample():
