@@ -51,8 +51,8 @@ SEED = 20240611
 # HBM bytes per launch of k_tgemm_f32 on the default workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
 # the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
-# 9.640e9 fetched + 4.445e8 written.  Valid for the default workload only.
-GEMM_TRAFFIC_BYTES = 10.08e9
+# 7.356e9 fetched + 4.445e8 written.  Valid for the default workload only.
+GEMM_TRAFFIC_BYTES = 7.80e9
 
 
 def cpu_sample():
@@ -76,23 +76,44 @@ def cpu_sample():
 
 
 def run_cpu_baseline(gpu_iterations, nsub_target):
+    """One single-threaded oracle process, then one per host core (at most 16)
+    solving concurrently: the all-cores rate is the sum of the per-process rates."""
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample"]
     t0 = time.time()
-    p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample"], capture_output=True, text=True,
-                       env=env, timeout=900)
+    p = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
     if p.returncode != 0:
         return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port",
                 "sample": "failed: " + p.stderr[-300:]}
     s = json.loads(p.stdout.strip().splitlines()[-1])
-    # per subdomain-iteration cost of the sample, scaled to the workload
-    per = s["seconds"] / (s["nsub"] * max(1, s["iterations"] + 3))
-    t_full = per * nsub_target * (gpu_iterations + 3)
-    return {"value": 1.0 / t_full, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": ("oracle (numpy/scipy, SuperLU K_reg, implicit K+ per iteration) on 2x2x2 subdomains of "
-                       "10^3 elements: %.3f s per solve (mean of %d) for %d iterations; scaled per subdomain-iteration "
-                       "to %d subdomains x %d iterations" % (s["seconds"], s["reps"], s["iterations"], nsub_target,
-                                                           gpu_iterations)),
-            "wall_s": round(time.time() - t0, 1)}
+
+    def scaled(seconds, iterations):
+        # per subdomain-iteration cost of the sample, scaled to the workload
+        per = seconds / (s["nsub"] * max(1, iterations + 3))
+        return 1.0 / (per * nsub_target * (gpu_iterations + 3))
+
+    out = {"value": scaled(s["seconds"], s["iterations"]), "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": ("oracle (numpy/scipy, SuperLU K_reg, implicit K+ per iteration) on 2x2x2 subdomains of "
+                      "10^3 elements: %.3f s per solve (mean of %d) for %d iterations; scaled per subdomain-iteration "
+                      "to %d subdomains x %d iterations" % (s["seconds"], s["reps"], s["iterations"], nsub_target,
+                                                          gpu_iterations))}
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    if ncores > 1:
+        t1 = time.time()
+        procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+                 for _ in range(ncores)]
+        res = [pr.communicate(timeout=900) for pr in procs]
+        if all(pr.returncode == 0 for pr in procs):
+            per = [json.loads(o.strip().splitlines()[-1]) for o, _ in res]
+            out["all_cores"] = {"value": sum(scaled(q["seconds"], q["iterations"]) for q in per), "cores": ncores,
+                                "sample": "%d concurrent single-threaded oracle processes on the same sample, "
+                                          "%.3f s per solve on average" % (ncores, sum(q["seconds"] for q in per) / ncores),
+                                "wall_s": round(time.time() - t1, 1)}
+        else:
+            out["all_cores"] = {"value": None, "cores": ncores,
+                                "sample": "failed: " + next(e for (_, e), pr in zip(res, procs) if pr.returncode)[-300:]}
+    out["wall_s"] = round(time.time() - t0, 1)
+    return out
 
 
 def log(*a):
@@ -270,6 +291,12 @@ def main():
     # roofline of the dominant kernel: the FP32 tile GEMM over 64 columns
     g_ms, g_flops = mesh.time_stream(2, reps=10)
     s_ms, s_flops = mesh.time_stream(3, reps=10)
+    # sustained matrix-core rates on register data with random operands (no memory
+    # traffic): the ceiling the FP32 tile GEMM runs against on real data
+    ceil = {}
+    for name, which in (("f64_16x16x4", 5), ("f32_32x32x2", 6)):
+        c_ms, c_fl = mesh.time_stream(which, reps=3)
+        ceil[name] = c_fl / (c_ms * 1e-3) / 1e12
     tune = mesh.tuning()
     achieved = g_flops / (g_ms * 1e-3) / 1e12
     lat = np.array(blk_ms)
@@ -306,6 +333,8 @@ def main():
                      "kernel": "k_tgemm_f32 (deflated K+_bb, packed symmetric 64x64 tiles, 64 columns)",
                      "kernel_ms": g_ms, "flops_per_launch": g_flops,
                      "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12,
+                     "measured_mfma_ceiling_TFs": ceil,
+                     "frac_of_measured_f32_ceiling": achieved / ceil["f32_32x32x2"],
                      "autotune": {"row_tiles_per_workgroup": tune["row_tiles"],
                                   "candidate_ms": [tune["ms"][1], tune["ms"][2]]}},
     }

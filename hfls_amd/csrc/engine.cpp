@@ -1209,6 +1209,27 @@ void Engine::kplus_local(const double* f_h, double* x_h) {
 
 double Engine::time_stream(int which, int reps, double* bytes) {
   if (which >= 2 && which <= 4) return time_block_gemm(which - 2, reps, bytes);
+  if (which == 5 || which == 6) {  // matrix-core ceilings, FP64 / FP32, on register data
+    const int nwg = 8 * 256, iters = which == 5 ? 400 : 800;
+    DVec<double> out;
+    out.alloc((size_t)nwg * 256);
+    double fl = 0;
+    HC(launch_mfma_ceiling(st, which == 5, nwg, iters, out.p, &fl));  // warm-up
+    hipEvent_t a, b;
+    HC(hipEventCreate(&a));
+    HC(hipEventCreate(&b));
+    HC(hipEventRecord(a, st));
+    for (int i = 0; i < reps; i++) HC(launch_mfma_ceiling(st, which == 5, nwg, iters, out.p, &fl));
+    HC(hipEventRecord(b, st));
+    HC(hipEventSynchronize(b));
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (bytes) *bytes = fl;
+    return ms / reps;
+  }
+  if (which < 0 || which > 6) throw std::runtime_error("time_stream: which must be 0..6");
   const int N = P->nsub;
   const int64_t nl = P->nlam;
   std::vector<double> ones(nl, 1.0);

@@ -168,6 +168,11 @@ hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, con
                            const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
 hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
                            const double* A, const int64_t* aoff, const int* lda, double* P, const int64_t* poff);
+// matrix-core ceiling on register data with random operands (no memory traffic): nwg workgroups of 4
+// waves, each wave `iters` rounds of independent MFMAs; one value per thread to out
+// (nwg * 256 doubles).  f64: v_mfma_f64_16x16x4f64, else v_mfma_f32_32x32x2f32.
+// Returns the flops of the launch in *flops.
+hipError_t launch_mfma_ceiling(hipStream_t st, bool f64, int nwg, int iters, double* out, double* flops);
 
 // ---------------------------------------------------------------- block PCPG (block.hip)
 constexpr int KMAX = 64;  // load cases per block

@@ -193,21 +193,36 @@ static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i
 
 // RT row tiles (64 rows each) per workgroup share the X chunk in LDS: RT = 2
 // halves the X staging and LDS reads per matrix-core operation (chosen per mesh
-// by the autotuner of the block solve)
+// by the autotuner of the block solve).
+//
+// Workgroup order: the hardware hands consecutive workgroups to the 8 XCDs in
+// turn, each XCD with its own L2.  The linear workgroup id is mapped so that all
+// row blocks of one subdomain run on one XCD (slot s = id / 8 of XCD id % 8 is row
+// block s % nslot of subdomain 8 (s / nslot) + id % 8), and row block i takes the
+// tile columns in the order j = (t - r_i) mod Ko, t = 0, 1, ..., with r_i = i & ~1
+// (the even row block of its pair): at step t, row blocks i and j = t - r_i need
+// the same symmetric tile (max(i,j), min(i,j)) within a step of each other, one
+// directly, one transposed, so the second read is served by the shared L2, as is
+// the X block of the subdomain.  Both tile shapes (RT = 1, 2) use the same r_i, so
+// they add every row's products in the same order and give bit-identical results.
 template <int RT>
-__global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* __restrict__ mrows,
+__global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, int nslot,
+                                                   const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const float* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const float* __restrict__ X,
                                                    const int64_t* __restrict__ xoff, double* __restrict__ Y,
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
                                                    int kp, float alpha, const int* __restrict__ gate) {
-  const int d = blockIdx.y;
+  const int sl = blockIdx.x >> 3;
+  const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
+  if (d >= nsub) return;
   if (gate && *gate == 0) return;
   const int Ra = (mrows[d] + 63) / 64, Ca = (mcols[d] + 63) / 64;
   const int Ro = op == 1 && mode != 0 ? Ca : Ra;
   const int Ko = op == 1 && mode != 0 ? Ra : Ca;
-  const int i0 = blockIdx.x * RT;
+  const int i0 = (sl % nslot) * RT;
   if (i0 >= Ro) return;
+  const int rot = (i0 & ~1) % Ko;  // K chunk c covers tile column (c/2 - rot) mod Ko
   const int nrt = min(RT, Ro - i0);
   const int c0 = blockIdx.z * 64;
   __shared__ float As[RT][32 * 65];
@@ -225,12 +240,15 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
   StageF s[RT];
   int kind[RT];
   auto fetch = [&](int c) {
+    int tj = (c >> 1) - rot;
+    if (tj < 0) tj += Ko;
+    const int cc = 2 * tj + (c & 1);
 #pragma unroll
     for (int q = 0; q < RT; q++) {
       if (q < nrt) {
         int64_t ti;
-        kind[q] = chunk_kind(mode, op, Ra, i0 + q, c, &ti);
-        fetch_f32(Ad + ti * 4096, kind[q], (c & 1) * 32, Xd + (int64_t)((c >> 1) * 64) * kp, kp, s[q]);
+        kind[q] = chunk_kind(mode, op, Ra, i0 + q, cc, &ti);
+        fetch_f32(Ad + ti * 4096, kind[q], (c & 1) * 32, Xd + (int64_t)(tj * 64) * kp, kp, s[q]);
       }
     }
   };
@@ -278,9 +296,13 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, const int* 
 
 // S > 1 (split K, one output row block): workgroup s of a subdomain takes K chunks
 // [s*nch/S, (s+1)*nch/S) and writes its partial 64 x kp block to Ypart at
-// (d*S + s)*64*kp; k_splitk_sum adds the partials in a fixed order
+// (d*S + s)*64*kp; k_splitk_sum adds the partials in a fixed order.
+// Workgroups are grouped per XCD as in k_tgemm_f32 (nslot = row blocks, or S);
+// with symmetric tiles and no split, row block i takes the tile columns in the
+// order (t - i) mod Ko, so the two reads of every off-diagonal tile meet in L2.
 template <bool CM>  // compact column-major A (mode 2) or tiles
-__global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* __restrict__ mrows,
+__global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, int nslot,
+                                                   const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const double* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const int* __restrict__ lda,
                                                    const double* __restrict__ X,
@@ -288,15 +310,18 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
                                                    int kp, double alpha, const int* __restrict__ gate, int S,
                                                    double* __restrict__ Ypart) {
-  const int d = blockIdx.y;
+  const int sl = blockIdx.x >> 3;
+  const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
+  if (d >= nsub) return;
   if (gate && *gate == 0) return;
   const int m = mrows[d], n = mcols[d];
   const int Ra = (m + 63) / 64, Ca = (n + 63) / 64;
   const int Ro = (mode != 0 && op == 1) ? Ca : Ra;
   const int Ko = (mode != 0 && op == 1) ? Ra : Ca;
-  const int i = S > 1 ? 0 : blockIdx.x;
-  const int sp = S > 1 ? blockIdx.x : 0;
+  const int i = S > 1 ? 0 : sl % nslot;
+  const int sp = S > 1 ? sl % nslot : 0;
   if (i >= Ro) return;
+  const int rot = (mode == 0 && S == 1) ? i : 0;  // K chunk c covers tile column (c/2 - rot) mod Ko
   const int c0 = blockIdx.z * 64;
   __shared__ double As[32 * 65];
   __shared__ double Xs[32 * 64];
@@ -317,23 +342,20 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, const int* 
     cend = op == 0 ? 2 * (i + 1) : nall;
   }
   StageD s;
-  if (cend > cbeg) {
+  auto fetch = [&](int c) {
+    int tj = (c >> 1) - rot;
+    if (tj < 0) tj += Ko;
     int64_t ti;
-    const int kind = chunk_kind(mode, op, Ra, i, cbeg, &ti);
-    fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, cbeg >> 1, kind, (cbeg & 1) * 32,
-                  Xd + (int64_t)((cbeg >> 1) * 64) * kp, kp, s);
-    stash_f64<CM>(kind, s, As, Xs);
-  }
+    const int kind = chunk_kind(mode, op, Ra, i, 2 * tj + (c & 1), &ti);
+    fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, tj, kind, (c & 1) * 32, Xd + (int64_t)(tj * 64) * kp, kp, s);
+    return kind;
+  };
+  if (cend > cbeg) stash_f64<CM>(fetch(cbeg), s, As, Xs);
   __syncthreads();
   for (int c = cbeg; c < cend; c++) {
     const bool more = c + 1 < cend;
     int kind_n = 0;
-    if (more) {
-      int64_t ti;
-      kind_n = chunk_kind(mode, op, Ra, i, c + 1, &ti);
-      fetch_f64<CM>(Ad + ti * 4096, Ad, la, m, n, i, (c + 1) >> 1, kind_n, ((c + 1) & 1) * 32,
-                    Xd + (int64_t)(((c + 1) >> 1) * 64) * kp, kp, s);
-    }
+    if (more) kind_n = fetch(c + 1);
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 4) {
       const double a = As[(kk + (lane >> 4)) * 65 + rb + (lane & 15)];
@@ -390,12 +412,15 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
                             const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt) {
   (void)lda;
   if (kp % 64 || mode == 2 || mode == 3) return hipErrorInvalidValue;  // FP32: modes 0, 1
-  if (rt == 2)
-    hipLaunchKernelGGL(k_tgemm_f32<2>, dim3((maxRo + 1) / 2, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols,
-                       A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
-  else
-    hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(maxRo, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
-                       X, xoff, Y, yoff, Z, kp, alpha, gate);
+  const int nsub8 = (nsub + 7) / 8;  // subdomain groups of 8, one subdomain of each per XCD
+  if (rt == 2) {
+    const int nslot = (maxRo + 1) / 2;
+    hipLaunchKernelGGL(k_tgemm_f32<2>, dim3(8 * nslot * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, nslot,
+                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
+  } else {
+    hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(8 * maxRo * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, maxRo,
+                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
+  }
   return hipGetLastError();
 }
 
@@ -406,13 +431,14 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
   if (S > 1 && (maxRo != 1 || !Ypart || mode == 3)) return hipErrorInvalidValue;
-  const int gx = S > 1 ? S : maxRo;
+  const int nslot = S > 1 ? S : maxRo;
+  const dim3 grid(8 * nslot * ((nsub + 7) / 8), 1, kp / 64);
   if (mode == 2)
-    hipLaunchKernelGGL(k_tgemm_f64<true>, dim3(gx, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A, aoff,
-                       lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
+    hipLaunchKernelGGL(k_tgemm_f64<true>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
+                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
   else
-    hipLaunchKernelGGL(k_tgemm_f64<false>, dim3(gx, nsub, kp / 64), dim3(256), 0, st, mode, op, mrows, mcols, A,
-                       aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
+    hipLaunchKernelGGL(k_tgemm_f64<false>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
+                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
   if (S > 1)
     hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub), dim3(256), 0, st, S, kp, Ypart, Y, yoff, Z,
                        gate);
@@ -469,6 +495,76 @@ hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, con
                            const double* A, const int64_t* aoff, const int* lda, double* P, const int64_t* poff) {
   hipLaunchKernelGGL(k_pack_any<double>, dim3(maxtiles, nsub), dim3(256), 0, st, mode, mrows, mcols, A, aoff, lda, P,
                      poff);
+  return hipGetLastError();
+}
+
+// matrix-core ceiling: 8 (f64) or 4 (f32) independent accumulator chains per wave,
+// operands in registers drawn per round from a xorshift stream (uniform in
+// [-0.5, 0.5) with random mantissas, so the datapaths toggle as on real data), so
+// the launch measures the sustained matrix-core rate under the clock and power the
+// device holds for it
+static __device__ __forceinline__ unsigned xorshift(unsigned h) {
+  h ^= h << 13;
+  h ^= h >> 17;
+  h ^= h << 5;
+  return h;
+}
+static __device__ __forceinline__ float unit_half(unsigned bits) {
+  return __uint_as_float((bits & 0x007fffffu) | 0x3f800000u) - 1.5f;
+}
+
+__global__ __launch_bounds__(256) void k_mfma_ceiling_f64(double* __restrict__ out, int iters, unsigned seed) {
+  unsigned h = seed ^ (threadIdx.x * 2654435761u) ^ (blockIdx.x * 40503u);
+  doublex4 c[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) c[i][r] = 0.0;
+  for (int it = 0; it < iters; it++) {
+    h = xorshift(h);
+    const double a = unit_half(h), b = unit_half(h >> 9);
+#pragma unroll
+    for (int i = 0; i < 8; i++) c[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c[i], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) s += c[i][r];
+  out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_mfma_ceiling_f32(double* __restrict__ out, int iters, unsigned seed) {
+  unsigned h = seed ^ (threadIdx.x * 2654435761u) ^ (blockIdx.x * 40503u);
+  floatx16 c[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) c[i][r] = 0.0f;
+  for (int it = 0; it < iters; it++) {
+    h = xorshift(h);
+    const float a = unit_half(h), b = unit_half(h >> 9);
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c[i], 0, 0, 0);
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) s += c[i][r];
+  out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+hipError_t launch_mfma_ceiling(hipStream_t st, bool f64, int nwg, int iters, double* out, double* flops) {
+  if (nwg < 1 || iters < 1) return hipErrorInvalidValue;
+  const double waves = 4.0 * nwg;
+  if (f64) {
+    hipLaunchKernelGGL(k_mfma_ceiling_f64, dim3(nwg), dim3(256), 0, st, out, iters, 12345u);
+    if (flops) *flops = waves * iters * 8 * (2.0 * 16 * 16 * 4);
+  } else {
+    hipLaunchKernelGGL(k_mfma_ceiling_f32, dim3(nwg), dim3(256), 0, st, out, iters, 12345u);
+    if (flops) *flops = waves * iters * 4 * (2.0 * 32 * 32 * 2);
+  }
   return hipGetLastError();
 }
 

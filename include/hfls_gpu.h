@@ -182,7 +182,9 @@ int hfls_server_stats(hfls_server s, double* out);
  * (inside mult_Ff) / S_i (inside Preconditioning), *bytes = algorithmic bytes of
  * one launch; which = 2 / 3: the FP32 tile GEMM of the block solve over 64
  * columns with K+_bb / S_i, *bytes = algorithmic flops of one launch; which = 4:
- * a plain device copy (reference for traffic counters), *bytes = bytes read. */
+ * a plain device copy (reference for traffic counters), *bytes = bytes read;
+ * which = 5 / 6: the sustained FP64 / FP32 matrix-core rate on register data with
+ * random operands (no memory traffic), *bytes = flops of one launch. */
 int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
 
 #ifdef __cplusplus

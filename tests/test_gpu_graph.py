@@ -62,3 +62,20 @@ def test_autotuned_gemm_shapes_agree(name):
         out[rows] = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     assert np.array_equal(out[1]["u"], out[2]["u"]) and np.array_equal(out[1]["u"], r_auto["u"])
     m.close()
+
+
+def test_measurement_hooks():
+    """The timing hook: every kind returns a positive time and work figure, the
+    matrix-core ceilings report their flops, and an unknown kind is an error."""
+    m = hfls_amd.Mesh(**CASES["cfg0"])
+    F = m.traction(79, 0, 2)
+    m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    for which in range(7):
+        ms, work = m.time_stream(which, reps=2)
+        print("hook", which, ms, work, flush=True)
+        assert ms > 0 and work > 0
+    ms5, fl5 = m.time_stream(5, reps=1)
+    assert fl5 == 4.0 * 8 * 256 * 400 * 8 * (2 * 16 * 16 * 4)
+    with pytest.raises(hfls_amd.HflsError):
+        m.time_stream(7, reps=1)
+    m.close()
