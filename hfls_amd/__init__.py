@@ -91,6 +91,22 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
+def _vec(x, n, name):
+    """x as a contiguous float64 vector of exactly n entries (the C ABI reads n)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if x.ndim != 1 or x.shape[0] != n:
+        raise ValueError("%s must have shape (%d,), got %s" % (name, n, x.shape))
+    return x
+
+
+def _rows(X, n, name):
+    """X as a contiguous float64 (k, n) block with k >= 1 rows."""
+    X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.float64)
+    if X.ndim != 2 or X.shape[1] != n or X.shape[0] < 1:
+        raise ValueError("%s must have shape (k >= 1, %d), got %s" % (name, n, X.shape))
+    return X
+
+
 class Problem:
     """Host-side mesh, numbering and gluing (Mesh.cpp:69-289, Data.cpp:111-144,
     Cluster.cpp:682-959) computed by the native library."""
@@ -237,25 +253,25 @@ class Mesh:
         return f
 
     def mult_Ff(self, w):
-        w = np.ascontiguousarray(w, dtype=np.float64)
+        w = _vec(w, self.nlam, "w")
         out = np.zeros(self.nlam)
         _check(load().hfls_apply_F(self._h, _ptr(w), _ptr(out)))
         return out
 
     def projection(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _vec(x, self.nlam, "x")
         Px, alpha = np.zeros(self.nlam), np.zeros(self.ncol)
         _check(load().hfls_project(self._h, _ptr(x), _ptr(Px), _ptr(alpha)))
         return Px, alpha
 
     def preconditioning(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _vec(x, self.nlam, "x")
         out = np.zeros(self.nlam)
         _check(load().hfls_precondition(self._h, _ptr(x), _ptr(out)))
         return out
 
     def mult_Kplus_f(self, f_packed):
-        f = np.ascontiguousarray(f_packed, dtype=np.float64)
+        f = _vec(f_packed, self.npacked, "f_packed")
         out = np.zeros(self.npacked)
         _check(load().hfls_kplus(self._h, _ptr(f), _ptr(out)))
         return out
@@ -269,7 +285,7 @@ class Mesh:
         u = np.zeros(3 * self.nPoints)
         lam = np.zeros(self.nlam) if want_lambda else None
         st = Stats()
-        f = None if f_packed is None else np.ascontiguousarray(f_packed, dtype=np.float64)
+        f = None if f_packed is None else _vec(f_packed, self.npacked, "f_packed")
         _check(load().hfls_solve(self._h, ctypes.c_double(eps), int(max_iter), _ptr(f), _ptr(u), _ptr(lam),
                                  ctypes.byref(st)))
         return {"u": u, "lambda": lam, "iterations": st.iterations, "converged": bool(st.converged),
@@ -283,7 +299,7 @@ class Mesh:
 
     def apply_block(self, op, W, precision=64):
         """op 0 F, 1 M, 2 P applied to the rows of W (k, nlam) at precision 64 or 32."""
-        W = np.ascontiguousarray(np.atleast_2d(W), dtype=np.float64)
+        W = _rows(W, self.nlam, "W")
         out = np.zeros_like(W)
         _check(load().hfls_apply_block(self._h, int(op), W.shape[0], int(precision), _ptr(W), _ptr(out)))
         return out
@@ -302,9 +318,9 @@ class Mesh:
         copies of the body-force load.  Returns u (k, 3*nPoints), per-case
         iterations/status/residual and the block timings."""
         if F is not None:
-            F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+            F = _rows(F, self.npacked, "F")
             k = F.shape[0]
-        k = int(k or 1)
+        k = 1 if k is None else int(k)
         u = np.zeros((k, 3 * self.nPoints))
         lam = np.zeros((k, self.nlam)) if want_lambda else None
         cs = (CaseStats * k)()
@@ -321,8 +337,8 @@ class Mesh:
         """F (nblocks*k, npacked): nblocks blocks of k load cases solved concurrently,
         one block context (own stream) each.  solve_ms is the device time span of all
         blocks; per-case results in F's order."""
-        F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
-        if F.shape[0] % k:
+        F = _rows(F, self.npacked, "F")
+        if k < 1 or F.shape[0] % k:
             raise ValueError("the number of load cases must be a multiple of k")
         nb = F.shape[0] // k
         u = np.zeros((nb * k, 3 * self.nPoints))
@@ -371,9 +387,9 @@ class Server:
     def submit(self, opts, F=None, k=None, seed=0, case0=0):
         n, kk, v = _opts(opts)
         if F is not None:
-            F = np.ascontiguousarray(np.atleast_2d(F), dtype=np.float64)
+            F = _rows(F, self.sizes(**opts)[0], "F")
             k = F.shape[0]
-        k = int(k or 1)
+        k = 1 if k is None else int(k)
         rid = ctypes.c_int64()
         _check(load().hfls_server_submit(self._h, n, kk, v, k, _ptr(F), ctypes.c_uint64(seed), int(case0),
                                          ctypes.byref(rid)))

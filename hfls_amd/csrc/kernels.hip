@@ -1165,7 +1165,10 @@ __global__ __launch_bounds__(256) void k_reduce(int nparts, const double* __rest
       sc->gPz = s;
       sc->norm0 = sqrt(s);
       sc->it = 0;
-      sc->done = (sc->max_iter <= 0 || !(sqrt(s) >= sc->eps * sc->norm0)) ? 1 : 0;
+      // stopping test at the top of the loop (Cluster.cpp:1425); a zero initial
+      // residual (zero load) is already the solution: no iteration
+      sc->done = (sc->max_iter <= 0 || !(s > 0.0) || !(sqrt(s) >= sc->eps * sc->norm0)) ? 1 : 0;
+      if (!(s == s)) sc->err = 1;
     } else {
       sc->gPg0 = s;
     }

@@ -121,3 +121,18 @@ def test_slab_draw_uses_only_the_choices_and_all_of_them():
     assert p.nPoints == m.nPoints
     for d in range(0, m.nsub, 37):
         assert p.sub_info(d)["n"] == 3 * len(np.unique(m.elements[m.part == d]))
+
+
+def test_mirror_shape_checks():
+    """The Python mirror rejects arrays whose size differs from what the C ABI reads."""
+    from hfls_amd import _vec, _rows
+    assert _vec(np.arange(4.0), 4, "x").flags["C_CONTIGUOUS"]
+    assert _rows(np.zeros(5), 5, "F").shape == (1, 5)
+    with pytest.raises(ValueError):
+        _vec(np.zeros(3), 4, "x")
+    with pytest.raises(ValueError):
+        _vec(np.zeros((2, 2)), 4, "x")
+    with pytest.raises(ValueError):
+        _rows(np.zeros((0, 5)), 5, "F")
+    with pytest.raises(ValueError):
+        _rows(np.zeros((2, 4)), 5, "F")
