@@ -113,7 +113,8 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
 // takes GQ positions per step; the 16 wave sums are added in a fixed order, so a
 // column's value does not depend on the other columns.
 constexpr int GTW = 16;
-__global__ __launch_bounds__(64 * GTW) void kb_gtx(int k, int kp, const int* __restrict__ ns,
+// (at most 64 VGPRs, so two 16-wave workgroups share a CU)
+__global__ __launch_bounds__(64 * GTW) __attribute__((amdgpu_waves_per_eu(8))) void kb_gtx(int k, int kp, const int* __restrict__ ns,
                                                    const int64_t* __restrict__ boff, const int* __restrict__ gptr,
                                                    const int* __restrict__ gidx, const double* __restrict__ Rb,
                                                    const int64_t* __restrict__ offRb, const double* __restrict__ x,
@@ -154,68 +155,6 @@ hipError_t launch_kb_gtx(hipStream_t st, int nsub, int k, int kp, const int* ns,
                          const int* gate) {
   if (k > 64 || k > kp) return hipErrorInvalidValue;
   hipLaunchKernelGGL(kb_gtx, dim3(nsub), dim3(64 * GTW), 0, st, k, kp, ns, boff, gptr, gidx, Rb, offRb, x, out, gate);
-  return hipGetLastError();
-}
-
-// The preconditioner's gather of P x without P x in HBM:
-// xb[q][j] = sum_e sign_e wgt[r_e] (x[r_e][j] - c[r_e][j]), c[r] = G_r,a . h_a - G_r,b . h_b
-// (the same arithmetic as kb_gh followed by kb_gather)
-template <bool W>
-__global__ __launch_bounds__(256) void kb_gather_px(int64_t nb, int k, int kp, const int* __restrict__ qrow,                                                     const int* __restrict__ gptr, const int* __restrict__ gidx,
-                                                    const double* __restrict__ x, const double* __restrict__ wgt,
-                                                    const int* __restrict__ rsa, const int* __restrict__ rsb,
-                                                    const double* __restrict__ Gr, const double* __restrict__ h,
-                                                    double* __restrict__ xb, float* __restrict__ xb32,
-                                                    const int* __restrict__ gate) {
-  if (gate && *gate == 0) return;
-  const int j = threadIdx.x & 63;
-  const int64_t nwaves = (int64_t)gridDim.x * 4;
-  const int64_t ngroups = (nb + GQ - 1) / GQ;
-  if (j >= k) return;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int64_t grp = (int64_t)blockIdx.x * 4 + wv; grp < ngroups; grp += nwaves) {
-    const int64_t q0 = grp * GQ;
-    const int nq = (int)std::min<int64_t>(GQ, nb - q0);
-    double s[GQ];
-    // a dual row with one subdomain reads h_a twice against the zero half of its G row
-    bt_row_sums<GQ, 4>(q0, nq, gptr, gidx,
-                       [&](int r) {
-                         const int a = rsa[r], b0 = rsb[r];
-                         const int b = b0 >= 0 ? b0 : a;
-                         const double* G = Gr + (int64_t)r * 12;
-                         const double* ha = h + (int64_t)6 * a * k + j;
-                         const double* hb = h + (int64_t)6 * b * k + j;
-                         double c = 0.0;
-#pragma unroll
-                         for (int m = 0; m < 6; m++) c += G[m] * ha[(int64_t)m * k];
-#pragma unroll
-                         for (int m = 0; m < 6; m++) c -= G[6 + m] * hb[(int64_t)m * k];
-                         const double val = x[(int64_t)r * k + j] - c;
-                         return W ? val * wgt[r] : val;
-                       },
-                       s);
-#pragma unroll
-    for (int u = 0; u < GQ; u++) {
-      if (u >= nq) break;
-      const int64_t o = (int64_t)qrow[q0 + u] * kp + j;
-      if (xb) xb[o] = s[u];
-      if (xb32) xb32[o] = (float)s[u];
-    }
-  }
-}
-
-hipError_t launch_kb_gather_px(hipStream_t st, int64_t nb, int k, int kp, const int* qrow, const int* gptr, const int* gidx, const double* x,
-                               const double* wgt, const int* rsa, const int* rsb, const double* Gr, const double* h,
-                               double* xb, float* xb32, const int* gate) {
-  if (k > 64) return hipErrorInvalidValue;
-  const int64_t groups = (nb + GQ - 1) / GQ;
-  const dim3 grid((unsigned)std::min<int64_t>((groups + 3) / 4, kStrideGrid));
-  if (wgt)
-    hipLaunchKernelGGL(kb_gather_px<true>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, x, wgt,
-                       rsa, rsb, Gr, h, xb, xb32, gate);
-  else
-    hipLaunchKernelGGL(kb_gather_px<false>, grid, dim3(256), 0, st, nb, k, kp, qrow, gptr, gidx, x, wgt,
-                       rsa, rsb, Gr, h, xb, xb32, gate);
   return hipGetLastError();
 }
 
@@ -450,7 +389,12 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
 
 // out[r*k+j] = x[r*k+j] + sgn * (G_r,a . h_a[:,j] - G_r,b . h_b[:,j]); partial dots, where
 // Gr[r*12 + m] = R_b,a[pos_a, m] and Gr[r*12 + 6 + m] = R_b,b[pos_b, m] (0 for Dirichlet
-// rows) are the rows of G_f restricted to the two subdomains of dual row r
+// rows) are the rows of G_f restricted to the two subdomains of dual row r.
+// Wave g of a workgroup takes the 16 consecutive dual rows 64 blockIdx + 16 g + q; its
+// x and dotv loads are issued first, and h_a / h_b are reloaded only when the
+// subdomain pair changes from one row to the next (consecutive dual rows mostly join
+// the same two subdomains), so a row costs two vector loads and a store.
+constexpr int GHR = 16;  // dual rows per wave
 __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __restrict__ rsa,
                                              const int* __restrict__ rsb, const double* __restrict__ Gr,
                                              const double* __restrict__ h, const double* __restrict__ x,
@@ -461,36 +405,41 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
   const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double s = 0.0;
   if (j < k) {
-    // GH rows per step with every load issued before the arithmetic; a row with a
-    // single subdomain reads h_a twice against the zero half of its G row
-    constexpr int GH = 4;
-    for (int q0 = 0; q0 < 16; q0 += GH) {
-      double c[GH], xv[GH], dv[GH];
+    const int64_t r0 = (int64_t)blockIdx.x * (4 * GHR) + g * GHR;
+    const int nr = (int)std::max<int64_t>(0, std::min<int64_t>(GHR, nlam - r0));
+    double xv[GHR], dv[GHR];
 #pragma unroll
-      for (int u = 0; u < GH; u++) {
-        const int64_t r0 = (int64_t)blockIdx.x * 64 + g + 4 * (q0 + u);
-        const int64_t r = r0 < nlam ? r0 : nlam - 1;
+    for (int q = 0; q < GHR; q++) {
+      xv[q] = (x && q < nr) ? x[(r0 + q) * k + j] : 0.0;
+      dv[q] = (dotv && q < nr) ? dotv[(r0 + q) * k + j] : 0.0;
+    }
+    int pa = -1, pb = -1;
+    double ha[6], hb[6];
+#pragma unroll
+    for (int q = 0; q < GHR; q++) {
+      if (q < nr) {
+        const int64_t r = r0 + q;
         const int a = rsa[r], b0 = rsb[r];
-        const int b = b0 >= 0 ? b0 : a;
+        const int b = b0 >= 0 ? b0 : a;  // a row with one subdomain: zero half of its G row
+        if (a != pa) {
+#pragma unroll
+          for (int m = 0; m < 6; m++) ha[m] = h[((int64_t)6 * a + m) * k + j];
+          pa = a;
+        }
+        if (b != pb) {
+#pragma unroll
+          for (int m = 0; m < 6; m++) hb[m] = h[((int64_t)6 * b + m) * k + j];
+          pb = b;
+        }
         const double* G = Gr + r * 12;
-        const double* ha = h + (int64_t)6 * a * k + j;
-        const double* hb = h + (int64_t)6 * b * k + j;
         double v = 0.0;
 #pragma unroll
-        for (int m = 0; m < 6; m++) v += G[m] * ha[(int64_t)m * k];
+        for (int m = 0; m < 6; m++) v += G[m] * ha[m];
 #pragma unroll
-        for (int m = 0; m < 6; m++) v -= G[6 + m] * hb[(int64_t)m * k];
-        c[u] = v;
-        xv[u] = x ? x[r * k + j] : 0.0;
-        dv[u] = dotv ? dotv[r * k + j] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < GH; u++) {
-        const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * (q0 + u);
-        if (r >= nlam) continue;
-        const double o = xv[u] + sgn * c[u];
+        for (int m = 0; m < 6; m++) v -= G[6 + m] * hb[m];
+        const double o = xv[q] + sgn * v;
         out[r * k + j] = o;
-        s += o * dv[u];
+        s += o * dv[q];
       }
     }
   }
@@ -505,8 +454,8 @@ hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, con
                         const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
                         const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, rsa, rsb, Gr, h, x, out, sgn,
-                     dotv, part, gate);
+  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 4 * GHR - 1) / (4 * GHR))), dim3(256), 0, st, nlam, k, rsa, rsb, Gr,
+                     h, x, out, sgn, dotv, part, gate);
   return hipGetLastError();
 }
 
@@ -779,6 +728,7 @@ __global__ __launch_bounds__(256) void kb_permute(int dir, int k, int kp, int64_
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int i = (int)(t / k), j = (int)(t % k);
   if (i >= n) return;
+  if (i < r && !Vr) return;  // interior rows left out
   const int l = perm[noff[d] + i];
   double* v = V + (int64_t)j * npacked + noff[d] + l;
   double* s = i < r ? Vr + (orrow[d] + i) * kp + j : Vb + (obrow[d] + (i - r)) * kp + j;

@@ -386,13 +386,14 @@ void Engine::bP(int k, const double* x, double* out, double* hbuf, const double*
   HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, out, -1.0, dotv, b_part.p, gate));
 }
 
-// out = M P x (Cluster.cpp:1043-1112 applied to the projection of x): P x is formed
-// inside the preconditioner's gather and never stored
+// out = M P x (Cluster.cpp:1043-1112 applied to the projection of x); P x goes
+// through the dual temporary b_Pg
 void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, const int* gate) {
   const int N = P->nsub;
   b_gtx_coarse(k, x, hbuf, gate);
-  HC(launch_kb_gather_px(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p, d_rsa.p,
-                         d_rsb.p, b_Gr.p, hbuf, prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, b_Pg.p, -1.0, nullptr, nullptr, gate));
+  HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, b_Pg.p, d_wgt.p,
+                      prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
@@ -430,18 +431,22 @@ void Engine::b_rr_solve(double* X) {
                       nullptr, KP, 1.0, nullptr));
 }
 
-// From the local block V (FP64, column major): b_Vr = v_r, b_xb = t_b = v_b - (K_rr^-1 K_rb)^T v_r,
+// From the local block V (FP64, column major): b_Vr = K_rr^-1 v_r, b_xb = t_b = v_b - K_br K_rr^-1 v_r,
 // b_yb = (K^+ (v - Bc^T lam_c))_b (+ R_b alpha_c if with_R), b_la = [lam_c; alpha_c],
 // b_v6 = R^T v (Cluster.cpp:967-1040).  With interior_too, b_X = K^+(v - Bc^T lam_c)
 // in local order (packed, column major).
+// interior_too is the recovery, whose V = f - Bf^T lam differs from the loads f only
+// at B-touched DOFs: its v_r is the loads' f_r, and b_Vr = K_rr^-1 f_r as the last
+// initialisation left it (every join initialises all slots from their loads) is used
+// again instead of a second interior solve.
 void Engine::b_kplus(int k, bool interior_too, double* V) {
   const int N = P->nsub, NC = P->ncl;
   const int64_t np = off_n[N];
-  HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, V, b_Vr.p, b_orrow.p,
-                       b_xb.p, b_obrow.p));
+  HC(launch_kb_permute(st, N, maxn, 0, k, KP, np, d_n.p, d_nr.p, d_perm1.p, d_off_n.p, V,
+                       interior_too ? nullptr : b_Vr.p, b_orrow.p, b_xb.p, b_obrow.p));
   // b_Vr = K_rr^-1 v_r = W^T (W v_r) ; t_b = v_b - K_br K_rr^-1 v_r
   if (maxTr > 0) {
-    b_rr_solve(b_Vr.p);
+    if (!interior_too) b_rr_solve(b_Vr.p);
     HC(launch_csr_apply(st, 0, NB, 0, k, d_brp.p, d_brc.p, d_brv.p, d_bsub.p, d_bloc.p, b_Vr.p, b_roff.p, KP, 1,
                         b_xb.p, b_xoff.p, KP, 1, nullptr, -1.0, 1.0));
   }

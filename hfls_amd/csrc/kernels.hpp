@@ -197,9 +197,6 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
 hipError_t launch_kb_gtx(hipStream_t st, int nsub, int k, int kp, const int* ns, const int64_t* boff, const int* gptr,
                          const int* gidx, const double* Rb, const int64_t* offRb, const double* x, double* out,
                          const int* gate = nullptr);
-hipError_t launch_kb_gather_px(hipStream_t st, int64_t nb, int k, int kp, const int* qrow, const int* gptr, const int* gidx, const double* x,
-                               const double* wgt, const int* rsa, const int* rsb, const double* Gr, const double* h,
-                               double* xb, float* xb32, const int* gate = nullptr);
 hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b);
 hipError_t launch_kb_copy(hipStream_t st, int64_t n, const double* a, double* b);
 hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
