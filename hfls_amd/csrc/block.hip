@@ -185,28 +185,37 @@ hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b
 }
 
 // a[(64d + m)*kp + j] = sum_l R[l + m*n] * V[j*npacked + noff_d + l]   (R_i^T v_i, full local)
+// one workgroup per (subdomain, column): v is read once for the six products
 __global__ __launch_bounds__(256) void kb_rtf(int k, int kp, int64_t npacked, const int* __restrict__ nn,
                                               const int64_t* __restrict__ noff, const double* __restrict__ R,
                                               const int64_t* __restrict__ roff, const double* __restrict__ V,
                                               double* __restrict__ a) {
-  const int d = blockIdx.y;
-  const int m = blockIdx.x;  // 0..5
-  const int j = blockIdx.z;
+  const int d = blockIdx.x;
+  const int j = blockIdx.y;
   const int n = nn[d];
-  __shared__ double sh[4];
-  const double* col = R + roff[d] + (int64_t)m * n;
+  __shared__ double sh[6][4];
+  const double* Rd = R + roff[d];
   const double* v = V + (int64_t)j * npacked + noff[d];
-  double s = 0.0;
-  for (int l = threadIdx.x; l < n; l += 256) s += col[l] * v[l];
-  s = wsum64(s);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int l = threadIdx.x; l < n; l += 256) {
+    const double x = v[l];
+#pragma unroll
+    for (int m = 0; m < 6; m++) s[m] += Rd[l + (int64_t)m * n] * x;
+  }
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const double t = wsum64(s[m]);
+    if ((threadIdx.x & 63) == 0) sh[m][threadIdx.x >> 6] = t;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) a[((int64_t)64 * d + m) * kp + j] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x < 6)
+    a[((int64_t)64 * d + threadIdx.x) * kp + j] =
+        sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
 }
 
 hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
                          const double* R, const int64_t* roff, const double* V, double* a) {
-  hipLaunchKernelGGL(kb_rtf, dim3(6, nsub, k), dim3(256), 0, st, k, kp, npacked, n, noff, R, roff, V, a);
+  hipLaunchKernelGGL(kb_rtf, dim3(nsub, k), dim3(256), 0, st, k, kp, npacked, n, noff, R, roff, V, a);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,6 +22,7 @@
 #include <functional>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace hfls {
 
@@ -298,6 +300,30 @@ void Engine::configure(const std::string& key, const std::string& value) {
   gx_blk_key = -1;
 }
 
+// fn(d) for d in [0, n) on up to 16 host threads (subdomains claimed one at a time);
+// fn must write only data owned by its d
+static void parallel_for(int n, const std::function<void(int)>& fn) {
+  const int nt = std::max(1, std::min<int>({16, n, (int)std::thread::hardware_concurrency()}));
+  if (nt <= 1) {
+    for (int d = 0; d < n; d++) fn(d);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> errs(nt);
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t] {
+      try {
+        for (int d = next++; d < n; d = next++) fn(d);
+      } catch (...) {
+        errs[t] = std::current_exception();
+      }
+    });
+  for (auto& x : th) x.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
 void Engine::build_meta() {
   const int N = P->nsub;
   h_n.resize(N); h_nr.resize(N); h_ns.resize(N); h_nbs.resize(N); h_nss.resize(N); h_ldb.resize(N);
@@ -358,7 +384,7 @@ void Engine::build_meta() {
   std::vector<int> perm(off_n[N]), gdof(off_n[N]);
   std::vector<unsigned char> owner(off_n[N]);
   const int64_t nx1 = P->NE[0] + 1, nxy1 = nx1 * (P->NE[1] + 1);
-  for (int d = 0; d < N; d++) {
+  parallel_for(N, [&](int d) {
     const Subdomain& s = P->subs[d];
     int q = 0;
     for (int l : s.rdofs) perm[off_n[d] + q++] = l;
@@ -370,7 +396,7 @@ void Engine::build_meta() {
       gdof[off_n[d] + l] = (int)(3 * (ii + jj * nx1 + kk * nxy1) + c);
       owner[off_n[d] + l] = s.owner[l];
     }
-  }
+  });
   d_perm1.upload(perm, st);
   d_gdof.upload(gdof, st);
   d_owner.upload(owner, st);
@@ -378,12 +404,11 @@ void Engine::build_meta() {
   // step along every axis inside the subdomain box), columns ascending
   NR = off_r[N];
   {
-    std::vector<int64_t> brp(NB + 1, 0), rbp(NR + 1, 0);
-    std::vector<int> brc, rbc, bloc(NB), rsub(NR), rloc(NR);
-    std::vector<int> cols;
-    for (int d = 0; d < N; d++) {
+    // per-subdomain rows built in parallel, then concatenated in subdomain order
+    std::vector<std::vector<int>> brc_d(N), rbc_d(N), brn_d(N), rbn_d(N);
+    parallel_for(N, [&](int d) {
       const Subdomain& s = P->subs[d];
-      std::vector<int> rpos(s.n, -1);
+      std::vector<int> rpos(s.n, -1), cols;
       for (int q = 0; q < s.n_r; q++) rpos[s.rdofs[q]] = q;
       auto neighbours = [&](int l, const std::vector<int>& pos) {
         cols.clear();
@@ -402,44 +427,66 @@ void Engine::build_meta() {
             }
         std::sort(cols.begin(), cols.end());
       };
+      brn_d[d].resize(s.n_s);
       for (int p = 0; p < s.n_s; p++) {
         neighbours(s.border[p], rpos);
-        brc.insert(brc.end(), cols.begin(), cols.end());
-        brp[off_b[d] + p + 1] = (int64_t)brc.size();
+        brc_d[d].insert(brc_d[d].end(), cols.begin(), cols.end());
+        brn_d[d][p] = (int)cols.size();
+      }
+      rbn_d[d].resize(s.n_r);
+      for (int q = 0; q < s.n_r; q++) {
+        neighbours(s.rdofs[q], s.bpos);
+        rbc_d[d].insert(rbc_d[d].end(), cols.begin(), cols.end());
+        rbn_d[d][q] = (int)cols.size();
+      }
+    });
+    std::vector<int64_t> brp(NB + 1, 0), rbp(NR + 1, 0);
+    std::vector<int> brc, rbc, bloc(NB), rsub(NR), rloc(NR);
+    for (int d = 0; d < N; d++) {
+      const Subdomain& s = P->subs[d];
+      for (int p = 0; p < s.n_s; p++) {
+        brp[off_b[d] + p + 1] = brp[off_b[d] + p] + brn_d[d][p];
         bloc[off_b[d] + p] = p;
       }
       for (int q = 0; q < s.n_r; q++) {
-        neighbours(s.rdofs[q], s.bpos);
-        rbc.insert(rbc.end(), cols.begin(), cols.end());
-        rbp[off_r[d] + q + 1] = (int64_t)rbc.size();
+        rbp[off_r[d] + q + 1] = rbp[off_r[d] + q] + rbn_d[d][q];
         rsub[off_r[d] + q] = d;
         rloc[off_r[d] + q] = q;
       }
+      brc.insert(brc.end(), brc_d[d].begin(), brc_d[d].end());
+      rbc.insert(rbc.end(), rbc_d[d].begin(), rbc_d[d].end());
+      std::vector<int>().swap(brc_d[d]);
+      std::vector<int>().swap(rbc_d[d]);
     }
     d_brp.upload(brp, st); d_brc.upload(brc, st); d_bloc.upload(bloc, st);
     d_rbp.upload(rbp, st); d_rbc.upload(rbc, st); d_rsub.upload(rsub, st); d_rloc.upload(rloc, st);
     d_brv.alloc(std::max<size_t>(brc.size(), 1));
     d_rbv.alloc(std::max<size_t>(rbc.size(), 1));
   }
-  // gather lists over packed b positions and dual-row entries
-  std::vector<std::vector<int>> lists(NB);
+  // gather lists over packed b positions and dual-row entries: the entries of b
+  // position q at gidx[gptr[q] .. gptr[q+1]), in the order of the subdomain's Bf
+  // entries.  A dual row's + and - entries belong to different subdomains, so the
+  // per-subdomain passes write disjoint data.
   const int64_t nl = P->nlam;
   std::vector<int> ra(nl, -1), rb(nl, -1), rsa(nl, -1), rsb(nl, -1), rpa(nl, 0), rpb(nl, 0);
-  for (int d = 0; d < N; d++) {
+  std::vector<int> gptr(NB + 1, 0);
+  parallel_for(N, [&](int d) {
     const Subdomain& s = P->subs[d];
+    for (size_t e = 0; e < s.bf_row.size(); e++) gptr[off_b[d] + s.bpos[s.bf_dof[e]] + 1]++;
+  });
+  for (int64_t q = 0; q < NB; q++) gptr[q + 1] += gptr[q];
+  // the batched Bf^T walks read up to 7 entries past a group's end
+  std::vector<int> gidx((size_t)gptr[NB] + 8, 0);
+  parallel_for(N, [&](int d) {
+    const Subdomain& s = P->subs[d];
+    std::vector<int> fill(s.n_s, 0);
     for (size_t e = 0; e < s.bf_row.size(); e++) {
-      int r = s.bf_row[e], p = s.bpos[s.bf_dof[e]];
-      lists[off_b[d] + p].push_back(s.bf_sign[e] > 0 ? r : ~r);
+      const int r = s.bf_row[e], p = s.bpos[s.bf_dof[e]];
+      gidx[gptr[off_b[d] + p] + fill[p]++] = s.bf_sign[e] > 0 ? r : ~r;
       if (s.bf_sign[e] > 0) { ra[r] = (int)(off_b[d] + p); rsa[r] = d; rpa[r] = p; }
       else { rb[r] = (int)(off_b[d] + p); rsb[r] = d; rpb[r] = p; }
     }
-  }
-  std::vector<int> gptr(NB + 1, 0), gidx;
-  for (int64_t q = 0; q < NB; q++) {
-    gptr[q + 1] = gptr[q] + (int)lists[q].size();
-    for (int c : lists[q]) gidx.push_back(c);
-  }
-  gidx.resize(gidx.size() + 8, 0);  // the batched Bf^T walks read up to 7 entries past a group's end
+  });
   d_gptr.upload(gptr, st); d_gidx.upload(gidx, st);
   d_ra.upload(ra, st); d_rb.upload(rb, st); d_rsa.upload(rsa, st); d_rsb.upload(rsb, st);
   d_rpa.upload(rpa, st); d_rpb.upload(rpb, st);

@@ -197,7 +197,11 @@ hipError_t launch_assemble(hipStream_t st, int nmat, const AsmSub* subs, const i
 // k_panel_diag: one workgroup (64 threads) per matrix factors the <=64 wide
 // diagonal block in LDS, writes it back, and writes the inverse of the factor,
 // W = L^-1 (lower, column major, zero outside w x w), for the TRSM below.
-// L[c*64 + r] holds the diagonal block (column major).
+// L[c*64 + r] holds the diagonal block (column major).  The factorisation is left
+// looking: thread t forms L[t][j] = (A[t][j] - sum_{k<j} L[t][k] L[j][k]) / L[j][j]
+// from LDS reads only (no store inside the sum, so its loads pipeline); the
+// products are subtracted from A[t][j] in ascending k, the order of a
+// right-looking update.
 __global__ __launch_bounds__(64) void k_panel_diag(const PanelDesc* __restrict__ desc,
                                                    const int64_t* __restrict__ moff, const int* __restrict__ ld,
                                                    double* __restrict__ M, double* __restrict__ Winv,
@@ -212,24 +216,29 @@ __global__ __launch_bounds__(64) void k_panel_diag(const PanelDesc* __restrict__
   const int64_t lda = ld[mi];
   __shared__ double L[64 * 64];
   __shared__ double Y[64 * 65];
-  for (int c = 0; c < w; c++) L[c * 64 + t] = (t < w && t >= c) ? A[(k0 + t) + (int64_t)(k0 + c) * lda] : 0.0;
+  {  // all 64 column loads in flight before the LDS stores
+    double col[64];
+#pragma unroll
+    for (int c = 0; c < 64; c++) col[c] = (c < w && t < w && t >= c) ? A[(k0 + t) + (int64_t)(k0 + c) * lda] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 64; c++)
+      if (c < w) L[c * 64 + t] = col[c];
+  }
   __syncthreads();
   for (int j = 0; j < w; j++) {
-    const double djj = L[j * 64 + j];
-    __syncthreads();
-    double dj = 1.0;
-    if (djj > 0.0) {
-      dj = sqrt(djj);
-    } else if (t == 0) {
-      atomicOr(err, 1);
+    double sj = 0.0;
+    if (t >= j && t < w) {
+      sj = L[j * 64 + t];
+      for (int k = 0; k < j; k++) sj -= L[k * 64 + t] * L[k * 64 + j];
     }
-    if (t == j) L[j * 64 + j] = dj;
-    if (t > j && t < w) L[j * 64 + t] /= dj;
-    __syncthreads();
-    if (t > j && t < w) {
-      const double ltj = L[j * 64 + t];
-      for (int k = j + 1; k <= t; k++) L[k * 64 + t] -= ltj * L[j * 64 + k];
+    if (t == j) {
+      double dj = 1.0;
+      if (sj > 0.0) dj = sqrt(sj);
+      else atomicOr(err, 1);
+      L[j * 64 + j] = dj;
     }
+    __syncthreads();
+    if (t > j && t < w) L[j * 64 + t] = sj / L[j * 64 + j];
     __syncthreads();
   }
   if (t < w)
@@ -267,6 +276,7 @@ __global__ __launch_bounds__(256) void k_panel_trsm(const PanelDesc* __restrict_
   const double* W = Winv + (int64_t)mi * 4096;
   __shared__ double As[64 * 65];  // [k][row] of the rows being solved
   __shared__ double Bs[64 * 65];  // [k][c] = W[c][k]
+#pragma unroll
   for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
     const int k = idx >> 6, r = idx & 63;
     As[k * 65 + r] = (k < w && ri + r < end) ? A[(ri + r) + (int64_t)(k0 + k) * lda] : 0.0;
@@ -613,7 +623,7 @@ hipError_t launch_coldot(hipStream_t st, int nmat, int maxcols, const double* L,
 
 // ============================================================ triangular solves
 // Backward: solve L^T X = B in place (L lower, m x m, column major), nrhs <= 8.
-__global__ __launch_bounds__(256) void k_bwd(const double* __restrict__ L, const int64_t* __restrict__ loff,
+__global__ __launch_bounds__(1024) void k_bwd(const double* __restrict__ L, const int64_t* __restrict__ loff,
                                              const int* __restrict__ ld, const int* __restrict__ mm,
                                              double* __restrict__ X, const int64_t* __restrict__ xoff,
                                              const int* __restrict__ ldx, int nrhs) {
@@ -629,15 +639,22 @@ __global__ __launch_bounds__(256) void k_bwd(const double* __restrict__ L, const
     const int j0 = j1 > 64 ? j1 - 64 : 0;
     const int bw = j1 - j0;
     // X[i] -= sum_{r >= j1} L[r,i] X[r]   (column i of L below the block)
-    for (int i = j0 + wv; i < j1; i += 4) {
+    for (int i = j0 + wv; i < j1; i += 16) {
       double acc[8];
 #pragma unroll
       for (int o = 0; o < 8; o++) acc[o] = 0.0;
-      for (int r = j1 + lane; r < m; r += 64) {
-        const double a = Lm[r + (int64_t)i * l];
+      // four rows per lane and step, their loads issued together; a lane adds its
+      // rows r, r + 64, r + 128, ... in ascending order
+      for (int r = j1 + lane; r < m; r += 256) {
+        double a[4];
 #pragma unroll
-        for (int o = 0; o < 8; o++)
-          if (o < nrhs) acc[o] += a * Xm[r + o * lx];
+        for (int u = 0; u < 4; u++) a[u] = r + 64 * u < m ? Lm[r + 64 * u + (int64_t)i * l] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (r + 64 * u < m)
+#pragma unroll
+            for (int o = 0; o < 8; o++)
+              if (o < nrhs) acc[o] += a[u] * Xm[r + 64 * u + o * lx];
       }
 #pragma unroll
       for (int o = 0; o < 8; o++)
@@ -646,7 +663,7 @@ __global__ __launch_bounds__(256) void k_bwd(const double* __restrict__ L, const
           if (lane == 0) Xm[i + o * lx] -= s;
         }
     }
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 1024) {
       const int c = idx >> 6, r = idx & 63;
       Lb[idx] = (c < bw && r < bw && r >= c) ? Lm[(j0 + r) + (int64_t)(j0 + c) * l] : 0.0;
     }
@@ -666,7 +683,7 @@ __global__ __launch_bounds__(256) void k_bwd(const double* __restrict__ L, const
 hipError_t launch_bwd(hipStream_t st, int nmat, const double* L, const int64_t* loff, const int* ld,
                       const int* m, double* X, const int64_t* xoff, const int* ldx, int nrhs) {
   if (nrhs > 8) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_bwd, dim3(nmat), dim3(256), 0, st, L, loff, ld, m, X, xoff, ldx, nrhs);
+  hipLaunchKernelGGL(k_bwd, dim3(nmat), dim3(1024), 0, st, L, loff, ld, m, X, xoff, ldx, nrhs);
   return hipGetLastError();
 }
 
