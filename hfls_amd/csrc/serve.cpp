@@ -51,6 +51,14 @@ const Problem& Server::host_problem(const std::string& key, const Options& o) {
 
 // device bytes a set-up mesh keeps (plain path + block buffers), from the host
 // problem; the transient factorisation scratch is sized by the free memory
+// relative set-up work of a mesh: the dense elimination of every subdomain
+// matrix, sum of n_i^3
+static double setup_work(const Problem& p) {
+  double w = 0.0;
+  for (const Subdomain& s : p.subs) w += (double)s.n * s.n * s.n;
+  return w;
+}
+
 size_t Server::estimate(const Problem& p) const {
   double b = 0.0;
   const double nl = (double)p.nlam, ng = 3.0 * (double)p.nPoints;
@@ -201,9 +209,27 @@ bool Server::step() {
   //    first; with nothing else to run, the loop waits for the running set-up.
   finish_setup(false);
   auto admission = [&] {
+    // requests for cached meshes, in arrival order
     for (auto it = wait_.begin(); it != wait_.end();) {
-      if (admit(req_[*it])) it = wait_.erase(it);
+      if (mesh_.count(req_[*it].key) && admit(req_[*it])) it = wait_.erase(it);
       else ++it;
+    }
+    // one set-up at a time: the earliest waiting request's mesh, unless a later
+    // request needs a mesh with less than an eighth of its set-up work (then the
+    // smallest such goes first: it delays the large one little and saves its own
+    // requests from waiting behind it)
+    if (!pend_ && !wait_.empty()) {
+      const double head = setup_work(*host_[req_[wait_.front()].key]);
+      int64_t best = wait_.front();
+      double bw = head;
+      for (int64_t id : wait_) {
+        const double w = setup_work(*host_[req_[id].key]);
+        if (w < head / 8.0 && w < bw) {
+          best = id;
+          bw = w;
+        }
+      }
+      admit(req_[best]);  // starts the set-up (or waits for room)
     }
   };
   admission();

@@ -10,7 +10,9 @@
 // streams, so the work of different meshes overlaps on the device.  A case's
 // arithmetic is independent of the other slots, so its result is the same alone
 // and in the mix.  A cold mesh is set up on a worker thread (its own stream) while
-// the cached meshes keep iterating; one set-up runs at a time.
+// the cached meshes keep iterating; one set-up runs at a time, in arrival order
+// except that a mesh with less than an eighth of the set-up work of the earliest
+// waiting one may go first.
 #pragma once
 #include <cstdint>
 #include <deque>
