@@ -357,10 +357,14 @@ hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const i
 
 // out[r*k+j] = wgt[r]*(y_a[pos_a][j] - y_b[pos_b][j]) - sub[r*k+j]; per-column
 // partial dots with dotw, 64 dual rows per workgroup: part[blk*k + j]
+// Wave g of a workgroup takes the 16 consecutive dual rows 64 blockIdx + 16 g + q in
+// two batches of 8, every load of a batch issued before any is used.  TY: the
+// b-block's element type (FP64, or the FP32 result of the mixed-precision GEMM).
+template <typename TY>
 __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, const int* __restrict__ rsa,
                                                   const int* __restrict__ rsb, const int* __restrict__ rpa,
                                                   const int* __restrict__ rpb, const int64_t* __restrict__ obrow,
-                                                  const double* __restrict__ y, const double* __restrict__ wgt,
+                                                  const TY* __restrict__ y, const double* __restrict__ wgt,
                                                   const double* __restrict__ sub, double* __restrict__ out,
                                                   const double* __restrict__ dotw, double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
@@ -368,16 +372,36 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
   const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double s = 0.0;
   if (j < k) {
-    for (int q = 0; q < 16; q++) {
-      const int64_t r = (int64_t)blockIdx.x * 64 + g + 4 * q;
-      if (r >= nlam) break;
-      const int a = rsa[r], b = rsb[r];
-      double v = y[(obrow[a] + rpa[r]) * kp + j];
-      if (b >= 0) v -= y[(obrow[b] + rpb[r]) * kp + j];
-      if (wgt) v *= wgt[r];
-      if (sub) v -= sub[r * k + j];
-      out[r * k + j] = v;
-      if (dotw) s += v * dotw[r * k + j];
+    constexpr int SB = 8;
+    const int64_t r0 = (int64_t)blockIdx.x * 64 + g * 16;
+#pragma unroll
+    for (int q0 = 0; q0 < 16; q0 += SB) {
+      double va[SB], vb[SB], ws[SB], sv[SB], dv[SB];
+#pragma unroll
+      for (int u = 0; u < SB; u++) {
+        const int64_t r = r0 + q0 + u;
+        va[u] = vb[u] = sv[u] = dv[u] = 0.0;
+        ws[u] = 1.0;
+        if (r < nlam) {
+          const int a = rsa[r], b = rsb[r];
+          va[u] = (double)y[(obrow[a] + rpa[r]) * kp + j];
+          if (b >= 0) vb[u] = (double)y[(obrow[b] + rpb[r]) * kp + j];
+          if (wgt) ws[u] = wgt[r];
+          if (sub) sv[u] = sub[r * k + j];
+          if (dotw) dv[u] = dotw[r * k + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < SB; u++) {
+        const int64_t r = r0 + q0 + u;
+        if (r >= nlam) break;
+        double v = va[u];
+        if (rsb[r] >= 0) v -= vb[u];
+        if (wgt) v *= ws[u];
+        if (sub) v -= sv[u];
+        out[r * k + j] = v;
+        if (dotw) s += v * dv[u];
+      }
     }
   }
   if (dotw) {
@@ -389,10 +413,16 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
 
 hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
                              const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
-                             const double* sub, double* out, const double* dotw, double* part, const int* gate) {
+                             const double* sub, double* out, const double* dotw, double* part, const int* gate,
+                             const float* yf) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_scatter, dim3((unsigned)((nlam + 63) / 64)), dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb,
-                     obrow, y, wgt, sub, out, dotw, part, gate);
+  const dim3 grid((unsigned)((nlam + 63) / 64));
+  if (yf)
+    hipLaunchKernelGGL(kb_scatter<float>, grid, dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb, obrow, yf, wgt,
+                       sub, out, dotw, part, gate);
+  else
+    hipLaunchKernelGGL(kb_scatter<double>, grid, dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb, obrow, y, wgt,
+                       sub, out, dotw, part, gate);
   return hipGetLastError();
 }
 

@@ -79,7 +79,7 @@ struct BlockCtx {
   hipGraphExec_t gx = nullptr;
   int64_t gx_key = -1;
   int k = 0, prec = 64;
-  DVec<float> xb32;
+  DVec<float> xb32, yb32;
   DVec<double> xb, yb, cr, coef, kpart, v6, gc, la, h, hg, hgf, lam, g, w, Fw, Pg, z, Pz, d, g64, part, gt, Vr, Xr, Tr,
       V, V2, X, U;
   DVec<BScalars> sc;
@@ -237,7 +237,7 @@ class Engine {
   DVec<int> b_crn, b_qrow;
   DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
   DVec<int64_t> b_crto;
-  DVec<float> b_Kt32, b_St32, b_xb32;
+  DVec<float> b_Kt32, b_St32, b_xb32, b_yb32;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_hgf, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;
@@ -249,7 +249,7 @@ class Engine {
   void bM(int k, int prec, const double* x, double* out, const int* gate = nullptr);
   void bPM(int k, int prec, const double* x, double* out, double* hbuf, const int* gate = nullptr);
   void b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate);
-  void b_corr(int k, int with_R, int defl, const int* gate);
+  void b_corr(int k, int with_R, int defl, const int* gate, const float* yf = nullptr);
   void crq_products(const int* gate);
   void b_kplus(int k, bool interior_too, double* V);  // K_c^+ pieces of the local block V
   void b_rr_solve(double* X);

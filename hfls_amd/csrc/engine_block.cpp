@@ -227,6 +227,7 @@ void Engine::alloc_ctx_buffers() {
   zero_alloc(b_xb, NBP * KP, st);
   zero_alloc(b_yb, NBP * KP, st);
   zero_alloc(b_xb32, NBP * KP, st);
+  zero_alloc(b_yb32, NBP * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
   zero_alloc(b_kpart, (int64_t)N * 8 * 64 * KP, st);
@@ -256,6 +257,7 @@ void Engine::swap_ctx(BlockCtx& c) {
   std::swap(blk_k, c.k);
   std::swap(blk_prec, c.prec);
   b_xb32.swap(c.xb32);
+  b_yb32.swap(c.yb32);
   b_xb.swap(c.xb); b_yb.swap(c.yb); b_cr.swap(c.cr); b_coef.swap(c.coef); b_kpart.swap(c.kpart); b_v6.swap(c.v6);
   b_gc.swap(c.gc); b_la.swap(c.la); b_h.swap(c.h); b_hg.swap(c.hg); b_hgf.swap(c.hgf); b_lam.swap(c.lam);
   b_g.swap(c.g); b_w.swap(c.w); b_Fw.swap(c.Fw); b_Pg.swap(c.Pg); b_z.swap(c.z); b_Pz.swap(c.Pz); b_d.swap(c.d);
@@ -299,7 +301,7 @@ void Engine::autotune() {
   for (int rt = 1; rt <= 2; rt++) {
     auto launch = [&] {
       HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p,
-                          b_yb.p, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt));
+                          nullptr, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt, b_yb32.p));
     };
     launch();
     HC(hipEventRecord(a, st));
@@ -318,7 +320,7 @@ void Engine::autotune() {
 size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
-  s += (b_Kt32.n + b_St32.n + b_xb32.n) * sizeof(float);
+  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n) * sizeof(float);
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
@@ -332,9 +334,9 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   const int N = P->nsub, NC = P->ncl;
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
                       prec == 32 ? b_xb32.p : nullptr, gate));
-  if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
+  if (prec == 32) {  // FP32 result into b_yb32; the FP64 correction below adds it into b_yb
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
@@ -347,19 +349,19 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
                           b_lcpos.p, b_cr.p, b_abase_cr.p, b_gc.p, b_gcrow.p, gate));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
                       b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate));
-  b_corr(k, 1, prec == 32 ? 1 : 0, gate);
+  b_corr(k, 1, prec == 32 ? 1 : 0, gate, prec == 32 ? b_yb32.p : nullptr);
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
                        dotw, b_part.p, gate));
 }
 
 // y_b += R_b alpha (with_R) - C lam_c (+ Q (Yh^T x) + Yh (Q^T x) if defl) as one
 // FP64 tile GEMM of CRQ with the coefficient block gathered from la and cr
-void Engine::b_corr(int k, int with_R, int defl, const int* gate) {
+void Engine::b_corr(int k, int with_R, int defl, const int* gate, const float* yf) {
   const int N = P->nsub;
   HC(launch_kb_coef(st, N, maxnlc + 18, k, KP, d_nlc.p, d_off_lc.p, b_lcrow.p, b_arow.p, b_la.p, with_R, defl, b_cr.p,
                     b_crrow.p, b_coef.p, gate));
   HC(launch_tgemm_f64(st, N, maxTb, 1, 0, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_coef.p, b_croff.p, b_yb.p,
-                      b_xoff.p, b_yb.p, KP, 1.0, gate));
+                      b_xoff.p, yf ? nullptr : b_yb.p, KP, 1.0, gate, 1, nullptr, yf));
 }
 
 // cr = CRQ^T x_b per subdomain (c = C^T x, a = R_b^T x, Q^T x, Yh^T x), from the
@@ -395,14 +397,14 @@ void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, co
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, b_Pg.p, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
   }
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
-                       out, nullptr, nullptr, gate));
+                       out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
 }
 
 // out = W Bf S Bf^T W x (Cluster.cpp:1043-1112)
@@ -411,14 +413,14 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32));
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate));
   }
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
-                       out, nullptr, nullptr, gate));
+                       out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
 }
 
 // X <- K_rr^-1 X = W^T (W X) on an interior block (rows orrow, ld KP), as two
@@ -499,8 +501,8 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
     if (which == 2)
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
     else
-      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, b_yb.p,
-                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32));
+      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32, b_yb32.p));
   };
   launch();
   hipEvent_t a, b;

@@ -159,11 +159,13 @@ hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff,
                             double* Y, const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate,
-                            int rt = 1);  // rt: 64-row output tiles per workgroup (1 or 2)
+                            int rt = 1,              // rt: 64-row output tiles per workgroup (1 or 2)
+                            float* Yf = nullptr);    // Yf: store the FP32 result there instead of Y (Z unused)
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
-                            double alpha, const int* gate, int S = 1, double* Ypart = nullptr);  // S: split K (maxRo 1)
+                            double alpha, const int* gate, int S = 1, double* Ypart = nullptr,  // S: split K (maxRo 1)
+                            const float* Zf = nullptr);  // Zf: an FP32 addend in place of Z
 hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
                            const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
 hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
@@ -215,7 +217,8 @@ hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const i
                                  const int* nlc, float* P32);
 hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
                              const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
-                             const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr);
+                             const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr,
+                             const float* yf = nullptr);  // yf: FP32 b-block in place of y
 hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const double* Gr,
                         const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
                         const int* gate = nullptr);

@@ -212,7 +212,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
                                                    const int64_t* __restrict__ aoff, const float* __restrict__ X,
                                                    const int64_t* __restrict__ xoff, double* __restrict__ Y,
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
-                                                   int kp, float alpha, const int* __restrict__ gate) {
+                                                   int kp, float alpha, const int* __restrict__ gate,
+                                                   float* __restrict__ Yf) {
   const int sl = blockIdx.x >> 3;
   const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
   if (d >= nsub) return;
@@ -278,6 +279,19 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
     if (more) stash();
     __syncthreads();
   }
+  if (Yf) {  // FP32 result: half the store traffic, the same values
+    float* Yfd = Yf + yoff[d] + c0;
+#pragma unroll
+    for (int q = 0; q < RT; q++) {
+      if (q >= nrt) break;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int row = (i0 + q) * 64 + rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        Yfd[(int64_t)row * kp + cb + (lane & 31)] = alpha * acc[q][r];
+      }
+    }
+    return;
+  }
   double* Yd = Y + yoff[d] + c0;
   const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
 #pragma unroll
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
                                                    const int64_t* __restrict__ xoff, double* __restrict__ Y,
                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
                                                    int kp, double alpha, const int* __restrict__ gate, int S,
-                                                   double* __restrict__ Ypart) {
+                                                   double* __restrict__ Ypart, const float* __restrict__ Zf) {
   const int sl = blockIdx.x >> 3;
   const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
   if (d >= nsub) return;
@@ -380,6 +394,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
   }
   double* Yd = Y + yoff[d] + c0;
   const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
+  const float* Zfd = Zf ? Zf + yoff[d] + c0 : nullptr;
 #pragma unroll
   for (int cc = 0; cc < 4; cc++)
 #pragma unroll
@@ -387,7 +402,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
       const int row = i * 64 + rb + 4 * r + (lane >> 4);
       const int col = 16 * cc + (lane & 15);
       double v = alpha * acc[cc][r];
-      if (Zd) v += Zd[(int64_t)row * kp + col];
+      if (Zfd) v += (double)Zfd[(int64_t)row * kp + col];
+      else if (Zd) v += Zd[(int64_t)row * kp + col];
       Yd[(int64_t)row * kp + col] = v;
     }
 }
@@ -409,17 +425,18 @@ __global__ __launch_bounds__(256) void k_splitk_sum(int S, int kp, const double*
 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, double* Y,
-                            const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt) {
+                            const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt,
+                            float* Yf) {
   (void)lda;
   if (kp % 64 || mode == 2 || mode == 3) return hipErrorInvalidValue;  // FP32: modes 0, 1
   const int nsub8 = (nsub + 7) / 8;  // subdomain groups of 8, one subdomain of each per XCD
   if (rt == 2) {
     const int nslot = (maxRo + 1) / 2;
     hipLaunchKernelGGL(k_tgemm_f32<2>, dim3(8 * nslot * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, nslot,
-                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
+                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
   } else {
     hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(8 * maxRo * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, maxRo,
-                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate);
+                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
   }
   return hipGetLastError();
 }
@@ -427,18 +444,18 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X, const int64_t* xoff, double* Y,
                             const int64_t* yoff, const double* Z, int kp, double alpha, const int* gate, int S,
-                            double* Ypart) {
+                            double* Ypart, const float* Zf) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
-  if (S > 1 && (maxRo != 1 || !Ypart || mode == 3)) return hipErrorInvalidValue;
+  if (S > 1 && (maxRo != 1 || !Ypart || mode == 3 || Zf)) return hipErrorInvalidValue;
   const int nslot = S > 1 ? S : maxRo;
   const dim3 grid(8 * nslot * ((nsub + 7) / 8), 1, kp / 64);
   if (mode == 2)
     hipLaunchKernelGGL(k_tgemm_f64<true>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
-                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
+                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
   else
     hipLaunchKernelGGL(k_tgemm_f64<false>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
-                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart);
+                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
   if (S > 1)
     hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub), dim3(256), 0, st, S, kp, Ypart, Y, yoff, Z,
                        gate);
