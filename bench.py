@@ -51,8 +51,8 @@ SEED = 20240611
 # HBM bytes per launch of k_tgemm_f32 on the default workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
 # the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
-# 7.356e9 fetched + 4.445e8 written.  Valid for the default workload only.
-GEMM_TRAFFIC_BYTES = 7.80e9
+# 7.317e9 fetched + 2.222e8 written (FP32 results).  Valid for the default workload only.
+GEMM_TRAFFIC_BYTES = 7.54e9
 
 
 def cpu_sample():
