@@ -238,6 +238,8 @@ class Engine {
   DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32, b_yb32;
+  DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
+  int n_xob = 0, n_xor = 0;
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_hgf, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;

@@ -64,6 +64,18 @@ void Engine::block_prepare() {
     v6off[d] = v6row[d] * KP;
   }
   NBP = rb;
+  {  // XCD orders balancing the tile GEMMs' work over the 8 XCDs
+    std::vector<double> wb(N), wr(N);
+    for (int d = 0; d < N; d++) {
+      wb[d] = (double)h_ns[d] * h_ns[d];
+      wr[d] = (double)h_nr[d] * h_nr[d];
+    }
+    const std::vector<int> ob = xcd_order(wb), orr = xcd_order(wr);
+    b_xob.upload(ob, st);
+    b_xor.upload(orr, st);
+    n_xob = (int)ob.size();
+    n_xor = (int)orr.size();
+  }
   NRP = rr;
   CRP = rc;
   std::vector<int64_t> gcrow(NC), gcoff(NC), croff_el(N);
@@ -165,7 +177,7 @@ void Engine::block_prepare() {
     dY.alloc(off_Rb[N]);
     HC(launch_kb_cols(st, N, maxns, 0, 6, KP, d_ns.p, b_obrow.p, dQ.p, d_off_Rb.p, d_ns.p, b_xb.p));
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0, nullptr));
+                        b_xoff.p, nullptr, KP, 1.0, nullptr, 1, nullptr, nullptr, b_xob.p, n_xob));
     HC(launch_kb_cols(st, N, maxns, 1, 6, KP, d_ns.p, b_obrow.p, dY.p, d_off_Rb.p, d_ns.p, b_yb.p));
     dY.download(Y, st);
     for (int d = 0; d < N; d++) {
@@ -301,7 +313,7 @@ void Engine::autotune() {
   for (int rt = 1; rt <= 2; rt++) {
     auto launch = [&] {
       HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p,
-                          nullptr, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt, b_yb32.p));
+                          nullptr, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt, b_yb32.p, b_xob.p, n_xob));
     };
     launch();
     HC(hipEventRecord(a, st));
@@ -336,10 +348,10 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
                       prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {  // FP32 result into b_yb32; the FP64 correction below adds it into b_yb
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0, gate));
+                        b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
   }
   // HTFETI correction (Cluster.cpp:1129-1186): [c; a] = [C_i | R_b,i]^T x_b,
   // [lam_c; alpha] = Ac^-1 [c; -a], y_b += R_b alpha - C lam_c; in FP32 also the
@@ -361,7 +373,7 @@ void Engine::b_corr(int k, int with_R, int defl, const int* gate, const float* y
   HC(launch_kb_coef(st, N, maxnlc + 18, k, KP, d_nlc.p, d_off_lc.p, b_lcrow.p, b_arow.p, b_la.p, with_R, defl, b_cr.p,
                     b_crrow.p, b_coef.p, gate));
   HC(launch_tgemm_f64(st, N, maxTb, 1, 0, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_coef.p, b_croff.p, b_yb.p,
-                      b_xoff.p, yf ? nullptr : b_yb.p, KP, 1.0, gate, 1, nullptr, yf));
+                      b_xoff.p, yf ? nullptr : b_yb.p, KP, 1.0, gate, 1, nullptr, yf, b_xob.p, n_xob));
 }
 
 // cr = CRQ^T x_b per subdomain (c = C^T x, a = R_b^T x, Q^T x, Yh^T x), from the
@@ -370,7 +382,7 @@ void Engine::crq_products(const int* gate) {
   const int N = P->nsub;
   const int S = maxTcr == 1 ? 8 : 1;
   HC(launch_tgemm_f64(st, N, maxTcr, 1, 1, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_xb.p, b_xoff.p, b_cr.p,
-                      b_croff.p, nullptr, KP, 1.0, gate, S, b_kpart.p));
+                      b_croff.p, nullptr, KP, 1.0, gate, S, b_kpart.p, nullptr, b_xob.p, n_xob));
 }
 
 // hbuf = H (G^T G)^-1 G^T x per subdomain, with G^T x = R_b^T (Bf^T x) taken
@@ -398,10 +410,10 @@ void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, co
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0, gate));
+                        b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
   }
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
                        out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
@@ -414,10 +426,10 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p));
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
-                        b_xoff.p, nullptr, KP, 1.0, gate));
+                        b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
   }
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
                        out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
@@ -428,9 +440,9 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
 void Engine::b_rr_solve(double* X) {
   const int N = P->nsub;
   HC(launch_tgemm_f64(st, N, maxTr, 3, 0, d_nr.p, d_nr.p, d_Lt.p, d_off_L.p, nullptr, X, b_roff.p, b_Tr.p, b_roff.p,
-                      nullptr, KP, 1.0, nullptr));
+                      nullptr, KP, 1.0, nullptr, 1, nullptr, nullptr, b_xor.p, n_xor));
   HC(launch_tgemm_f64(st, N, maxTr, 3, 1, d_nr.p, d_nr.p, d_Lt.p, d_off_L.p, nullptr, b_Tr.p, b_roff.p, X, b_roff.p,
-                      nullptr, KP, 1.0, nullptr));
+                      nullptr, KP, 1.0, nullptr, 1, nullptr, nullptr, b_xor.p, n_xor));
 }
 
 // From the local block V (FP64, column major): b_Vr = K_rr^-1 v_r, b_xb = t_b = v_b - K_br K_rr^-1 v_r,
@@ -453,7 +465,7 @@ void Engine::b_kplus(int k, bool interior_too, double* V) {
                         b_xb.p, b_xoff.p, KP, 1, nullptr, -1.0, 1.0));
   }
   HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
-                      b_xoff.p, nullptr, KP, 1.0, nullptr));
+                      b_xoff.p, nullptr, KP, 1.0, nullptr, 1, nullptr, nullptr, b_xob.p, n_xob));
   crq_products(nullptr);
   HC(launch_kb_rtf(st, N, k, KP, np, d_n.p, d_off_n.p, d_R.p, d_off_R.p, V, b_v6.p));
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
@@ -502,7 +514,7 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
     else
       HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32, b_yb32.p));
+                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32, b_yb32.p, b_xob.p, n_xob));
   };
   launch();
   hipEvent_t a, b;

@@ -3,6 +3,8 @@
 // named *off hold per-matrix (per-subdomain) offsets into packed buffers.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <cstdint>
 
 namespace hfls {
@@ -154,18 +156,25 @@ hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const
                               const double* part, const int64_t* partoff, double* y, const int* done);
 
 // ---------------------------------------------------------------- tile GEMM (tgemm.hip)
+// The tile GEMMs put all row blocks of a subdomain on one XCD: entry 8 g + x of an
+// XCD order (length norder, a multiple of 8) is the g-th subdomain of XCD x, -1 for
+// none.  Without one, subdomain d goes to XCD d % 8.  xcd_order() balances the
+// XCDs' work (longest first, each to the least loaded XCD).
+std::vector<int> xcd_order(const std::vector<double>& work);
 // Y_d = alpha op(A_d) X_d (+ Z_d); A_d: mode 0 symmetric lower 64x64 tiles, mode 1
 // general tiles, mode 2 compact column major (lda); X, Y, Z row major, ld kp.
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff,
                             double* Y, const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate,
                             int rt = 1,              // rt: 64-row output tiles per workgroup (1 or 2)
-                            float* Yf = nullptr);    // Yf: store the FP32 result there instead of Y (Z unused)
+                            float* Yf = nullptr,     // Yf: store the FP32 result there instead of Y (Z unused)
+                            const int* xcd_order = nullptr, int norder = 0);  // see xcd_order()
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
                             double alpha, const int* gate, int S = 1, double* Ypart = nullptr,  // S: split K (maxRo 1)
-                            const float* Zf = nullptr);  // Zf: an FP32 addend in place of Z
+                            const float* Zf = nullptr,  // Zf: an FP32 addend in place of Z
+                            const int* xcd_order = nullptr, int norder = 0);  // see xcd_order()
 hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
                            const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
 hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,

@@ -21,6 +21,8 @@
 // four 16-column blocks.  Operand/result lane layouts (checked on gfx950):
 //   f32 32x32x2: A[l%32][l/32], B[l/32][l%32], D row 8(r/4)+4(l/32)+r%4, col l%32
 //   f64 16x16x4: A[l%16][l/16], B[l/16][l%16], D row 4r+l/16, col l%16
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace hfls {
@@ -198,7 +200,8 @@ static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i
 // Workgroup order: the hardware hands consecutive workgroups to the 8 XCDs in
 // turn, each XCD with its own L2.  The linear workgroup id is mapped so that all
 // row blocks of one subdomain run on one XCD (slot s = id / 8 of XCD id % 8 is row
-// block s % nslot of subdomain 8 (s / nslot) + id % 8), and row block i takes the
+// block s % nslot of the subdomain at entry 8 (s / nslot) + id % 8 of the XCD
+// order, or of subdomain 8 (s / nslot) + id % 8 without one), and row block i takes the
 // tile columns in the order j = (t - r_i) mod Ko, t = 0, 1, ..., with r_i = i & ~1
 // (the even row block of its pair): at step t, row blocks i and j = t - r_i need
 // the same symmetric tile (max(i,j), min(i,j)) within a step of each other, one
@@ -207,6 +210,7 @@ static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i
 // they add every row's products in the same order and give bit-identical results.
 template <int RT>
 __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, int nslot,
+                                                   const int* __restrict__ order, int norder,
                                                    const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const float* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const float* __restrict__ X,
@@ -215,8 +219,9 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
                                                    int kp, float alpha, const int* __restrict__ gate,
                                                    float* __restrict__ Yf) {
   const int sl = blockIdx.x >> 3;
-  const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
-  if (d >= nsub) return;
+  const int gi = (sl / nslot) * 8 + (blockIdx.x & 7);
+  const int d = order ? (gi < norder ? order[gi] : -1) : (gi < nsub ? gi : -1);
+  if (d < 0) return;
   if (gate && *gate == 0) return;
   const int Ra = (mrows[d] + 63) / 64, Ca = (mcols[d] + 63) / 64;
   const int Ro = op == 1 && mode != 0 ? Ca : Ra;
@@ -316,6 +321,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
 // order (t - i) mod Ko, so the two reads of every off-diagonal tile meet in L2.
 template <bool CM>  // compact column-major A (mode 2) or tiles
 __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, int nslot,
+                                                   const int* __restrict__ order, int norder,
                                                    const int* __restrict__ mrows,
                                                    const int* __restrict__ mcols, const double* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const int* __restrict__ lda,
@@ -325,8 +331,9 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
                                                    int kp, double alpha, const int* __restrict__ gate, int S,
                                                    double* __restrict__ Ypart, const float* __restrict__ Zf) {
   const int sl = blockIdx.x >> 3;
-  const int d = (sl / nslot) * 8 + (blockIdx.x & 7);
-  if (d >= nsub) return;
+  const int gi = (sl / nslot) * 8 + (blockIdx.x & 7);
+  const int d = order ? (gi < norder ? order[gi] : -1) : (gi < nsub ? gi : -1);
+  if (d < 0) return;
   if (gate && *gate == 0) return;
   const int m = mrows[d], n = mcols[d];
   const int Ra = (m + 63) / 64, Ca = (n + 63) / 64;
@@ -426,17 +433,19 @@ __global__ __launch_bounds__(256) void k_splitk_sum(int S, int kp, const double*
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const float* A, const int64_t* aoff, const int* lda, const float* X, const int64_t* xoff, double* Y,
                             const int64_t* yoff, const double* Z, int kp, float alpha, const int* gate, int rt,
-                            float* Yf) {
+                            float* Yf, const int* order, int norder) {
   (void)lda;
   if (kp % 64 || mode == 2 || mode == 3) return hipErrorInvalidValue;  // FP32: modes 0, 1
-  const int nsub8 = (nsub + 7) / 8;  // subdomain groups of 8, one subdomain of each per XCD
+  if (order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
+  // subdomain groups of 8, one subdomain of each per XCD
+  const int ngroups = order ? norder / 8 : (nsub + 7) / 8;
   if (rt == 2) {
     const int nslot = (maxRo + 1) / 2;
-    hipLaunchKernelGGL(k_tgemm_f32<2>, dim3(8 * nslot * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, nslot,
-                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
+    hipLaunchKernelGGL(k_tgemm_f32<2>, dim3(8 * nslot * ngroups, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, nslot,
+                       order, norder, mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
   } else {
-    hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(8 * maxRo * nsub8, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, maxRo,
-                       mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
+    hipLaunchKernelGGL(k_tgemm_f32<1>, dim3(8 * maxRo * ngroups, 1, kp / 64), dim3(256), 0, st, mode, op, nsub, maxRo,
+                       order, norder, mrows, mcols, A, aoff, X, xoff, Y, yoff, Z, kp, alpha, gate, Yf);
   }
   return hipGetLastError();
 }
@@ -444,18 +453,19 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X, const int64_t* xoff, double* Y,
                             const int64_t* yoff, const double* Z, int kp, double alpha, const int* gate, int S,
-                            double* Ypart, const float* Zf) {
+                            double* Ypart, const float* Zf, const int* order, int norder) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
   if (S > 1 && (maxRo != 1 || !Ypart || mode == 3 || Zf)) return hipErrorInvalidValue;
+  if (order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
   const int nslot = S > 1 ? S : maxRo;
-  const dim3 grid(8 * nslot * ((nsub + 7) / 8), 1, kp / 64);
+  const dim3 grid(8 * nslot * (order ? norder / 8 : (nsub + 7) / 8), 1, kp / 64);
   if (mode == 2)
-    hipLaunchKernelGGL(k_tgemm_f64<true>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
-                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
+    hipLaunchKernelGGL(k_tgemm_f64<true>, grid, dim3(256), 0, st, mode, op, nsub, nslot, order, norder, mrows, mcols, A,
+                       aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
   else
-    hipLaunchKernelGGL(k_tgemm_f64<false>, grid, dim3(256), 0, st, mode, op, nsub, nslot, mrows, mcols, A, aoff, lda,
-                       X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
+    hipLaunchKernelGGL(k_tgemm_f64<false>, grid, dim3(256), 0, st, mode, op, nsub, nslot, order, norder, mrows, mcols,
+                       A, aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
   if (S > 1)
     hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub), dim3(256), 0, st, S, kp, Ypart, Y, yoff, Z,
                        gate);
@@ -570,6 +580,28 @@ __global__ __launch_bounds__(256) void k_mfma_ceiling_f32(double* __restrict__ o
 #pragma unroll
     for (int r = 0; r < 16; r++) s += c[i][r];
   out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+std::vector<int> xcd_order(const std::vector<double>& work) {
+  const int n = (int)work.size();
+  std::vector<int> idx(n);
+  for (int d = 0; d < n; d++) idx[d] = d;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return work[a] > work[b]; });
+  std::vector<std::vector<int>> per(8);
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int d : idx) {
+    int x = 0;
+    for (int y = 1; y < 8; y++)
+      if (load[y] < load[x]) x = y;
+    per[x].push_back(d);
+    load[x] += work[d];
+  }
+  size_t g = 0;
+  for (auto& v : per) g = std::max(g, v.size());
+  std::vector<int> out(8 * std::max<size_t>(g, 1), -1);
+  for (int x = 0; x < 8; x++)
+    for (size_t i = 0; i < per[x].size(); i++) out[8 * i + x] = per[x][i];
+  return out;
 }
 
 hipError_t launch_mfma_ceiling(hipStream_t st, bool f64, int nwg, int iters, double* out, double* flops) {
