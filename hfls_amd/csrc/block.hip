@@ -107,27 +107,31 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
   return hipGetLastError();
 }
 
-// G^T x of the projector without an interface block in HBM: one workgroup per
-// subdomain, out[(6d + m)*kp + j] = sum_p R_b[p][m] * (sum_e sign_e x[r_e][j]) over
-// the b positions p of d (R_b column major at Rb[offRb[d] + p + m*n_s]).  Each wave
-// takes GQ positions per step; the 16 wave sums are added in a fixed order, so a
+// G^T x of the projector without an interface block in HBM, in chunks of at most
+// GTC b positions of one subdomain (chunk c: subdomain csub[c], positions
+// cp0[c] .. min(cp0[c] + GTC, n_s)), so large and small subdomains spread evenly over
+// the device: part[(6c + m)*kp + j] = sum_p R_b[p][m] * (sum_e sign_e x[r_e][j]) over the
+// chunk's positions (R_b column major at Rb[offRb[d] + p + m*n_s]).  Each wave takes
+// GQ positions per step; the 4 wave sums are added in a fixed order.  kb_gtx_sum
+// then adds a subdomain's chunks in ascending order: out[(6d + m)*kp + j].  A
 // column's value does not depend on the other columns.
-constexpr int GTW = 16;
-// (at most 64 VGPRs, so two 16-wave workgroups share a CU)
-__global__ __launch_bounds__(64 * GTW) __attribute__((amdgpu_waves_per_eu(8))) void kb_gtx(int k, int kp, const int* __restrict__ ns,
-                                                   const int64_t* __restrict__ boff, const int* __restrict__ gptr,
-                                                   const int* __restrict__ gidx, const double* __restrict__ Rb,
-                                                   const int64_t* __restrict__ offRb, const double* __restrict__ x,
-                                                   double* __restrict__ out, const int* __restrict__ gate) {
+constexpr int GTC = 256;
+__global__ __launch_bounds__(256) void kb_gtx(int k, int kp, const int* __restrict__ ns,
+                                              const int* __restrict__ csub, const int* __restrict__ cp0,
+                                              const int64_t* __restrict__ boff, const int* __restrict__ gptr,
+                                              const int* __restrict__ gidx, const double* __restrict__ Rb,
+                                              const int64_t* __restrict__ offRb, const double* __restrict__ x,
+                                              double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ double red[GTW][6][64];
-  const int d = blockIdx.x, j = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n = ns[d];
+  __shared__ double red[4][6][64];
+  const int c = blockIdx.x, j = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int d = csub[c], n = ns[d];
+  const int pbeg = cp0[c], pend = std::min(n, pbeg + GTC);
   const double* R = Rb + offRb[d];
   double acc[6] = {0, 0, 0, 0, 0, 0};
   if (j < k) {
-    for (int p0 = w * GQ; p0 < n; p0 += GTW * GQ) {
-      const int nq = std::min(GQ, n - p0);
+    for (int p0 = pbeg + w * GQ; p0 < pend; p0 += 4 * GQ) {
+      const int nq = std::min(GQ, pend - p0);
       double xq[GQ];
       bt_row_sums<GQ, 8>(boff[d] + p0, nq, gptr, gidx, [&](int r) { return x[(int64_t)r * k + j]; }, xq);
 #pragma unroll
@@ -141,20 +145,32 @@ __global__ __launch_bounds__(64 * GTW) __attribute__((amdgpu_waves_per_eu(8))) v
 #pragma unroll
   for (int m = 0; m < 6; m++) red[w][m][j] = acc[m];
   __syncthreads();
-  for (int t = threadIdx.x; t < 6 * 64; t += 64 * GTW) {
+  for (int t = threadIdx.x; t < 6 * 64; t += 256) {
     const int m = t >> 6, jj = t & 63;
     if (jj >= k) continue;
-    double sum = 0.0;
-    for (int ww = 0; ww < GTW; ww++) sum += red[ww][m][jj];
-    out[((int64_t)6 * d + m) * kp + jj] = sum;
+    part[((int64_t)6 * c + m) * kp + jj] = red[0][m][jj] + red[1][m][jj] + red[2][m][jj] + red[3][m][jj];
   }
 }
 
-hipError_t launch_kb_gtx(hipStream_t st, int nsub, int k, int kp, const int* ns, const int64_t* boff, const int* gptr,
-                         const int* gidx, const double* Rb, const int64_t* offRb, const double* x, double* out,
+__global__ __launch_bounds__(384) void kb_gtx_sum(int k, int kp, const int* __restrict__ cbeg,
+                                                  const double* __restrict__ part, double* __restrict__ out,
+                                                  const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int d = blockIdx.x, m = threadIdx.x >> 6, j = threadIdx.x & 63;
+  if (j >= k) return;
+  double s = 0.0;
+  for (int c = cbeg[d]; c < cbeg[d + 1]; c++) s += part[((int64_t)6 * c + m) * kp + j];
+  out[((int64_t)6 * d + m) * kp + j] = s;
+}
+
+hipError_t launch_kb_gtx(hipStream_t st, int nsub, int nchunk, int k, int kp, const int* ns, const int* csub,
+                         const int* cp0, const int* cbeg, const int64_t* boff, const int* gptr, const int* gidx,
+                         const double* Rb, const int64_t* offRb, const double* x, double* part, double* out,
                          const int* gate) {
-  if (k > 64 || k > kp) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gtx, dim3(nsub), dim3(64 * GTW), 0, st, k, kp, ns, boff, gptr, gidx, Rb, offRb, x, out, gate);
+  if (k > 64 || k > kp || nchunk < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_gtx, dim3(nchunk), dim3(256), 0, st, k, kp, ns, csub, cp0, boff, gptr, gidx, Rb, offRb, x,
+                     part, gate);
+  hipLaunchKernelGGL(kb_gtx_sum, dim3(nsub), dim3(384), 0, st, k, kp, cbeg, part, out, gate);
   return hipGetLastError();
 }
 

@@ -80,6 +80,7 @@ struct BlockCtx {
   int64_t gx_key = -1;
   int k = 0, prec = 64;
   DVec<float> xb32, yb32;
+  DVec<double> gtp;
   DVec<double> xb, yb, cr, coef, kpart, v6, gc, la, h, hg, hgf, lam, g, w, Fw, Pg, z, Pz, d, g64, part, gt, Vr, Xr, Tr,
       V, V2, X, U;
   DVec<BScalars> sc;
@@ -240,6 +241,9 @@ class Engine {
   DVec<float> b_Kt32, b_St32, b_xb32, b_yb32;
   DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
   int n_xob = 0, n_xor = 0;
+  DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)
+  int n_gchunk = 0;
+  DVec<double> b_gtp;  // G^T x chunk partials
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_hgf, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;
   DVec<BScalars> b_sc;

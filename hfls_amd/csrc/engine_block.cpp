@@ -76,6 +76,20 @@ void Engine::block_prepare() {
     n_xob = (int)ob.size();
     n_xor = (int)orr.size();
   }
+  {  // chunks of at most 256 b positions for G^T x
+    std::vector<int> csub, cp0, cbeg(N + 1, 0);
+    for (int d = 0; d < N; d++) {
+      for (int p = 0; p < h_ns[d]; p += 256) {
+        csub.push_back(d);
+        cp0.push_back(p);
+      }
+      cbeg[d + 1] = (int)csub.size();
+    }
+    b_gcsub.upload(csub, st);
+    b_gcp0.upload(cp0, st);
+    b_gcbeg.upload(cbeg, st);
+    n_gchunk = (int)csub.size();
+  }
   NRP = rr;
   CRP = rc;
   std::vector<int64_t> gcrow(NC), gcoff(NC), croff_el(N);
@@ -252,6 +266,7 @@ void Engine::alloc_ctx_buffers() {
   for (DVec<double>* v : {&b_lam, &b_g, &b_w, &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64}) zero_alloc(*v, nl * KMAX, st);
   zero_alloc(b_part, (int64_t)b_nparts * KMAX, st);
   zero_alloc(b_gt, (int64_t)6 * N * KP, st);
+  zero_alloc(b_gtp, (int64_t)6 * std::max(n_gchunk, 1) * KP, st);
   zero_alloc(b_Vr, NRP * KP, st);
   zero_alloc(b_Xr, NRP * KP, st);
   zero_alloc(b_Tr, NRP * KP, st);
@@ -273,7 +288,7 @@ void Engine::swap_ctx(BlockCtx& c) {
   b_xb.swap(c.xb); b_yb.swap(c.yb); b_cr.swap(c.cr); b_coef.swap(c.coef); b_kpart.swap(c.kpart); b_v6.swap(c.v6);
   b_gc.swap(c.gc); b_la.swap(c.la); b_h.swap(c.h); b_hg.swap(c.hg); b_hgf.swap(c.hgf); b_lam.swap(c.lam);
   b_g.swap(c.g); b_w.swap(c.w); b_Fw.swap(c.Fw); b_Pg.swap(c.Pg); b_z.swap(c.z); b_Pz.swap(c.Pz); b_d.swap(c.d);
-  b_g64.swap(c.g64); b_part.swap(c.part); b_gt.swap(c.gt); b_Vr.swap(c.Vr); b_Xr.swap(c.Xr); b_Tr.swap(c.Tr);
+  b_g64.swap(c.g64); b_part.swap(c.part); b_gt.swap(c.gt); b_gtp.swap(c.gtp); b_Vr.swap(c.Vr); b_Xr.swap(c.Xr); b_Tr.swap(c.Tr);
   b_V.swap(c.V); b_V2.swap(c.V2); b_X.swap(c.X); b_U.swap(c.U);
   b_sc.swap(c.sc);
 }
@@ -333,7 +348,7 @@ size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
   s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n) * sizeof(float);
-  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
+  for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_gtp, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
   return s;
@@ -389,7 +404,8 @@ void Engine::crq_products(const int* gate) {
 // straight from the dual block (Cluster.cpp:1314-1329)
 void Engine::b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate) {
   const int N = P->nsub;
-  HC(launch_kb_gtx(st, N, k, KP, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, d_Rb.p, d_off_Rb.p, x, b_gt.p, gate));
+  HC(launch_kb_gtx(st, N, n_gchunk, k, KP, d_ns.p, b_gcsub.p, b_gcp0.p, b_gcbeg.p, d_off_b.p, d_gptr.p, d_gidx.p,
+                   d_Rb.p, d_off_Rb.p, x, b_gtp.p, b_gt.p, gate));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_gt.p, nullptr, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0,
                            gate));
 }

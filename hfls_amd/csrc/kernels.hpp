@@ -205,8 +205,12 @@ struct BScalars {         // per-column PCPG scalars of a block solve
 
 hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int* qrow, const int* gptr, const int* gidx, const double* v,
                             const double* wgt, double* xb, float* xb32, const int* gate = nullptr);
-hipError_t launch_kb_gtx(hipStream_t st, int nsub, int k, int kp, const int* ns, const int64_t* boff, const int* gptr,
-                         const int* gidx, const double* Rb, const int64_t* offRb, const double* x, double* out,
+// G^T x per subdomain (out rows 6d + m) through chunk partials (part, 6 nchunk rows);
+// chunk c covers positions cp0[c] .. cp0[c] + 255 of subdomain csub[c], subdomain d
+// owns chunks cbeg[d] .. cbeg[d+1] - 1
+hipError_t launch_kb_gtx(hipStream_t st, int nsub, int nchunk, int k, int kp, const int* ns, const int* csub,
+                         const int* cp0, const int* cbeg, const int64_t* boff, const int* gptr, const int* gidx,
+                         const double* Rb, const int64_t* offRb, const double* x, double* part, double* out,
                          const int* gate = nullptr);
 hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b);
 hipError_t launch_kb_copy(hipStream_t st, int64_t n, const double* a, double* b);
