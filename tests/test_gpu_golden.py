@@ -1,0 +1,58 @@
+"""GPU solves of the 2x2x2 x 10^3 cube (the subdomain size of config 1) against the
+committed oracle displacement (tests/golden/cube8_10e3_u.npz, eps 1e-10):
+
+* plain FP64 path at eps 1e-10: displacement within 1e-8 relative; at eps 1e-8 the
+  iteration count within +-2 of the oracle's;
+* block FP64 at eps 1e-10: within 1e-8;
+* block mixed precision at eps 1e-8 (body force as one of three load cases):
+  certified below 1e-8 and within 1e-7 of the FP64 fixture.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import hfls_amd
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CUBE8 = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+             load="body", body_force="-1.0")
+
+
+@pytest.fixture(scope="module")
+def cube():
+    print("golden cube 2x2x2 x 10^3", flush=True)
+    with np.load(os.path.join(HERE, "cube8_10e3_u.npz")) as f:
+        u, it8 = f["u"].copy(), int(f["iterations_1e8"])
+    m = hfls_amd.Mesh(**CUBE8)
+    yield m, u, it8
+    m.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_plain_fp64(cube):
+    m, u, it8 = cube
+    r = m.pcpg(eps=1e-10, max_iter=1000)
+    print("plain 1e-10: it", r["iterations"], "err %.2e" % rel(r["u"], u), flush=True)
+    assert r["converged"] and rel(r["u"], u) <= 1e-8
+    r8 = m.pcpg(eps=1e-8, max_iter=1000)
+    print("plain 1e-8: it", r8["iterations"], "oracle", it8, flush=True)
+    assert abs(r8["iterations"] - it8) <= 2
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_block(cube, precision):
+    m, u, _ = cube
+    F = m.traction(5, 0, 3)
+    F[1] = m.load_vector()
+    eps = 1e-10 if precision == 64 else 1e-8
+    b = m.pcpg_block(F, eps=eps, max_iter=1000, precision=precision)
+    err = rel(b["u"][1], u)
+    print("block", precision, "it", b["iterations"], "err %.2e" % err, flush=True)
+    assert np.all(b["status"] == 2) and np.all(b["residual"] < eps)
+    assert err <= (1e-8 if precision == 64 else 1e-7)
