@@ -242,13 +242,15 @@ def main():
     log("loads: %d seeded face tractions" % F.shape[0])
 
     # ---- plain FP64: one load case at a time
-    plain_ms, plain_it = [], []
+    plain_ms, plain_it, plain_res, plain_pg = [], [], [], []
     for i in range(a.warmup + a.plain_steps):
         r = mesh.pcpg(eps=a.eps, max_iter=1000, f_packed=F[i], want_lambda=False)
         assert r["converged"], "plain PCPG did not converge"
         if i >= a.warmup:
             plain_ms.append(r["solve_ms"])
             plain_it.append(r["iterations"])
+            plain_res.append(r["residual"])
+            plain_pg.append(r["pg_rel"])
     log("plain fp64: %s ms, iterations %s" % (["%.1f" % x for x in plain_ms], plain_it))
     plain_rate = 1e3 / float(np.mean(plain_ms))
 
@@ -322,7 +324,8 @@ def main():
                    "fp64_refinement_points_per_block": float(np.mean(certs)), "sweeps_per_block": float(np.mean(sweeps)),
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3)},
         "plain": {"value": plain_rate, "unit": "solves/s", "ms_per_solve": float(np.mean(plain_ms)),
-                  "iterations": plain_it[-1], "dtype": "f64"},
+                  "iterations": plain_it[-1], "dtype": "f64", "residual_max": float(max(plain_res)),
+                  "projected_gradient_rel_max": float(max(plain_pg))},
         "speedup_vs_plain": value / plain_rate,
         "concurrent_blocks": conc,
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),

@@ -46,7 +46,7 @@ class HflsError(RuntimeError):
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double),
-                ("solve_ms", ctypes.c_double), ("pcpg_ms", ctypes.c_double)]
+                ("solve_ms", ctypes.c_double), ("pcpg_ms", ctypes.c_double), ("pg_rel", ctypes.c_double)]
 
 
 class CaseStats(ctypes.Structure):
@@ -289,7 +289,7 @@ class Mesh:
         _check(load().hfls_solve(self._h, ctypes.c_double(eps), int(max_iter), _ptr(f), _ptr(u), _ptr(lam),
                                  ctypes.byref(st)))
         return {"u": u, "lambda": lam, "iterations": st.iterations, "converged": bool(st.converged),
-                "residual": st.residual, "solve_ms": st.solve_ms, "pcpg_ms": st.pcpg_ms}
+                "residual": st.residual, "pg_rel": st.pg_rel, "solve_ms": st.solve_ms, "pcpg_ms": st.pcpg_ms}
 
     def traction(self, seed, case0, k):
         """k seeded smoothed tractions on the x = 1 face, packed local loads (k, npacked)."""

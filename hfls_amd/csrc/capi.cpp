@@ -268,6 +268,7 @@ extern "C" int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f
       stats->residual = info.residual;
       stats->solve_ms = info.solve_ms;
       stats->pcpg_ms = info.pcpg_ms;
+      stats->pg_rel = info.pg_rel;
     }
   });
 }

@@ -1116,6 +1116,7 @@ void Engine::setup() {
   xloc.alloc(off_n[N]);
   ug.alloc(3 * P->nPoints);
   sc.alloc(1);
+  pgn.alloc(2);
   HC(hipStreamSynchronize(st));
   sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   size_t freeb = 0, totb = 0;
@@ -1357,6 +1358,7 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   // g0 = F lambda0 - d
   F_apply(lam.p, g.p, dvec.p, nullptr, nullptr);
   proj(g.p, Pg.p, hg.p, alpha.p, nullptr, nullptr);
+  HC(launch_sqnorm(st, nl, Pg.p, pgn.p));  // |P g0|^2
   M_apply(Pg.p, z.p, nullptr);
   proj(z.p, Pz.p, h.p, nullptr, g.p, nullptr);
   HC(launch_reduce(st, nparts, part.p, sc.p, 2));
@@ -1389,6 +1391,7 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
     if (sh.done || launched > max_iter + chunk) break;
   }
   HC(hipEventRecord(e2, st));
+  HC(launch_sqnorm(st, nl, Pg.p, pgn.p + 1));  // P g of the last iteration
   // recovery: u = K_c^+(f - Bf^T lambda) - Rf alpha  (Cluster.cpp:1648-1678)
   HC(hipMemcpyAsync(vloc.p, f, off_n[N] * sizeof(double), hipMemcpyDeviceToDevice, st));
   HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, lam.p, nullptr,
@@ -1408,8 +1411,11 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   if (u_h) HC(hipMemcpyAsync(u_h, ug.p, 3 * P->nPoints * sizeof(double), hipMemcpyDeviceToHost, st));
   if (lam_h) HC(hipMemcpyAsync(lam_h, lam.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
   HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+  double pg2[2] = {0.0, 0.0};
+  HC(hipMemcpyAsync(pg2, pgn.p, sizeof(pg2), hipMemcpyDeviceToHost, st));
   HC(hipStreamSynchronize(st));
   if (info) {
+    info->pg_rel = pg2[0] > 0 ? std::sqrt(pg2[1] / pg2[0]) : 0.0;
     info->iterations = sh.it;
     info->residual = sh.norm0 > 0 ? std::sqrt(std::fabs(sh.gPz)) / sh.norm0 : 0.0;
     info->norm0 = sh.norm0;

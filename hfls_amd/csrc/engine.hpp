@@ -27,6 +27,7 @@ namespace hfls {
 struct SolveInfo {
   int iterations = 0;
   double residual = 0.0;     // sqrt(g'Pz)/sqrt(g0'Pz0) at exit
+  double pg_rel = 0.0;       // |P g| / |P g0| at exit
   double norm0 = 0.0;
   double solve_ms = 0.0;     // device time of the whole load-case solve
   double pcpg_ms = 0.0;      // device time of the iteration loop
@@ -243,6 +244,7 @@ class Engine {
   int n_xob = 0, n_xor = 0;
   DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)
   int n_gchunk = 0;
+  DVec<double> pgn;    // |P g0|^2 and |P g|^2 of the plain solve
   DVec<double> b_gtp;  // G^T x chunk partials
   DVec<double> b_xb, b_yb, b_cr, b_coef, b_v6, b_gc, b_la, b_h, b_hg, b_hgf, b_lam, b_g, b_w, b_Fw, b_Pg, b_z, b_Pz, b_d, b_g64, b_part,
       b_Vr, b_Xr, b_Tr, b_V, b_V2, b_X, b_U;

@@ -26,8 +26,8 @@ int main(int argc, char* argv[]) {
     const hfls::Cluster& c = driver.cluster;
     double umax = 0;
     for (double v : c.solution) umax = v < -umax ? -v : (v > umax ? v : umax);
-    std::printf("iterations %d converged %d residual %.3e max|u| %.6e\n", c.stats.iterations, c.stats.converged,
-                c.stats.residual, umax);
+    std::printf("iterations %d converged %d residual %.3e |Pg|/|Pg0| %.3e max|u| %.6e\n", c.stats.iterations,
+                c.stats.converged, c.stats.residual, c.stats.pg_rel, umax);
     // operator round trip through the facade: F is symmetric
     hfls::Cluster& cc = driver.cluster;
     std::vector<double> a(cc.get_nLam_f()), b(cc.get_nLam_f()), Fa, Fb;

@@ -155,6 +155,9 @@ hipError_t launch_symv_tiles(hipStream_t st, int nsub, int maxT, const int* ns, 
 hipError_t launch_symv_reduce(hipStream_t st, int64_t nb, const int* bsub, const int* ns, const int64_t* boff,
                               const double* part, const int64_t* partoff, double* y, const int* done);
 
+// out[0] = |x|^2 (one workgroup, deterministic order)
+hipError_t launch_sqnorm(hipStream_t st, int64_t n, const double* x, double* out);
+
 // ---------------------------------------------------------------- tile GEMM (tgemm.hip)
 // The tile GEMMs put all row blocks of a subdomain on one XCD: entry 8 g + x of an
 // XCD order (length norder, a multiple of 8) is the g-th subdomain of XCD x, -1 for

@@ -279,4 +279,24 @@ hipError_t launch_recover(hipStream_t st, int nsub, int maxn, const int* n, cons
   return hipGetLastError();
 }
 
+// out[0] = sum_i x[i]^2 over n entries: one workgroup, each thread a fixed strided
+// range, then a fixed-shape tree (deterministic)
+__global__ __launch_bounds__(1024) void k_sqnorm(int64_t n, const double* __restrict__ x, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += x[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+hipError_t launch_sqnorm(hipStream_t st, int64_t n, const double* x, double* out) {
+  hipLaunchKernelGGL(k_sqnorm, dim3(1), dim3(1024), 0, st, n, x, out);
+  return hipGetLastError();
+}
+
 }  // namespace hfls

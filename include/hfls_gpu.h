@@ -31,6 +31,7 @@ typedef struct {
   double residual;   /* sqrt(g'Pz)/sqrt(g0'Pz0) at exit */
   double solve_ms;   /* device time of the whole load-case solve */
   double pcpg_ms;    /* device time of the iteration loop */
+  double pg_rel;     /* |P g| / |P g0| at exit (projected gradient, reported beside residual) */
 } hfls_stats;
 
 const char* hfls_last_error(void);

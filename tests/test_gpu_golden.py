@@ -1,8 +1,8 @@
 """GPU solves of the 2x2x2 x 10^3 cube (the subdomain size of config 1) against the
 committed oracle displacement (tests/golden/cube8_10e3_u.npz, eps 1e-10):
 
-* plain FP64 path at eps 1e-10: displacement within 1e-8 relative; at eps 1e-8 the
-  iteration count within +-2 of the oracle's;
+* plain FP64 path at eps 1e-10: displacement within 1e-8 relative and |P g|/|P g0|
+  below 1e-6; at eps 1e-8 the iteration count within +-2 of the oracle's;
 * block FP64 at eps 1e-10: within 1e-8;
 * block mixed precision at eps 1e-8 (body force as one of three load cases):
   certified below 1e-8 and within 1e-7 of the FP64 fixture.
@@ -38,8 +38,10 @@ def rel(a, b):
 def test_plain_fp64(cube):
     m, u, it8 = cube
     r = m.pcpg(eps=1e-10, max_iter=1000)
-    print("plain 1e-10: it", r["iterations"], "err %.2e" % rel(r["u"], u), flush=True)
+    print("plain 1e-10: it", r["iterations"], "err %.2e" % rel(r["u"], u), "|Pg|/|Pg0| %.2e" % r["pg_rel"],
+          flush=True)
     assert r["converged"] and rel(r["u"], u) <= 1e-8
+    assert 0.0 < r["pg_rel"] < 1e-6  # the projected gradient falls with the preconditioned residual
     r8 = m.pcpg(eps=1e-8, max_iter=1000)
     print("plain 1e-8: it", r8["iterations"], "oracle", it8, flush=True)
     assert abs(r8["iterations"] - it8) <= 2
