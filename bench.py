@@ -112,8 +112,20 @@ def run_cpu_baseline(gpu_iterations, nsub_target):
         else:
             out["all_cores"] = {"value": None, "cores": ncores,
                                 "sample": "failed: " + next(e for (_, e), pr in zip(res, procs) if pr.returncode)[-300:]}
+    out["host"] = {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
     out["wall_s"] = round(time.time() - t0, 1)
     return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(*a):
