@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <thread>
@@ -546,6 +547,10 @@ void Engine::partial_cholesky(double* M, const std::vector<int64_t>& moff, const
 void Engine::factor_subdomains() {
   const int N = P->nsub;
   const Options& o = P->o;
+  auto ms_since = [](std::chrono::steady_clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+  };
+  const auto talloc = std::chrono::steady_clock::now();  // allocations count as assembly time
   d_kdiag.alloc(off_n[N]);
   d_f.alloc(off_n[N]);
   HC(hipMemsetAsync(d_f.p, 0, off_n[N] * sizeof(double), st));
@@ -589,6 +594,8 @@ void Engine::factor_subdomains() {
   }
   DVec<double> M1, M2, R1, R1T, LS;  // LS: the chunk's L_rr tiles (W = L^-1 is kept)
   M1.alloc(cap1); M2.alloc(cap2); R1.alloc(cap3); R1T.alloc(cap3); LS.alloc(std::max<int64_t>(capL, 1));
+  HC(hipStreamSynchronize(st));
+  sinfo.assemble_ms += ms_since(talloc);
   for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
     const int c0 = cuts[ci], c1 = cuts[ci + 1];
     const int nm = c1 - c0;
@@ -732,6 +739,9 @@ void Engine::factor_subdomains() {
     HC(hipStreamSynchronize(st));
     sinfo.factor_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  const auto trel = std::chrono::steady_clock::now();
+  M1.release(); M2.release(); R1.release(); R1T.release(); LS.release();
+  sinfo.assemble_ms += ms_since(trel);
   int err = 0;
   HC(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
   if (err) throw std::runtime_error("non-positive pivot in the subdomain factorisation");
@@ -756,7 +766,8 @@ void Engine::build_bc_and_coarse() {
   std::vector<std::vector<Row>> brow(N);
   std::vector<int> nlcl(P->ncl, 0);
   auto Rbv = [&](int d, int p, int k) { return Rb[off_Rb[d] + p + (int64_t)k * h_ns[d]]; };
-  for (int cl = 0; cl < P->ncl; cl++) {
+  // clusters in parallel: a cluster's rows touch only its own subdomains
+  parallel_for(P->ncl, [&](int cl) {
     const int d0 = cl * P->nsub_cl, d1 = d0 + P->nsub_cl;
     int cnt = 0;
     if (typeBc == "ker") {  // Cluster.cpp:779-819
@@ -835,7 +846,7 @@ void Engine::build_bc_and_coarse() {
       throw std::runtime_error("unknown typeBc " + typeBc);
     }
     nlcl[cl] = cnt;
-  }
+  });
   // per-subdomain dense Bc^T (b-order, ld = ldb), row maps
   h_nlc.assign(N, 0);
   off_C.assign(N + 1, 0);
@@ -858,20 +869,22 @@ void Engine::build_bc_and_coarse() {
     h_Aoff[cl + 1] = h_Aoff[cl] + align32((int64_t)len * len);
     max_cl_len = std::max(max_cl_len, len);
   }
-  std::vector<double> BcT(off_C[N], 0.0);
+  std::unique_ptr<double[]> BcT(new double[std::max<int64_t>(off_C[N], 1)]);  // zeroed per subdomain below
   std::vector<int> lcmap(off_lc[N]);
   std::vector<int> gc_src;
   std::vector<int64_t> gc_off(P->ncl);
   std::vector<std::vector<int>> contrib(h_la_off[P->ncl]);
-  for (int d = 0; d < N; d++) {
-    int cl = h_cl[d];
+  parallel_for(N, [&](int d) {
+    std::fill(BcT.get() + off_C[d], BcT.get() + off_C[d + 1], 0.0);
+    const int cl = h_cl[d];
     for (int k = 0; k < h_nlc[d]; k++) {
       const Row& r = brow[d][k];
       for (auto& e : r.e) BcT[off_C[d] + e.first + (int64_t)k * h_ldb[d]] += e.second;
       lcmap[off_lc[d] + k] = (int)(h_la_off[cl] + r.l);
-      contrib[h_la_off[cl] + r.l].push_back((int)(off_lc[d] + k));
     }
-  }
+  });
+  for (int d = 0; d < N; d++)
+    for (int k = 0; k < h_nlc[d]; k++) contrib[h_la_off[h_cl[d]] + brow[d][k].l].push_back((int)(off_lc[d] + k));
   int64_t gcount = 0;
   for (int cl = 0; cl < P->ncl; cl++) {
     gc_off[cl] = gcount;
@@ -883,7 +896,10 @@ void Engine::build_bc_and_coarse() {
       gcount++;
     }
   }
-  d_BcT.upload(BcT, st);
+  d_BcT.alloc(std::max<int64_t>(off_C[N], 1));
+  HC(hipMemcpyAsync(d_BcT.p, BcT.get(), off_C[N] * sizeof(double), hipMemcpyHostToDevice, st));
+  HC(hipStreamSynchronize(st));
+  BcT.reset();
   d_lcmap.upload(lcmap, st);
   d_nlc.upload(h_nlc, st);
   d_off_C.upload(std::vector<int64_t>(off_C.begin(), off_C.end() - 1), st);
@@ -938,7 +954,6 @@ void Engine::build_bc_and_coarse() {
   d_S.release();
   // cluster matrices, kernel H of Gc, augmented saddle-point matrices
   std::vector<double> Hall((size_t)N * 36);
-  std::vector<double> Aaug;
   std::vector<int64_t> aug_off(P->ncl);
   std::vector<int> aug_ld(P->ncl), aug_nlc(P->ncl), aug_nr(P->ncl), aug_end(P->ncl);
   int64_t aug_total = 0;
@@ -947,9 +962,12 @@ void Engine::build_bc_and_coarse() {
     aug_off[cl] = aug_total;
     aug_total += align32((int64_t)L2 * L2);
   }
-  Aaug.assign(aug_total, 0.0);
-  for (int cl = 0; cl < P->ncl; cl++) {
+  std::unique_ptr<double[]> Aaug(new double[std::max<int64_t>(aug_total, 1)]);  // zeroed per cluster below
+  // clusters in parallel: each writes its own slice of Aaug and the Hall rows of its subdomains
+  parallel_for(P->ncl, [&](int cl) {
     const int d0 = cl * P->nsub_cl, Ns = P->nsub_cl, nlc = nlcl[cl], nr = 6 * Ns;
+    std::fill(Aaug.get() + aug_off[cl], Aaug.get() + aug_off[cl] + align32((int64_t)(2 * (nlc + nr)) * (2 * (nlc + nr))),
+              0.0);
     std::vector<double> Fc((size_t)nlc * nlc, 0.0), Gc((size_t)nlc * nr, 0.0);
     for (int m = 0; m < Ns; m++) {
       int d = d0 + m;
@@ -979,7 +997,7 @@ void Engine::build_bc_and_coarse() {
     for (int c = 0; c < nr; c++) tg += GtG[(size_t)c * nr + c];
     double sig = (nlc > 0 && tf > 0) ? (tg / nr) / (tf / nlc) : 1.0;
     const int L2 = 2 * (nlc + nr);
-    double* A = Aaug.data() + aug_off[cl];
+    double* A = Aaug.get() + aug_off[cl];
     auto at = [&](int r, int c) -> double& { return A[r + (int64_t)c * L2]; };  // lower triangle, column major
     for (int i = 0; i < nlc; i++)
       for (int j = 0; j <= i; j++) at(i, j) = 0.5 * (Fc[(size_t)i * nlc + j] + Fc[(size_t)j * nlc + i]);
@@ -995,11 +1013,14 @@ void Engine::build_bc_and_coarse() {
     for (int l = 0; l < nlc; l++) at(nlc + nr + l, l) = 1.0;
     for (int a = 0; a < nr; a++) at(2 * nlc + nr + a, nlc + a) = 1.0;
     aug_ld[cl] = L2; aug_nlc[cl] = nlc; aug_nr[cl] = nr; aug_end[cl] = L2;
-  }
+  });
   d_H.upload(Hall, st);
   {
     DVec<double> dA;
-    dA.upload(Aaug, st);
+    dA.alloc(std::max<int64_t>(aug_total, 1));
+    HC(hipMemcpyAsync(dA.p, Aaug.get(), aug_total * sizeof(double), hipMemcpyHostToDevice, st));
+    HC(hipStreamSynchronize(st));
+    Aaug.reset();
     std::vector<int> zero(P->ncl, 0), startB(P->ncl), row0(P->ncl), nlenv(P->ncl);
     for (int cl = 0; cl < P->ncl; cl++) { startB[cl] = aug_nlc[cl]; row0[cl] = aug_nlc[cl] + aug_nr[cl]; nlenv[cl] = row0[cl]; }
     partial_cholesky(dA.p, aug_off, aug_ld, zero, aug_nlc, aug_end);
@@ -1018,14 +1039,25 @@ void Engine::build_bc_and_coarse() {
   std::vector<double> GtG((size_t)ncol * ncol, 0.0);
   {
     std::vector<double> RH((size_t)NB * 6);  // R_b,i[p,:] H_i per packed b position
-    for (int d = 0; d < N; d++)
+    parallel_for(N, [&](int d) {
       for (int p = 0; p < h_ns[d]; p++)
         for (int b = 0; b < 6; b++) {
           double s = 0.0;
           for (int a = 0; a < 6; a++) s += Rbv(d, p, a) * Hall[(size_t)d * 36 + a * 6 + b];
           RH[(size_t)(off_b[d] + p) * 6 + b] = s;
         }
-    for (int64_t r = 0; r < P->nlam; r++) {
+    });
+    // the dual rows in 16 fixed chunks, each summed into its own GtG, the chunks added
+    // in order (the same result for any thread count)
+    constexpr int NCH = 16;
+    const bool chunked = (int64_t)ncol * ncol * NCH <= (int64_t)1 << 26;
+    const int nch = chunked ? NCH : 1;
+    std::vector<std::vector<double>> part(nch);
+    parallel_for(nch, [&](int ch) {
+    std::vector<double>& Gp = nch > 1 ? part[ch] : GtG;
+    if (nch > 1) Gp.assign((size_t)ncol * ncol, 0.0);
+    const int64_t r0 = P->nlam * ch / nch, r1 = P->nlam * (ch + 1) / nch;
+    for (int64_t r = r0; r < r1; r++) {
       double row[12];
       int cols[2];
       int nc = 0;
@@ -1055,8 +1087,12 @@ void Engine::build_bc_and_coarse() {
         for (int qb = 0; qb < nc; qb++)
           for (int a = 0; a < 6; a++)
             for (int b = 0; b < 6; b++)
-              GtG[(size_t)(6 * cols[qa] + a) * ncol + 6 * cols[qb] + b] += row[qa * 6 + a] * row[qb * 6 + b];
+              Gp[(size_t)(6 * cols[qa] + a) * ncol + 6 * cols[qb] + b] += row[qa * 6 + a] * row[qb * 6 + b];
     }
+    });
+    if (nch > 1)
+      for (int ch = 0; ch < nch; ch++)
+        for (size_t i = 0; i < GtG.size(); i++) GtG[i] += part[ch][i];
   }
   {
     const int L2 = 2 * ncol;
