@@ -279,6 +279,7 @@ void Engine::configure(const std::string& key, const std::string& value) {
     use_graph = value == "1" || value == "true";
   } else if (key == "refine_tol") {
     P->o["refine_tol"] = value;
+
   } else if (key == "gemm_rows") {  // override the autotuner: 1, 2 or auto
     if (value == "auto") {
       gemm_rows_forced = 0;
@@ -298,6 +299,11 @@ void Engine::configure(const std::string& key, const std::string& value) {
   if (gx_plain) (void)hipGraphExecDestroy(gx_plain);
   if (gx_blk) (void)hipGraphExecDestroy(gx_blk);
   gx_plain = gx_blk = nullptr;
+  for (auto& c : ctxs_)  // the graphs of the other block contexts were captured with the old settings
+    if (c && c->gx) {
+      (void)hipGraphExecDestroy(c->gx);
+      c->gx = nullptr;
+    }
   gx_blk_key = -1;
 }
 
