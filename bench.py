@@ -229,7 +229,7 @@ def main():
         cpu_sample()
         return 0
     if a.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path.",
               file=sys.stderr)
         return 2
     import hfls_amd

@@ -231,7 +231,7 @@ Engine::Engine(const Options& o, int dev) : device(dev) {
   HC(hipSetDevice(device));
   HC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   P.reset(new Problem(o));
-  lumped = opt(o, "preconditioner") == "lumped";
+  lumped = opt(o, "preconditioner") == "lumped";  // value checked by Problem
 }
 
 Engine::~Engine() {

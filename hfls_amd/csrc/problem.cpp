@@ -127,6 +127,19 @@ void Problem::node_coords(int64_t ii, int64_t jj, int64_t kk, double* x) const {
 double Problem::young(int d) const { return subs[d].E; }
 
 Problem::Problem(const Options& oin) : o(oin) {
+  // htfeti.cpp:18 documents Dirichlet_explicit, Dirichlet_implicit and lumped.
+  // Both Dirichlet forms apply S_bb (Cluster.cpp:1061-1102; explicit and implicit
+  // are equal in exact arithmetic, here S_bb is always explicit).  The reference
+  // takes any other string as lumped (Cluster.cpp:1104-1107); a misspelt value is
+  // rejected here instead of silently switching the preconditioner.
+  {
+    const std::string prec = opt(o, "preconditioner");
+    if (prec != "Dirichlet_explicit" && prec != "Dirichlet_implicit" && prec != "lumped")
+      throw std::runtime_error("unknown preconditioner " + prec +
+                               " (Dirichlet_explicit, Dirichlet_implicit or lumped)");
+    const std::string tb = opt(o, "typeBc");
+    if (tb != "ker" && tb != "cor" && tb != "all") throw std::runtime_error("unknown typeBc " + tb);
+  }
   const char* ax = "xyz";
   for (int a = 0; a < 3; a++) {
     C[a] = opti(o, std::string("C") + ax[a]);

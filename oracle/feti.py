@@ -143,12 +143,16 @@ class HTFETI:
         inv = sla.cho_solve(c, np.eye(ncol))
         self.invGfTGf = 0.5 * (inv + inv.T)
         # ---- preconditioner (Cluster.cpp:140-148, Matrix.cpp:1708-1871) ----
+        # Dirichlet_explicit and Dirichlet_implicit build the same operator S_bb
+        # (explicit vs implicit application only); every other value applies K
+        # (the lumped branch, Cluster.cpp:1104-1107)
         self.prec = o["preconditioner"]
+        dirichlet = self.prec in ("Dirichlet_explicit", "Dirichlet_implicit")
         self.S = []
         for d in range(self.nsub):
             b = g.Bf[d]["j_col_cmpr"]
             K = s.K[d]
-            if self.prec.startswith("Dirichlet"):
+            if dirichlet:
                 r = np.setdiff1d(np.arange(K.shape[0]), b)
                 Kd = K.toarray() if hasattr(K, "toarray") else K
                 Kbb = Kd[np.ix_(b, b)]

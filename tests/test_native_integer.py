@@ -32,9 +32,21 @@ def test_unknown_option_is_an_error():
         hfls_amd.Problem(not_an_option="1")
 
 
+def test_option_values_of_the_reference():
+    """htfeti.cpp:18 lists Dirichlet_explicit, Dirichlet_implicit and lumped; both Dirichlet
+    forms build S_bb (Cluster.cpp:140-142).  A value outside the list is an error (the
+    reference would silently take it as lumped, Cluster.cpp:1104-1107)."""
+    for v in ("Dirichlet_explicit", "Dirichlet_implicit", "lumped"):
+        assert hfls_amd.Problem(preconditioner=v, nx=2, ny=2, nz=2).nsub == 8
+    with pytest.raises(hfls_amd.HflsError, match="unknown preconditioner"):
+        hfls_amd.Problem(preconditioner="Dirichlet")
+    with pytest.raises(hfls_amd.HflsError, match="unknown typeBc"):
+        hfls_amd.Problem(typeBc="corners")
+
+
 CASES = [
     dict(),  # reference defaults: 2x2x2, 5^3, symmetry BC, checkerboard E
-    dict(bc="clamp_x0", material="uniform", young_modulus="2.1e5", load="body"),
+    dict(bc="clamp_x0", material="uniform", young_modulus="2.1e5", load="body"),  # 2x2x2 x 5^3 clamped cube, body force
     dict(Nx=3, Ny=2, Nz=1, nx=2, ny=3, nz=2),
     dict(Cx=2, Cy=1, Cz=2, Nx=2, Ny=2, Nz=1, nx=3, ny=2, nz=2, bc="clamp_x0"),
     dict(Cx=1, Cy=2, Cz=1, Nx=2, Ny=1, Nz=2, slabs_x="2,4", slabs_y="3,1", slabs_z="2,3",
@@ -77,7 +89,7 @@ def test_integer_structures_bit_exact(kw):
 
 @pytest.mark.slow
 def test_config1_integer_hashes():
-    """BASELINE config 1 decomposition: 8 clusters x 4^3 subdomains of 10^3 elements;
+    """512-subdomain cube decomposition: 8 clusters x 4^3 subdomains of 10^3 elements;
     sizes derived from the reference's construction rules."""
     kw = dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4, nx=10, ny=10, nz=10, bc="clamp_x0")
     p = hfls_amd.Problem(**kw)

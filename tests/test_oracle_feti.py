@@ -75,7 +75,7 @@ def test_generalised_inverse_invariance(cfg0):
 
 
 @pytest.mark.parametrize("kw", [dict(typeBc="cor"), dict(preconditioner="lumped"),
-                                dict(Cx=2, Nx=1, Ny=2, Nz=2, nx=3, ny=3, nz=3)])
+                                dict(preconditioner="Dirichlet_implicit"), dict(Cx=2, Nx=1, Ny=2, Nz=2, nx=3, ny=3, nz=3)])
 def test_variants_match_direct(kw):
     k = dict(CFG0)
     k.update(nx=3, ny=3, nz=3)
