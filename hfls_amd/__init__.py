@@ -36,7 +36,7 @@ DEFAULTS = {
     "eps_iter": "1e-6", "max_iter": "200", "Ac_extended_by_kerGc": "true",
     "slabs_x": "", "slabs_y": "", "slabs_z": "",
     "linear_solver": "hip", "vtkWithinIter": "false", "print_matrices": "0",
-    "precision": "fp64", "nrhs": "1", "device": "0", "refine_tol": "1e-4", "slab_choices": "",
+    "precision": "fp64", "nrhs": "1", "device": "0", "refine_tol": "1e-4", "slab_choices": "", "block_gemm": "f16x2",
 }
 
 

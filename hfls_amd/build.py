@@ -26,6 +26,7 @@ SOURCES = [
     ("kernels2.hip", HIP_FLAGS),
     ("symv.hip", HIP_FLAGS),
     ("tgemm.hip", HIP_FLAGS),
+    ("tgemm16.hip", HIP_FLAGS),
     ("lrr.hip", HIP_FLAGS),
     ("block.hip", HIP_FLAGS),
     ("problem.cpp", HOST_FLAGS + ["-ffp-contract=off"]),

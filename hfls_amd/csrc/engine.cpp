@@ -231,7 +231,8 @@ Engine::Engine(const Options& o, int dev) : device(dev) {
   HC(hipSetDevice(device));
   HC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   P.reset(new Problem(o));
-  lumped = opt(o, "preconditioner") == "lumped";  // value checked by Problem
+  lumped = opt(o, "preconditioner") == "lumped";  // values checked by Problem
+  gemm16 = opt(o, "block_gemm") == "f16x2";
 }
 
 Engine::~Engine() {

@@ -80,7 +80,7 @@ struct BlockCtx {
   hipGraphExec_t gx = nullptr;
   int64_t gx_key = -1;
   int k = 0, prec = 64;
-  DVec<float> xb32, yb32;
+  DVec<float> xb32, yb32, sx;
   DVec<double> gtp;
   DVec<double> xb, yb, cr, coef, kpart, v6, gc, la, h, hg, hgf, lam, g, w, Fw, Pg, z, Pz, d, g64, part, gt, Vr, Xr, Tr,
       V, V2, X, U;
@@ -154,13 +154,14 @@ class Engine {
   void blk_recover(const BScalars* state = nullptr);  // state: the scalars of the last blk_sync
   void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
-  // autotuner result: output row tiles per workgroup of the FP32 block GEMM and
-  // the measured ms of each candidate (1, 2)
+  // autotuner result of the block GEMM (FP32 format: output row tiles per
+  // workgroup 1 or 2; split-FP16 format: K chunk depth 32 or 64 as 1 or 2) and the
+  // measured ms of each candidate
   int tuned_rows() const { return tune_rt32; }
   double tuned_ms(int rt) const { return tune_ms32[rt - 1]; }
   // runtime settings: hip_graph (1/0: capture and replay the iteration chunks),
   // refine_tol (mixed-precision refinement factor), poll_chunk (iterations
-  // between host polls), gemm_rows (1, 2 or auto: block GEMM tile shape)
+  // between host polls), gemm_rows (1, 2 or auto: block GEMM tile shape, see tuned_rows)
   void configure(const std::string& key, const std::string& value);
   hipStream_t stream() const { return st; }
 
@@ -232,6 +233,9 @@ class Engine {
   void blk_set_status(const std::vector<int>& slots, int status);
   void blk_enqueue_sweeps(int n);
   void autotune();
+  // b_yb32 = A b_xb32 on the matrix cores, A = K+_bb (which 0) or S_i (which 1) in
+  // the block format (split FP16 or FP32)
+  void gemm_lo(int which, const int* gate);
   int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
@@ -240,6 +244,10 @@ class Engine {
   DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32, b_yb32;
+  // split-FP16 block GEMM (option block_gemm=f16x2): b_Kt32/b_St32 hold hi and lo FP16
+  // planes of A_d / s_d, b_sK/b_sS the power-of-two s_d, b_sx the per-column scales of X
+  bool gemm16 = true;
+  DVec<float> b_sK, b_sS, b_sx;
   DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
   int n_xob = 0, n_xor = 0;
   DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)

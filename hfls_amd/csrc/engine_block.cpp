@@ -241,6 +241,12 @@ void Engine::block_prepare() {
   b_St32.alloc(off_P[N]);
   HC(launch_kb_deflate_f32(st, N, maxtiles, d_ns.p, d_Kt.p, d_off_P.p, b_CR.p, b_cro.p, d_ldb.p, d_nlc.p, b_Kt32.p));
   HC(launch_kb_to_f32(st, off_P[N], d_St.p, b_St32.p));
+  if (gemm16) {  // in place: FP32 tiles -> hi and lo FP16 planes with a power-of-two scale per subdomain
+    b_sK.alloc(N);
+    b_sS.alloc(N);
+    HC(launch_split16(st, N, maxtiles, d_ns.p, d_off_P.p, b_Kt32.p, b_sK.p));
+    HC(launch_split16(st, N, maxtiles, d_ns.p, d_off_P.p, b_St32.p, b_sS.p));
+  }
   HC(hipStreamSynchronize(st));
   autotune();
   blk_ready = true;
@@ -254,6 +260,7 @@ void Engine::alloc_ctx_buffers() {
   zero_alloc(b_yb, NBP * KP, st);
   zero_alloc(b_xb32, NBP * KP, st);
   zero_alloc(b_yb32, NBP * KP, st);
+  zero_alloc(b_sx, (int64_t)N * KP, st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
   zero_alloc(b_kpart, (int64_t)N * 8 * 64 * KP, st);
@@ -285,6 +292,7 @@ void Engine::swap_ctx(BlockCtx& c) {
   std::swap(blk_prec, c.prec);
   b_xb32.swap(c.xb32);
   b_yb32.swap(c.yb32);
+  b_sx.swap(c.sx);
   b_xb.swap(c.xb); b_yb.swap(c.yb); b_cr.swap(c.cr); b_coef.swap(c.coef); b_kpart.swap(c.kpart); b_v6.swap(c.v6);
   b_gc.swap(c.gc); b_la.swap(c.la); b_h.swap(c.h); b_hg.swap(c.hg); b_hgf.swap(c.hgf); b_lam.swap(c.lam);
   b_g.swap(c.g); b_w.swap(c.w); b_Fw.swap(c.Fw); b_Pg.swap(c.Pg); b_z.swap(c.z); b_Pz.swap(c.Pz); b_d.swap(c.d);
@@ -317,9 +325,10 @@ int Engine::blk_contexts(int n) {
   return (int)ctxs_.size();
 }
 
-// Autotuner of the block GEMM tile shape: every candidate (64 or 128 output rows
-// per workgroup) is timed on this mesh's own FP32 K+_bb tiles over a 64-column
-// block with HIP events, and the fastest is used for K+_bb and S_i from then on.
+// Autotuner of the block GEMM tile shape: every candidate is timed on this mesh's
+// own K+_bb tiles over a 64-column block with HIP events, and the fastest is used
+// for K+_bb and S_i from then on.  Candidates 1, 2: FP32 format 64 or 128 output rows
+// per workgroup; split-FP16 format K chunks of 32 or 64 per LDS stage.
 void Engine::autotune() {
   const int N = P->nsub;
   hipEvent_t a, b;
@@ -327,8 +336,12 @@ void Engine::autotune() {
   HC(hipEventCreate(&b));
   for (int rt = 1; rt <= 2; rt++) {
     auto launch = [&] {
-      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p,
-                          nullptr, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt, b_yb32.p, b_xob.p, n_xob));
+      if (gemm16)
+        HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, b_Kt32.p, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f,
+                            nullptr, b_yb32.p, b_xob.p, n_xob, 32 * rt));
+      else
+        HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p,
+                            nullptr, b_xoff.p, nullptr, KP, 1.0f, nullptr, rt, b_yb32.p, b_xob.p, n_xob));
     };
     launch();
     HC(hipEventRecord(a, st));
@@ -347,7 +360,7 @@ void Engine::autotune() {
 size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
-  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n) * sizeof(float);
+  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n + b_sK.n + b_sS.n + b_sx.n) * sizeof(float);
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_gtp, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
@@ -362,8 +375,7 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, win, nullptr, b_xb.p,
                       prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {  // FP32 result into b_yb32; the FP64 correction below adds it into b_yb
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_Kt32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
+    gemm_lo(0, gate);
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_Kt.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
@@ -379,6 +391,17 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   b_corr(k, 1, prec == 32 ? 1 : 0, gate, prec == 32 ? b_yb32.p : nullptr);
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
                        dotw, b_part.p, gate));
+}
+
+void Engine::gemm_lo(int which, const int* gate) {
+  const int N = P->nsub;
+  const float* A = which == 0 ? b_Kt32.p : b_St32.p;
+  if (gemm16)
+    HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
+                        KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 32 * tune_rt32));
+  else
+    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
 }
 
 // y_b += R_b alpha (with_R) - C lam_c (+ Q (Yh^T x) + Yh (Q^T x) if defl) as one
@@ -425,8 +448,7 @@ void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, co
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, b_Pg.p, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
+    gemm_lo(1, gate);
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
@@ -441,8 +463,7 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, x, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
-    HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, b_St32.p, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                        b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
+    gemm_lo(1, gate);
   } else {
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
@@ -523,14 +544,12 @@ void Engine::apply_block(int op, int k, int prec, const double* x_h, double* y_h
 double Engine::time_block_gemm(int which, int reps, double* flops) {
   block_prepare();
   const int N = P->nsub;
-  const float* A = which == 0 ? b_Kt32.p : b_St32.p;
   const int64_t ncopy = b_xb.n & ~(int64_t)1;
   auto launch = [&] {
     if (which == 2)
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
     else
-      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
-                          b_xoff.p, nullptr, KP, 1.0f, nullptr, tune_rt32, b_yb32.p, b_xob.p, n_xob));
+      gemm_lo(which, nullptr);
   };
   launch();
   hipEvent_t a, b;

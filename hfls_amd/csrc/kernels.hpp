@@ -228,6 +228,16 @@ hipError_t launch_kb_coef(hipStream_t st, int nsub, int maxrows, int k, int kp, 
                           const double* cr, const int64_t* crow, double* coef, const int* gate = nullptr);
 hipError_t launch_kb_cols(hipStream_t st, int nsub, int maxns, int dir, int ncol, int kp, const int* ns,
                           const int64_t* obrow, double* A, const int64_t* aoff, const int* lda, double* X);
+// split-FP16 form of packed FP32 tiles, in place (tgemm16.hip): scale[d] = power of
+// two mapping max |A_d| into [2^14, 2^15); each tile becomes hi and lo FP16 planes
+hipError_t launch_split16(hipStream_t st, int nsub, int maxtiles, const int* ns, const int64_t* poff, float* P32,
+                          float* scale);
+// Yf_d = alpha A_d X_d, A_d split-FP16 symmetric tiles (mode 0) with scale sA[d], X FP32
+// row major (ld kp); sx (nsub * kp) receives the per-column scales of X
+hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, const float* A, const int64_t* aoff,
+                            const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
+                            const int* gate, float* Yf, const int* xcd_order = nullptr, int norder = 0,
+                            int kc = 32);  // kc: K chunk depth staged per step (32 or 64)
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
                                  const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
                                  const int* nlc, float* P32);

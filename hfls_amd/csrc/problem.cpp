@@ -24,7 +24,8 @@ Options default_options() {
       {"eps_iter", "1e-6"}, {"max_iter", "200"}, {"Ac_extended_by_kerGc", "true"},
       {"slabs_x", ""}, {"slabs_y", ""}, {"slabs_z", ""},
       {"linear_solver", "hip"}, {"vtkWithinIter", "false"}, {"print_matrices", "0"},
-      {"precision", "fp64"}, {"nrhs", "1"}, {"device", "0"}, {"refine_tol", "1e-4"}, {"slab_choices", ""}};
+      {"precision", "fp64"}, {"nrhs", "1"}, {"device", "0"}, {"refine_tol", "1e-4"}, {"slab_choices", ""},
+      {"block_gemm", "f16x2"}};
 }
 
 std::string opt(const Options& o, const std::string& k) {
@@ -139,6 +140,10 @@ Problem::Problem(const Options& oin) : o(oin) {
                                " (Dirichlet_explicit, Dirichlet_implicit or lumped)");
     const std::string tb = opt(o, "typeBc");
     if (tb != "ker" && tb != "cor" && tb != "all") throw std::runtime_error("unknown typeBc " + tb);
+    // matrix-core format of K+_bb and S_i in mixed-precision blocks: split FP16
+    // (hi + lo planes) or FP32
+    const std::string bg = opt(o, "block_gemm");
+    if (bg != "f16x2" && bg != "f32") throw std::runtime_error("unknown block_gemm " + bg + " (f16x2 or f32)");
   }
   const char* ax = "xyz";
   for (int a = 0; a < 3; a++) {

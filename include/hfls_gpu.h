@@ -84,12 +84,13 @@ int hfls_mesh_load(hfls_mesh m, double* f);
  * "hip_graph" "1"/"0": replay the PCPG iteration chunks from captured HIP graphs
  * (default on); "refine_tol": mixed-precision refinement factor (default 1e-4);
  * "poll_chunk": iterations between host polls of the stopping flags (default 8);
- * "gemm_rows": "1", "2" or "auto" (default): output row tiles per workgroup of the
- * FP32 block GEMM, overriding the autotuner */
+ * "gemm_rows": "1", "2" or "auto" (default): tile shape of the mixed-precision block
+ * GEMM, overriding the autotuner (FP32 format: 1 or 2 output row tiles per workgroup;
+ * split-FP16 format, the default, option block_gemm: K chunks of 32 or 64) */
 int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value);
-/* autotuner result (after the first block call): [0] output row tiles per
- * workgroup chosen for the FP32 block GEMM, [1], [2] measured ms per launch of
- * the candidates with 1 and 2 row tiles */
+/* autotuner result (after the first block call): [0] tile shape chosen for the
+ * mixed-precision block GEMM (1 or 2, as for "gemm_rows"), [1], [2] measured ms per
+ * launch of candidates 1 and 2 */
 int hfls_mesh_tuning(hfls_mesh m, double* out);
 
 /* operators: Cluster::mult_Ff (Cluster.cpp:1114-1188), Projection (1314-1329,

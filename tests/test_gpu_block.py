@@ -78,6 +78,39 @@ def test_block_F_fp64_and_fp32(pair):
         assert e32 <= 2e-5
 
 
+@pytest.mark.parametrize("fmt", ["f16x2", "f32"])
+def test_matrix_core_formats(fmt):
+    """K+_bb and S_i of the mixed-precision operators in both matrix-core formats
+    (block_gemm: split FP16 hi + lo planes with power-of-two scales, or FP32): F and M at
+    precision 32 within 2e-5 of FP64 per column, also for columns scaled by 1e-12 and 1e6
+    (per-column scales of the right-hand sides), and a mixed solve to eps 1e-8 within
+    1e-7 of the FP64 solve at eps 1e-10."""
+    kw = dict(UNEVEN, block_gemm=fmt)
+    print("block_gemm", fmt, kw, flush=True)
+    gpu = hfls_amd.Mesh(**kw)
+    try:
+        rng = np.random.default_rng(11)
+        W = rng.standard_normal((4, gpu.nlam))
+        W[1] *= 1e-12
+        W[2] *= 1e6
+        for op in (0, 1):
+            A64 = gpu.apply_block(op, W, 64)
+            A32 = gpu.apply_block(op, W, 32)
+            for j in range(4):
+                e = np.linalg.norm(A32[j] - A64[j]) / np.linalg.norm(A64[j])
+                print(fmt, "op", op, "column", j, "rel %.2e" % e, flush=True)
+                assert e <= 2e-5
+        F = gpu.traction(SEED, 0, 3)
+        r = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=32)
+        r64 = gpu.pcpg_block(F, eps=1e-10, max_iter=500, precision=64)
+        for j in range(3):
+            e = np.linalg.norm(r["u"][j] - r64["u"][j]) / np.linalg.norm(r64["u"][j])
+            print(fmt, "mixed case", j, "iterations", r["iterations"][j], "rel %.2e" % e, flush=True)
+            assert r["status"][j] == 2 and e <= 1e-7
+    finally:
+        gpu.close()
+
+
 def test_block_fp64_matches_oracle(pair):
     """Iterations at eps 1e-8 within +-2 of the oracle's; displacements compared at
     eps 1e-10, where both solves are converged well below the 1e-8 bar (at 1e-8 the
