@@ -91,22 +91,37 @@ __global__ __launch_bounds__(256) void kb_split16(const int* __restrict__ ns, co
   }
 }
 
-// sx[d*kp + c] = scale of column c of X_d (rows 0 .. ns[d]-1)
-__global__ __launch_bounds__(256) void kb_colscale(const int* __restrict__ ns, const int64_t* __restrict__ xoff,
-                                                   int kp, const float* __restrict__ X, float* __restrict__ sx,
-                                                   const int* __restrict__ gate) {
+// sx[d*kp + c] = scale of column c of X_d (rows 0 .. ns[d]-1); 16 waves per
+// subdomain, each lane with 8 independent loads in flight per step
+__global__ __launch_bounds__(1024) void kb_colscale(const int* __restrict__ ns, const int64_t* __restrict__ xoff,
+                                                    int kp, const float* __restrict__ X, float* __restrict__ sx,
+                                                    const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ float red[4][64];
+  __shared__ float red[16][64];
   const int d = blockIdx.x, c = blockIdx.y * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
   const float* Xd = X + xoff[d] + c;
   const int n = ns[d];
-  float m = 0.0f;
-  for (int r = g; r < n; r += 4) m = fmaxf(m, fabsf(Xd[(int64_t)r * kp]));
-  red[g][threadIdx.x & 63] = m;
+  float m[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) m[u] = 0.0f;
+  int r = g;
+  for (; r + 112 < n; r += 128) {  // rows r + 16 u, u = 0..7, all below n
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = Xd[(int64_t)(r + 16 * u) * kp];
+#pragma unroll
+    for (int u = 0; u < 8; u++) m[u] = fmaxf(m[u], fabsf(v[u]));
+  }
+  for (; r < n; r += 16) m[0] = fmaxf(m[0], fabsf(Xd[(int64_t)r * kp]));
+#pragma unroll
+  for (int u = 1; u < 8; u++) m[0] = fmaxf(m[0], m[u]);
+  red[g][threadIdx.x & 63] = m[0];
   __syncthreads();
   if (g == 0) {
-    m = fmaxf(fmaxf(red[0][c & 63], red[1][c & 63]), fmaxf(red[2][c & 63], red[3][c & 63]));
-    sx[(int64_t)d * kp + c] = pow2_scale(m);
+    float v = red[0][threadIdx.x];
+#pragma unroll
+    for (int q = 1; q < 16; q++) v = fmaxf(v, red[q][threadIdx.x]);
+    sx[(int64_t)d * kp + c] = pow2_scale(v);
   }
 }
 
@@ -241,7 +256,7 @@ hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, c
   if (nsub <= 0 || maxT <= 0) return hipSuccess;
   if (kp <= 0 || kp % 64 || (kc != 32 && kc != 64)) return hipErrorInvalidValue;
   if (xcd_order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(256), 0, st, ns, xoff, kp, X, sx, gate);
+  hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(1024), 0, st, ns, xoff, kp, X, sx, gate);
   const int groups = xcd_order ? norder / 8 : (nsub + 7) / 8;  // subdomain groups of 8, one per XCD
   if (kc == 64)
     hipLaunchKernelGGL(k_tgemm_h16<64>, dim3(8 * maxT * groups, 1, kp / 64), dim3(256), 0, st, nsub, maxT,
