@@ -205,9 +205,11 @@ static __device__ __forceinline__ int chunk_kind(int mode, int op, int Ra, int i
 // tile columns in the order j = (t - r_i) mod Ko, t = 0, 1, ..., with r_i = i & ~1
 // (the even row block of its pair): at step t, row blocks i and j = t - r_i need
 // the same symmetric tile (max(i,j), min(i,j)) within a step of each other, one
-// directly, one transposed, so the second read is served by the shared L2, as is
-// the X block of the subdomain.  Both tile shapes (RT = 1, 2) use the same r_i, so
-// they add every row's products in the same order and give bit-identical results.
+// directly, one transposed.  The X block of the subdomain is served by the XCD's
+// L2; the second tile read mostly is not (FETCH_SIZE is about twice the packed
+// tiles: the workgroups drift apart by more than the L2 holds).  Both tile shapes
+// (RT = 1, 2) use the same r_i, so they add every row's products in the same order
+// and give bit-identical results.
 template <int RT>
 __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, int nslot,
                                                    const int* __restrict__ order, int norder,

@@ -128,9 +128,11 @@ __global__ __launch_bounds__(1024) void kb_colscale(const int* __restrict__ ns, 
 // One workgroup: 64 output rows (row block i) x 64 columns, K in chunks of KC (32:
 // half a tile, 64: a whole tile) through LDS, A and X split into hi/lo as they are
 // staged; the workgroup order and the rotated tile order are those of k_tgemm_f32
-// (all row blocks of a subdomain on one XCD, tile columns from (i & ~1)).  Both
-// chunk sizes issue the same matrix-core operations in the same order, so they give
-// bit-identical results.
+// (all row blocks of a subdomain on one XCD, so X_d is fetched once per launch; tile
+// columns from (i & ~1)).  Every stored off-diagonal tile is read by two row blocks;
+// FETCH_SIZE shows both reads come from HBM (2 x the packed tiles), so the kernel is
+// bound by that traffic.  Both chunk sizes issue the same matrix-core operations in
+// the same order, so they give bit-identical results.
 template <int KC>
 __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const int* __restrict__ order, int norder,
                                                    const int* __restrict__ mrows, const float* __restrict__ A,
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
   const int T = (mrows[d] + 63) / 64;
   const int i0 = sl % nslot;
   if (i0 >= T) return;
-  const int rot = (i0 & ~1) % T;
+  const int rot = (i0 & ~1) % T;  // K chunk c covers tile column (c / CPT - rot) mod T
   const int c0 = blockIdx.z * 64;
   __shared__ __attribute__((aligned(16))) _Float16 Ah[64 * LDH], Al[64 * LDH], Xh[64 * LDH], Xl[64 * LDH];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
