@@ -9,8 +9,9 @@ relative dual residual.  Load cases: seeded random smoothed tractions on the x=1
 face (one load case = one right-hand side).
 
   * batched mixed precision (the headline `value`): 64 load cases per block,
-    F_i and S_i applied in FP32 on the matrix cores (FP32 accumulation),
-    everything else FP64, every case certified on its FP64 true residual;
+    K+_bb and S_i applied on the FP16 matrix cores in split FP16 (each value as
+    hi + lo FP16 with power-of-two scales, FP32 accumulation), everything else
+    FP64, every case certified on its FP64 true residual;
   * plain FP64 (`plain`): the same load cases solved one at a time.
 
 A step is one solve (a block of 64 cases, or one case for the plain form) against
@@ -46,13 +47,14 @@ SAMPLE = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="un
               poissons_ratio="0.3", load="body", body_force="-1.0", typeBc="ker",
               preconditioner="Dirichlet_explicit")
 HBM_PEAK_GBS = 8000.0
-FP32_MFMA_PEAK_TFS = 157.3
+FP16_MFMA_PEAK_TFS = 2500.0  # dense FP16 matrix cores, published
 SEED = 20240611
-# HBM bytes per launch of k_tgemm_f32 on the default workload, from rocprofv3 --pmc
+# HBM bytes per launch of k_tgemm_h16 on the default workload, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
 # the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
-# 7.317e9 fetched + 2.222e8 written (FP32 results).  Valid for the default workload only.
-GEMM_TRAFFIC_BYTES = 7.54e9
+# 7.249e9 fetched + 2.222e8 written (FP32 results), K chunk depth 64.  Valid for the
+# default workload only.
+GEMM_TRAFFIC_BYTES = 7.47e9
 
 
 def cpu_sample():
@@ -221,6 +223,8 @@ def main():
     ap.add_argument("--serve-edges", default="8,9,10,11,12", help="elements per subdomain edge of the served meshes")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0, help="HBM budget of the server")
     ap.add_argument("--serve-warm", action="store_true", help="set every mesh up before the stream starts")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the iteration kernels directly instead of replaying captured HIP graphs")
     ap.add_argument("--kernel-only", action="store_true",
                     help="setup, then only the dominant kernel (FP32 tile GEMM, 4 launches) and a device copy of "
                          "known size (4 launches; calibration of HBM traffic counters)")
@@ -241,6 +245,8 @@ def main():
     t0 = time.perf_counter()
     mesh = hfls_amd.Mesh(device=0, **cfg)
     setup_s = time.perf_counter() - t0
+    if a.no_graph:
+        mesh.configure(hip_graph=0)
     log("setup %.2f s (factor %.2f s), nlam %d, device bytes %.2f GB" %
         (setup_s, mesh.factor_ms / 1e3, mesh.nlam, mesh.device_bytes / 1e9))
     if a.kernel_only:
@@ -302,9 +308,12 @@ def main():
     rb = mesh.pcpg_block(F[a.warmup * a.k:a.warmup * a.k + 1], eps=a.eps, max_iter=1000, precision=32)
     agree = float(np.linalg.norm(rb["u"][0] - r1["u"]) / np.linalg.norm(r1["u"]))
 
-    # roofline of the dominant kernel: the FP32 tile GEMM over 64 columns
-    g_ms, g_flops = mesh.time_stream(2, reps=10)
-    s_ms, s_flops = mesh.time_stream(3, reps=10)
+    # roofline of the dominant kernel: the split-FP16 tile GEMM over 64 columns,
+    # bound by HBM (its matrix-core time at the published FP16 peak is a fraction
+    # of the time to stream the packed tiles once)
+    g_ms, g_bytes = mesh.time_stream(7, reps=10)
+    _, g_flops = mesh.time_stream(2, reps=1)
+    s_ms, s_bytes = mesh.time_stream(8, reps=10)
     # sustained matrix-core rates on register data with random operands (no memory
     # traffic): the ceiling the FP32 tile GEMM runs against on real data
     ceil = {}
@@ -312,7 +321,7 @@ def main():
         c_ms, c_fl = mesh.time_stream(which, reps=3)
         ceil[name] = c_fl / (c_ms * 1e-3) / 1e12
     tune = mesh.tuning()
-    achieved = g_flops / (g_ms * 1e-3) / 1e12
+    achieved = g_bytes / (g_ms * 1e-3) / 1e9
     lat = np.array(blk_ms)
     out = {
         "metric": "load-case solves/s at 1e-8 dual residual, batched mixed precision (64 cases/block) vs plain FP64",
@@ -327,11 +336,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f16x2+f64",
         "data": "synthetic (structured cube mesh and seeded face tractions generated in-process)",
         "config": {"workload": workload, "subdomains": mesh.nsub, "dual_rows": mesh.nlam,
                    "global_dof": 3 * mesh.nPoints, "load_cases_per_block": a.k, "eps": a.eps,
-                   "precision": "F_i, S_i FP32 on MFMA; factorisation, kernel, coarse, dots, projector FP64",
+                   "precision": "K+_bb, S_i as split FP16 (hi + lo, FP32 accumulation) on MFMA; factorisation, "
+                                "kernel, coarse, corrections, dots, projector, certification FP64",
                    "iterations_min_max": [int(its.min()), int(its.max())], "max_true_residual": float(max(res)),
                    "fp64_refinement_points_per_block": float(np.mean(certs)), "sweeps_per_block": float(np.mean(sweeps)),
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3)},
@@ -342,15 +352,16 @@ def main():
         "concurrent_blocks": conc,
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),
         "block_vs_plain_displacement_rel_diff": agree,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_MFMA_PEAK_TFS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
                      "traffic": GEMM_TRAFFIC_BYTES if (a.elements == 10 and not a.small) else None,
-                     "kernel": "k_tgemm_f32 (deflated K+_bb, packed symmetric 64x64 tiles, 64 columns)",
-                     "kernel_ms": g_ms, "flops_per_launch": g_flops,
-                     "S_gemm_TFs": s_flops / (s_ms * 1e-3) / 1e12,
+                     "kernel": "k_tgemm_h16 (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 columns)",
+                     "kernel_ms": g_ms, "algorithmic_bytes": g_bytes, "flops_per_launch": g_flops,
+                     "algorithmic_TFs": g_flops / (g_ms * 1e-3) / 1e12,
+                     "fp16_mfma_frac": 3.0 * g_flops / (g_ms * 1e-3) / 1e12 / FP16_MFMA_PEAK_TFS,
+                     "S_gemm_ms": s_ms, "S_gemm_GBs": s_bytes / (s_ms * 1e-3) / 1e9,
                      "measured_mfma_ceiling_TFs": ceil,
-                     "frac_of_measured_f32_ceiling": achieved / ceil["f32_32x32x2"],
-                     "autotune": {"row_tiles_per_workgroup": tune["row_tiles"],
+                     "autotune": {"k_chunk": 32 * tune["row_tiles"],
                                   "candidate_ms": [tune["ms"][1], tune["ms"][2]]}},
     }
     mesh.close()

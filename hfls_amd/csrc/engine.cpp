@@ -1300,6 +1300,18 @@ void Engine::kplus_local(const double* f_h, double* x_h) {
 
 double Engine::time_stream(int which, int reps, double* bytes) {
   if (which >= 2 && which <= 4) return time_block_gemm(which - 2, reps, bytes);
+  if (which == 7 || which == 8) {  // the same GEMMs; algorithmic HBM bytes of one launch
+    double fl = 0;
+    const double ms = time_block_gemm(which - 7, reps, &fl);
+    if (bytes) {
+      // packed lower tiles read once (4 B per element: FP32, or FP16 hi + lo), the
+      // FP32 right-hand sides read once and the FP32 results written once
+      double nb = 0;
+      for (int d = 0; d < P->nsub; d++) nb += h_ns[d];
+      *bytes = 4.0 * (double)off_P[P->nsub] + 2.0 * 4.0 * KMAX * nb;  // leading dimension KMAX
+    }
+    return ms;
+  }
   if (which == 5 || which == 6) {  // matrix-core ceilings, FP64 / FP32, on register data
     const int nwg = 8 * 256, iters = which == 5 ? 400 : 800;
     DVec<double> out;
@@ -1320,7 +1332,7 @@ double Engine::time_stream(int which, int reps, double* bytes) {
     if (bytes) *bytes = fl;
     return ms / reps;
   }
-  if (which < 0 || which > 6) throw std::runtime_error("time_stream: which must be 0..6");
+  if (which < 0 || which > 6) throw std::runtime_error("time_stream: which must be 0..8");
   const int N = P->nsub;
   const int64_t nl = P->nlam;
   std::vector<double> ones(nl, 1.0);

@@ -111,7 +111,8 @@ class Engine {
   // (which=1)) with HIP events on the library stream; returns ms per launch and
   // the algorithmic bytes one launch moves.
   double time_stream(int which, int reps, double* bytes);
-  // Time the FP32 tile GEMM of the block solve over 64 columns (which = 0: the
+  // Time the tile GEMM of the block solve (split-FP16 or FP32 format; the kernel
+  // alone, after an untimed call that sets the column scales) over 64 columns (which = 0: the
   // deflated K+_bb tiles, 1: S_i) with HIP events on the library stream; returns ms
   // per launch and the algorithmic flops of one launch (2 * 64 * sum n_s^2).
   // which = 2: a plain device copy of known size (*flops = bytes read = written).

@@ -237,7 +237,8 @@ hipError_t launch_split16(hipStream_t st, int nsub, int maxtiles, const int* ns,
 hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, const float* A, const int64_t* aoff,
                             const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
                             const int* gate, float* Yf, const int* xcd_order = nullptr, int norder = 0,
-                            int kc = 32);  // kc: K chunk depth staged per step (32 or 64)
+                            int kc = 32,     // kc: K chunk depth staged per step (32 or 64)
+                            int scales = 1); // scales 0: sx already holds the scales of this X
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
                                  const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
                                  const int* nlc, float* P32);

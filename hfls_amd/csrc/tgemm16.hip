@@ -254,11 +254,11 @@ hipError_t launch_split16(hipStream_t st, int nsub, int maxtiles, const int* ns,
 
 hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, const float* A, const int64_t* aoff,
                             const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
-                            const int* gate, float* Yf, const int* xcd_order, int norder, int kc) {
+                            const int* gate, float* Yf, const int* xcd_order, int norder, int kc, int scales) {
   if (nsub <= 0 || maxT <= 0) return hipSuccess;
   if (kp <= 0 || kp % 64 || (kc != 32 && kc != 64)) return hipErrorInvalidValue;
   if (xcd_order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(1024), 0, st, ns, xoff, kp, X, sx, gate);
+  if (scales) hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(1024), 0, st, ns, xoff, kp, X, sx, gate);
   const int groups = xcd_order ? norder / 8 : (nsub + 7) / 8;  // subdomain groups of 8, one per XCD
   if (kc == 64)
     hipLaunchKernelGGL(k_tgemm_h16<64>, dim3(8 * maxT * groups, 1, kp / 64), dim3(256), 0, st, nsub, maxT,

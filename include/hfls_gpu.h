@@ -182,8 +182,11 @@ int hfls_server_stats(hfls_server s, double* out);
  * the dominant kernel over reps back-to-back launches timed with HIP events on
  * the library's stream.  which = 0 / 1: the packed-tile GEMV streaming K+_bb
  * (inside mult_Ff) / S_i (inside Preconditioning), *bytes = algorithmic bytes of
- * one launch; which = 2 / 3: the FP32 tile GEMM of the block solve over 64
- * columns with K+_bb / S_i, *bytes = algorithmic flops of one launch; which = 4:
+ * one launch; which = 2 / 3: the tile GEMM of the mixed-precision block solve
+ * (split-FP16 or FP32 format, option block_gemm) over 64 columns with K+_bb / S_i,
+ * *bytes = algorithmic flops of one launch; which = 7 / 8: the same GEMMs, *bytes =
+ * algorithmic HBM bytes of one launch (packed tiles, right-hand sides and results
+ * each moved once); which = 4:
  * a plain device copy (reference for traffic counters), *bytes = bytes read;
  * which = 5 / 6: the sustained FP64 / FP32 matrix-core rate on register data with
  * random operands (no memory traffic), *bytes = flops of one launch. */

@@ -66,16 +66,24 @@ def test_autotuned_gemm_shapes_agree(name):
 
 def test_measurement_hooks():
     """The timing hook: every kind returns a positive time and work figure, the
-    matrix-core ceilings report their flops, and an unknown kind is an error."""
+    matrix-core ceilings report their flops, the block GEMM its algorithmic flops
+    (2 * 64 * sum n_s^2) and HBM bytes (packed 64x64 tiles at 4 B per element, 64
+    FP32 columns of right-hand sides and results per interface DOF), and an unknown
+    kind is an error."""
     m = hfls_amd.Mesh(**CASES["cfg0"])
     F = m.traction(79, 0, 2)
     m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
-    for which in range(7):
+    for which in range(9):
         ms, work = m.time_stream(which, reps=2)
         print("hook", which, ms, work, flush=True)
         assert ms > 0 and work > 0
     ms5, fl5 = m.time_stream(5, reps=1)
     assert fl5 == 4.0 * 8 * 256 * 400 * 8 * (2 * 16 * 16 * 4)
+    p = hfls_amd.Problem(**CASES["cfg0"])
+    ns = [p.sub_info(d)["n_s"] for d in range(p.nsub)]
+    tiles = sum(((n + 63) // 64) * ((n + 63) // 64 + 1) // 2 for n in ns)
+    assert m.time_stream(2, reps=1)[1] == sum(2.0 * 64 * n * n for n in ns)
+    assert m.time_stream(7, reps=1)[1] == 4.0 * 4096 * tiles + 2 * 4.0 * 64 * sum(ns)
     with pytest.raises(hfls_amd.HflsError):
-        m.time_stream(7, reps=1)
+        m.time_stream(9, reps=1)
     m.close()
