@@ -30,12 +30,13 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// power of two s with m / s in [2^14, 2^15) (1 for m == 0)
+// power of two s with m / s in [2^14, 2^15) (1 for m == 0), at least 2^-126 so that
+// 1 / s stays finite (values below 2^-111 then keep fewer significant bits)
 static __device__ __forceinline__ float pow2_scale(float m) {
   if (!(m > 0.0f)) return 1.0f;
   int e;
   (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
-  return ldexpf(1.0f, e - 15);
+  return ldexpf(1.0f, max(e - 15, -126));
 }
 
 static __device__ __forceinline__ float block_max256(float m, float* red) {
@@ -235,12 +236,13 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
     __syncthreads();
   }
   const int col = cb + (lane & 31);
-  const float f = alpha * sA[d] * sx[(int64_t)d * kp + c0 + col];
+  // the two scales applied one after the other: their product alone may underflow
+  const float fa = alpha * sA[d], fx = sx[(int64_t)d * kp + c0 + col];
   float* Yd = Yf + xoff[d] + c0;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int row = i0 * 64 + rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    Yd[(int64_t)row * kp + col] = f * acc[r];
+    Yd[(int64_t)row * kp + col] = (acc[r] * fa) * fx;
   }
 }
 

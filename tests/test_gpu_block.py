@@ -83,7 +83,7 @@ def test_matrix_core_formats(fmt):
     """K+_bb and S_i of the mixed-precision operators in both matrix-core formats
     (block_gemm: split FP16 hi + lo planes with power-of-two scales, or FP32): F and M at
     precision 32 within 2e-5 of FP64 per column, also for columns scaled by 1e-12 and 1e6
-    (per-column scales of the right-hand sides), and a mixed solve to eps 1e-8 within
+    (per-column scales of the right-hand sides) and, for M, 1e-35, and a mixed solve to eps 1e-8 within
     1e-7 of the FP64 solve at eps 1e-10."""
     kw = dict(UNEVEN, block_gemm=fmt)
     print("block_gemm", fmt, kw, flush=True)
@@ -100,6 +100,13 @@ def test_matrix_core_formats(fmt):
                 e = np.linalg.norm(A32[j] - A64[j]) / np.linalg.norm(A64[j])
                 print(fmt, "op", op, "column", j, "rel %.2e" % e, flush=True)
                 assert e <= 2e-5
+        # a column below 2^-111 (smallest power-of-two scale); through M its results
+        # stay in the FP32 normal range
+        Wt = W[:1] * 1e-35
+        e = np.linalg.norm(gpu.apply_block(1, Wt, 32)[0] - gpu.apply_block(1, Wt, 64)[0])
+        e /= np.linalg.norm(gpu.apply_block(1, Wt, 64)[0])
+        print(fmt, "op 1 column scaled 1e-35 rel %.2e" % e, flush=True)
+        assert e <= 2e-5
         F = gpu.traction(SEED, 0, 3)
         r = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=32)
         r64 = gpu.pcpg_block(F, eps=1e-10, max_iter=500, precision=64)
