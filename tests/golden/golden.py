@@ -7,12 +7,12 @@ Run from the repository root to (re)write the fixtures next to this file:
 Written from the oracle (oracle/, the CPU restatement of the reference, pinned by
 its own tests against closed forms and a global direct solve):
 
-* ``cfg0_integer.npz`` -- config 0 (2x2x2 subdomains of 5^3 hex8, x = 0 clamped):
+* ``cube8_5e3_integer.npz`` -- the 2x2x2 cube of 5^3-element subdomains (x = 0 clamped):
   Dirichlet and corner DOFs, the Bf row table (Cluster.cpp:846-924: subdomain and
   global DOF of each row's +1 and -1 entry, weights), per-subdomain l2g
   (Data.cpp:111-144), compressed Bf (rows, local columns, signs;
   Cluster.cpp:931-959) and j_col_cmpr (Cluster.cpp:756-765), as full arrays.
-* ``cfg1_integer_sha256.json`` -- the same arrays for config 1 (8 clusters x 4^3
+* ``cube512_10e3_integer_sha256.json`` -- the same arrays for the 512-subdomain cube (8 clusters x 4^3
   subdomains of 10^3 hex8), as SHA-256 digests of their little-endian bytes.
 * ``cube8_10e3_u.npz`` -- displacement of the 2x2x2 x 10^3 cube (x = 0 clamped,
   uniform E, body force) from the oracle's PCPG at eps 1e-10 (sparse LU K_reg,
@@ -34,10 +34,10 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-CFG0 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+CUBE8_5E3 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
             load="body", body_force="-1.0")
-CFG1 = dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4, nx=10, ny=10, nz=10, bc="clamp_x0")
-CUBE8 = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+CUBE512_10E3 = dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4, nx=10, ny=10, nz=10, bc="clamp_x0")
+CUBE8_10E3 = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
              load="body", body_force="-1.0")
 KEYS = ["dirichlet", "corners", "row_a_sub", "row_a_dof", "row_b_sub", "weight", "l2g", "l2g_ptr", "bf_rows",
         "bf_col", "bf_sign", "bf_ptr", "jcol", "jcol_ptr"]
@@ -105,16 +105,16 @@ def digests(arrays):
 
 
 def write_all():
-    a0 = oracle_arrays(CFG0)
-    np.savez_compressed(os.path.join(HERE, "cfg0_integer.npz"), **a0)
-    a1 = oracle_arrays(CFG1)
-    meta = {"options": CFG1, "nlam": int(len(a1["weight"])), "nsub": int(len(a1["l2g_ptr"]) - 1),
+    a0 = oracle_arrays(CUBE8_5E3)
+    np.savez_compressed(os.path.join(HERE, "cube8_5e3_integer.npz"), **a0)
+    a1 = oracle_arrays(CUBE512_10E3)
+    meta = {"options": CUBE512_10E3, "nlam": int(len(a1["weight"])), "nsub": int(len(a1["l2g_ptr"]) - 1),
             "sha256": digests(a1)}
-    with open(os.path.join(HERE, "cfg1_integer_sha256.json"), "w") as f:
+    with open(os.path.join(HERE, "cube512_10e3_integer_sha256.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     from oracle.mesh import opts
     from oracle.feti import HTFETI
-    h = HTFETI(opts(eps_iter="1e-10", **CUBE8), sparse=True)
+    h = HTFETI(opts(eps_iter="1e-10", **CUBE8_10E3), sparse=True)
     r = h.pcpg(eps=1e-10, max_iter=1000)
     r8 = h.pcpg(eps=1e-8, max_iter=1000)
     np.savez(os.path.join(HERE, "cube8_10e3_u.npz"), u=np.asarray(r["u"], dtype=np.float64),

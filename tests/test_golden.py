@@ -1,10 +1,10 @@
 """CPU checks against the committed golden fixtures (tests/golden/golden.py wrote
 them from the oracle).
 
-* config 0: the oracle and the product's host library both reproduce the full
+* the 2x2x2 x 5^3 cube: the oracle and the product's host library both reproduce the full
   integer arrays of the bit-exact contract (Dirichlet/corner DOFs, Bf row table and
   weights, l2g, compressed Bf, j_col_cmpr) bit for bit;
-* config 1 (512 subdomains): the product's arrays have the fixture's SHA-256
+* the 512-subdomain cube: the product's arrays have the fixture's SHA-256
   digests;
 * the 2x2x2 x 10^3 displacement fixture agrees with the global FP64 direct solve
   (1e-8 relative), which pins the fixture itself.
@@ -22,22 +22,22 @@ import golden  # noqa: E402
 
 
 def _fixture0():
-    with np.load(os.path.join(HERE, "cfg0_integer.npz")) as f:
+    with np.load(os.path.join(HERE, "cube8_5e3_integer.npz")) as f:
         return {k: f[k] for k in golden.KEYS}
 
 
 @pytest.mark.parametrize("source", ["oracle", "native"])
-def test_cfg0_integer_arrays(source):
+def test_cube8_5e3_integer_arrays(source):
     ref = _fixture0()
-    got = golden.oracle_arrays(golden.CFG0) if source == "oracle" else golden.native_arrays(golden.CFG0)
+    got = golden.oracle_arrays(golden.CUBE8_5E3) if source == "oracle" else golden.native_arrays(golden.CUBE8_5E3)
     assert len(ref["weight"]) == 1878  # 1,446 glue rows + 432 Dirichlet rows
     for k in golden.KEYS:
         assert got[k].dtype.kind == ref[k].dtype.kind, k
         assert np.array_equal(got[k], ref[k]), k
 
 
-def test_cfg1_native_digests():
-    with open(os.path.join(HERE, "cfg1_integer_sha256.json")) as f:
+def test_cube512_10e3_native_digests():
+    with open(os.path.join(HERE, "cube512_10e3_integer_sha256.json")) as f:
         meta = json.load(f)
     got = golden.native_arrays(meta["options"])
     assert len(got["weight"]) == meta["nlam"] == 588846 and len(got["l2g_ptr"]) - 1 == meta["nsub"] == 512
@@ -50,7 +50,7 @@ def test_cube8_displacement_fixture_matches_direct_solve():
     with np.load(os.path.join(HERE, "cube8_10e3_u.npz")) as f:
         u = f["u"]
         it8 = int(f["iterations_1e8"])
-    m = Mesh(opts(**golden.CUBE8))
+    m = Mesh(opts(**golden.CUBE8_10E3))
     u_ref, _, _ = global_solve(m)
     assert u.shape == u_ref.shape == (3 * 21 ** 3,)
     assert np.linalg.norm(u - u_ref) <= 1e-8 * np.linalg.norm(u_ref)

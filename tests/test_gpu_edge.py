@@ -14,14 +14,14 @@ import hfls_amd
 
 pytestmark = pytest.mark.gpu
 
-CFG0 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform",
+CUBE8_5E3 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform",
             young_modulus="2.1e5", load="body", body_force="-1.0")
 
 
 @pytest.fixture(scope="module")
 def mesh():
-    print("edge case mesh", CFG0, flush=True)
-    m = hfls_amd.Mesh(**CFG0)
+    print("edge case mesh", CUBE8_5E3, flush=True)
+    m = hfls_amd.Mesh(**CUBE8_5E3)
     yield m
     m.close()
 

@@ -1,4 +1,4 @@
-"""GPU solves of the 2x2x2 x 10^3 cube (the subdomain size of config 1) against the
+"""GPU solves of the 2x2x2 x 10^3 cube (the subdomain size of the 512-subdomain cube) against the
 committed oracle displacement (tests/golden/cube8_10e3_u.npz, eps 1e-10):
 
 * plain FP64 path at eps 1e-10: displacement within 1e-8 relative and |P g|/|P g0|
@@ -17,7 +17,7 @@ import hfls_amd
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CUBE8 = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+CUBE8_10E3 = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
              load="body", body_force="-1.0")
 
 
@@ -26,7 +26,7 @@ def cube():
     print("golden cube 2x2x2 x 10^3", flush=True)
     with np.load(os.path.join(HERE, "cube8_10e3_u.npz")) as f:
         u, it8 = f["u"].copy(), int(f["iterations_1e8"])
-    m = hfls_amd.Mesh(**CUBE8)
+    m = hfls_amd.Mesh(**CUBE8_10E3)
     yield m, u, it8
     m.close()
 

@@ -14,7 +14,7 @@ import hfls_amd
 pytestmark = pytest.mark.gpu
 
 CASES = {
-    "cfg0": dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
+    "cube8_5e3": dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
                  load="body"),
     "uneven": dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slabs_x="3,5,4,6", slabs_y="4,3", slabs_z="5,3",
                    bc="clamp_x0", material="lognormal", young_modulus="2.1e5", seed="9", mat_sigma="1.5",
@@ -70,7 +70,7 @@ def test_measurement_hooks():
     (2 * 64 * sum n_s^2) and HBM bytes (packed 64x64 tiles at 4 B per element, 64
     FP32 columns of right-hand sides and results per interface DOF), and an unknown
     kind is an error."""
-    m = hfls_amd.Mesh(**CASES["cfg0"])
+    m = hfls_amd.Mesh(**CASES["cube8_5e3"])
     F = m.traction(79, 0, 2)
     m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     for which in range(9):
@@ -79,7 +79,7 @@ def test_measurement_hooks():
         assert ms > 0 and work > 0
     ms5, fl5 = m.time_stream(5, reps=1)
     assert fl5 == 4.0 * 8 * 256 * 400 * 8 * (2 * 16 * 16 * 4)
-    p = hfls_amd.Problem(**CASES["cfg0"])
+    p = hfls_amd.Problem(**CASES["cube8_5e3"])
     ns = [p.sub_info(d)["n_s"] for d in range(p.nsub)]
     tiles = sum(((n + 63) // 64) * ((n + 63) // 64 + 1) // 2 for n in ns)
     assert m.time_stream(2, reps=1)[1] == sum(2.0 * 64 * n * n for n in ns)

@@ -19,16 +19,16 @@ from oracle.direct import global_solve
 
 pytestmark = pytest.mark.gpu
 
-CFG0 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform",
+CUBE8_5E3 = dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0", material="uniform",
             young_modulus="2.1e5", load="body", body_force="-1.0")
 MULTI = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, nx=3, ny=3, nz=4, bc="clamp_x0", material="lognormal",
              young_modulus="2.1e5", seed="3", mat_sigma="0.5", load="body")
 # uneven batching in small: slab sizes drawn from {2,3,4}, lognormal E (sigma 1.5, contrast 1e3)
 UNEVEN = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", bc="clamp_x0", material="lognormal",
               young_modulus="2.1e5", seed="5", mat_sigma="1.5", mat_contrast="1e3", load="body")
-CASES = {"cfg0": CFG0, "multi": MULTI, "refdefault": dict(nx=3, ny=3, nz=3, ratio_mat="10"), "uneven": UNEVEN,
+CASES = {"cube8_5e3": CUBE8_5E3, "multi": MULTI, "refdefault": dict(nx=3, ny=3, nz=3, ratio_mat="10"), "uneven": UNEVEN,
          # the other gluing and preconditioner of the reference (Cluster.cpp:846-903, 1106-1107)
-         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CFG0, preconditioner="lumped")}
+         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CUBE8_5E3, preconditioner="lumped")}
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))

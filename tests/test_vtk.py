@@ -11,7 +11,7 @@ import hfls_amd
 from oracle.mesh import Mesh, opts
 
 CASES = {
-    "cfg0": dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0"),
+    "cube8_5e3": dict(Nx=2, Ny=2, Nz=2, nx=5, ny=5, nz=5, bc="clamp_x0"),
     "clusters_uneven": dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", seed="5", bc="clamp_x0"),
 }
 
@@ -81,6 +81,6 @@ def test_vtk_matches_oracle_mesh(name, tmp_path):
 
 
 def test_vtk_rejects_wrong_size(tmp_path):
-    p = hfls_amd.Problem(**CASES["cfg0"])
+    p = hfls_amd.Problem(**CASES["cube8_5e3"])
     with pytest.raises(ValueError):
         p.write_vtk(np.zeros(5), tmp_path)
