@@ -42,6 +42,11 @@ def test_option_values_of_the_reference():
         hfls_amd.Problem(preconditioner="Dirichlet")
     with pytest.raises(hfls_amd.HflsError, match="unknown typeBc"):
         hfls_amd.Problem(typeBc="corners")
+    # matrix-core format of the mixed-precision block GEMM (no counterpart in the reference)
+    for v in ("f16x2", "f32"):
+        assert hfls_amd.Problem(block_gemm=v, nx=2, ny=2, nz=2).nsub == 8
+    with pytest.raises(hfls_amd.HflsError, match="unknown block_gemm"):
+        hfls_amd.Problem(block_gemm="bf16")
 
 
 CASES = [
