@@ -24,6 +24,8 @@ EXPORTS = [
     "hfls_server_destroy", "hfls_server_sizes", "hfls_server_submit", "hfls_server_step", "hfls_server_state",
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
     "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk", "hfls_solve_blocks",
+    "hfls_server_cancel", "hfls_mult_Gf", "hfls_mult_GfT", "hfls_mult_RfT", "hfls_recover", "hfls_mesh_rf",
+    "hfls_mesh_counts",
 ]
 
 DEFAULTS = {
@@ -276,6 +278,49 @@ class Mesh:
         _check(load().hfls_kplus(self._h, _ptr(f), _ptr(out)))
         return out
 
+    def mult_Gf(self, alpha):
+        """Gf alpha (Cluster.cpp:1275-1283); alpha has 6 entries per cluster."""
+        alpha = _vec(alpha, self.ncol, "alpha")
+        out = np.zeros(self.nlam)
+        _check(load().hfls_mult_Gf(self._h, _ptr(alpha), _ptr(out)))
+        return out
+
+    def mult_GfT(self, lam):
+        """Gf^T lambda (Cluster.cpp:1285-1293)."""
+        lam = _vec(lam, self.nlam, "lambda")
+        out = np.zeros(self.ncol)
+        _check(load().hfls_mult_GfT(self._h, _ptr(lam), _ptr(out)))
+        return out
+
+    def mult_RfT(self, f_packed):
+        """sum_d Rf_d^T f_d (Cluster.cpp:1295-1312), f packed local."""
+        f = _vec(f_packed, self.npacked, "f_packed")
+        out = np.zeros(self.ncol)
+        _check(load().hfls_mult_RfT(self._h, _ptr(f), _ptr(out)))
+        return out
+
+    def recover(self, f_packed, lam, alpha):
+        """Recovery of printVTK (Cluster.cpp:1648-1678): (x packed local, u global)
+        with x_d = K_c^+ (f - Bf^T lambda)_d - Rf_d alpha, alpha as from projection()."""
+        f = _vec(f_packed, self.npacked, "f_packed")
+        lam = _vec(lam, self.nlam, "lambda")
+        alpha = _vec(alpha, self.ncol, "alpha")
+        x, u = np.zeros(self.npacked), np.zeros(3 * self.nPoints)
+        _check(load().hfls_recover(self._h, _ptr(f), _ptr(lam), _ptr(alpha), _ptr(x), _ptr(u)))
+        return x, u
+
+    def Rf(self):
+        """Rf_d = R_d H_d per subdomain, packed local n_d x 6 column major."""
+        out = np.zeros(6 * self.npacked)
+        _check(load().hfls_mesh_rf(self._h, _ptr(out)))
+        return out
+
+    def counts(self):
+        """The Cluster getters (Cluster.hpp:26-31)."""
+        c = np.zeros(6, dtype=np.int64)
+        _check(load().hfls_mesh_counts(self._h, _ptr(c)))
+        return dict(zip(["nSubClst", "neqClst", "nLam_c", "nLam_f", "nRBM_f", "nRBM_c"], (int(x) for x in c)))
+
     def time_stream(self, which=0, reps=20):
         ms, nbytes = ctypes.c_double(), ctypes.c_double()
         _check(load().hfls_time_stream(self._h, int(which), int(reps), ctypes.byref(ms), ctypes.byref(nbytes)))
@@ -316,7 +361,8 @@ class Mesh:
     def pcpg_block(self, F=None, k=None, eps=1e-8, max_iter=200, precision=32, want_lambda=False):
         """Block of load cases: F (k, npacked) packed local loads, or None for k
         copies of the body-force load.  Returns u (k, 3*nPoints), per-case
-        iterations/status/residual and the block timings."""
+        iterations/status/residual and the block timings.  Status per case: 2
+        converged, 3 iteration limit, 6 stopped on a non-finite residual."""
         if F is not None:
             F = _rows(F, self.npacked, "F")
             k = F.shape[0]
@@ -407,18 +453,32 @@ class Server:
         return s.value
 
     def result(self, rid):
+        """Displacements and per-case stats of a finished request (status 2
+        converged, 3 iteration limit, 6 non-finite residual).  A request whose mesh
+        set-up failed raises HflsError (its per-case status, 7, is on the error's
+        ``status`` attribute)."""
         k, ng = self._k.pop(rid)
         u = np.zeros((k, ng))
         cs = (CaseStats * k)()
         lat = np.zeros(2)
-        _check(load().hfls_server_result(self._h, ctypes.c_int64(rid), _ptr(u), cs, _ptr(lat)))
+        rc = load().hfls_server_result(self._h, ctypes.c_int64(rid), _ptr(u), cs, _ptr(lat))
+        if rc != 0:
+            err = HflsError(load().hfls_last_error().decode())
+            err.status = np.array([c.status for c in cs])
+            raise err
         return {"u": u, "iterations": np.array([c.iterations for c in cs]),
                 "status": np.array([c.status for c in cs]), "residual": np.array([c.residual for c in cs]),
                 "latency_ms": float(lat[0]), "queue_ms": float(lat[1])}
 
+    def cancel(self, rid):
+        """Drop a request: queued cases removed, running cases leave their slots."""
+        _check(load().hfls_server_cancel(self._h, ctypes.c_int64(rid)))
+        self._k.pop(rid, None)
+
     def stats(self):
-        out = np.zeros(8)
+        out = np.zeros(10)
         _check(load().hfls_server_stats(self._h, _ptr(out)))
-        keys = ["meshes", "bytes", "setups", "evictions", "cases_done", "requests_done", "joins", "setup_ms"]
+        keys = ["meshes", "bytes", "setups", "evictions", "cases_done", "requests_done", "joins", "setup_ms",
+                "failed_setups", "cancelled"]
         return dict(zip(keys, out.tolist()))
 

@@ -582,7 +582,8 @@ hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int 
 // Per-column reductions of the partial dots (fixed order) and the PCPG scalar
 // logic (Cluster.cpp:1398-1448).  Column status: 0 running, 1 waiting for an FP64
 // true-residual evaluation, 2 converged, 3 iteration limit, 4 empty slot,
-// 5 joining (initialised by mode 2).
+// 5 joining (initialised by mode 2), 6 stopped on a non-finite residual (the
+// column is frozen like a converged one; err records that some column hit it).
 //   mode 0: wFw, rho = -gPz / wFw
 //   mode 1: gamma, iteration count; the stopping test sqrt(gPz) < eps*norm0 gives
 //           status 2 (status 1 when certification is on), a recursive residual
@@ -634,9 +635,11 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->gPz[j] = s;
         sc->gamma[j] = s / sc->gPz_prev[j];
         sc->it[j] += 1;
-        if (!(s == s)) sc->err = 1;
         const double r = sqrt(fabs(s));
-        if (!(r >= sc->eps * sc->norm0[j])) sc->done[j] = sc->cert ? 1 : 2;
+        if (!isfinite(s)) {
+          sc->done[j] = 6;
+          sc->err = 1;
+        } else if (!(r >= sc->eps * sc->norm0[j])) sc->done[j] = sc->cert ? 1 : 2;
         else if (r < sc->rtol * sc->ref[j]) sc->done[j] = 1;
         else if (sc->it[j] >= sc->max_iter) sc->done[j] = 3;
       }
@@ -659,8 +662,8 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->ncert[j] = 0;
         sc->hgok[j] = 0;
         sc->waitn[j] = 0;
-        if (!(s == s)) sc->err = 1;
-        sc->done[j] = !(s > 0.0) ? 2 : (sc->max_iter <= 0 ? 3 : 0);
+        if (!isfinite(s)) sc->err = 1;
+        sc->done[j] = !isfinite(s) ? 6 : (!(s > 0.0) ? 2 : (sc->max_iter <= 0 ? 3 : 0));
       }
     } else if (mode == 3) {
       sc->repl[j] = 0;
@@ -670,7 +673,10 @@ __global__ __launch_bounds__(64) void kb_reduce(int k, BScalars* sc, int mode, c
         sc->res[j] = sc->norm0[j] > 0 ? r / sc->norm0[j] : 0.0;
         sc->ncert[j] += 1;
         sc->waitn[j] = 0;
-        if (r < sc->eps * sc->norm0[j]) {
+        if (!isfinite(s)) {
+          sc->done[j] = 6;
+          sc->err = 1;
+        } else if (r < sc->eps * sc->norm0[j]) {
           sc->done[j] = 2;
           sc->certnow[j] = 1;
           sc->hgok[j] = 1;

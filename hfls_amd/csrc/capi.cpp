@@ -1,5 +1,6 @@
 // C ABI (include/hfls_gpu.h) over the host problem and the device engine.
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <stdexcept>
 #include <string>
@@ -38,9 +39,30 @@ static int guard(F&& f) {
   }
 }
 
+// the object behind a handle; a NULL handle is an error, not a crash
+static hfls::Problem* prob(hfls_problem h) {
+  if (!h || !h->p) throw std::runtime_error("null problem handle");
+  return h->p;
+}
+static hfls::Engine* eng(hfls_mesh m) {
+  if (!m || !m->e) throw std::runtime_error("null mesh handle");
+  return m->e;
+}
+static hfls::Server* srv(hfls_server s) {
+  if (!s || !s->s) throw std::runtime_error("null server handle");
+  return s->s;
+}
+template <class T>
+static T* need(T* p, const char* what) {
+  if (!p) throw std::runtime_error(std::string("null argument: ") + what);
+  return p;
+}
+
 static hfls::Options make_options(int n, const char* const* k, const char* const* v) {
   hfls::Options o = hfls::default_options();
+  if (n > 0 && (!k || !v)) throw std::runtime_error("null option arrays");
   for (int i = 0; i < n; i++) {
+    if (!k[i] || !v[i]) throw std::runtime_error("null option key or value");
     if (!o.count(k[i])) throw std::runtime_error(std::string("unknown option ") + k[i]);
     o[k[i]] = v[i];
   }
@@ -49,8 +71,10 @@ static hfls::Options make_options(int n, const char* const* k, const char* const
 
 extern "C" int hfls_problem_create(int nopt, const char* const* keys, const char* const* vals, hfls_problem* out) {
   return guard([&] {
-    auto* h = new hfls_problem_s;
-    h->p = new hfls::Problem(make_options(nopt, keys, vals));
+    need(out, "out");
+    std::unique_ptr<hfls::Problem> p(new hfls::Problem(make_options(nopt, keys, vals)));
+    auto* h = new hfls_problem_s;  // after the problem is built: nothing to leak if it throws
+    h->p = p.release();
     *out = h;
   });
 }
@@ -66,19 +90,21 @@ extern "C" int hfls_problem_destroy(hfls_problem p) {
 
 extern "C" int hfls_problem_sizes(hfls_problem h, int64_t* s) {
   return guard([&] {
-    const hfls::Problem& p = *h->p;
+    need(s, "sizes");
+    const hfls::Problem& p = *prob(h);
     s[0] = p.nsub; s[1] = p.nlam; s[2] = p.n_glue; s[3] = p.nPoints; s[4] = p.ncl; s[5] = p.nsub_cl;
     s[6] = (int64_t)p.dirichlet_dofs.size(); s[7] = (int64_t)p.corner_dofs.size();
   });
 }
 
 static const hfls::Subdomain& sub(hfls_problem h, int d) {
-  if (d < 0 || d >= h->p->nsub) throw std::runtime_error("subdomain index out of range");
-  return h->p->subs[d];
+  if (d < 0 || d >= prob(h)->nsub) throw std::runtime_error("subdomain index out of range");
+  return prob(h)->subs[d];
 }
 
 extern "C" int hfls_problem_sub_info(hfls_problem h, int d, int* info) {
   return guard([&] {
+    need(info, "info");
     const hfls::Subdomain& s = sub(h, d);
     info[0] = s.n; info[1] = s.n_r; info[2] = s.n_s; info[3] = s.n_bs; info[4] = s.n_ss; info[5] = s.cluster;
     info[6] = (int)s.bf_row.size(); info[7] = 0;
@@ -87,6 +113,7 @@ extern "C" int hfls_problem_sub_info(hfls_problem h, int d, int* info) {
 
 extern "C" int hfls_problem_sub_bf(hfls_problem h, int d, int* rows, int* dofs, int* signs) {
   return guard([&] {
+    need(rows, "rows"); need(dofs, "dofs"); need(signs, "signs");
     const hfls::Subdomain& s = sub(h, d);
     for (size_t i = 0; i < s.bf_row.size(); i++) {
       rows[i] = s.bf_row[i];
@@ -98,6 +125,7 @@ extern "C" int hfls_problem_sub_bf(hfls_problem h, int d, int* rows, int* dofs, 
 
 extern "C" int hfls_problem_sub_bdofs(hfls_problem h, int d, int* b) {
   return guard([&] {
+    need(b, "bdofs");
     const hfls::Subdomain& s = sub(h, d);
     std::memcpy(b, s.bdofs.data(), s.bdofs.size() * sizeof(int));
   });
@@ -105,6 +133,7 @@ extern "C" int hfls_problem_sub_bdofs(hfls_problem h, int d, int* b) {
 
 extern "C" int hfls_problem_sub_sset(hfls_problem h, int d, int* b) {
   return guard([&] {
+    need(b, "sset");
     const hfls::Subdomain& s = sub(h, d);
     std::memcpy(b, s.sset.data(), s.sset.size() * sizeof(int));
   });
@@ -112,7 +141,8 @@ extern "C" int hfls_problem_sub_sset(hfls_problem h, int d, int* b) {
 
 extern "C" int hfls_problem_sub_l2g(hfls_problem h, int d, int64_t* l2g) {
   return guard([&] {
-    const hfls::Problem& p = *h->p;
+    need(l2g, "l2g");
+    const hfls::Problem& p = *prob(h);
     const hfls::Subdomain& s = sub(h, d);
     const int64_t nx1 = p.NE[0] + 1, nxy1 = nx1 * (p.NE[1] + 1);
     for (int l = 0; l < s.n; l++) {
@@ -125,19 +155,19 @@ extern "C" int hfls_problem_sub_l2g(hfls_problem h, int d, int64_t* l2g) {
 }
 
 extern "C" int hfls_problem_sub_young(hfls_problem h, int d, double* E) {
-  return guard([&] { *E = sub(h, d).E; });
+  return guard([&] { *need(E, "E") = sub(h, d).E; });
 }
 
 extern "C" int hfls_problem_write_vtk(hfls_problem h, const double* u, const char* folder, int iter) {
   return guard([&] {
     if (!u || !folder) throw std::runtime_error("null argument");
-    hfls::write_vtk(*h->p, u, folder, iter);
+    hfls::write_vtk(*prob(h), u, folder, iter);
   });
 }
 
 extern "C" int hfls_problem_rows(hfls_problem h, int* a, int* b, int64_t* dof, double* w) {
   return guard([&] {
-    const hfls::Problem& p = *h->p;
+    const hfls::Problem& p = *prob(h);
     for (int64_t r = 0; r < p.nlam; r++) {
       if (a) a[r] = p.row_sub_a[r];
       if (b) b[r] = p.row_sub_b[r];
@@ -149,14 +179,16 @@ extern "C" int hfls_problem_rows(hfls_problem h, int* a, int* b, int64_t* dof, d
 
 extern "C" int hfls_problem_dirichlet(hfls_problem h, int64_t* dofs) {
   return guard([&] {
-    const auto& v = h->p->dirichlet_dofs;
+    need(dofs, "dofs");
+    const auto& v = prob(h)->dirichlet_dofs;
     std::memcpy(dofs, v.data(), v.size() * sizeof(int64_t));
   });
 }
 
 extern "C" int hfls_problem_corners(hfls_problem h, int64_t* dofs) {
   return guard([&] {
-    const auto& v = h->p->corner_dofs;
+    need(dofs, "dofs");
+    const auto& v = prob(h)->corner_dofs;
     std::memcpy(dofs, v.data(), v.size() * sizeof(int64_t));
   });
 }
@@ -165,16 +197,12 @@ extern "C" int hfls_problem_corners(hfls_problem h, int64_t* dofs) {
 extern "C" int hfls_mesh_create(int device, int nopt, const char* const* keys, const char* const* vals,
                                 hfls_mesh* out) {
   return guard([&] {
+    need(out, "out");
     hfls::Options o = make_options(nopt, keys, vals);
-    auto* e = new hfls::Engine(o, device);
-    try {
-      e->setup();
-    } catch (...) {
-      delete e;
-      throw;
-    }
+    std::unique_ptr<hfls::Engine> e(new hfls::Engine(o, device));
+    e->setup();
     auto* h = new hfls_mesh_s;
-    h->e = e;
+    h->e = e.release();
     *out = h;
   });
 }
@@ -190,7 +218,8 @@ extern "C" int hfls_mesh_destroy(hfls_mesh m) {
 
 extern "C" int hfls_mesh_sizes(hfls_mesh m, int64_t* s) {
   return guard([&] {
-    const hfls::Engine& e = *m->e;
+    need(s, "sizes");
+    const hfls::Engine& e = *eng(m);
     s[0] = e.problem().nsub; s[1] = e.nlam(); s[2] = e.ncol_G(); s[3] = e.packed_n();
     s[4] = e.problem().nPoints; s[5] = (int64_t)e.setup_info().device_bytes;
   });
@@ -198,7 +227,8 @@ extern "C" int hfls_mesh_sizes(hfls_mesh m, int64_t* s) {
 
 extern "C" int hfls_mesh_setup_times(hfls_mesh m, double* t) {
   return guard([&] {
-    const hfls::SetupInfo& i = m->e->setup_info();
+    need(t, "times");
+    const hfls::SetupInfo& i = eng(m)->setup_info();
     t[0] = i.setup_ms;
     t[1] = i.factor_ms;
     t[2] = i.meta_ms;
@@ -210,7 +240,7 @@ extern "C" int hfls_mesh_setup_times(hfls_mesh m, double* t) {
 extern "C" int hfls_mesh_write_vtk(hfls_mesh m, const double* u, const char* folder, int iter) {
   return guard([&] {
     if (!u || !folder) throw std::runtime_error("null argument");
-    hfls::write_vtk(m->e->problem(), u, folder, iter);
+    hfls::write_vtk(eng(m)->problem(), u, folder, iter);
   });
 }
 
@@ -219,15 +249,15 @@ extern "C" int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect) {
     std::vector<double> r;
     std::vector<int> f, d;
     if (R) {
-      m->e->get_R(r);
+      eng(m)->get_R(r);
       std::memcpy(R, r.data(), r.size() * sizeof(double));
     }
     if (fix) {
-      m->e->get_fix(f);
+      eng(m)->get_fix(f);
       std::memcpy(fix, f.data(), f.size() * sizeof(int));
     }
     if (defect) {
-      m->e->get_defect(d);
+      eng(m)->get_defect(d);
       std::memcpy(defect, d.data(), d.size() * sizeof(int));
     }
   });
@@ -235,33 +265,72 @@ extern "C" int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect) {
 
 extern "C" int hfls_mesh_load(hfls_mesh m, double* f) {
   return guard([&] {
+    need(f, "f");
     std::vector<double> v;
-    m->e->get_load(v);
+    eng(m)->get_load(v);
     std::memcpy(f, v.data(), v.size() * sizeof(double));
   });
 }
 
 extern "C" int hfls_apply_F(hfls_mesh m, const double* w, double* Fw) {
-  return guard([&] { m->e->apply_F(w, Fw); });
+  return guard([&] { eng(m)->apply_F(need(w, "w"), need(Fw, "Fw")); });
 }
 
 extern "C" int hfls_project(hfls_mesh m, const double* x, double* Px, double* alpha) {
-  return guard([&] { m->e->project(x, Px, alpha); });
+  return guard([&] { eng(m)->project(need(x, "x"), need(Px, "Px"), alpha); });
 }
 
 extern "C" int hfls_precondition(hfls_mesh m, const double* x, double* Mx) {
-  return guard([&] { m->e->precondition(x, Mx); });
+  return guard([&] { eng(m)->precondition(need(x, "x"), need(Mx, "Mx")); });
 }
 
 extern "C" int hfls_kplus(hfls_mesh m, const double* f, double* x) {
-  return guard([&] { m->e->kplus_local(f, x); });
+  return guard([&] { eng(m)->kplus_local(need(f, "f"), need(x, "x")); });
+}
+
+extern "C" int hfls_mult_Gf(hfls_mesh m, const double* alpha, double* lam) {
+  return guard([&] { eng(m)->mult_Gf(need(alpha, "alpha"), need(lam, "lambda")); });
+}
+
+extern "C" int hfls_mult_GfT(hfls_mesh m, const double* lam, double* e) {
+  return guard([&] { eng(m)->mult_GfT(need(lam, "lambda"), need(e, "e")); });
+}
+
+extern "C" int hfls_mult_RfT(hfls_mesh m, const double* f, double* e) {
+  return guard([&] { eng(m)->mult_RfT(need(f, "f_local"), need(e, "e")); });
+}
+
+extern "C" int hfls_recover(hfls_mesh m, const double* f, const double* lam, const double* alpha, double* x_local,
+                            double* u) {
+  return guard([&] { eng(m)->recover(need(f, "f_local"), need(lam, "lambda"), need(alpha, "alpha"), x_local, u); });
+}
+
+extern "C" int hfls_mesh_rf(hfls_mesh m, double* Rf) {
+  return guard([&] {
+    std::vector<double> r;
+    eng(m)->get_Rf(r);
+    std::memcpy(need(Rf, "Rf"), r.data(), r.size() * sizeof(double));
+  });
+}
+
+extern "C" int hfls_mesh_counts(hfls_mesh m, int64_t* c) {
+  return guard([&] {
+    need(c, "counts");
+    const hfls::Engine& e = *eng(m);
+    c[0] = e.problem().nsub;
+    c[1] = e.packed_n();
+    c[2] = e.nlam_c();
+    c[3] = e.nlam();
+    c[4] = e.ncol_G();
+    c[5] = 6 * (int64_t)e.problem().nsub;
+  });
 }
 
 extern "C" int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f, double* u, double* lam,
                           hfls_stats* stats) {
   return guard([&] {
     hfls::SolveInfo info;
-    m->e->solve(eps, max_iter, f, u, lam, &info);
+    eng(m)->solve(eps, max_iter, f, u, lam, &info);
     if (stats) {
       stats->iterations = info.iterations;
       stats->converged = info.converged;
@@ -274,7 +343,7 @@ extern "C" int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f
 }
 
 extern "C" int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes) {
-  return guard([&] { *ms = m->e->time_stream(which, reps, bytes); });
+  return guard([&] { *need(ms, "ms") = eng(m)->time_stream(which, reps, bytes); });
 }
 
 extern "C" int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, int precision, const double* f,
@@ -284,7 +353,7 @@ extern "C" int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, in
     std::vector<int> it(k), stt(k);
     std::vector<double> res(k);
     hfls::BlockInfo info;
-    m->e->solve_block(eps, max_iter, k, precision, f, u, lam, it.data(), stt.data(), res.data(), &info);
+    eng(m)->solve_block(eps, max_iter, k, precision, f, u, lam, it.data(), stt.data(), res.data(), &info);
     if (cases)
       for (int j = 0; j < k; j++) {
         cases[j].iterations = it[j];
@@ -312,7 +381,7 @@ extern "C" int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblo
     std::vector<int> it(kt), stt(kt);
     std::vector<double> res(kt);
     hfls::BlockInfo info;
-    m->e->solve_blocks(nblocks, eps, max_iter, k, precision, f, u, it.data(), stt.data(), res.data(), &info);
+    eng(m)->solve_blocks(nblocks, eps, max_iter, k, precision, f, u, it.data(), stt.data(), res.data(), &info);
     if (cases)
       for (size_t j = 0; j < kt; j++) {
         cases[j].iterations = it[j];
@@ -331,20 +400,20 @@ extern "C" int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblo
 }
 
 extern "C" int hfls_apply_block(hfls_mesh m, int op, int k, int precision, const double* x, double* y) {
-  return guard([&] { m->e->apply_block(op, k, precision, x, y); });
+  return guard([&] { eng(m)->apply_block(op, k, precision, need(x, "x"), need(y, "y")); });
 }
 
 extern "C" int hfls_problem_traction(hfls_problem p, uint64_t seed, int case0, int k, double* f) {
   return guard([&] {
     if (k < 1 || case0 < 0) throw std::runtime_error("invalid load-case range");
-    hfls::traction_loads(*p->p, seed, case0, k, f);
+    hfls::traction_loads(*prob(p), seed, case0, k, need(f, "f"));
   });
 }
 
 extern "C" int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, double* f) {
   return guard([&] {
     if (k < 1 || case0 < 0) throw std::runtime_error("invalid load-case range");
-    hfls::traction_loads(m->e->problem(), seed, case0, k, f);
+    hfls::traction_loads(eng(m)->problem(), seed, case0, k, need(f, "f"));
   });
 }
 
@@ -352,14 +421,12 @@ extern "C" int hfls_mesh_traction(hfls_mesh m, uint64_t seed, int case0, int k, 
 extern "C" int hfls_server_create(int device, double budget_gb, int slots, int precision, double eps, int max_iter,
                                   int chunk, hfls_server* out) {
   return guard([&] {
+    need(out, "out");
     if (!(budget_gb > 0)) throw std::runtime_error("HBM budget must be positive");
+    std::unique_ptr<hfls::Server> sv(
+        new hfls::Server(device, (size_t)(budget_gb * 1e9), slots, precision, eps, max_iter, chunk));
     auto* h = new hfls_server_s;
-    try {
-      h->s = new hfls::Server(device, (size_t)(budget_gb * 1e9), slots, precision, eps, max_iter, chunk);
-    } catch (...) {
-      delete h;
-      throw;
-    }
+    h->s = sv.release();
     *out = h;
   });
 }
@@ -375,70 +442,86 @@ extern "C" int hfls_server_destroy(hfls_server s) {
 
 extern "C" int hfls_server_sizes(hfls_server s, int nopt, const char* const* keys, const char* const* vals,
                                  int64_t* npacked, int64_t* nglob, int64_t* bytes) {
-  return guard([&] { s->s->sizes(make_options(nopt, keys, vals), npacked, nglob, bytes); });
+  return guard([&] { srv(s)->sizes(make_options(nopt, keys, vals), npacked, nglob, bytes); });
 }
 
 extern "C" int hfls_server_submit(hfls_server s, int nopt, const char* const* keys, const char* const* vals, int k,
                                   const double* f, uint64_t seed, int case0, int64_t* id) {
-  return guard([&] { *id = s->s->submit(make_options(nopt, keys, vals), k, f, seed, case0); });
+  return guard([&] { *need(id, "id") = srv(s)->submit(make_options(nopt, keys, vals), k, f, seed, case0); });
 }
 
 extern "C" int hfls_server_step(hfls_server s, int* busy) {
   return guard([&] {
-    const bool b = s->s->step();
+    const bool b = srv(s)->step();
     if (busy) *busy = b ? 1 : 0;
   });
 }
 
 extern "C" int hfls_server_state(hfls_server s, int64_t id, int* state) {
-  return guard([&] { *state = s->s->state(id); });
+  return guard([&] { *need(state, "state") = srv(s)->state(id); });
 }
 
 extern "C" int hfls_server_result(hfls_server s, int64_t id, double* u, hfls_case_stats* cases, double* latency_ms) {
   return guard([&] {
-    const int k = s->s->request_k(id);
+    const int k = srv(s)->request_k(id);
     std::vector<int> it(k), stt(k);
     std::vector<double> res(k);
     double lat[2] = {0, 0};
-    s->s->result(id, u, it.data(), stt.data(), res.data(), lat, nullptr);
-    if (cases)
-      for (int j = 0; j < k; j++) {
-        cases[j].iterations = it[j];
-        cases[j].status = stt[j];
-        cases[j].residual = res[j];
+    auto copy_out = [&] {
+      if (cases)
+        for (int j = 0; j < k; j++) {
+          cases[j].iterations = it[j];
+          cases[j].status = stt[j];
+          cases[j].residual = res[j];
+        }
+      if (latency_ms) {
+        latency_ms[0] = lat[0];
+        latency_ms[1] = lat[1];
       }
-    if (latency_ms) {
-      latency_ms[0] = lat[0];
-      latency_ms[1] = lat[1];
+    };
+    try {
+      srv(s)->result(id, u, it.data(), stt.data(), res.data(), lat, nullptr);
+    } catch (...) {
+      copy_out();  // a failed set-up: statuses 7, then the error
+      throw;
     }
+    copy_out();
   });
+}
+
+extern "C" int hfls_server_cancel(hfls_server s, int64_t id) {
+  return guard([&] { srv(s)->cancel(id); });
 }
 
 extern "C" int hfls_server_stats(hfls_server s, double* out) {
   return guard([&] {
-    const hfls::ServeStats& t = s->s->stats();
-    out[0] = s->s->meshes();
-    out[1] = (double)s->s->bytes_in_use();
+    need(out, "out");
+    const hfls::ServeStats& t = srv(s)->stats();
+    out[0] = srv(s)->meshes();
+    out[1] = (double)srv(s)->bytes_in_use();
     out[2] = (double)t.setups;
     out[3] = (double)t.evictions;
     out[4] = (double)t.cases_done;
     out[5] = (double)t.requests_done;
     out[6] = (double)t.joins;
     out[7] = t.setup_ms;
+    out[8] = (double)t.failed_setups;
+    out[9] = (double)t.cancelled;
   });
 }
 
 extern "C" int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value) {
   return guard([&] {
     if (!key || !value) throw std::runtime_error("null setting");
-    m->e->configure(key, value);
+    eng(m)->configure(key, value);
   });
 }
 
 extern "C" int hfls_mesh_tuning(hfls_mesh m, double* out) {
   return guard([&] {
-    out[0] = m->e->tuned_rows();
-    out[1] = m->e->tuned_ms(1);
-    out[2] = m->e->tuned_ms(2);
+    need(out, "out");
+    out[0] = eng(m)->tuned_rows();
+    out[1] = eng(m)->tuned_ms(1);
+    out[2] = eng(m)->tuned_ms(2);
   });
 }

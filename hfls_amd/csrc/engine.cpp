@@ -12,6 +12,7 @@
 // so F_i = B_i K_reg^-1 B_i^T is applied as B_i (K+_bb (B_i^T w)), streaming the
 // n_s x n_s block instead of an nlam_i x nlam_i one.
 #include "engine.hpp"
+#include "vtk.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -135,6 +136,12 @@ static void chol_solve(int n, const std::vector<double>& L, double* b) {
 // Gram-Schmidt.  The s-set is the 6 columns of up to 8 subdomains spread over the
 // cluster (deterministic instead of the reference's time-seeded shuffle).
 static std::vector<double> kernel_small(int n, const std::vector<double>& K, int nsub_cl, int* defect) {
+  if (nsub_cl == 1) {  // one subdomain: no cluster constraints, G_c^T G_c = 0, kernel = I (6)
+    *defect = n;
+    std::vector<double> I((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.0;
+    return I;
+  }
   std::vector<double> dg(n);
   for (int i = 0; i < n; i++) dg[i] = std::sqrt(K[(size_t)i * n + i]);
   std::vector<int> sub;
@@ -627,6 +634,9 @@ void Engine::factor_subdomains() {
       r0s[i] = s.n_r;
       for (int a = 0; a < 3; a++) { asub[i].e0[a] = s.e0[a]; asub[i].ne[a] = s.ne[a]; }
       asub[i].n = s.n; asub[i].nr = 0; asub[i].E = s.E;  // plain K (no identity coupling)
+      for (int a = 0; a < 3; a++) asub[i].mn[a] = P->mat_per_element ? P->geo_n[a] : 0;
+      asub[i].mN[0] = P->geo_N[0]; asub[i].mN[1] = P->geo_N[1];
+      asub[i].E1 = P->mat_E1;
     }
     // host-side extent checks of every block a kernel of this chunk touches:
     // (offset, ld, rows, cols) must stay inside its buffer
@@ -823,7 +833,7 @@ void Engine::build_bc_and_coarse() {
           }
         }
       }
-    } else if (typeBc == "cor" || typeBc == "all") {  // Cluster.cpp:846-903, corners only
+    } else if (typeBc == "cor" || typeBc == "all" || typeBc == "cor+edge") {  // Cluster.cpp:846-903, corners
       for (int d = d0; d < d1; d++) {
         const Subdomain& s = P->subs[d];
         for (int l = 0; l < s.n; l++) {
@@ -846,6 +856,60 @@ void Engine::build_bc_and_coarse() {
             brow[t].push_back(Row{cnt, {{u.bpos[lu], -1.0}}});
             cnt++;
             if (full_rank) break;
+          }
+        }
+      }
+      if (typeBc == "cor+edge") {
+        // averaged edges (no counterpart in the reference; corners as above plus one
+        // averaged constraint per edge, component and neighbour pair): for subdomain d
+        // (ascending), its 12 box edges in the order axis a = x, y, z, then the other
+        // two coordinates (b < c) at (lo, lo), (hi, lo), (lo, hi), (hi, hi); an edge's
+        // nodes are those strictly between its two end vertices (m of them).  For the
+        // first t > d of the cluster whose box holds the whole edge (every such t when
+        // Bc_fullRank=false) one row per component: sum_nodes u_d / m - u_t / m.
+        auto lidx = [](const Subdomain& x, int64_t ii, int64_t jj, int64_t kk, int c) {
+          return 3 * (int)((ii - x.e0[0]) + (jj - x.e0[1]) * x.nn[0] + (kk - x.e0[2]) * x.nn[0] * x.nn[1]) + c;
+        };
+        for (int d = d0; d < d1; d++) {
+          const Subdomain& s = P->subs[d];
+          for (int a = 0; a < 3; a++) {
+            const int b = a == 0 ? 1 : 0, c3 = a == 2 ? 1 : 2;
+            const int m = s.ne[a] - 1;
+            if (m < 1) continue;
+            for (int q = 0; q < 4; q++) {
+              int64_t p0[3], p1[3];
+              p0[a] = s.e0[a] + 1;
+              p1[a] = s.e0[a] + s.ne[a] - 1;
+              p0[b] = p1[b] = s.e0[b] + ((q & 1) ? s.ne[b] : 0);
+              p0[c3] = p1[c3] = s.e0[c3] + ((q & 2) ? s.ne[c3] : 0);
+              int list[8];
+              const int nl = P->subs_of_node(p0[0], p0[1], p0[2], list);
+              for (int w = 0; w < nl; w++) {
+                const int t = list[w];
+                if (t <= d || t >= d1) continue;
+                const Subdomain& u = P->subs[t];
+                bool holds = true;
+                for (int x = 0; x < 3; x++)
+                  if (p1[x] < u.e0[x] || p1[x] > u.e0[x] + u.ne[x]) holds = false;
+                if (!holds) continue;
+                for (int c = 0; c < 3; c++) {
+                  Row rd{cnt, {}}, rt{cnt, {}};
+                  for (int e = 0; e < m; e++) {
+                    int64_t pt[3] = {p0[0], p0[1], p0[2]};
+                    pt[a] += e;
+                    const int ls = s.bpos[lidx(s, pt[0], pt[1], pt[2], c)];
+                    const int lu = u.bpos[lidx(u, pt[0], pt[1], pt[2], c)];
+                    if (ls < 0 || lu < 0) throw std::runtime_error("edge DOF outside the interface set");
+                    rd.e.push_back({ls, 1.0 / m});
+                    rt.e.push_back({lu, -1.0 / m});
+                  }
+                  brow[d].push_back(rd);
+                  brow[t].push_back(rt);
+                  cnt++;
+                }
+                if (full_rank) break;
+              }
+            }
           }
         }
       }
@@ -970,6 +1034,18 @@ void Engine::build_bc_and_coarse() {
     aug_total += align32((int64_t)L2 * L2);
   }
   std::unique_ptr<double[]> Aaug(new double[std::max<int64_t>(aug_total, 1)]);  // zeroed per cluster below
+  // Ac^-1 as used by K_c^+ (Cluster.cpp:967-1040).  Both forms are generalised
+  // inverses of the singular Ac = [Fc Gc; Gc^T 0], so P F P and u do not depend on
+  // the choice:
+  //  * Ac_extended_by_kerGc=true (Cluster.cpp:592-660): the inverse of
+  //    [Fc Gc 0; Gc^T 0 H; 0 H^T 0] restricted to (lambda_c, alpha).  Formed as
+  //    A_r^-1 + diag(0, H H^T / sig) with A_r = [Fc Gc; Gc^T -sig H H^T]: A_r^-1 gives
+  //    the same lambda_c and alpha - H H^T b / sig, which the second term cancels.
+  //  * false (the reference default, htfeti.cpp:39: the singular Ac factorised with
+  //    its kernel regularised, Cluster.cpp:309-317): A_r^-1 itself, which satisfies
+  //    Ac A_r^-1 Ac = Ac because A_r [0; H y] = [0; -sig H y].
+  const bool extended = opt(o, "Ac_extended_by_kerGc") == "true";
+  const std::string dump = opti(o, "print_matrices") > 0 ? opt(o, "path2data") : std::string();
   // clusters in parallel: each writes its own slice of Aaug and the Hall rows of its subdomains
   parallel_for(P->ncl, [&](int cl) {
     const int d0 = cl * P->nsub_cl, Ns = P->nsub_cl, nlc = nlcl[cl], nr = 6 * Ns;
@@ -1015,12 +1091,37 @@ void Engine::build_bc_and_coarse() {
         double hh = 0.0;
         for (int k = 0; k < 6; k++) hh += H[(size_t)a * 6 + k] * H[(size_t)b * 6 + k];
         at(nlc + a, nlc + b) = -sig * hh;                         // block (2,2): -sig H H^T
-        at(2 * nlc + nr + a, 2 * nlc + nr + b) = -hh / sig;       // block (4,4) start: -H H^T / sig
+        if (extended) at(2 * nlc + nr + a, 2 * nlc + nr + b) = -hh / sig;  // block (4,4) start: -H H^T / sig
       }
     for (int l = 0; l < nlc; l++) at(nlc + nr + l, l) = 1.0;
     for (int a = 0; a < nr; a++) at(2 * nlc + nr + a, nlc + a) = 1.0;
     aug_ld[cl] = L2; aug_nlc[cl] = nlc; aug_nr[cl] = nr; aug_end[cl] = L2;
+    if (!dump.empty()) {
+      // print_matrices >= 1: the cluster matrices of Cluster.cpp:220-277, cluster index
+      // as the file suffix (the reference has one cluster, index 0)
+      write_matrix_market(dump, "Fc_clust", cl, nlc, nlc, Fc.data(), true);
+      write_matrix_market(dump, "Gc_clust", cl, nlc, nr, Gc.data(), false);
+      write_matrix_market(dump, "GcTGc_clust", cl, nr, nr, GtG.data(), true);
+      write_matrix_market(dump, "kerGc", cl, nr, 6, H.data(), false);
+      const int na = nlc + nr + (extended ? 6 : 0);
+      std::vector<double> Ac((size_t)na * na, 0.0);
+      for (int i = 0; i < nlc; i++)
+        for (int j = 0; j < nlc; j++) Ac[(size_t)i * na + j] = Fc[(size_t)i * nlc + j];
+      for (int l = 0; l < nlc; l++)
+        for (int c = 0; c < nr; c++) {
+          Ac[(size_t)l * na + nlc + c] = Gc[(size_t)l * nr + c];
+          Ac[(size_t)(nlc + c) * na + l] = Gc[(size_t)l * nr + c];
+        }
+      if (extended)
+        for (int c = 0; c < nr; c++)
+          for (int k = 0; k < 6; k++) {
+            Ac[(size_t)(nlc + c) * na + nlc + nr + k] = H[(size_t)c * 6 + k];
+            Ac[(size_t)(nlc + nr + k) * na + nlc + c] = H[(size_t)c * 6 + k];
+          }
+      write_matrix_market(dump, "Ac_clust", cl, na, na, Ac.data(), true);
+    }
   });
+  if (!dump.empty()) write_array(dump, "weigth", P->weight.data(), (int64_t)P->weight.size());
   d_H.upload(Hall, st);
   {
     DVec<double> dA;
@@ -1318,17 +1419,13 @@ double Engine::time_stream(int which, int reps, double* bytes) {
     out.alloc((size_t)nwg * 256);
     double fl = 0;
     HC(launch_mfma_ceiling(st, which == 5, nwg, iters, out.p, &fl));  // warm-up
-    hipEvent_t a, b;
-    HC(hipEventCreate(&a));
-    HC(hipEventCreate(&b));
-    HC(hipEventRecord(a, st));
+    Events ev(2);
+    HC(hipEventRecord(ev[0], st));
     for (int i = 0; i < reps; i++) HC(launch_mfma_ceiling(st, which == 5, nwg, iters, out.p, &fl));
-    HC(hipEventRecord(b, st));
-    HC(hipEventSynchronize(b));
+    HC(hipEventRecord(ev[1], st));
+    HC(hipEventSynchronize(ev[1]));
     float ms = 0;
-    HC(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    HC(hipEventElapsedTime(&ms, ev[0], ev[1]));
     if (bytes) *bytes = fl;
     return ms / reps;
   }
@@ -1340,18 +1437,14 @@ double Engine::time_stream(int which, int reps, double* bytes) {
   HC(launch_gather(st, NB, d_gptr.p, d_gidx.p, w.p, which ? d_wgt.p : nullptr, xb.p, nullptr));
   const double* A = which == 0 ? d_Kt.p : d_St.p;
   HC(launch_symv_tiles(st, N, maxT, d_ns.p, A, d_off_P.p, xb.p, d_off_b.p, partbuf.p, d_off_part.p, nullptr));
-  hipEvent_t a, b;
-  HC(hipEventCreate(&a));
-  HC(hipEventCreate(&b));
-  HC(hipEventRecord(a, st));
+  Events ev(2);
+  HC(hipEventRecord(ev[0], st));
   for (int i = 0; i < reps; i++)
     HC(launch_symv_tiles(st, N, maxT, d_ns.p, A, d_off_P.p, xb.p, d_off_b.p, partbuf.p, d_off_part.p, nullptr));
-  HC(hipEventRecord(b, st));
-  HC(hipEventSynchronize(b));
+  HC(hipEventRecord(ev[1], st));
+  HC(hipEventSynchronize(ev[1]));
   float ms = 0;
-  HC(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  HC(hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (bytes) {
     // algorithmic: the lower half of each symmetric n_s x n_s block (8 B each),
     // the interface vector and the tile-row partials written
@@ -1380,11 +1473,138 @@ void Engine::get_defect(std::vector<int>& d) const { d = h_def; }
 void Engine::get_load(std::vector<double>& f) const { d_f.download(f, st); }
 
 // ------------------------------------------------------------ PCPG (Cluster.cpp:1363-1458)
+// u = K_c^+ (f - Bf^T lam) - Rf alpha (Cluster.cpp:1648-1678) from the device lam and
+// hg (H alpha' per subdomain with alpha' = -alpha): u[gdof] = x where owner is set
+void Engine::recover_u(const double* f, const int* gdof, const unsigned char* owner, double* u) {
+  const int N = P->nsub;
+  HC(hipMemcpyAsync(vloc.p, f, off_n[N] * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, lam.p, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vloc.p));
+  prep_t(vloc.p);
+  cluster_solve_t(vloc.p);
+  kplus_b_from_t();
+  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
+                       d_ns.p, d_off_b.p, nullptr, la.p, yb.p, nullptr));
+  interior(xloc.p);
+  if (owner == d_owner.p) HC(hipMemsetAsync(u, 0, 3 * P->nPoints * sizeof(double), st));
+  HC(launch_recover(st, N, maxn, d_n.p, d_off_n.p, d_R.p, d_off_R.p, d_alpha_idx.p, la.p, hg.p, xloc.p, gdof, owner,
+                    u));
+}
+
+// per subdomain d: hv_d = sgn * H_d a_cl(d) (H_d: 6 x 6 rows of the cluster kernel)
+void Engine::coarse_to_sub(const double* a, double sgn, std::vector<double>& hv) const {
+  const int N = P->nsub;
+  std::vector<double> H;
+  d_H.download(H, st);
+  hv.assign(6 * (size_t)N, 0.0);
+  for (int d = 0; d < N; d++)
+    for (int i = 0; i < 6; i++) {
+      double t = 0.0;
+      for (int k = 0; k < 6; k++) t += H[36 * (size_t)d + 6 * i + k] * a[6 * h_cl[d] + k];
+      hv[6 * (size_t)d + i] = sgn * t;
+    }
+}
+
+// e_cl = sum_{d in cl} H_d^T v_d, subdomains ascending
+void Engine::sub_to_coarse(const std::vector<double>& v, double* e) const {
+  const int N = P->nsub;
+  std::vector<double> H;
+  d_H.download(H, st);
+  std::fill(e, e + 6 * P->ncl, 0.0);
+  for (int d = 0; d < N; d++)
+    for (int k = 0; k < 6; k++) {
+      double t = 0.0;
+      for (int i = 0; i < 6; i++) t += H[36 * (size_t)d + 6 * i + k] * v[6 * (size_t)d + i];
+      e[6 * h_cl[d] + k] += t;
+    }
+}
+
+void Engine::mult_Gf(const double* a_h, double* lam_h) {
+  std::vector<double> hv;
+  coarse_to_sub(a_h, 1.0, hv);
+  HC(hipMemcpyAsync(h.p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HC(launch_gh(st, P->nlam, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, h.p, nullptr, Fw.p, 1.0,
+               nullptr, nullptr, nullptr));
+  HC(hipMemcpyAsync(lam_h, Fw.p, P->nlam * sizeof(double), hipMemcpyDeviceToHost, st));
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::mult_GfT(const double* lam_h, double* e_h) {
+  const int N = P->nsub;
+  HC(hipMemcpyAsync(g.p, lam_h, P->nlam * sizeof(double), hipMemcpyHostToDevice, st));
+  HC(launch_proj_dots(st, N, d_Rb.p, d_off_Rb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, g.p, v6.p, nullptr));
+  std::vector<double> v;
+  v6.download(v, st);
+  sub_to_coarse(v, e_h);
+}
+
+void Engine::mult_RfT(const double* f_h, double* e_h) {
+  const int N = P->nsub;
+  HC(hipMemcpyAsync(vloc.p, f_h, off_n[N] * sizeof(double), hipMemcpyHostToDevice, st));
+  HC(launch_tdots(st, N, 6, d_n.p, d_sixes.p, d_ones.p, d_R.p, d_off_R.p, d_ldn.p, vloc.p, d_off_n.p, d_ldn.p, avec.p,
+                  d_v6off.p, d_ns.p, 1.0));
+  std::vector<double> v;
+  avec.download(v, st);
+  sub_to_coarse(v, e_h);
+}
+
+void Engine::recover(const double* f_h, const double* lam_h, const double* a_h, double* x_h, double* u_h) {
+  const int N = P->nsub;
+  DVec<double> fin;
+  fin.alloc(off_n[N]);
+  HC(hipMemcpyAsync(fin.p, f_h, off_n[N] * sizeof(double), hipMemcpyHostToDevice, st));
+  HC(hipMemcpyAsync(lam.p, lam_h, P->nlam * sizeof(double), hipMemcpyHostToDevice, st));
+  std::vector<double> hv;
+  coarse_to_sub(a_h, -1.0, hv);  // the device adds R H alpha' with alpha' = -alpha
+  HC(hipMemcpyAsync(hg.p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if (u_h) {
+    recover_u(fin.p, d_gdof.p, d_owner.p, ug.p);
+    HC(hipMemcpyAsync(u_h, ug.p, 3 * P->nPoints * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  if (x_h) {  // every local DOF of every subdomain: identity map, all owners
+    DVec<int> iota_;
+    std::vector<int> io(off_n[N]);
+    std::iota(io.begin(), io.end(), 0);
+    iota_.upload(io, st);
+    DVec<unsigned char> all1;
+    all1.upload(std::vector<unsigned char>(off_n[N], 1), st);
+    DVec<double> out;
+    out.alloc(off_n[N]);
+    recover_u(fin.p, iota_.p, all1.p, out.p);
+    HC(hipMemcpyAsync(x_h, out.p, off_n[N] * sizeof(double), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+  }
+  HC(hipStreamSynchronize(st));
+}
+
+void Engine::get_Rf(std::vector<double>& Rf) const {
+  const int N = P->nsub;
+  std::vector<double> R, H;
+  get_R(R);
+  d_H.download(H, st);
+  Rf.assign(R.size(), 0.0);
+  for (int d = 0; d < N; d++) {
+    const int64_t o = 6 * off_n[d], n = h_n[d];
+    for (int k = 0; k < 6; k++)
+      for (int64_t l = 0; l < n; l++) {
+        double t = 0.0;
+        for (int i = 0; i < 6; i++) t += R[o + l + i * n] * H[36 * (size_t)d + 6 * i + k];
+        Rf[o + l + k * n] = t;
+      }
+  }
+}
+
+int64_t Engine::nlam_c() const {
+  int64_t s = 0;
+  for (int v : h_nlcl) s += v;
+  return s;
+}
+
 void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, double* lam_h, SolveInfo* info) {
   const int N = P->nsub;
   const int64_t nl = P->nlam;
-  hipEvent_t e0, e1, e2;
-  HC(hipEventCreate(&e0)); HC(hipEventCreate(&e1)); HC(hipEventCreate(&e2));
+  Events ev(4);
+  hipEvent_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
   const double* f = d_f.p;
   DVec<double> fin;
   if (f_h) {
@@ -1420,48 +1640,56 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
   HC(hipMemcpyAsync(w.p, Pz.p, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
   HC(hipEventRecord(e1, st));
   Scalars sh;
-  const int chunk = poll_chunk;
-  int launched = 0;
-  if (gx_plain && gx_plain_n != chunk) {
-    (void)hipGraphExecDestroy(gx_plain);
-    gx_plain = nullptr;
-  }
-  gx_plain_n = chunk;
-  for (;;) {
-    run_graph(&gx_plain, [&] {
-      for (int k = 0; k < chunk; k++) {
-        F_apply(w.p, Fw.p, nullptr, w.p, done);
-        HC(launch_reduce(st, nparts, part.p, sc.p, 0));
-        HC(launch_axpy2(st, nl, lam.p, g.p, w.p, Fw.p, sc.p));
-        proj(g.p, Pg.p, hg.p, alpha.p, nullptr, done);
-        M_apply(Pg.p, z.p, done);
-        proj(z.p, Pz.p, h.p, nullptr, g.p, done);
-        HC(launch_reduce(st, nparts, part.p, sc.p, 1));
-        HC(launch_update_w(st, nl, w.p, Pz.p, sc.p));
+  auto iteration = [&] {  // Cluster.cpp:1422-1446, skipped on the device once done
+    F_apply(w.p, Fw.p, nullptr, w.p, done);
+    HC(launch_reduce(st, nparts, part.p, sc.p, 0));
+    HC(launch_axpy2(st, nl, lam.p, g.p, w.p, Fw.p, sc.p));
+    proj(g.p, Pg.p, hg.p, alpha.p, nullptr, done);
+    M_apply(Pg.p, z.p, done);
+    proj(z.p, Pz.p, h.p, nullptr, g.p, done);
+    HC(launch_reduce(st, nparts, part.p, sc.p, 1));
+    HC(launch_update_w(st, nl, w.p, Pz.p, sc.p));
+  };
+  const std::string folder = opt(P->o, "path2data");
+  if (!folder.empty() && opt(P->o, "vtkWithinIter") == "true") {
+    // vtkWithinIter (Cluster.cpp:1448-1449): after every iteration the displacements
+    // of the current lambda and alpha are recovered and written as box<it>.vtk;
+    // one eager iteration per host poll
+    std::vector<double> uh(3 * P->nPoints);
+    for (int done_it = 0;;) {
+      iteration();
+      HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+      HC(hipStreamSynchronize(st));
+      if (sh.it > done_it) {
+        recover_u(f, d_gdof.p, d_owner.p, ug.p);
+        HC(hipMemcpyAsync(uh.data(), ug.p, uh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HC(hipStreamSynchronize(st));
+        write_vtk(*P, uh.data(), folder, sh.it - 1);
+        done_it = sh.it;
       }
-    });
-    launched += chunk;
-    HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-    HC(hipStreamSynchronize(st));
-    if (sh.done || launched > max_iter + chunk) break;
+      if (sh.done || done_it > max_iter) break;
+    }
+  } else {
+    const int chunk = poll_chunk;
+    int launched = 0;
+    if (gx_plain && gx_plain_n != chunk) {
+      (void)hipGraphExecDestroy(gx_plain);
+      gx_plain = nullptr;
+    }
+    gx_plain_n = chunk;
+    for (;;) {
+      run_graph(&gx_plain, [&] {
+        for (int k = 0; k < chunk; k++) iteration();
+      });
+      launched += chunk;
+      HC(hipMemcpyAsync(&sh, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+      HC(hipStreamSynchronize(st));
+      if (sh.done || launched > max_iter + chunk) break;
+    }
   }
   HC(hipEventRecord(e2, st));
   HC(launch_sqnorm(st, nl, Pg.p, pgn.p + 1));  // P g of the last iteration
-  // recovery: u = K_c^+(f - Bf^T lambda) - Rf alpha  (Cluster.cpp:1648-1678)
-  HC(hipMemcpyAsync(vloc.p, f, off_n[N] * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HC(launch_sub_bt(st, N, maxns, d_nr.p, d_ns.p, d_perm1.p, d_off_n.p, d_off_b.p, d_gptr.p, d_gidx.p, lam.p, nullptr,
-                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, vloc.p));
-  prep_t(vloc.p);
-  cluster_solve_t(vloc.p);
-  kplus_b_from_t();
-  HC(launch_corr_apply(st, N, maxns, d_C.p, d_off_C.p, d_ldb.p, d_nlc.p, d_off_lc.p, d_lcmap.p, d_Rb.p, d_off_Rb.p,
-                       d_ns.p, d_off_b.p, nullptr, la.p, yb.p, nullptr));
-  interior(xloc.p);
-  HC(hipMemsetAsync(ug.p, 0, 3 * P->nPoints * sizeof(double), st));
-  HC(launch_recover(st, N, maxn, d_n.p, d_off_n.p, d_R.p, d_off_R.p, d_alpha_idx.p, la.p, hg.p, xloc.p, d_gdof.p,
-                    d_owner.p, ug.p));
-  hipEvent_t e3;
-  HC(hipEventCreate(&e3));
+  recover_u(f, d_gdof.p, d_owner.p, ug.p);
   HC(hipEventRecord(e3, st));
   if (u_h) HC(hipMemcpyAsync(u_h, ug.p, 3 * P->nPoints * sizeof(double), hipMemcpyDeviceToHost, st));
   if (lam_h) HC(hipMemcpyAsync(lam_h, lam.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1481,10 +1709,13 @@ void Engine::solve(double eps, int max_iter, const double* f_h, double* u_h, dou
     info->pcpg_ms = ms;
     info->converged = (sh.it < max_iter || std::sqrt(std::fabs(sh.gPz)) < eps * sh.norm0) ? 1 : 0;
   }
-  int err = 0;
-  HC(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
-  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
   if (sh.err) throw std::runtime_error("PCPG produced a non-finite residual");
+  if (!folder.empty()) {  // the final displacements as box.vtk (Cluster.cpp:1455-1456)
+    std::vector<double> uh(3 * P->nPoints);
+    HC(hipMemcpyAsync(uh.data(), ug.p, uh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    write_vtk(*P, uh.data(), folder, -1);
+  }
 }
 
 }  // namespace hfls

@@ -24,6 +24,32 @@ namespace hfls {
                                __FILE__ + ":" + std::to_string(__LINE__) + ": " #x);           \
   } while (0)
 
+// HIP events destroyed when the scope ends, also when a HIP call in it throws
+struct Events {
+  std::vector<hipEvent_t> e;
+  explicit Events(size_t n) : e(n, nullptr) {
+    for (auto& x : e) {
+      const hipError_t r = hipEventCreate(&x);
+      if (r != hipSuccess) {
+        x = nullptr;
+        release();
+        throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(r) + ": hipEventCreate");
+      }
+    }
+  }
+  Events(const Events&) = delete;
+  Events& operator=(const Events&) = delete;
+  ~Events() { release(); }
+  void release() {
+    for (auto& x : e)
+      if (x) {
+        (void)hipEventDestroy(x);
+        x = nullptr;
+      }
+  }
+  hipEvent_t operator[](size_t i) const { return e[i]; }
+};
+
 struct SolveInfo {
   int iterations = 0;
   double residual = 0.0;     // sqrt(g'Pz)/sqrt(g0'Pz0) at exit
@@ -102,6 +128,17 @@ class Engine {
   void project(const double* x, double* Px, double* alpha);
   void precondition(const double* x, double* Mx);
   void kplus_local(const double* f_local, double* x_local);  // K_c^+ f (packed local vectors)
+  // Gf alpha (Cluster.cpp:1275-1283), Gf^T lambda (1285-1293), Rf^T f (1295-1312);
+  // coarse vectors have 6 entries per cluster
+  void mult_Gf(const double* alpha, double* lam);
+  void mult_GfT(const double* lam, double* e);
+  void mult_RfT(const double* f_local, double* e);
+  // recovery of printVTK (Cluster.cpp:1648-1678) from host lambda and alpha (the
+  // alpha of Projection): x_local = K_c^+ (f - Bf^T lambda) - Rf alpha per subdomain
+  // (packed local) and u its scatter to the global DOFs; either may be null
+  void recover(const double* f_local, const double* lam, const double* alpha, double* x_local, double* u);
+  void get_Rf(std::vector<double>& Rf) const;  // packed local, n x 6 column major per subdomain: R_d H_d
+  int64_t nlam_c() const;                      // cluster constraints, all clusters
 
   // One load case: body-force load assembled on the device, or the given packed
   // local loads (sum over subdomains in l2g order).
@@ -122,8 +159,9 @@ class Engine {
   // at j*3*nPoints, of lam at j*nlam).  precision 64: F_i and S_i applied in FP64;
   // 32: F_i and S_i applied in FP32 on the matrix cores with FP32 accumulation,
   // everything else FP64, and each column's stopping test certified by its FP64
-  // true residual.  Per column: iterations, status (2 converged, 3 limit) and the
-  // final FP64 residual sqrt(g'PMPg)/sqrt(g0'PMPg0).
+  // true residual.  Per column: iterations, status (2 converged, 3 limit, 6 stopped
+  // on a non-finite residual; the other columns are unaffected) and the final FP64
+  // residual sqrt(g'PMPg)/sqrt(g0'PMPg0).
   void solve_block(double eps, int max_iter, int k, int precision, const double* f_local, double* u, double* lam,
                    int* iters, int* status, double* res, BlockInfo* info);
   // nb blocks of k load cases solved concurrently, one block context each (f_local:
@@ -155,6 +193,8 @@ class Engine {
   void blk_recover(const BScalars* state = nullptr);  // state: the scalars of the last blk_sync
   void blk_collect(const std::vector<int>& slots, double* u, double* lam);
   int blk_width() const { return blk_k; }
+  // free slots without recovering them (a cancelled case); they take new cases later
+  void blk_release(const std::vector<int>& slots) { blk_set_status(slots, 4); }
   // autotuner result of the block GEMM (FP32 format: output row tiles per
   // workgroup 1 or 2; split-FP16 format: K chunk depth 32 or 64 as 1 or 2) and the
   // measured ms of each candidate
@@ -291,6 +331,9 @@ class Engine {
   void interior(double* xout);               // xout = [K_rr^-1 v_r - K_rr^-1 K_rb yb ; yb] in local order
   // X <- K_rr^-1 X = W^T W X on the interior rows of every subdomain (element (i, j)
   // of d at X[xbase[d] + i*rs + j*cs]); xbase holds the subdomains' bases from sub0 on
+  void recover_u(const double* f, const int* gdof, const unsigned char* owner, double* u);
+  void coarse_to_sub(const double* a, double sgn, std::vector<double>& hv) const;
+  void sub_to_coarse(const std::vector<double>& v, double* e) const;
   void rr_solve(int nsub, int sub0, int k, double* X, const int64_t* xbase, int64_t rs, int64_t cs, const int* csv,
                 double* tmp, const int64_t* tbase);  // tmp: same layout as X
 };

@@ -331,9 +331,8 @@ int Engine::blk_contexts(int n) {
 // per workgroup; split-FP16 format K chunks of 32 or 64 per LDS stage.
 void Engine::autotune() {
   const int N = P->nsub;
-  hipEvent_t a, b;
-  HC(hipEventCreate(&a));
-  HC(hipEventCreate(&b));
+  Events ev(2);
+  hipEvent_t a = ev[0], b = ev[1];
   for (int rt = 1; rt <= 2; rt++) {
     auto launch = [&] {
       if (gemm16)
@@ -352,8 +351,6 @@ void Engine::autotune() {
     HC(hipEventElapsedTime(&ms, a, b));
     tune_ms32[rt - 1] = ms / 3;
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   tune_rt32 = gemm_rows_forced ? gemm_rows_forced : (tune_ms32[1] < tune_ms32[0] ? 2 : 1);
 }
 
@@ -559,17 +556,13 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
   };
   if (which != 2) gemm_lo(which, nullptr);
   launch();
-  hipEvent_t a, b;
-  HC(hipEventCreate(&a));
-  HC(hipEventCreate(&b));
-  HC(hipEventRecord(a, st));
+  Events ev(2);
+  HC(hipEventRecord(ev[0], st));
   for (int i = 0; i < reps; i++) launch();
-  HC(hipEventRecord(b, st));
-  HC(hipEventSynchronize(b));
+  HC(hipEventRecord(ev[1], st));
+  HC(hipEventSynchronize(ev[1]));
   float ms = 0;
-  HC(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  HC(hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (flops) {
     double f = 0;
     for (int d = 0; d < N; d++) f += 2.0 * KP * (double)h_ns[d] * h_ns[d];
@@ -708,7 +701,7 @@ void Engine::blk_recover(const BScalars* sh) {
   // column has it
   bool saved = sh != nullptr && prec == 32;
   for (int j = 0; saved && j < k; j++)
-    if (sh->done[j] == 3 || (sh->done[j] == 2 && !sh->hgok[j])) saved = false;
+    if (sh->done[j] == 3 || sh->done[j] == 6 || (sh->done[j] == 2 && !sh->hgok[j])) saved = false;
   if (!saved) {
     bF(k, 64, b_lam.p, b_g64.p, b_d.p, nullptr);
     bPM(k, prec, b_g64.p, b_z.p, b_hg.p);
@@ -757,8 +750,7 @@ void Engine::solve_blocks(int nb, double eps, int max_iter, int k, int prec, con
     blk_load(cols, f_h + (size_t)c * k * np);
     HC(hipStreamSynchronize(st));
   }
-  std::vector<hipEvent_t> ev(nb + 4);
-  for (auto& e : ev) HC(hipEventCreate(&e));
+  Events ev(nb + 4);
   // ev[0]: start (context 0), ev[1]: init done (context 0), ev[2]: loop done
   // (context 0), ev[3]: span end; ev[4 + c]: end of context c
   use_ctx(0);
@@ -790,7 +782,7 @@ void Engine::solve_blocks(int nb, double eps, int max_iter, int k, int prec, con
         bool r = false;
         for (int j = 0; j < k; j++)
           if (sh[c].done[j] == 0 || sh[c].done[j] == 1) r = true;
-        if (sh[c].err || !r || sweeps[c] > max_iter + 2 * chunk) run[c] = 0;
+        if (!r || sweeps[c] > max_iter + 2 * chunk) run[c] = 0;
         else any = true;
       }
   }
@@ -841,7 +833,6 @@ void Engine::solve_blocks(int nb, double eps, int max_iter, int k, int prec, con
       for (int j = 0; j < k; j++) cs += sh[c].ncert[j];
     info->cert_steps = cs;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 void Engine::solve_block(double eps, int max_iter, int k, int prec, const double* f_h, double* u_h, double* lam_h,
@@ -849,8 +840,7 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
   blk_begin(k, eps, max_iter, prec);
   std::vector<int> cols(k);
   for (int j = 0; j < k; j++) cols[j] = j;
-  hipEvent_t ev[4];
-  for (auto& e : ev) HC(hipEventCreate(&e));
+  Events ev(4);
   blk_load(cols, f_h);  // host loads to HBM: outside the timed solve
   HC(hipEventRecord(ev[0], st));
   blk_init(cols);
@@ -862,7 +852,6 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
     blk_sweeps(chunk);
     sweeps += chunk;
     blk_sync(&sh);
-    if (sh.err) break;
     bool running = false;
     for (int j = 0; j < k; j++)
       if (sh.done[j] == 0 || sh.done[j] == 1) running = true;
@@ -902,8 +891,6 @@ void Engine::solve_block(double eps, int max_iter, int k, int prec, const double
     for (int j = 0; j < k; j++) nc += sh.ncert[j];
     info->cert_steps = nc;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (sh.err) throw std::runtime_error("block PCPG produced a non-finite residual");
 }
 
 }  // namespace hfls

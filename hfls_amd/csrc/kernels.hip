@@ -57,8 +57,6 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmSub* __restrict__ sub
   double* fd = f ? f + f_off[mi] : nullptr;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nnx = s.ne[0] + 1, nny = s.ne[1] + 1;
-  const double c0 = s.E / ((1.0 + nu) * (1.0 - 2.0 * nu));
-  const double l2 = (1.0 - nu) * c0, l1 = nu * c0, G = (0.5 - nu) * c0;
   const double PT = 0.577350269189626;
   const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0}, oy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
   const double acc[3] = {ax, ay, az};
@@ -69,6 +67,14 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmSub* __restrict__ sub
     const int cnt = ncx * ncy * ncz;
     for (int q = wave; q < cnt; q += 4) {
       const int ex = cx + 2 * (q % ncx), ey = cy + 2 * ((q / ncx) % ncy), ez = cz + 2 * (q / (ncx * ncy));
+      double Ee = s.E;
+      if (s.mn[0] > 0) {
+        const int pid = (s.e0[0] + ex) / s.mn[0] + ((s.e0[1] + ey) / s.mn[1]) * s.mN[0] +
+                        ((s.e0[2] + ez) / s.mn[2]) * s.mN[0] * s.mN[1];
+        if (pid & 1) Ee = s.E1;
+      }
+      const double c0 = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+      const double l2 = (1.0 - nu) * c0, l1 = nu * c0, G = (0.5 - nu) * c0;
       double X[8], Y[8], Z[8];
       int ln[8];
 #pragma unroll
@@ -1177,7 +1183,10 @@ __global__ __launch_bounds__(256) void k_reduce(int nparts, const double* __rest
       sc->gamma = s / sc->gPz_prev;
       sc->it += 1;
       if (!(sqrt(fabs(s)) >= sc->eps * sc->norm0) || sc->it >= sc->max_iter) sc->done = 1;
-      if (!(s == s)) sc->err = 1;
+      if (!isfinite(s)) {  // stop on a non-finite residual; the host reports the error
+        sc->done = 1;
+        sc->err = 1;
+      }
     } else if (mode == 2) {
       sc->gPz = s;
       sc->norm0 = sqrt(s);
@@ -1185,7 +1194,10 @@ __global__ __launch_bounds__(256) void k_reduce(int nparts, const double* __rest
       // stopping test at the top of the loop (Cluster.cpp:1425); a zero initial
       // residual (zero load) is already the solution: no iteration
       sc->done = (sc->max_iter <= 0 || !(s > 0.0) || !(sqrt(s) >= sc->eps * sc->norm0)) ? 1 : 0;
-      if (!(s == s)) sc->err = 1;
+      if (!isfinite(s)) {
+        sc->done = 1;
+        sc->err = 1;
+      }
     } else {
       sc->gPg0 = s;
     }

@@ -20,6 +20,11 @@ struct SyrkDesc {  // update by columns [k0,k0+w) of rows/cols t0 <= c <= r < en
 struct AsmSub {  // box subdomain for element assembly
   int e0[3], ne[3], n, nr;  // nr > 0: also write the identity coupling M[n+i][i] = 1, i < nr
   double E;
+  // per-element material (mn[0] > 0): an element in geometric partition
+  // pid = gi/mn[0] + (gj/mn[1]) mN[0] + (gk/mn[2]) mN[0] mN[1] (global element indices)
+  // has E when pid is even and E1 when odd (Mesh.cpp:271-288, Data.cpp:47-48)
+  int mn[3], mN[2];
+  double E1;
 };
 
 struct Scalars {  // PCPG scalars kept on the device

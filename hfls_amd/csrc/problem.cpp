@@ -25,7 +25,10 @@ Options default_options() {
       {"slabs_x", ""}, {"slabs_y", ""}, {"slabs_z", ""},
       {"linear_solver", "hip"}, {"vtkWithinIter", "false"}, {"print_matrices", "0"},
       {"precision", "fp64"}, {"nrhs", "1"}, {"device", "0"}, {"refine_tol", "1e-4"}, {"slab_choices", ""},
-      {"block_gemm", "f16x2"}};
+      {"block_gemm", "f16x2"},
+      // the rest of the reference's options map (htfeti.cpp:12-57)
+      {"metis", "false"}, {"nparts", "1"}, {"path2data", ""}, {"Ac_clust_explicit", "true"},
+      {"GcTGc_assembl_block_by_block", "true"}, {"create_analytic_ker_K", "false"}};
 }
 
 std::string opt(const Options& o, const std::string& k) {
@@ -127,6 +130,55 @@ void Problem::node_coords(int64_t ii, int64_t jj, int64_t kk, double* x) const {
 
 double Problem::young(int d) const { return subs[d].E; }
 
+int Problem::material_partition(int64_t ii, int64_t jj, int64_t kk) const {
+  if (geo_n[0] > 0)
+    return (int)(ii / geo_n[0] + (jj / geo_n[1]) * geo_N[0] + (kk / geo_n[2]) * geo_N[0] * geo_N[1]);
+  const int I = (int)(std::upper_bound(starts[0].begin(), starts[0].end(), ii) - starts[0].begin()) - 1;
+  const int J = (int)(std::upper_bound(starts[1].begin(), starts[1].end(), jj) - starts[1].begin()) - 1;
+  const int K = (int)(std::upper_bound(starts[2].begin(), starts[2].end(), kk) - starts[2].begin()) - 1;
+  return sub_id(I, J, K);
+}
+
+static bool is_bool(const std::string& v) { return v == "true" || v == "false"; }
+
+// The keys of the reference's options map that select code paths (htfeti.cpp:12-57):
+// every value the reference accepts is either honoured or rejected with the reason.
+static void check_reference_options(const Options& o) {
+  // linear_solver picks the sparse direct solver behind Matrix::sym_factor /
+  // num_factor / solve_system (Matrix.cpp:970-985, 1205-1233); here that seam is
+  // the device Cholesky for both the reference's default (pardiso) and hip
+  const std::string ls = opt(o, "linear_solver");
+  if (ls == "dissection")
+    throw std::runtime_error("linear_solver=dissection (Matrix.cpp:886-968) is not available; use pardiso or hip");
+  if (ls != "pardiso" && ls != "hip") throw std::runtime_error("unknown linear_solver " + ls + " (pardiso or hip)");
+  const std::string me = opt(o, "metis");
+  if (!is_bool(me)) throw std::runtime_error("metis must be true or false");
+  if (me == "true" && opti(o, "nparts") > 1)
+    throw std::runtime_error(
+        "metis=true with nparts > 1 partitions the mesh with METIS_PartMeshDual (Mesh.cpp:392-412), which this "
+        "library does not provide; use metis=false (geometric Nx x Ny x Nz decomposition) or nparts=1");
+  // Ac_clust_explicit=false forms Sc = Gc^T Fc^-1 Gc and exits (Cluster.cpp:325-342)
+  const std::string ace = opt(o, "Ac_clust_explicit");
+  if (ace == "false")
+    throw std::runtime_error("Ac_clust_explicit=false: the reference forms Sc_clust and exits (Cluster.cpp:325-342); "
+                             "there is no solve on that path");
+  if (ace != "true") throw std::runtime_error("Ac_clust_explicit must be true or false");
+  // GcTGc_assembl_block_by_block picks one of two assemblies of the same Gc^T Gc
+  // (Cluster.cpp:236-245); create_analytic_ker_K is read by no solver branch
+  // (Cluster.cpp:84-86 is `if (0)`), and htfeti.cpp:60-66 can leave it empty
+  for (const char* k : {"Ac_extended_by_kerGc", "GcTGc_assembl_block_by_block", "Bc_fullRank", "vtkWithinIter"})
+    if (!is_bool(opt(o, k))) throw std::runtime_error(std::string(k) + " must be true or false");
+  // print_matrices 1: MatrixMarket dumps of the cluster matrices (Cluster.cpp:220-277);
+  // 2 and 3 add per-subdomain CSR dumps of K, Bc, Bf, R, Gc, Rf, Gf (Cluster.cpp:126-212,
+  // 362-370), which this library does not write (it never forms K_i in CSR)
+  const int pm = opti(o, "print_matrices");
+  if (pm < 0 || pm > 3) throw std::runtime_error("print_matrices must be 0..3");
+  if (pm > 1)
+    throw std::runtime_error("print_matrices > 1 (per-subdomain CSR dumps, Cluster.cpp:126-212) is not supported; "
+                             "use 0 or 1");
+  if (pm > 0 && opt(o, "path2data").empty()) throw std::runtime_error("print_matrices > 0 needs path2data");
+}
+
 Problem::Problem(const Options& oin) : o(oin) {
   // htfeti.cpp:18 documents Dirichlet_explicit, Dirichlet_implicit and lumped.
   // Both Dirichlet forms apply S_bb (Cluster.cpp:1061-1102; explicit and implicit
@@ -139,7 +191,9 @@ Problem::Problem(const Options& oin) : o(oin) {
       throw std::runtime_error("unknown preconditioner " + prec +
                                " (Dirichlet_explicit, Dirichlet_implicit or lumped)");
     const std::string tb = opt(o, "typeBc");
-    if (tb != "ker" && tb != "cor" && tb != "all") throw std::runtime_error("unknown typeBc " + tb);
+    if (tb != "ker" && tb != "cor" && tb != "all" && tb != "cor+edge")
+      throw std::runtime_error("unknown typeBc " + tb + " (ker, cor, all or cor+edge)");
+    check_reference_options(o);
     // matrix-core format of K+_bb and S_i in mixed-precision blocks: split FP16
     // (hi + lo planes) or FP32
     const std::string bg = opt(o, "block_gemm");
@@ -155,14 +209,12 @@ Problem::Problem(const Options& oin) : o(oin) {
     NE[a] = starts[a].back();
     dx[a] = 1.0 / NE[a];
   }
-  nsub_cl = N[0] * N[1] * N[2];
-  ncl = C[0] * C[1] * C[2];
-  nsub = nsub_cl * ncl;
   const int64_t nx1 = NE[0] + 1, ny1 = NE[1] + 1, nz1 = NE[2] + 1;
   nPoints = nx1 * ny1 * nz1;
   const int64_t nxy1 = nx1 * ny1;
 
-  // corner DOFs (Mesh.cpp:175-208)
+  // corner DOFs (Mesh.cpp:175-208): from the geometric Nx x Ny x Nz grid, which
+  // createMesh uses before any METIS partition
   {
     std::set<int64_t> cube = {0, NE[0], NE[1] * nx1, NE[1] * nx1 + NE[0]};
     std::set<int64_t> top;
@@ -176,6 +228,31 @@ Problem::Problem(const Options& oin) : o(oin) {
           for (int c = 0; c < 3; c++) corner_dofs.push_back(3 * n + c);
         }
   }
+  // metis=true with nparts <= 1: Mesh::ddm_metis puts every element in partition 0
+  // without calling METIS (Mesh.cpp:413-421), i.e. the whole cube is one subdomain
+  if (opt(o, "metis") == "true") {
+    if (C[0] * C[1] * C[2] != 1) throw std::runtime_error("metis=true needs one cluster (Cx = Cy = Cz = 1)");
+    if (!opt(o, "slabs_x").empty() || !opt(o, "slabs_y").empty() || !opt(o, "slabs_z").empty() ||
+        !opt(o, "slab_choices").empty())
+      throw std::runtime_error("metis=true takes the uniform Nx x Ny x Nz grid of nx x ny x nz elements");
+    // the material ids stay those of the geometric partition (createMesh sets them,
+    // ddm_metis changes only the partition ids, Mesh.cpp:271-288, 425-427)
+    for (int a = 0; a < 3; a++) geo_n[a] = slabs[a][0];
+    geo_N[0] = N[0];
+    geo_N[1] = N[1];
+    if (opt(o, "material") == "reference") {
+      mat_per_element = true;
+      mat_E1 = optd(o, "young_modulus");
+    }
+    for (int a = 0; a < 3; a++) {
+      N[a] = 1;
+      slabs[a].assign(1, NE[a]);
+      starts[a] = {0, NE[a]};
+    }
+  }
+  nsub_cl = N[0] * N[1] * N[2];
+  ncl = C[0] * C[1] * C[2];
+  nsub = nsub_cl * ncl;
   // Dirichlet DOFs (Mesh.cpp:221-238), points ascending, components x,y,z
   {
     std::string bc = opt(o, "bc");

@@ -59,6 +59,16 @@ struct Problem {
   std::vector<int> row_sub_a, row_sub_b;
   std::vector<int64_t> row_dof;   // global DOF of the row
   std::vector<double> weight;
+  // metis=true: the geometric Nx x Ny x Nz partition that createMesh made before
+  // ddm_metis merged it (geo_n elements per partition and axis, geo_N partitions
+  // along x and y; zeros otherwise).  It keeps the material ids (Mesh.cpp:271-288):
+  // with the reference material an element of an odd geometric partition has
+  // mat_E1, an even one its subdomain's E (mat_per_element)
+  int geo_n[3] = {0, 0, 0}, geo_N[2] = {0, 0};
+  bool mat_per_element = false;
+  double mat_E1 = 0.0;
+  // geometric partition of element (ii, jj, kk): geo_n set, else the subdomain id
+  int material_partition(int64_t ii, int64_t jj, int64_t kk) const;
 
   explicit Problem(const Options& o);
   int sub_id(int I, int J, int K) const;

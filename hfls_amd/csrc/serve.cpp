@@ -97,10 +97,38 @@ bool Server::any_work() const {
   return false;
 }
 
+void Server::fail_waiting(const std::string& key, const std::string& msg) {
+  const double t = now();
+  for (auto it = wait_.begin(); it != wait_.end();) {
+    ServeRequest& r = req_[*it];
+    if (r.key != key) {
+      ++it;
+      continue;
+    }
+    r.error = msg;
+    std::fill(r.status.begin(), r.status.end(), 7);
+    r.remaining = 0;
+    r.t_done = t;
+    st_.requests_done++;
+    it = wait_.erase(it);
+  }
+}
+
 bool Server::finish_setup(bool wait) {
   if (!pend_) return false;
   if (!wait && pend_->fut.wait_for(std::chrono::seconds(0)) != std::future_status::ready) return false;
-  std::unique_ptr<ServeMesh> m = pend_->fut.get();  // rethrows a failed set-up
+  std::unique_ptr<ServeMesh> m;
+  try {
+    m = pend_->fut.get();  // rethrows a failed set-up
+  } catch (const std::exception& ex) {
+    // the waiting requests of this mesh end with status 7; the server goes on
+    const std::string key = pend_->key;
+    st_.setup_ms += now() - pend_->t0;
+    st_.failed_setups++;
+    pend_.reset();
+    fail_waiting(key, std::string("set-up of the mesh failed: ") + ex.what());
+    return false;
+  }
   st_.setup_ms += now() - pend_->t0;
   st_.setups++;
   m->last_used = now();
@@ -274,10 +302,11 @@ bool Server::step() {
     if (!m.swept) continue;
     BScalars sh;
     m.e->blk_sync(&sh);
-    if (sh.err) throw std::runtime_error("PCPG produced a non-finite residual in a served block");
+    // stopped: converged, iteration limit, or non-finite (a bad case ends alone;
+    // the other slots' arithmetic does not depend on it)
     std::vector<int> cols;
     for (int j = 0; j < slots_; j++)
-      if (m.slot[j].req >= 0 && (sh.done[j] == 2 || sh.done[j] == 3)) cols.push_back(j);
+      if (m.slot[j].req >= 0 && (sh.done[j] == 2 || sh.done[j] == 3 || sh.done[j] == 6)) cols.push_back(j);
     if (cols.empty()) continue;
     std::vector<double> U(cols.size() * (size_t)m.ng);
     m.e->blk_finish(cols, U.data(), nullptr, &sh);
@@ -340,6 +369,35 @@ void Server::result(int64_t id, double* u, int* it, int* status, double* res, do
     latency_ms[0] = r.t_done - r.t_submit;
     latency_ms[1] = r.t_admit - r.t_submit;
   }
+  const std::string err = r.error;
+  req_.erase(f);
+  if (!err.empty()) throw std::runtime_error(err);
+}
+
+void Server::cancel(int64_t id) {
+  auto f = req_.find(id);
+  if (f == req_.end()) throw std::runtime_error("unknown request");
+  for (auto it = wait_.begin(); it != wait_.end();)
+    if (*it == id) it = wait_.erase(it);
+    else ++it;
+  auto mi = mesh_.find(f->second.key);
+  if (mi != mesh_.end()) {
+    ServeMesh& m = *mi->second;
+    for (auto it = m.queue.begin(); it != m.queue.end();)
+      if (it->req == id) it = m.queue.erase(it);
+      else ++it;
+    std::vector<int> cols;
+    for (int j = 0; j < (int)m.slot.size(); j++)
+      if (m.slot[j].req == id) {
+        cols.push_back(j);
+        m.slot[j] = ServeSlot();
+      }
+    if (!cols.empty()) {
+      HC(hipSetDevice(device_));
+      m.e->blk_release(cols);  // stream ordered after the mesh's last chunk
+    }
+  }
+  if (f->second.t_done < 0) st_.cancelled++;
   req_.erase(f);
 }
 

@@ -36,6 +36,7 @@ struct ServeRequest {
   std::vector<double> U;               // k x 3*nPoints
   std::vector<int> it, status;
   std::vector<double> res;
+  std::string error;                   // set-up failure of the request's mesh
 };
 
 struct ServeSlot {
@@ -57,6 +58,7 @@ struct ServeMesh {
 
 struct ServeStats {
   int64_t cases_done = 0, requests_done = 0, setups = 0, evictions = 0, joins = 0, finishes = 0;
+  int64_t failed_setups = 0, cancelled = 0;
   double setup_ms = 0;
 };
 
@@ -71,7 +73,12 @@ class Server {
   bool step();                                   // one scheduling round; false when idle
   int state(int64_t id) const;                   // 0 queued, 1 running, 2 done, -1 unknown
   int request_k(int64_t id) const;
+  // per-case status: 2 converged, 3 iteration limit, 6 non-finite residual, 7 the
+  // mesh's set-up failed (the message is thrown after the outputs are written)
   void result(int64_t id, double* u, int* it, int* status, double* res, double* latency_ms, int* k);
+  // drop a request: its queued cases are removed, its running cases leave their
+  // slots (without recovery); the id is unknown afterwards
+  void cancel(int64_t id);
   const ServeStats& stats() const { return st_; }
   size_t bytes_in_use() const;
   int meshes() const { return (int)mesh_.size(); }
@@ -95,6 +102,7 @@ class Server {
   };
   std::unique_ptr<PendingSetup> pend_;
   bool finish_setup(bool wait);                  // install a finished set-up; true if one was installed
+  void fail_waiting(const std::string& key, const std::string& msg);  // requests of a failed set-up
   bool any_work() const;                         // a cached mesh has running or queued cases
   double now() const;
   const Problem& host_problem(const std::string& key, const Options& o);

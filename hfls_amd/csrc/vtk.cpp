@@ -57,9 +57,39 @@ void write_vtk(const Problem& P, const double* u, const std::string& folder, int
       for (int64_t ii = 0; ii < NX; ii++, e++) part[e] = P.sub_id(slab[0][ii], slab[1][jj], slab[2][kk]);
   std::fprintf(o, "\nCELL_DATA %lld\nSCALARS decomposition int 1\nLOOKUP_TABLE decomposition\n", (long long)ne);
   for (int64_t e = 0; e < ne; e++) std::fprintf(o, "%d\n", part[e]);
+  // material id from the partition createMesh made (the geometric one also after
+  // metis=true merged it, Mesh.cpp:271-288)
   std::fprintf(o, "SCALARS materialId int 1\nLOOKUP_TABLE materialId\n");
-  for (int64_t e = 0; e < ne; e++) std::fprintf(o, "%d\n", (int)std::remainder((double)part[e], 2.0));
+  for (int64_t kk = 0; kk < NZ; kk++)
+    for (int64_t jj = 0; jj < NY; jj++)
+      for (int64_t ii = 0; ii < NX; ii++)
+        std::fprintf(o, "%d\n", (int)std::remainder((double)P.material_partition(ii, jj, kk), 2.0));
   if (std::ferror(o)) throw std::runtime_error("write error on " + name);
+}
+
+void write_matrix_market(const std::string& folder, const std::string& name, int idx, int rows, int cols,
+                         const double* A, bool symmetric) {
+  const std::string path = folder + "/dump_" + name + "_" + std::to_string(idx) + ".txt";
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path.c_str(), "w"), &std::fclose);
+  if (!f) throw std::runtime_error("cannot open " + path + " for writing");
+  int64_t nnz = 0;
+  for (int i = 0; i < rows; i++)
+    for (int j = symmetric ? i : 0; j < cols; j++)
+      if (A[(size_t)i * cols + j] != 0.0) nnz++;
+  std::fprintf(f.get(), "%%%%MatrixMarket matrix coordinate real %s\n", symmetric ? "symmetric" : "general");
+  std::fprintf(f.get(), "%d\t%d\t%lld\n", rows, cols, (long long)nnz);
+  for (int i = 0; i < rows; i++)
+    for (int j = symmetric ? i : 0; j < cols; j++) {
+      const double v = A[(size_t)i * cols + j];
+      if (v != 0.0) std::fprintf(f.get(), "%d\t%d\t%.16e\n", i + 1, j + 1, v);
+    }
+}
+
+void write_array(const std::string& folder, const std::string& name, const double* v, int64_t n) {
+  const std::string path = folder + "/dump_" + name + ".txt";
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path.c_str(), "w"), &std::fclose);
+  if (!f) throw std::runtime_error("cannot open " + path + " for writing");
+  for (int64_t i = 0; i < n; i++) std::fprintf(f.get(), "%.16e\n", v[i]);
 }
 
 }  // namespace hfls

@@ -8,6 +8,7 @@
 // methods move dual vectors (nLam_f doubles, reference Bf row order) and
 // per-subdomain vectors (l2g order, Data.cpp:111-144) between host and device.
 #pragma once
+#include <cmath>
 #include <cstdlib>
 #include <map>
 #include <stdexcept>
@@ -35,8 +36,11 @@ struct KV {
   }
 };
 
-// Mesh::createMesh (Mesh.cpp:69-289); ddm_metis is a no-op (geometric
-// decomposition only, as with metis=false, Mesh.cpp:355-357).
+// Mesh::createMesh (Mesh.cpp:69-289).  Mesh::ddm_metis (Mesh.cpp:352-452) is
+// folded into createMesh: metis=false keeps the geometric Nx x Ny x Nz partition,
+// metis=true with nparts <= 1 makes the whole cube one subdomain (what ddm_metis
+// does without calling METIS, Mesh.cpp:413-421), metis=true with nparts > 1 is an
+// error (METIS is not available).
 class Mesh {
  public:
   Mesh() = default;
@@ -94,9 +98,32 @@ class Cluster {
     if (run_pcpg) pcpg();
   }
 
+  // Cluster.hpp:26-31
   int get_nSubClst() { return nSubClst; }
+  int get_neqClst() { return (int)counts(1); }
+  int get_nLam_c() { return (int)counts(2); }
   int get_nLam_f() { return nLam_f; }
   int get_nRBM_f() { return nRBM_f; }
+  int get_nRBM_c() { return (int)counts(5); }
+
+  // lambda = Gf alpha (Cluster.cpp:1275-1283)
+  void mult_Gf(Vector const& alpha, Vector& lambda) {
+    if ((int)alpha.size() != nRBM_f) throw std::runtime_error("alpha has the wrong size");
+    lambda.resize(nLam_f);
+    check(hfls_mult_Gf(m, alpha.data(), lambda.data()));
+  }
+  // alpha = Gf^T lambda (Cluster.cpp:1285-1293)
+  void mult_GfT(Vector const& lambda, Vector& alpha) {
+    if ((int)lambda.size() != nLam_f) throw std::runtime_error("lambda has the wrong size");
+    alpha.resize(nRBM_f);
+    check(hfls_mult_GfT(m, lambda.data(), alpha.data()));
+  }
+  // alpha = sum_d Rf_d^T x_d (Cluster.cpp:1295-1312)
+  void mult_RfT(std::vector<Vector> const& x_in, Vector& alpha) {
+    Vector f = pack(x_in);
+    alpha.resize(nRBM_f);
+    check(hfls_mult_RfT(m, f.data(), alpha.data()));
+  }
 
   // x = K_c^+ f (Cluster.cpp:967-1040)
   void mult_Kplus_f(std::vector<Vector>& rhs_in, std::vector<Vector>& x_out) {
@@ -143,7 +170,9 @@ class Cluster {
   void scale(Vector& x) {
     for (int i = 0; i < nLam_f; i++) x[i] *= weigth[i];
   }
-  // Cluster::pcpg (Cluster.cpp:1363-1458) and the final recovery of printVTK
+  // Cluster::pcpg (Cluster.cpp:1363-1458) and the final recovery of printVTK; with
+  // path2data set the library writes <path2data>/box.vtk, and with
+  // vtkWithinIter=true also box<it>.vtk after every iteration (Cluster.cpp:1448-1456)
   void pcpg() {
     double eps = options2.count("eps_iter") ? std::atof(options2["eps_iter"].c_str()) : 1e-6;
     int max_iter = options2.count("max_iter") ? std::atoi(options2["max_iter"].c_str()) : 200;
@@ -152,11 +181,28 @@ class Cluster {
     solution.assign(3 * s[4], 0.0);
     lambda.assign(nLam_f, 0.0);
     check(hfls_solve(m, eps, max_iter, nullptr, solution.data(), lambda.data(), &stats));
-    // final solution to <path2data>/box.vtk (Cluster.cpp:1455-1456)
-    if (options2.count("path2data") && !options2["path2data"].empty()) printVTK(-1);
   }
-  // Cluster::printVTK (Cluster.cpp:1648-1678): the recovered displacements as VTK
-  // (<path2data>/box.vtk for it < 0, box<it>.vtk otherwise)
+  // Cluster::printVTK (Cluster.cpp:1648-1678): yy = rhs - Bf^T lambda, xx = K_c^+ yy -
+  // Rf alpha per subdomain, the scatter of xx written as <path2data>/box.vtk (it < 0)
+  // or box<it>.vtk; rhs is the options' body force
+  void printVTK(std::vector<Vector>& yy, std::vector<Vector>& xx, Vector& lambda_in, Vector& alpha, int it) {
+    if ((int)lambda_in.size() != nLam_f || (int)alpha.size() != nRBM_f)
+      throw std::runtime_error("lambda or alpha has the wrong size");
+    Vector f(nPacked), x(nPacked);
+    check(hfls_mesh_load(m, f.data()));
+    std::vector<Vector> bt;
+    mult_BfT(lambda_in, bt);
+    yy = unpack(f);
+    for (int d = 0; d < nSubClst; d++)
+      for (size_t i = 0; i < yy[d].size(); i++) yy[d][i] -= bt[d][i];
+    int64_t s[6];
+    check(hfls_mesh_sizes(m, s));
+    Vector u(3 * s[4]);
+    check(hfls_recover(m, f.data(), lambda_in.data(), alpha.data(), x.data(), u.data()));
+    xx = unpack(x);
+    check(hfls_mesh_write_vtk(m, u.data(), options2["path2data"].c_str(), it));
+  }
+  // the displacements of the last pcpg() as VTK (<path2data>/box.vtk for it < 0)
   void printVTK(int it) {
     check(hfls_mesh_write_vtk(m, solution.data(), options2["path2data"].c_str(), it));
   }
@@ -173,6 +219,11 @@ class Cluster {
   int nSubClst = 0, nLam_f = 0, nRBM_f = 0;
   int64_t nPacked = 0;
   std::vector<int> n_sub;
+  int64_t counts(int i) {
+    int64_t c[6];
+    check(hfls_mesh_counts(m, c));
+    return c[i];
+  }
   Vector pack(const std::vector<Vector>& v) {
     Vector out;
     out.reserve(nPacked);

@@ -99,10 +99,32 @@ int hfls_apply_F(hfls_mesh m, const double* w, double* Fw);
 int hfls_project(hfls_mesh m, const double* x, double* Px, double* alpha);
 int hfls_precondition(hfls_mesh m, const double* x, double* Mx);
 int hfls_kplus(hfls_mesh m, const double* f_local, double* x_local);
+/* coarse operators: lambda = Gf alpha (Cluster::mult_Gf, Cluster.cpp:1275-1283),
+ * e = Gf^T lambda (mult_GfT, 1285-1293), e = Rf^T f (mult_RfT, 1295-1312); alpha and
+ * e have 6 entries per cluster (Rf_d = R_d H_d, H the kernel of the cluster's
+ * Gc^T Gc, Cluster.cpp:1332-1352) */
+int hfls_mult_Gf(hfls_mesh m, const double* alpha, double* lambda);
+int hfls_mult_GfT(hfls_mesh m, const double* lambda, double* e);
+int hfls_mult_RfT(hfls_mesh m, const double* f_local, double* e);
+/* recovery of Cluster::printVTK (Cluster.cpp:1648-1678): x_local = K_c^+ (f -
+ * Bf^T lambda) - Rf alpha per subdomain (packed local) and u (3*nPoints) its scatter
+ * by l2g where the last subdomain holding a DOF writes it; alpha as returned by
+ * hfls_project.  x_local or u may be NULL. */
+int hfls_recover(hfls_mesh m, const double* f_local, const double* lambda, const double* alpha, double* x_local,
+                 double* u);
+/* Rf (Cluster.cpp:1332-1352): packed local, n_d x 6 column major per subdomain
+ * (the 6 columns of the subdomain's cluster) */
+int hfls_mesh_rf(hfls_mesh m, double* Rf);
+/* the getters of Cluster.hpp:26-31: [0] nSubClst, [1] neqClst (sum of n_d),
+ * [2] nLam_c, [3] nLam_f, [4] nRBM_f (6 per cluster), [5] nRBM_c (6 per subdomain) */
+int hfls_mesh_counts(hfls_mesh m, int64_t* counts);
 
 /* one load case: Cluster::pcpg (Cluster.cpp:1363-1458) + the recovery of
  * printVTK (Cluster.cpp:1648-1678).  f_local may be NULL (body force of the
- * options); u has 3*nPoints entries, lambda nlam (either may be NULL). */
+ * options); u has 3*nPoints entries, lambda nlam (either may be NULL).  With the
+ * option path2data set, the final displacements are written to <path2data>/box.vtk
+ * and, with vtkWithinIter=true, those of every iteration to box<it>.vtk (one
+ * eager iteration per host poll, as Cluster.cpp:1448-1449). */
 int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f_local, double* u, double* lambda,
                hfls_stats* stats);
 
@@ -119,7 +141,10 @@ int hfls_solve(hfls_mesh m, double eps, int max_iter, const double* f_local, dou
  * (a column that fails continues from the true residual). */
 typedef struct {
   int iterations;    /* F applications of this column's PCPG loop */
-  int status;        /* 2 converged (certified), 3 iteration limit */
+  int status;        /* 2 converged (certified), 3 iteration limit, 6 stopped on a
+                        non-finite residual (NaN/Inf in the column's loads or
+                        iterates; the other columns are unaffected), 7 (server
+                        only) the mesh's set-up failed */
   double residual;   /* FP64 true residual sqrt(g'PMPg)/sqrt(g0'PMPg0) at exit */
 } hfls_case_stats;
 typedef struct {
@@ -175,8 +200,14 @@ int hfls_server_state(hfls_server s, int64_t id, int* state);
  * latency_ms[1] = submit -> admitted; releases the request */
 int hfls_server_result(hfls_server s, int64_t id, double* u, hfls_case_stats* cases, double* latency_ms);
 /* [0] cached meshes, [1] device bytes held by them, [2] mesh setups, [3] evictions,
- * [4] cases done, [5] requests done, [6] slot joins, [7] summed setup ms */
+ * [4] cases done, [5] requests done, [6] slot joins, [7] summed setup ms,
+ * [8] failed set-ups, [9] cancelled requests (out holds 10) */
 int hfls_server_stats(hfls_server s, double* out);
+/* drop a request (no counterpart in the reference): queued cases are removed,
+ * running cases leave their slots at once without recovery; the id is unknown
+ * afterwards.  A request whose mesh set-up failed ends with status 7 per case and
+ * hfls_server_result returns the set-up error (after writing the statuses). */
+int hfls_server_cancel(hfls_server s, int64_t id);
 
 /* Measurement hook (no counterpart in the reference): average device time of
  * the dominant kernel over reps back-to-back launches timed with HIP events on

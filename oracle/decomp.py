@@ -149,7 +149,7 @@ def build_Bc(mesh, subs, glue, R, typeBc="ker", full_rank=True):
                         ent[i][0].extend([cnt] * len(dofs)); ent[i][1].extend(li); ent[i][2].extend(A[k])
                         ent[j][0].extend([cnt] * len(dofs)); ent[j][1].extend(lj); ent[j][2].extend(-A[k])
                         cnt += 1
-        elif typeBc in ("cor", "all"):
+        elif typeBc in ("cor", "all", "cor+edge"):
             # Cluster.cpp:846-903 with cornersOnly=true, addDirConstr=false
             corner = np.zeros(3 * mesh.nPoints, dtype=bool)
             corner[mesh.corner_dofs] = True
@@ -167,9 +167,54 @@ def build_Bc(mesh, subs, glue, R, typeBc="ker", full_rank=True):
                             cnt += 1
                             if full_rank:
                                 break
+            if typeBc == "cor+edge":
+                cnt = _edge_rows(mesh, l2g, glue, d0, d1, ent, cnt, full_rank)
         else:
             raise ValueError(typeBc)
         rows = {d: (np.array(v[0], dtype=np.int64), np.array(v[1], dtype=np.int64), np.array(v[2]))
                 for d, v in ent.items()}
         out.append({"nlam": cnt, "rows": rows, "d0": d0, "d1": d1})
     return out
+
+
+def _edge_rows(mesh, l2g, glue, d0, d1, ent, cnt, full_rank):
+    """Averaged edge constraints of typeBc=cor+edge (an extension: the reference has
+    only ker/cor/all).  For subdomain d ascending, its 12 box edges in the order axis
+    a = x, y, z, then the other two axes (b < c) at (lo, lo), (hi, lo), (lo, hi),
+    (hi, hi); the edge's m nodes lie strictly between its end vertices.  For the
+    first t > d of the cluster holding all of them (every such t when
+    Bc_fullRank=false) one row per component: sum_nodes u_d / m - u_t / m."""
+    NX, NY, _ = mesh.NE
+    nx1, nxy1 = NX + 1, (NX + 1) * (NY + 1)
+    for d in range(d0, d1):
+        g = mesh.grid[l2g[d][0::3] // 3]
+        lo, hi = g.min(axis=0), g.max(axis=0)
+        for a in range(3):
+            b, c = [x for x in range(3) if x != a]
+            m = int(hi[a] - lo[a]) - 1
+            if m < 1:
+                continue
+            for q in range(4):
+                fix = {b: hi[b] if q & 1 else lo[b], c: hi[c] if q & 2 else lo[c]}
+                nodes = []
+                for e in range(m):
+                    p = [0, 0, 0]
+                    p[a] = lo[a] + 1 + e
+                    p[b], p[c] = fix[b], fix[c]
+                    nodes.append(int(p[0] + p[1] * nx1 + p[2] * nxy1))
+                owners = set(glue.subs_of(3 * nodes[0]).tolist())
+                for nd in nodes[1:]:
+                    owners &= set(glue.subs_of(3 * nd).tolist())
+                for t in sorted(owners):
+                    if t <= d or t >= d1:
+                        continue
+                    for comp in range(3):
+                        dofs = np.array([3 * nd + comp for nd in nodes], dtype=np.int64)
+                        ld = np.searchsorted(l2g[d], dofs)
+                        lt = np.searchsorted(l2g[t], dofs)
+                        ent[d][0].extend([cnt] * m); ent[d][1].extend(ld); ent[d][2].extend([1.0 / m] * m)
+                        ent[t][0].extend([cnt] * m); ent[t][1].extend(lt); ent[t][2].extend([-1.0 / m] * m)
+                        cnt += 1
+                    if full_rank:
+                        break
+    return cnt

@@ -18,7 +18,7 @@ def global_solve(mesh, f_sub=None, subs=None):
     is their sum; otherwise the body force of the options is assembled."""
     o = mesh.o
     mu = float(o["poissons_ratio"])
-    E_el = mesh.young_per_sub()[mesh.part]
+    E_el = mesh.young_per_element()
     nodes = mesh.elements
     ieq = np.concatenate([3 * nodes, 3 * nodes + 1, 3 * nodes + 2], axis=1)
     Ke, fe = hex8(mesh.points[nodes], E_el, mu, load_vector(o))

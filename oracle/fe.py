@@ -112,6 +112,7 @@ class Subdomains:
         o = mesh.o
         mu = float(o["poissons_ratio"])
         Esub = mesh.young_per_sub()
+        Eel = mesh.young_per_element()
         acc = load_vector(o)
         self.mesh = mesh
         self.E = Esub
@@ -122,7 +123,7 @@ class Subdomains:
             ieq = np.concatenate([3 * nodes, 3 * nodes + 1, 3 * nodes + 2], axis=1)  # (ne, 24)
             l2g = np.unique(ieq)
             loc = np.searchsorted(l2g, ieq)
-            Ke, fe = hex8(mesh.points[nodes], np.full(len(el), Esub[d]), mu, acc)
+            Ke, fe = hex8(mesh.points[nodes], Eel[el], mu, acc)
             n = len(l2g)
             rows = np.repeat(loc, 24, axis=1).ravel()
             cols = np.tile(loc, (1, 24)).ravel()

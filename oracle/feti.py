@@ -109,14 +109,33 @@ class HTFETI:
                 roff.append(off)
                 off += self.R[d].shape[1]
             GtG = Gc.T @ Gc
-            H, dH, _ = kernel_from_K(GtG, None, seed=seed + 100000 + cl)
+            if d1 - d0 == 1:
+                # one subdomain: no cluster constraints, Gc^T Gc = 0 and its kernel is
+                # the whole 6-dimensional alpha space
+                H = np.eye(nr)
+            else:
+                H, dH, _ = kernel_from_K(GtG, None, seed=seed + 100000 + cl)
             nh = H.shape[1]
-            Ac = np.zeros((nlc + nr + nh, nlc + nr + nh))
-            Ac[:nlc, :nlc] = Fc
-            Ac[:nlc, nlc:nlc + nr] = Gc
-            Ac[nlc:nlc + nr, :nlc] = Gc.T
-            Ac[nlc:nlc + nr, nlc + nr:] = H
-            Ac[nlc + nr:, nlc:nlc + nr] = H.T
+            if o.get("Ac_extended_by_kerGc", "true") == "true":
+                # extended non-singular form (Cluster.cpp:592-660)
+                Ac = np.zeros((nlc + nr + nh, nlc + nr + nh))
+                Ac[:nlc, :nlc] = Fc
+                Ac[:nlc, nlc:nlc + nr] = Gc
+                Ac[nlc:nlc + nr, :nlc] = Gc.T
+                Ac[nlc:nlc + nr, nlc + nr:] = H
+                Ac[nlc + nr:, nlc:nlc + nr] = H.T
+            else:
+                # the singular Ac = [Fc Gc; Gc^T 0] with its kernel [0; H] regularised:
+                # A_r = Ac - sig [0 0; 0 H H^T] is a generalised inverse of Ac (A_r [0; H y]
+                # = [0; -sig H y]), sig = (tr(Gc^T Gc)/nr) / (tr(Fc)/nlc)
+                tf, tg = np.trace(Fc), np.trace(GtG)
+                sig = (tg / nr) / (tf / nlc) if (nlc > 0 and tf > 0) else 1.0
+                Ac = np.zeros((nlc + nr + nh, nlc + nr + nh))
+                Ac[:nlc, :nlc] = Fc
+                Ac[:nlc, nlc:nlc + nr] = Gc
+                Ac[nlc:nlc + nr, :nlc] = Gc.T
+                Ac[nlc:nlc + nr, nlc:nlc + nr] = -sig * (H @ H.T)
+                Ac[nlc + nr:, nlc + nr:] = np.eye(nh)  # decoupled rows: multiplier 0
             self.clusters.append({"d0": d0, "d1": d1, "nlc": nlc, "nr": nr, "roff": roff, "H": H,
                                   "Fc": Fc, "Gc": Gc, "Ac_lu": sla.lu_factor(Ac), "nh": nh,
                                   "Ac": Ac})

@@ -27,6 +27,7 @@ DEFAULTS = {
     "typeBc": "ker", "Bc_fullRank": "true", "preconditioner": "Dirichlet_explicit",
     "eps_iter": "1e-6", "max_iter": "200", "Ac_extended_by_kerGc": "true",
     "slabs_x": "", "slabs_y": "", "slabs_z": "", "slab_choices": "",
+    "metis": "false", "nparts": "1",
 }
 
 
@@ -151,6 +152,23 @@ class Mesh:
         self.material_id = np.array([int(math.remainder(p, 2)) for p in range(self.nsub)])
         self.corner_dofs = self._corners()
         self.dirichlet_dofs = self._dirichlet(o["bc"])
+        # geometric partition of every element; it keeps the material ids when metis
+        # merges the partition (Mesh.cpp:271-288, 425-427)
+        self.geo_part = self.part.copy()
+        self.geo_material_id = self.material_id.copy()
+        if o.get("metis", "false") == "true":
+            # Mesh::ddm_metis with nparts <= 1 puts every element in partition 0
+            # without calling METIS (Mesh.cpp:413-421): one subdomain, one cluster
+            if int(o.get("nparts", "1")) > 1:
+                raise ValueError("metis=true with nparts > 1 needs METIS")
+            if self.ncl != 1:
+                raise ValueError("metis=true needs one cluster")
+            self.Nxyz = [1, 1, 1]
+            self.slabs = [[NX], [NY], [NZ]]
+            self.starts = [np.array([0, n], dtype=np.int64) for n in (NX, NY, NZ)]
+            self.nsub = self.nsub_cl = 1
+            self.part = np.zeros_like(self.part)
+            self.material_id = np.array([0])
 
     def sub_id(self, I, J, K):
         Nx, Ny, Nz = self.Nxyz
@@ -197,6 +215,15 @@ class Mesh:
         else:
             raise ValueError("unknown bc " + bc)
         return np.array(out, dtype=np.int64)
+
+    def young_per_element(self):
+        """E per element: the reference material follows the geometric partition's
+        material id (Data.cpp:47-48), the other modes the subdomain's E."""
+        if self.o["material"] == "reference":
+            E = float(self.o["young_modulus"])
+            r = float(self.o["ratio_mat"])
+            return np.where(self.geo_material_id[self.geo_part] == 0, E * r, E)
+        return self.young_per_sub()[self.part]
 
     def young_per_sub(self):
         """E per subdomain (Data.cpp:36,47-48 for the reference mode)."""

@@ -31,7 +31,8 @@ UNEVEN = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", bc="clam
               young_modulus="2.1e5", seed="5", mat_sigma="1.5", mat_contrast="1e3", load="body")
 CASES = {"cube8_5e3": CUBE8_5E3, "multi": MULTI, "uneven": UNEVEN,
          # the other gluing and preconditioner of the reference (Cluster.cpp:846-903, 1106-1107)
-         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CUBE8_5E3, preconditioner="lumped")}
+         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CUBE8_5E3, preconditioner="lumped"),
+         "cube8_5e3_cor_edge": dict(CUBE8_5E3, typeBc="cor+edge")}
 SEED = 20240611
 K = 5
 

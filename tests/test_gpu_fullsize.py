@@ -1,0 +1,83 @@
+"""GPU solves at full size against the oracle's full-size fixtures
+(tests/golden/<name>_u.npz, written by tests/golden/golden_full.py from the
+oracle's sparse-LU PCPG at eps 1e-10):
+
+* cube512_10e3 -- the 512-subdomain cube (8 clusters x 4^3 subdomains of 10^3
+  hex8, 1.6 M DOF): body force and two seeded face tractions;
+* cube64_12e3 -- one cluster of 4^3 subdomains of 12^3 hex8 (the subdomain and
+  cluster shape of the 512 x 12^3 cube): four face tractions;
+* uneven512 -- 8 clusters x 4^3 subdomains, per-slab element counts drawn from
+  {6,8,10,12,14}, lognormal E (sigma 1.5, 1e3 contrast): body force.
+
+Per load case: plain FP64 at eps 1e-10 within 1e-8 relative of the oracle on the
+16,384 sampled DOFs and in |u|_2; plain FP64 at eps 1e-8 within +-2 iterations of
+the oracle's count at 1e-8; the mixed-precision block at eps 1e-8 certified on
+its FP64 true residual and within 1e-7 of the oracle.  The error of each result is
+printed.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import hfls_amd
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = ["cube512_10e3", "cube64_12e3", "uneven512"]
+
+
+def fixture_path(name):
+    return os.path.join(HERE, name + "_u.npz")
+
+
+@pytest.fixture(scope="module", params=NAMES)
+def workload(request):
+    name = request.param
+    path = fixture_path(name)
+    if not os.path.exists(path):
+        pytest.skip("fixture %s not written" % path)
+    with np.load(path) as f:
+        fx = {k: f[k].copy() for k in f.files}
+    kw = json.loads(str(fx["options"]))
+    cases = json.loads(str(fx["cases"]))
+    print("full-size workload", name, kw, "cases", cases, flush=True)
+    m = hfls_amd.Mesh(**kw)
+    F = np.zeros((len(cases), m.npacked))
+    for j, c in enumerate(cases):
+        F[j] = m.load_vector() if c == "body" else m.traction(int(fx["traction_seed"]), int(c), 1)[0]
+    yield name, m, fx, cases, F
+    m.close()
+
+
+def rel_sample(u, fx, j):
+    ref = fx["u_sample"][j]
+    return np.linalg.norm(u[fx["sample"]] - ref) / np.linalg.norm(ref)
+
+
+def test_plain_fp64(workload):
+    name, m, fx, cases, F = workload
+    for j, c in enumerate(cases):
+        r = m.pcpg(eps=1e-10, max_iter=2000, f_packed=F[j], want_lambda=False)
+        es = rel_sample(r["u"], fx, j)
+        en = abs(np.linalg.norm(r["u"]) - fx["u_norm"][j]) / fx["u_norm"][j]
+        r8 = m.pcpg(eps=1e-8, max_iter=2000, f_packed=F[j], want_lambda=False)
+        print("%s case %s: plain 1e-10 %d it (oracle %d), sample err %.2e, |u| err %.2e; 1e-8 %d it (oracle %d)" %
+              (name, c, r["iterations"], fx["iterations_1e10"][j], es, en, r8["iterations"],
+               fx["iterations_1e8"][j]), flush=True)
+        assert r["converged"] and r8["converged"]
+        assert es <= 1e-8 and en <= 1e-8
+        assert abs(r8["iterations"] - int(fx["iterations_1e8"][j])) <= 2
+
+
+def test_mixed_block(workload):
+    name, m, fx, cases, F = workload
+    b = m.pcpg_block(F, eps=1e-8, max_iter=2000, precision=32)
+    for j, c in enumerate(cases):
+        es = rel_sample(b["u"][j], fx, j)
+        print("%s case %s: mixed block %d it, status %d, true residual %.2e, sample err %.2e" %
+              (name, c, b["iterations"][j], b["status"][j], b["residual"][j], es), flush=True)
+        assert b["status"][j] == 2 and b["residual"][j] < 1e-8
+        assert es <= 1e-7

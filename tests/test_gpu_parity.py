@@ -28,7 +28,15 @@ UNEVEN = dict(Cx=2, Cy=1, Cz=1, Nx=2, Ny=2, Nz=2, slab_choices="2,3,4", bc="clam
               young_modulus="2.1e5", seed="5", mat_sigma="1.5", mat_contrast="1e3", load="body")
 CASES = {"cube8_5e3": CUBE8_5E3, "multi": MULTI, "refdefault": dict(nx=3, ny=3, nz=3, ratio_mat="10"), "uneven": UNEVEN,
          # the other gluing and preconditioner of the reference (Cluster.cpp:846-903, 1106-1107)
-         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CUBE8_5E3, preconditioner="lumped")}
+         "cor": dict(MULTI, typeBc="cor"), "lumped": dict(CUBE8_5E3, preconditioner="lumped"),
+         # the 2x2x2 x 5^3 cube with corner plus averaged-edge gluing, and the same gluing on two clusters
+         "cube8_5e3_cor_edge": dict(CUBE8_5E3, typeBc="cor+edge"), "multi_cor_edge": dict(MULTI, typeBc="cor+edge"),
+         # the reference's default cluster inverse: the singular Ac regularised by its kernel
+         "ac_regularized": dict(MULTI, Ac_extended_by_kerGc="false"),
+         # the reference's own map: metis=true with nparts=1 makes the 10x5x5 cube one
+         # subdomain (Mesh.cpp:413-421), checkerboard material of the geometric 2x1x1
+         # partition, symmetry planes, the reference body force
+         "metis_single": dict(Nx=2, Ny=1, Nz=1, nx=5, ny=5, nz=5, metis="true", nparts="1")}
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))
