@@ -25,7 +25,7 @@ EXPORTS = [
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
     "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk", "hfls_solve_blocks",
     "hfls_server_cancel", "hfls_mult_Gf", "hfls_mult_GfT", "hfls_mult_RfT", "hfls_recover", "hfls_mesh_rf",
-    "hfls_mesh_counts",
+    "hfls_mesh_counts", "hfls_project_noalloc",
 ]
 
 DEFAULTS = {

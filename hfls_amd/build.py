@@ -29,6 +29,7 @@ SOURCES = [
     ("tgemm16.hip", HIP_FLAGS),
     ("lrr.hip", HIP_FLAGS),
     ("block.hip", HIP_FLAGS),
+    ("noalloc.hip", HIP_FLAGS),
     ("problem.cpp", HOST_FLAGS + ["-ffp-contract=off"]),
     ("engine.cpp", HOST_FLAGS),
     ("engine_block.cpp", HOST_FLAGS),

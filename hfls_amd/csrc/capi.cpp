@@ -305,6 +305,13 @@ extern "C" int hfls_recover(hfls_mesh m, const double* f, const double* lam, con
   return guard([&] { eng(m)->recover(need(f, "f_local"), need(lam, "lambda"), need(alpha, "alpha"), x_local, u); });
 }
 
+extern "C" int hfls_project_noalloc(hfls_mesh m, const double* x_dev, double* Px_dev, double* alpha_dev,
+                                    void* stream) {
+  return guard([&] {
+    eng(m)->project_noalloc(need(x_dev, "x"), need(Px_dev, "Px"), alpha_dev, static_cast<hipStream_t>(stream));
+  });
+}
+
 extern "C" int hfls_mesh_rf(hfls_mesh m, double* Rf) {
   return guard([&] {
     std::vector<double> r;

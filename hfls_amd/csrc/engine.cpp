@@ -1262,6 +1262,9 @@ void Engine::setup() {
   sc.alloc(1);
   pgn.alloc(2);
   HC(hipStreamSynchronize(st));
+  // the scratch array of the allocation-free projection: its device address is
+  // looked up here, once per device, so that the entry point itself does no set-up
+  if (12 * (int64_t)N <= NOALLOC_SCRATCH_DOUBLES) (void)noalloc_scratch(device, 0);
   sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   size_t freeb = 0, totb = 0;
   HC(hipMemGetInfo(&freeb, &totb));
@@ -1592,6 +1595,19 @@ void Engine::get_Rf(std::vector<double>& Rf) const {
         Rf[o + l + k * n] = t;
       }
   }
+}
+
+void Engine::project_noalloc(const double* x, double* Px, double* alpha, hipStream_t stream) {
+  const int N = P->nsub;
+  double* ws = noalloc_scratch(device, 12 * (int64_t)N);  // every entry read below is written first
+  double* v = ws;                                       // R_b,d^T Bf_d^T x, 6 per subdomain
+  double* hb = ws + 6 * (int64_t)N;                     // H_d a_cl(d), 6 per subdomain
+  hipStream_t s = stream ? stream : st;
+  HC(launch_proj_dots(s, N, d_Rb.p, d_off_Rb.p, d_ns.p, d_off_b.p, d_gptr.p, d_gidx.p, x, v, nullptr));
+  HC(launch_proj_coarse(s, N, 6 * P->ncl, v, d_H.p, d_cl.p, d_invGtG.p, hb, alpha, 1.0, nullptr, -1.0));
+  HC(launch_gh(s, P->nlam, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, d_Rb.p, d_off_Rb.p, d_ns.p, hb, x, Px, -1.0, nullptr,
+               nullptr, nullptr));
+  if (!stream) HC(hipStreamSynchronize(st));
 }
 
 int64_t Engine::nlam_c() const {

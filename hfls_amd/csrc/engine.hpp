@@ -138,6 +138,13 @@ class Engine {
   // (packed local) and u its scatter to the global DOFs; either may be null
   void recover(const double* f_local, const double* lam, const double* alpha, double* x_local, double* u);
   void get_Rf(std::vector<double>& Rf) const;  // packed local, n x 6 column major per subdomain: R_d H_d
+  // Projection on the caller's device vectors (x, Px: nlam; alpha: 6 per cluster, the
+  // reference's alpha, may be null) with no allocation: the 12 doubles per subdomain
+  // of partial sums live in the library's file-scope scratch array (noalloc.hip).
+  // stream null: the library's stream, synchronised before return; otherwise the
+  // work is only enqueued on `stream`.  Calls that overlap on one GPU race on the
+  // shared scratch.
+  void project_noalloc(const double* x, double* Px, double* alpha, hipStream_t stream);
   int64_t nlam_c() const;                      // cluster constraints, all clusters
 
   // One load case: body-force load assembled on the device, or the given packed

@@ -1262,14 +1262,14 @@ hipError_t launch_proj_dots(hipStream_t st, int nsub, const double* Rb, const in
   return hipGetLastError();
 }
 
-// y = Gf^T x = [sum_{i in cl} H_i^T v_i]_cl ; alpha = (Gf^T Gf)^-1 y ;
-// h_i = sgn * H_i alpha_cl(i).  One workgroup; subdomains of a cluster are the
+// y = Gf^T x = [sum_{i in cl} H_i^T v_i]_cl ; a = (Gf^T Gf)^-1 y ; alpha = asgn * a ;
+// h_i = sgn * H_i a_cl(i).  One workgroup; subdomains of a cluster are the
 // contiguous ids [cl*nsub_cl, (cl+1)*nsub_cl).  (Cluster.cpp:1314-1329)
 __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const double* __restrict__ v,
                                                       const double* __restrict__ H, const int* __restrict__ cl,
                                                       const double* __restrict__ invGtG, double* __restrict__ h,
                                                       double* __restrict__ alpha, double sgn,
-                                                      const int* __restrict__ done) {
+                                                      const int* __restrict__ done, double asgn) {
   if (done && *done) return;
   __shared__ double y[1024];
   __shared__ double al[1024];
@@ -1292,7 +1292,7 @@ __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const 
     double s = 0.0;
     for (int j = 0; j < ncol; j++) s += invGtG[c + (int64_t)j * ncol] * y[j];
     al[c] = s;
-    if (alpha) alpha[c] = s;
+    if (alpha) alpha[c] = asgn * s;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < 6 * nsub; q += blockDim.x) {
@@ -1304,9 +1304,11 @@ __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const 
 }
 
 hipError_t launch_proj_coarse(hipStream_t st, int nsub, int ncol, const double* v, const double* H, const int* cl,
-                              const double* invGtG, double* h, double* alpha, double sgn, const int* done) {
+                              const double* invGtG, double* h, double* alpha, double sgn, const int* done,
+                              double asgn) {
   if (ncol > 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_proj_coarse, dim3(1), dim3(1024), 0, st, nsub, ncol, v, H, cl, invGtG, h, alpha, sgn, done);
+  hipLaunchKernelGGL(k_proj_coarse, dim3(1), dim3(1024), 0, st, nsub, ncol, v, H, cl, invGtG, h, alpha, sgn, done,
+                     asgn);
   return hipGetLastError();
 }
 

@@ -135,7 +135,7 @@ hipError_t launch_proj_dots(hipStream_t st, int nsub, const double* Rb, const in
                             double* v, const int* done);
 hipError_t launch_proj_coarse(hipStream_t st, int nsub, int ncol, const double* v, const double* H,
                               const int* cl, const double* invGtG, double* h, double* alpha, double sgn,
-                              const int* done);
+                              const int* done, double asgn = 1.0);
 hipError_t launch_gh(hipStream_t st, int64_t nlam, const int* rsub_a, const int* rsub_b, const int* rpos_a,
                      const int* rpos_b, const double* Rb, const int64_t* rboff, const int* ns, const double* h,
                      const double* x, double* out, double sgn, const double* dotv, double* part,
@@ -145,6 +145,12 @@ hipError_t launch_sub_bt(hipStream_t st, int nsub, int maxns, const int* nr, con
                          const int64_t* noff, const int64_t* boff, const int* gptr, const int* gidx,
                          const double* lam, const double* BcT, const int64_t* coff, const int* ldc, const int* nlc,
                          const int64_t* lcoff, const int* lcmap, const double* la, double* v);
+// device address of the file-scope scratch array of the allocation-free entry
+// point on the current device (looked up once per device), or an error if
+// `doubles` exceeds its size (NOALLOC_SCRATCH_DOUBLES)
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = 196608;  // 1.5 MiB: 12 doubles for each of 16384 subdomains
+double* noalloc_scratch(int device, int64_t doubles);
+
 hipError_t launch_recover(hipStream_t st, int nsub, int maxn, const int* n, const int64_t* noff, const double* R,
                           const int64_t* roff, const int64_t* alpha_idx, const double* la, const double* h,
                           const double* x, const int* gdof, const unsigned char* owner, double* u);

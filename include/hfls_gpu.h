@@ -112,6 +112,18 @@ int hfls_mult_RfT(hfls_mesh m, const double* f_local, double* e);
  * hfls_project.  x_local or u may be NULL. */
 int hfls_recover(hfls_mesh m, const double* f_local, const double* lambda, const double* alpha, double* x_local,
                  double* u);
+/* Projection on device memory with no allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * result as hfls_project, bit for bit: Px = x - Gf (Gf^T Gf)^-1 Gf^T x and alpha =
+ * -(Gf^T Gf)^-1 Gf^T x, with x_dev, Px_dev (nlam doubles) and alpha_dev (6 per
+ * cluster, may be NULL) the caller's device memory.  Its partial sums (12 doubles
+ * per subdomain) live in one file-scope device array per GPU of at most 196,608
+ * doubles (1.5 MiB, held until the process ends): meshes of more than 16,384
+ * subdomains are rejected with an error.  stream NULL: the library's stream,
+ * synchronised before return; otherwise (a hipStream_t) the work is only enqueued
+ * on that stream.  Calls that overlap on one GPU race on the shared array. */
+int hfls_project_noalloc(hfls_mesh m, const double* x_dev, double* Px_dev, double* alpha_dev, void* stream);
 /* Rf (Cluster.cpp:1332-1352): packed local, n_d x 6 column major per subdomain
  * (the 6 columns of the subdomain's cluster) */
 int hfls_mesh_rf(hfls_mesh m, double* Rf);
