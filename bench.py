@@ -134,6 +134,87 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def load_fixture(name):
+    """Oracle displacements of a full-size workload (tests/golden/<name>_u.npz: the
+    oracle's sparse-LU PCPG at eps 1e-10, u at a seeded sample of DOFs), or None."""
+    path = os.path.join(GOLDEN, name + "_u.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path) as f:
+        fx = {k: f[k].copy() for k in f.files}
+    fx["cases"] = json.loads(str(fx["cases"]))
+    fx["options"] = json.loads(str(fx["options"]))
+    return fx
+
+
+def fixture_error(fx, u, case):
+    """|u - u_oracle| / |u_oracle| on the fixture's sampled DOFs for load case `case`."""
+    j = fx["cases"].index(str(case))
+    ref = fx["u_sample"][j]
+    return float(np.linalg.norm(u[fx["sample"]] - ref) / np.linalg.norm(ref))
+
+
+def roofline_of(mesh):
+    """The split-FP16 tile GEMM over 64 columns with K+_bb (the dominant kernel):
+    HIP-event time per launch on the library stream and its algorithmic bytes."""
+    g_ms, g_bytes = mesh.time_stream(7, reps=10)
+    _, g_flops = mesh.time_stream(2, reps=1)
+    achieved = g_bytes / (g_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "kernel": "k_tgemm_h16", "kernel_ms": g_ms, "algorithmic_bytes": g_bytes,
+            "fp16_mfma_frac": 3.0 * g_flops / (g_ms * 1e-3) / 1e12 / FP16_MFMA_PEAK_TFS}
+
+
+def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fixture=None, fixture_cases=()):
+    """Set the mesh up, solve `warmup + steps` blocks of k load cases in mixed precision
+    and `warmup + plain_steps` single cases in FP64; per-block device times, rates,
+    iterations and, with a fixture, the error of the fixture's cases (columns
+    `fixture_cases` of the first block, and the plain solves of the same loads)
+    against the oracle."""
+    t0 = time.perf_counter()
+    mesh = hfls_amd.Mesh(device=0, **cfg)
+    setup_s = time.perf_counter() - t0
+    nblk = warmup + steps
+    if F is None:
+        F = mesh.traction(SEED, 0, k * nblk)
+    plain_ms, plain_it, err_plain = [], [], []
+    for i in range(warmup + plain_steps):
+        r = mesh.pcpg(eps=eps, max_iter=2000, f_packed=F[i], want_lambda=False)
+        assert r["converged"], "plain PCPG did not converge"
+        if i >= warmup:
+            plain_ms.append(r["solve_ms"])
+            plain_it.append(r["iterations"])
+        if fixture is not None and i < len(fixture_cases):
+            err_plain.append(fixture_error(fixture, r["u"], fixture_cases[i]))
+    blk_ms, its, res, err_mixed = [], [], [], []
+    for b in range(nblk):
+        r = mesh.pcpg_block(F[b * k:(b + 1) * k], eps=eps, max_iter=2000, precision=32)
+        assert np.all(r["status"] == 2), "a load case did not converge"
+        if b == 0 and fixture is not None:
+            err_mixed = [fixture_error(fixture, r["u"][j], c) for j, c in enumerate(fixture_cases)]
+        if b >= warmup:
+            blk_ms.append(r["solve_ms"])
+            its.append(r["iterations"])
+            res.append(float(r["residual"].max()))
+    its = np.concatenate(its)
+    rate = k * len(blk_ms) / (float(np.sum(blk_ms)) / 1e3)
+    plain_rate = 1e3 / float(np.mean(plain_ms))
+    out = {"solves_per_s": rate, "ms_per_step": float(np.mean(blk_ms)), "load_cases_per_block": k,
+           "subdomains": mesh.nsub, "dual_rows": mesh.nlam, "global_dof": 3 * mesh.nPoints,
+           "iterations_min_max": [int(its.min()), int(its.max())], "max_true_residual": max(res),
+           "plain": {"solves_per_s": plain_rate, "ms_per_solve": float(np.mean(plain_ms)), "iterations": plain_it[-1]},
+           "speedup_vs_plain": rate / plain_rate, "setup_s": round(setup_s, 3), "device_bytes": mesh.device_bytes,
+           "roofline": roofline_of(mesh)}
+    if fixture is not None:
+        out["error_vs_fp64_oracle"] = {"cases": [str(c) for c in fixture_cases], "mixed": err_mixed,
+                                       "plain_fp64": err_plain}
+    mesh.close()
+    return out
+
+
 FAMILIES = {64: dict(Cx=2, Cy=2, Cz=2, Nx=2, Ny=2, Nz=2), 216: dict(Cx=3, Cy=3, Cz=3, Nx=2, Ny=2, Nz=2),
             512: dict(Cx=2, Cy=2, Cz=2, Nx=4, Ny=4, Nz=4), 1000: dict(Cx=5, Cy=5, Cz=5, Nx=2, Ny=2, Nz=2)}
 
@@ -154,8 +235,14 @@ def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb, war
     (least recently used idle meshes are evicted); with warm, one single-case
     request per mesh runs before the stream starts.  Latency is submission to the
     last case recovered, set-up time of cold meshes included."""
-    rng = np.random.default_rng(seed)
     meshes = [serve_mesh(f, e, 100 + i) for i, (f, e) in enumerate((f, e) for f in families for e in edges)]
+    out = run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm)
+    out["meshes"] = ["%dx%d^3" % (f, e) for f in families for e in edges]
+    return out
+
+
+def run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm):
+    rng = np.random.default_rng(seed)
     srv = hfls_amd.Server(slots=64, precision=32, eps=eps, max_iter=1000, chunk=8, hbm_budget_gb=budget_gb)
     t0 = time.perf_counter()
     if warm:
@@ -195,8 +282,7 @@ def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb, war
     lat = np.array(lat)
     return {"requests": nreq, "load_cases": cases, "arrival_rate_per_s": rate, "wall_s": wall,
             "solves_per_s": cases / wall, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-            "meshes": ["%dx%d^3" % (f, e) for f in families for e in edges], "hbm_budget_gb": budget_gb,
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "hbm_budget_gb": budget_gb,
             "cache": "warmed before the stream" if warm else "cold start (set-ups inside the latencies)",
             "cache_warm_s": warm_s, "setups_in_stream": int(st["setups"] - st0["setups"]),
             "setup_s_in_stream": (st["setup_ms"] - st0["setup_ms"]) / 1e3,
@@ -223,6 +309,12 @@ def main():
     ap.add_argument("--serve-edges", default="8,9,10,11,12", help="elements per subdomain edge of the served meshes")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0, help="HBM budget of the server")
     ap.add_argument("--serve-warm", action="store_true", help="set every mesh up before the stream starts")
+    ap.add_argument("--configs", default="3,2",
+                    help="extra workloads: 3 = 512 x 12^3 cube, 2 = 512 uneven subdomains ('' for none)")
+    ap.add_argument("--config-steps", type=int, default=2, help="timed blocks of each extra workload")
+    ap.add_argument("--serve-saturated", type=int, default=48,
+                    help="requests of the saturated serving run on the cached 512 x 10^3 cube (0: none)")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 tile format (block_gemm=f32) run")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the iteration kernels directly instead of replaying captured HIP graphs")
     ap.add_argument("--kernel-only", action="store_true",
@@ -259,11 +351,17 @@ def main():
     F = mesh.traction(SEED, 0, a.k * max(1, nblk))
     log("loads: %d seeded face tractions" % F.shape[0])
 
+    # oracle displacements of the 512 x 10^3 cube (cases 0 and 1 are F[0], F[1])
+    fx = load_fixture("cube512_10e3") if (a.elements == 10 and not a.small and a.k >= 2) else None
+    err_plain, err_mixed = [], []
+
     # ---- plain FP64: one load case at a time
     plain_ms, plain_it, plain_res, plain_pg = [], [], [], []
     for i in range(a.warmup + a.plain_steps):
         r = mesh.pcpg(eps=a.eps, max_iter=1000, f_packed=F[i], want_lambda=False)
         assert r["converged"], "plain PCPG did not converge"
+        if fx is not None and i < 2:
+            err_plain.append(fixture_error(fx, r["u"], i))
         if i >= a.warmup:
             plain_ms.append(r["solve_ms"])
             plain_it.append(r["iterations"])
@@ -280,6 +378,8 @@ def main():
         r = mesh.pcpg_block(Fb, eps=a.eps, max_iter=1000, precision=32)
         w1 = time.perf_counter()
         assert np.all(r["status"] == 2), "a load case did not converge"
+        if s == 0 and fx is not None:
+            err_mixed = [fixture_error(fx, r["u"][j], j) for j in range(2)]
         log("block %d: %.1f ms device (init %.1f, pcpg %.1f, recover %.1f), wall %.1f ms, iterations %d..%d, "
             "max true residual %.2e, FP64 refinement points %d" %
             (s, r["solve_ms"], r["init_ms"], r["pcpg_ms"], r["recover_ms"], (w1 - w0) * 1e3, r["iterations"].min(),
@@ -352,6 +452,11 @@ def main():
         "concurrent_blocks": conc,
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),
         "block_vs_plain_displacement_rel_diff": agree,
+        "error_vs_fp64_oracle": (None if fx is None else
+                                 {"reference": "oracle (tests/golden/cube512_10e3_u.npz: sparse-LU PCPG at eps 1e-10, "
+                                               "16,384 sampled DOFs)",
+                                  "cases": ["traction 0", "traction 1"], "mixed_block": err_mixed,
+                                  "plain_fp64": err_plain, "max": max(err_mixed + err_plain)}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": GEMM_TRAFFIC_BYTES if (a.elements == 10 and not a.small) else None,
@@ -365,6 +470,46 @@ def main():
                                   "candidate_ms": [tune["ms"][1], tune["ms"][2]]}},
     }
     mesh.close()
+    if not a.small and not a.no_f32:
+        # the FP32 tile format (block_gemm=f32) of the same workload
+        cf = dict(cfg, block_gemm="f32")
+        r32 = block_workload(hfls_amd, cf, a.k, 1, 2, 1, a.eps)
+        out["block_gemm_f32"] = {"solves_per_s": r32["solves_per_s"], "ms_per_step": r32["ms_per_step"],
+                                 "iterations_min_max": r32["iterations_min_max"]}
+        log("block_gemm=f32: %.1f solves/s" % r32["solves_per_s"])
+    extra = {}
+    for c in [x for x in a.configs.split(",") if x] if not a.small else []:
+        if c == "3":  # 512 x 12^3, 64 face tractions per block
+            extra["cfg3_512sub_12e3_64cases"] = block_workload(hfls_amd, cube(12), a.k, 1, a.config_steps, 1, a.eps)
+        elif c == "2":  # 512 uneven subdomains, lognormal E; column 0 the body force (oracle fixture)
+            fu = load_fixture("uneven512")
+            if fu is not None:
+                cu = dict(fu["options"])  # the fixture's own options
+            else:
+                cu = dict(cube(10), slab_choices="6,8,10,12,14", seed=str(SEED), material="lognormal",
+                          mat_sigma="1.5", mat_contrast="1e3")
+                for key in ("nx", "ny", "nz"):
+                    cu.pop(key)
+            m2 = hfls_amd.Mesh(device=0, **cu)
+            Fu = m2.traction(SEED, 0, a.k * (1 + a.config_steps))
+            Fu[0] = m2.load_vector()
+            m2.close()
+            extra["cfg2_512sub_uneven_64cases"] = block_workload(
+                hfls_amd, cu, a.k, 1, a.config_steps, 1, a.eps, F=Fu, fixture=fu,
+                fixture_cases=("body",) if fu is not None else ())
+        else:
+            raise SystemExit("unknown extra config " + c)
+        log("config", c, json.dumps(extra[list(extra)[-1]]))
+    if extra:
+        out["configs"] = extra
+    if a.serve_saturated > 0 and not a.small:
+        # the served form at saturation: requests for the cached 512 x 10^3 cube arrive
+        # faster than they are served (Poisson, 200 per second), 1..64 cases each
+        sat = run_stream(hfls_amd, [cfg], a.serve_saturated, 200.0, SEED + 1, a.eps, 250.0, True)
+        sat["meshes"] = [workload]
+        sat["speedup_vs_plain"] = sat["solves_per_s"] / plain_rate
+        out["serving_saturated"] = sat
+        log("saturated serving:", json.dumps(sat))
     if a.serve_requests > 0:
         fam = [int(x) for x in a.serve_families.split(",")]
         edg = [int(x) for x in a.serve_edges.split(",")]

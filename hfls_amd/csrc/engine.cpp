@@ -297,6 +297,22 @@ void Engine::configure(const std::string& key, const std::string& value) {
       gemm_rows_forced = r;
     }
     if (blk_ready) tune_rt32 = gemm_rows_forced ? gemm_rows_forced : (tune_ms32[1] < tune_ms32[0] ? 2 : 1);
+  } else if (key == "gemm_lds_pad") {  // unused dynamic LDS bytes per workgroup of the split-FP16 GEMM
+    const int v = std::atoi(value.c_str());
+    if (v < 0 || v > 120 * 1024) throw std::runtime_error("gemm_lds_pad must be 0..122880");
+    gemm_lds_pad = v;
+  } else if (key == "gemm_pf") {  // K chunks in flight in the split-FP16 GEMM: 1 or 2
+    const int v = std::atoi(value.c_str());
+    if (v != 1 && v != 2) throw std::runtime_error("gemm_pf must be 1 or 2");
+    gemm_pf = v;
+  } else if (key == "gemm_rot") {  // tile order of the split-FP16 GEMM: pair, exact or a shift of odd rows
+    if (value == "pair") gemm_exact = 0;
+    else if (value == "exact") gemm_exact = 1;
+    else {
+      const int v = std::atoi(value.c_str());
+      if (v < 1 || v > 64) throw std::runtime_error("gemm_rot must be pair, exact or 1..64");
+      gemm_exact = v;
+    }
   } else if (key == "poll_chunk") {
     const int c = std::atoi(value.c_str());
     if (c < 1 || c > 64) throw std::runtime_error("poll_chunk must be 1..64");

@@ -270,6 +270,7 @@ class Engine {
   // ---- block state (allocated on the first block call)
   bool use_graph = true;
   int tune_rt32 = 1, gemm_rows_forced = 0;
+  int gemm_lds_pad = 0, gemm_exact = 0, gemm_pf = 1;  // split-FP16 GEMM: residency cap, tile order, prefetch
   double tune_ms32[2] = {0, 0};
   int poll_chunk = 8;
   hipGraphExec_t gx_plain = nullptr, gx_blk = nullptr;

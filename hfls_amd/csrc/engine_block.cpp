@@ -395,7 +395,7 @@ void Engine::gemm_lo(int which, const int* gate) {
   const float* A = which == 0 ? b_Kt32.p : b_St32.p;
   if (gemm16)
     HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
-                        KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 32 * tune_rt32));
+                        KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 32 * tune_rt32, 1, gemm_lds_pad, gemm_exact, gemm_pf));
   else
     HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
                         b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
@@ -550,7 +550,7 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
     else if (gemm16)
       HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, which == 0 ? b_Kt32.p : b_St32.p, d_off_P.p,
                           which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, nullptr, b_yb32.p,
-                          b_xob.p, n_xob, 32 * tune_rt32, 0));
+                          b_xob.p, n_xob, 32 * tune_rt32, 0, gemm_lds_pad, gemm_exact, gemm_pf));
     else
       gemm_lo(which, nullptr);
   };

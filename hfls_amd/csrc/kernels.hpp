@@ -249,7 +249,10 @@ hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, c
                             const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
                             const int* gate, float* Yf, const int* xcd_order = nullptr, int norder = 0,
                             int kc = 32,     // kc: K chunk depth staged per step (32 or 64)
-                            int scales = 1); // scales 0: sx already holds the scales of this X
+                            int scales = 1,
+                            int lds_pad = 0,  // unused dynamic LDS per workgroup (caps residency per CU)
+                            int exact = 0,    // tile order rotated by the row block itself (else its pair)
+                            int pf = 1);      // K chunks in flight while one is multiplied (1 or 2) // scales 0: sx already holds the scales of this X
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
                                  const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
                                  const int* nlc, float* P32);

@@ -134,13 +134,13 @@ __global__ __launch_bounds__(1024) void kb_colscale(const int* __restrict__ ns, 
 // FETCH_SIZE shows both reads come from HBM (2 x the packed tiles), so the kernel is
 // bound by that traffic.  Both chunk sizes issue the same matrix-core operations in
 // the same order, so they give bit-identical results.
-template <int KC>
+template <int KC, int PF>
 __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const int* __restrict__ order, int norder,
                                                    const int* __restrict__ mrows, const float* __restrict__ A,
                                                    const int64_t* __restrict__ aoff, const float* __restrict__ sA,
                                                    const float* __restrict__ X, const int64_t* __restrict__ xoff,
                                                    const float* __restrict__ sx, int kp, float alpha,
-                                                   const int* __restrict__ gate, float* __restrict__ Yf) {
+                                                   const int* __restrict__ gate, float* __restrict__ Yf, int exact) {
   constexpr int LDH = KC + 8;  // LDS row: KC halves + 8 of padding
   constexpr int NQ = KC / 32;  // 32-deep slices per chunk
   constexpr int CPT = 64 / KC; // chunks per tile
@@ -152,7 +152,8 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
   const int T = (mrows[d] + 63) / 64;
   const int i0 = sl % nslot;
   if (i0 >= T) return;
-  const int rot = (i0 & ~1) % T;  // K chunk c covers tile column (c / CPT - rot) mod T
+  // K chunk c covers tile column (c / CPT - rot) mod T; exact > 0: odd row blocks shifted by exact - 1 steps
+  const int rot = (exact ? i0 + (exact - 1) * (i0 & 1) : (i0 & ~1)) % T;
   const int c0 = blockIdx.z * 64;
   __shared__ __attribute__((aligned(16))) _Float16 Ah[64 * LDH], Al[64 * LDH], Xh[64 * LDH], Xl[64 * LDH];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -165,48 +166,51 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
   // X staging: column xc, rows k0 + 32 q + 8 xg .. + 7 of the chunk
   const int xc = t & 63, xg = t >> 6;
   const float xinv = 1.0f / sx[(int64_t)d * kp + c0 + xc];
-  half8 sah[NQ], sal[NQ];
-  float sxv[NQ][8];
-  bool tr = false;
-  auto fetch = [&](int c) {
+  // one staged K chunk in registers: A tile slices (hi, lo), X rows, orientation
+  struct Stage {
+    half8 sah[NQ], sal[NQ];
+    float sxv[NQ][8];
+    bool tr;
+  };
+  auto fetch = [&](int c, Stage& g) {
     int tj = c / CPT - rot;
     if (tj < 0) tj += T;
     const int k0 = (c % CPT) * KC;
-    tr = i0 < tj;  // op(A)(i0, tj) is stored tile (tj, i0) transposed
-    const int64_t ti = tr ? (int64_t)i0 * T - (int64_t)i0 * (i0 - 1) / 2 + (tj - i0)
-                          : (int64_t)tj * T - (int64_t)tj * (tj - 1) / 2 + (i0 - tj);
+    g.tr = i0 < tj;  // op(A)(i0, tj) is stored tile (tj, i0) transposed
+    const int64_t ti = g.tr ? (int64_t)i0 * T - (int64_t)i0 * (i0 - 1) / 2 + (tj - i0)
+                            : (int64_t)tj * T - (int64_t)tj * (tj - 1) / 2 + (i0 - tj);
     const _Float16* tile = Ad + ti * 8192;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       // direct: 8 rows (t&7)*8.. of column k0 + 32q + (t>>3); transposed: 8 k of row t>>2
       const int k = k0 + 32 * q;
-      const int64_t e = tr ? (int64_t)(t >> 2) * 64 + k + (t & 3) * 8 : (int64_t)(k + (t >> 3)) * 64 + (t & 7) * 8;
-      sah[q] = *(const half8*)(tile + e);
-      sal[q] = *(const half8*)(tile + 4096 + e);
+      const int64_t e = g.tr ? (int64_t)(t >> 2) * 64 + k + (t & 3) * 8 : (int64_t)(k + (t >> 3)) * 64 + (t & 7) * 8;
+      g.sah[q] = *(const half8*)(tile + e);
+      g.sal[q] = *(const half8*)(tile + 4096 + e);
       const float* xr = Xd + (int64_t)(tj * 64 + k + xg * 8) * kp + xc;
 #pragma unroll
-      for (int u = 0; u < 8; u++) sxv[q][u] = xr[(int64_t)u * kp];
+      for (int u = 0; u < 8; u++) g.sxv[q][u] = xr[(int64_t)u * kp];
     }
   };
-  auto stash = [&]() {
+  auto stash = [&](const Stage& g) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      if (tr) {
+      if (g.tr) {
         const int o = (t >> 2) * LDH + 32 * q + (t & 3) * 8;
-        *(half8*)(Ah + o) = sah[q];
-        *(half8*)(Al + o) = sal[q];
+        *(half8*)(Ah + o) = g.sah[q];
+        *(half8*)(Al + o) = g.sal[q];
       } else {
         const int kk = 32 * q + (t >> 3), r8 = (t & 7) * 8;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          Ah[(r8 + u) * LDH + kk] = sah[q][u];
-          Al[(r8 + u) * LDH + kk] = sal[q][u];
+          Ah[(r8 + u) * LDH + kk] = g.sah[q][u];
+          Al[(r8 + u) * LDH + kk] = g.sal[q][u];
         }
       }
       half8 h, l;
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const float x = sxv[q][u] * xinv;
+        const float x = g.sxv[q][u] * xinv;
         h[u] = (_Float16)x;
         l[u] = (_Float16)(x - (float)h[u]);
       }
@@ -214,15 +218,9 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
       *(half8*)(Xl + xc * LDH + 32 * q + xg * 8) = l;
     }
   };
-  const int nch = CPT * T;
-  fetch(0);
-  stash();
-  __syncthreads();
   const int ao = (rb + (lane & 31)) * LDH + 8 * (lane >> 5);
   const int bo = (cb + (lane & 31)) * LDH + 8 * (lane >> 5);
-  for (int c = 0; c < nch; c++) {
-    const bool more = c + 1 < nch;
-    if (more) fetch(c + 1);
+  auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 16) {
       const half8 ah = *(const half8*)(Ah + ao + kk), al = *(const half8*)(Al + ao + kk);
@@ -231,9 +229,41 @@ __global__ __launch_bounds__(256) void k_tgemm_h16(int nsub, int nslot, const in
       acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc, 0, 0, 0);
     }
-    __syncthreads();
-    if (more) stash();
-    __syncthreads();
+  };
+  const int nch = CPT * T;
+  Stage s0, s1;
+  fetch(0, s0);
+  if (PF == 2 && nch > 1) fetch(1, s1);
+  stash(s0);
+  __syncthreads();
+  if (PF == 1) {
+    // the next chunk's loads are in flight while the current one is multiplied
+    for (int c = 0; c < nch; c++) {
+      const bool more = c + 1 < nch;
+      if (more) fetch(c + 1, s0);
+      compute();
+      __syncthreads();
+      if (more) stash(s0);
+      __syncthreads();
+    }
+  } else {
+    // the loads of the two chunks after the current one are in flight; the loop
+    // body covers two chunks so that both register stages keep static names
+    for (int c = 0; c < nch; c += 2) {
+      if (c + 2 < nch) fetch(c + 2, s0);  // s0 held chunk c, already in LDS
+      compute();
+      __syncthreads();
+      if (c + 1 >= nch) break;
+      stash(s1);  // chunk c + 1
+      __syncthreads();
+      if (c + 3 < nch) fetch(c + 3, s1);
+      compute();
+      __syncthreads();
+      if (c + 2 < nch) {
+        stash(s0);  // chunk c + 2
+        __syncthreads();
+      }
+    }
   }
   const int col = cb + (lane & 31);
   // the two scales applied one after the other: their product alone may underflow
@@ -256,18 +286,25 @@ hipError_t launch_split16(hipStream_t st, int nsub, int maxtiles, const int* ns,
 
 hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, const float* A, const int64_t* aoff,
                             const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
-                            const int* gate, float* Yf, const int* xcd_order, int norder, int kc, int scales) {
+                            const int* gate, float* Yf, const int* xcd_order, int norder, int kc, int scales,
+                            int lds_pad, int exact, int pf) {
+  if (lds_pad < 0 || lds_pad > 120 * 1024 || (pf != 1 && pf != 2)) return hipErrorInvalidValue;
   if (nsub <= 0 || maxT <= 0) return hipSuccess;
   if (kp <= 0 || kp % 64 || (kc != 32 && kc != 64)) return hipErrorInvalidValue;
   if (xcd_order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
   if (scales) hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(1024), 0, st, ns, xoff, kp, X, sx, gate);
   const int groups = xcd_order ? norder / 8 : (nsub + 7) / 8;  // subdomain groups of 8, one per XCD
-  if (kc == 64)
-    hipLaunchKernelGGL(k_tgemm_h16<64>, dim3(8 * maxT * groups, 1, kp / 64), dim3(256), 0, st, nsub, maxT,
-                       xcd_order, norder, ns, A, aoff, sA, X, xoff, (const float*)sx, kp, alpha, gate, Yf);
-  else
-    hipLaunchKernelGGL(k_tgemm_h16<32>, dim3(8 * maxT * groups, 1, kp / 64), dim3(256), 0, st, nsub, maxT,
-                       xcd_order, norder, ns, A, aoff, sA, X, xoff, (const float*)sx, kp, alpha, gate, Yf);
+  // lds_pad: unused dynamic LDS per workgroup, which caps the workgroups resident per CU;
+  // pf: K chunks whose loads are in flight while one is multiplied (1 or 2)
+  const dim3 grid(8 * maxT * groups, 1, kp / 64);
+#define HFLS_TG16(KCv, PFv)                                                                                  \
+  hipLaunchKernelGGL((k_tgemm_h16<KCv, PFv>), grid, dim3(256), lds_pad, st, nsub, maxT, xcd_order, norder, ns, A, \
+                     aoff, sA, X, xoff, (const float*)sx, kp, alpha, gate, Yf, exact)
+  if (kc == 64 && pf == 2) HFLS_TG16(64, 2);
+  else if (kc == 64) HFLS_TG16(64, 1);
+  else if (pf == 2) HFLS_TG16(32, 2);
+  else HFLS_TG16(32, 1);
+#undef HFLS_TG16
   return hipGetLastError();
 }
 
