@@ -288,7 +288,9 @@ def run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm):
             "setup_s_in_stream": (st["setup_ms"] - st0["setup_ms"]) / 1e3,
             "evictions_in_stream": int(st["evictions"] - st0["evictions"]),
             "iterations_min_max": [int(min(iters)), int(max(iters))], "max_true_residual": worst,
-            "slot_joins": int(st["joins"])}
+            "slot_joins": int(st["joins"]), "rounds": int(st["steps"] - st0["steps"]),
+            "host_ms": {"join": st["join_ms"] - st0["join_ms"], "sweeps": st["sweep_ms"] - st0["sweep_ms"],
+                        "finish": st["finish_ms"] - st0["finish_ms"]}}
 
 
 def main():

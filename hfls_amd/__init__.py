@@ -476,9 +476,9 @@ class Server:
         self._k.pop(rid, None)
 
     def stats(self):
-        out = np.zeros(10)
+        out = np.zeros(14)
         _check(load().hfls_server_stats(self._h, _ptr(out)))
         keys = ["meshes", "bytes", "setups", "evictions", "cases_done", "requests_done", "joins", "setup_ms",
-                "failed_setups", "cancelled"]
+                "failed_setups", "cancelled", "steps", "join_ms", "sweep_ms", "finish_ms"]
         return dict(zip(keys, out.tolist()))
 

@@ -514,6 +514,10 @@ extern "C" int hfls_server_stats(hfls_server s, double* out) {
     out[7] = t.setup_ms;
     out[8] = (double)t.failed_setups;
     out[9] = (double)t.cancelled;
+    out[10] = (double)t.steps;
+    out[11] = t.join_ms;
+    out[12] = t.sweep_ms;
+    out[13] = t.finish_ms;
   });
 }
 

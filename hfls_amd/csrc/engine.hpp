@@ -202,6 +202,13 @@ class Engine {
   int blk_width() const { return blk_k; }
   // free slots without recovering them (a cancelled case); they take new cases later
   void blk_release(const std::vector<int>& slots) { blk_set_status(slots, 4); }
+  // seeded face-traction loads (traction_loads, problem.cpp) of load cases `cases`
+  // into the given slots: the smoothed face fields on the host, the lumping on the
+  // device (no packed local vector crosses PCIe)
+  void blk_load_traction(const std::vector<int>& slots, uint64_t seed, const std::vector<int>& cases);
+  // the recovered displacements of slot j (3 * nPoints, after blk_recover) to device
+  // memory dst, on the block's stream
+  void blk_copy_u(int j, double* dst);
   // autotuner result of the block GEMM (FP32 format: output row tiles per
   // workgroup 1 or 2; split-FP16 format: K chunk depth 32 or 64 as 1 or 2) and the
   // measured ms of each candidate
@@ -300,6 +307,10 @@ class Engine {
   DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
   int n_xob = 0, n_xor = 0;
   DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)
+  DVec<int> d_face_meta;                // subdomains on the x = 1 face (traction lumping)
+  DVec<int64_t> d_face_off;
+  DVec<double> d_trac;                  // face traction fields of the loads being joined
+  int n_face = -1;
   int n_gchunk = 0;
   DVec<double> pgn;    // |P g0|^2 and |P g|^2 of the plain solve
   DVec<double> b_gtp;  // G^T x chunk partials

@@ -619,6 +619,45 @@ void Engine::blk_load(const std::vector<int>& cols, const double* f_h) {
   }
 }
 
+void Engine::blk_load_traction(const std::vector<int>& cols, uint64_t seed, const std::vector<int>& cases) {
+  if (cols.size() != cases.size()) throw std::runtime_error("one load case per slot");
+  const int NY = P->NE[1], NZ = P->NE[2], NX = P->NE[0];
+  const int ny1 = NY + 1, nz1 = NZ + 1;
+  const int64_t np = off_n[P->nsub], field = 3 * (int64_t)ny1 * nz1;
+  if (n_face < 0) {  // the subdomains on the x = 1 face, once
+    std::vector<int> meta;
+    std::vector<int64_t> foff;
+    for (int d = 0; d < P->nsub; d++) {
+      const Subdomain& s = P->subs[d];
+      if (s.e0[0] + s.ne[0] != NX) continue;
+      meta.insert(meta.end(), {s.e0[1], s.ne[1], s.e0[2], s.ne[2], s.nn[0], s.nn[0] * s.nn[1]});
+      foff.push_back(off_n[d]);
+    }
+    n_face = (int)foff.size();
+    if (meta.empty()) meta.assign(6, 0), foff.assign(1, 0);
+    d_face_meta.upload(meta, st);
+    d_face_off.upload(foff, st);
+  }
+  if (d_trac.n < (size_t)(field * KMAX)) d_trac.alloc(field * KMAX);
+  std::vector<double> t((size_t)field * cols.size());
+  for (size_t c = 0; c < cols.size(); c++) traction_face(*P, seed, cases[c], t.data() + c * field);
+  HC(hipMemcpyAsync(d_trac.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  for (size_t c = 0; c < cols.size(); c++) {
+    if (cols[c] < 0 || cols[c] >= blk_k) throw std::runtime_error("slot out of range");
+    double* dst = b_V.p + (size_t)cols[c] * np;
+    HC(hipMemsetAsync(dst, 0, np * sizeof(double), st));
+    HC(launch_traction_lump(st, n_face, d_face_meta.p, d_face_off.p, ny1, nz1, 1.0 / NY, 1.0 / NZ,
+                            d_trac.p + c * field, dst));
+  }
+  HC(hipStreamSynchronize(st));  // the host field buffer is released on return
+}
+
+void Engine::blk_copy_u(int j, double* dst) {
+  if (j < 0 || j >= blk_k) throw std::runtime_error("slot out of range");
+  const int64_t ng = 3 * (int64_t)P->nPoints;
+  HC(hipMemcpyAsync(dst, b_U.p + (size_t)j * ng, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
 void Engine::blk_init(const std::vector<int>& cols) {
   const int N = P->nsub, k = blk_k, prec = blk_prec;
   const int64_t nl = P->nlam;

@@ -145,6 +145,10 @@ hipError_t launch_sub_bt(hipStream_t st, int nsub, int maxns, const int* nr, con
                          const int64_t* noff, const int64_t* boff, const int* gptr, const int* gidx,
                          const double* lam, const double* BcT, const int64_t* coff, const int* ldc, const int* nlc,
                          const int64_t* lcoff, const int* lcmap, const double* la, double* v);
+// f (one packed local load, zeroed before) += the lumped face traction t of one load
+// case (traction_loads' lumping) over the nface subdomains on the x = 1 face
+hipError_t launch_traction_lump(hipStream_t st, int nface, const int* meta, const int64_t* foff, int ny1, int nz1,
+                                double hy, double hz, const double* t, double* f);
 // device address of the file-scope scratch array of the allocation-free entry
 // point on the current device (looked up once per device), or an error if
 // `doubles` exceeds its size (NOALLOC_SCRATCH_DOUBLES)

@@ -299,4 +299,34 @@ hipError_t launch_sqnorm(hipStream_t st, int64_t n, const double* x, double* out
   return hipGetLastError();
 }
 
+// Lumped nodal forces of a smoothed face traction t (c, kk, jj) on the x = 1 face
+// into one packed local load vector f (zeroed before): for every subdomain on that
+// face, f = t * (cnt * hy * hz * 0.25) at its face nodes, cnt the subdomain's face
+// elements at the node -- the lumping of traction_loads (problem.cpp), the same
+// IEEE operations in the same order.  meta per face subdomain: e0 y, ne y, e0 z,
+// ne z, local nodes along x, local nodes in an x-y layer.
+__global__ __launch_bounds__(256) void k_traction_lump(const int* __restrict__ meta, const int64_t* __restrict__ foff,
+                                                       int ny1, int nz1, double hy, double hz,
+                                                       const double* __restrict__ t, double* __restrict__ f) {
+  const int q = blockIdx.x;
+  const int* mq = meta + 6 * q;
+  const int e0y = mq[0], ney = mq[1], e0z = mq[2], nez = mq[3], nn0 = mq[4], nn01 = mq[5];
+  double* fd = f + foff[q];
+  const int nodes = (ney + 1) * (nez + 1);
+  for (int p = threadIdx.x; p < nodes; p += 256) {
+    const int a = p % (ney + 1), b = p / (ney + 1);
+    const int c1 = (a > 0) + (a < ney), c2 = (b > 0) + (b < nez);
+    const double w = (double)(c1 * c2) * hy * hz * 0.25;
+    const int ln = (nn0 - 1) + a * nn0 + b * nn01;
+    for (int c = 0; c < 3; c++) fd[3 * ln + c] = t[((int64_t)c * nz1 + e0z + b) * ny1 + e0y + a] * w;
+  }
+}
+
+hipError_t launch_traction_lump(hipStream_t st, int nface, const int* meta, const int64_t* foff, int ny1, int nz1,
+                                double hy, double hz, const double* t, double* f) {
+  if (nface <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_traction_lump, dim3(nface), dim3(256), 0, st, meta, foff, ny1, nz1, hy, hz, t, f);
+  return hipGetLastError();
+}
+
 }  // namespace hfls

@@ -430,45 +430,56 @@ Problem::Problem(const Options& oin) : o(oin) {
 //   * lumped nodal forces per subdomain: f = t * (cnt * hy * hz * 0.25), cnt the
 //     number of the subdomain's face elements at the node, hy = 1/NY, hz = 1/NZ.
 // Output: k packed local vectors (column j at j * sum_d n_d), cases case0..case0+k-1.
+void traction_face(const Problem& P, uint64_t seed, int lcase, double* t) {
+  const int NY = P.NE[1], NZ = P.NE[2];
+  const int ny1 = NY + 1, nz1 = NZ + 1;
+  std::vector<double> u((size_t)3 * ny1 * nz1), cnt((size_t)ny1 * nz1);
+  auto at = [&](int c, int jj, int kk) -> size_t { return ((size_t)c * nz1 + kk) * ny1 + jj; };
+  const uint64_t cs = (uint64_t)lcase + 1;
+  uint64_t st = seed ^ (0xD1B54A32D192ED03ULL * cs);
+  for (int kk = 0; kk < nz1; kk++)
+    for (int jj = 0; jj < ny1; jj++)
+      for (int c = 0; c < 3; c++) {
+        double u0 = uniform01(st), u1 = uniform01(st);
+        t[at(c, jj, kk)] = std::sqrt(-2.0 * std::log(u0)) * std::cos(2.0 * M_PI * u1);
+      }
+  for (int kk = 0; kk < nz1; kk++)
+    for (int jj = 0; jj < ny1; jj++) {
+      double m = 1.0;
+      if (jj > 0) m += 1.0;
+      if (jj < NY) m += 1.0;
+      if (kk > 0) m += 1.0;
+      if (kk < NZ) m += 1.0;
+      cnt[(size_t)kk * ny1 + jj] = m;
+    }
+  double* a = t;
+  double* b = u.data();
+  for (int pass = 0; pass < 3; pass++) {
+    for (int c = 0; c < 3; c++)
+      for (int kk = 0; kk < nz1; kk++)
+        for (int jj = 0; jj < ny1; jj++) {
+          double v = a[at(c, jj, kk)];
+          if (jj > 0) v += a[at(c, jj - 1, kk)];
+          if (jj < NY) v += a[at(c, jj + 1, kk)];
+          if (kk > 0) v += a[at(c, jj, kk - 1)];
+          if (kk < NZ) v += a[at(c, jj, kk + 1)];
+          b[at(c, jj, kk)] = v / cnt[(size_t)kk * ny1 + jj];
+        }
+    std::swap(a, b);
+  }
+  if (a != t) std::copy(a, a + u.size(), t);  // three passes: the result is in u
+}
+
 void traction_loads(const Problem& P, uint64_t seed, int case0, int k, double* f) {
   const int NY = P.NE[1], NZ = P.NE[2], NX = P.NE[0];
   const int ny1 = NY + 1, nz1 = NZ + 1;
   const double hy = 1.0 / NY, hz = 1.0 / NZ;
   int64_t np = 0;
   for (const Subdomain& s : P.subs) np += s.n;
-  std::vector<double> t((size_t)3 * ny1 * nz1), u(t.size()), cnt((size_t)ny1 * nz1);
+  std::vector<double> t((size_t)3 * ny1 * nz1);
   auto at = [&](int c, int jj, int kk) -> size_t { return ((size_t)c * nz1 + kk) * ny1 + jj; };
   for (int j = 0; j < k; j++) {
-    const uint64_t cs = (uint64_t)(case0 + j) + 1;
-    uint64_t st = seed ^ (0xD1B54A32D192ED03ULL * cs);
-    for (int kk = 0; kk < nz1; kk++)
-      for (int jj = 0; jj < ny1; jj++)
-        for (int c = 0; c < 3; c++) {
-          double u0 = uniform01(st), u1 = uniform01(st);
-          t[at(c, jj, kk)] = std::sqrt(-2.0 * std::log(u0)) * std::cos(2.0 * M_PI * u1);
-        }
-    for (int pass = 0; pass < 3; pass++) {
-      for (int kk = 0; kk < nz1; kk++)
-        for (int jj = 0; jj < ny1; jj++) {
-          double m = 1.0;
-          if (jj > 0) m += 1.0;
-          if (jj < NY) m += 1.0;
-          if (kk > 0) m += 1.0;
-          if (kk < NZ) m += 1.0;
-          cnt[(size_t)kk * ny1 + jj] = m;
-        }
-      for (int c = 0; c < 3; c++)
-        for (int kk = 0; kk < nz1; kk++)
-          for (int jj = 0; jj < ny1; jj++) {
-            double v = t[at(c, jj, kk)];
-            if (jj > 0) v += t[at(c, jj - 1, kk)];
-            if (jj < NY) v += t[at(c, jj + 1, kk)];
-            if (kk > 0) v += t[at(c, jj, kk - 1)];
-            if (kk < NZ) v += t[at(c, jj, kk + 1)];
-            u[at(c, jj, kk)] = v / cnt[(size_t)kk * ny1 + jj];
-          }
-      t.swap(u);
-    }
+    traction_face(P, seed, case0 + j, t.data());
     double* fj = f + (size_t)j * np;
     int64_t off = 0;
     for (const Subdomain& s : P.subs) {

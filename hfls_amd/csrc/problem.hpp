@@ -85,5 +85,9 @@ double uniform01(uint64_t& state);
 // k seeded random smoothed tractions on the x = 1 face as packed local loads
 // (column j at j * sum_d n_d) for load cases case0 .. case0 + k - 1
 void traction_loads(const Problem& P, uint64_t seed, int case0, int k, double* f);
+// the smoothed traction field t[(c * (NZ + 1) + kk) * (NY + 1) + jj] of load case lcase
+// on the x = 1 face (the first part of traction_loads; the lumping to subdomain
+// loads can then run on the device)
+void traction_face(const Problem& P, uint64_t seed, int lcase, double* t);
 
 }  // namespace hfls

@@ -83,7 +83,7 @@ size_t Server::estimate(const Problem& p) const {
 }
 
 size_t Server::bytes_in_use() const {
-  size_t s = pend_ ? pend_->need : 0;
+  size_t s = (pend_ ? pend_->need : 0) + result_bytes_;
   for (auto& kv : mesh_) s += kv.second->bytes;
   return s;
 }
@@ -157,14 +157,17 @@ int64_t Server::submit(const Options& o, int k, const double* F, uint64_t seed, 
   r.id = next_id_++;
   r.key = key;
   r.k = k;
-  r.F.resize((size_t)k * np);
-  if (F)
+  if (F) {
+    r.F.resize((size_t)k * np);
     std::memcpy(r.F.data(), F, r.F.size() * sizeof(double));
-  else
-    traction_loads(p, seed, case0, k, r.F.data());
+  } else {  // generated at the join: fields on the host, lumping on the device
+    r.seeded = true;
+    r.seed = seed;
+    r.case0 = case0;
+  }
   r.t_submit = now();
   r.remaining = k;
-  r.U.assign((size_t)k * 3 * p.nPoints, 0.0);
+  r.ng = 3 * (int64_t)p.nPoints;
   r.it.assign(k, 0);
   r.status.assign(k, 0);
   r.res.assign(k, 0.0);
@@ -265,11 +268,22 @@ bool Server::step() {
     finish_setup(true);
     admission();
   }
-  // 2. free slots take queued cases (all meshes, asynchronously)
+  // 2. free slots take queued cases (all meshes, asynchronously).  A join runs the
+  //    initialisation over the whole block, so cases join in batches: when at least
+  //    a quarter of the slots (or every queued case) can start, or when the block
+  //    has nothing else to run
+  const int batch = std::max(1, slots_ / 4);
+  st_.steps++;
+  double tph = now();
   for (auto& kv : mesh_) {
     ServeMesh& m = *kv.second;
-    std::vector<int> cols;
+    if (m.queue.empty()) continue;
+    int nfree = 0, nbusy = 0;
+    for (auto& sl : m.slot) (sl.req < 0 ? nfree : nbusy)++;
+    if (nfree == 0 || (nbusy > 0 && nfree < std::min<int>((int)m.queue.size(), batch))) continue;
+    std::vector<int> cols, fcols;
     std::vector<double> F;
+    std::map<uint64_t, std::pair<std::vector<int>, std::vector<int>>> tcols;  // seed -> slots, load cases
     for (int j = 0; j < slots_ && !m.queue.empty(); j++) {
       if (m.slot[j].req >= 0) continue;
       ServeSlot c = m.queue.front();
@@ -277,15 +291,26 @@ bool Server::step() {
       m.slot[j] = c;
       ServeRequest& r = req_[c.req];
       cols.push_back(j);
-      F.insert(F.end(), r.F.begin() + (size_t)c.idx * m.np, r.F.begin() + (size_t)(c.idx + 1) * m.np);
+      if (r.seeded) {
+        auto& tc = tcols[r.seed];
+        tc.first.push_back(j);
+        tc.second.push_back(r.case0 + c.idx);
+      } else {
+        fcols.push_back(j);
+        F.insert(F.end(), r.F.begin() + (size_t)c.idx * m.np, r.F.begin() + (size_t)(c.idx + 1) * m.np);
+      }
       r.started++;
     }
     if (!cols.empty()) {
-      m.e->blk_join(cols, F.data());
+      if (!fcols.empty()) m.e->blk_load(fcols, F.data());
+      for (auto& tc : tcols) m.e->blk_load_traction(tc.second.first, tc.first, tc.second.second);
+      m.e->blk_init(cols);
       st_.joins += (int64_t)cols.size();
     }
   }
   // 3. one chunk of sweeps for every mesh with work (different streams overlap)
+  st_.join_ms += now() - tph;
+  tph = now();
   for (auto& kv : mesh_) {
     ServeMesh& m = *kv.second;
     m.swept = false;
@@ -302,14 +327,45 @@ bool Server::step() {
     if (!m.swept) continue;
     BScalars sh;
     m.e->blk_sync(&sh);
+    st_.sweep_ms += now() - tph;  // the meshes' chunks run concurrently: waited for once
+    tph = now();
     // stopped: converged, iteration limit, or non-finite (a bad case ends alone;
     // the other slots' arithmetic does not depend on it)
     std::vector<int> cols;
-    for (int j = 0; j < slots_; j++)
-      if (m.slot[j].req >= 0 && (sh.done[j] == 2 || sh.done[j] == 3 || sh.done[j] == 6)) cols.push_back(j);
+    int nrun = 0, maxw = 0;
+    if ((int)m.waited.size() != slots_) m.waited.assign(slots_, -1);
+    for (int j = 0; j < slots_; j++) {
+      if (m.slot[j].req < 0) continue;
+      if (sh.done[j] == 2 || sh.done[j] == 3 || sh.done[j] == 6) {
+        cols.push_back(j);
+        m.waited[j] = m.waited[j] < 0 ? 0 : m.waited[j] + 1;
+        maxw = std::max(maxw, m.waited[j]);
+      } else {
+        nrun++;
+      }
+    }
     if (cols.empty()) continue;
-    std::vector<double> U(cols.size() * (size_t)m.ng);
-    m.e->blk_finish(cols, U.data(), nullptr, &sh);
+    // the recovery also runs over the whole block: stopped cases are recovered
+    // together once a quarter of the slots have stopped, when nothing else runs,
+    // when queued cases need their slots, or after 4 chunks of waiting
+    int nfree = 0;
+    for (auto& sl : m.slot) nfree += sl.req < 0 ? 1 : 0;
+    const bool need = !m.queue.empty() && nfree < std::min<int>((int)m.queue.size(), batch);
+    if (!((int)cols.size() >= batch || nrun == 0 || need || maxw >= 4)) continue;
+    for (int j : cols) m.waited[j] = -1;
+    // recovery on the device, the displacements stay there (device to device into
+    // the request's result buffer) until result() fetches them
+    m.e->blk_recover(&sh);
+    for (int j : cols) {
+      ServeRequest& r = req_[m.slot[j].req];
+      if (!r.dU) {
+        r.dU = std::make_shared<DVec<double>>();
+        r.dU->alloc((size_t)r.k * r.ng);
+        result_bytes_ += r.dU->n * sizeof(double);
+      }
+      m.e->blk_copy_u(j, r.dU->p + (size_t)m.slot[j].idx * r.ng);
+    }
+    m.e->blk_release(cols);
     BScalars s2;
     m.e->blk_sync(&s2);
     const double t = now();
@@ -317,7 +373,6 @@ bool Server::step() {
       const int j = cols[c];
       ServeRequest& r = req_[m.slot[j].req];
       const int i = m.slot[j].idx;
-      std::copy(U.begin() + c * m.ng, U.begin() + (c + 1) * m.ng, r.U.begin() + (size_t)i * m.ng);
       r.it[i] = sh.it[j];
       r.status[i] = sh.done[j];
       r.res[i] = s2.res[j];
@@ -329,6 +384,8 @@ bool Server::step() {
       m.slot[j] = ServeSlot();
     }
     st_.finishes += (int64_t)cols.size();
+    st_.finish_ms += now() - tph;
+    tph = now();
   }
   bool busy = !wait_.empty() || pend_ != nullptr;
   for (auto& kv : mesh_) {
@@ -359,7 +416,15 @@ void Server::result(int64_t id, double* u, int* it, int* status, double* res, do
   ServeRequest& r = f->second;
   if (r.t_done < 0) throw std::runtime_error("request not finished");
   if (k) *k = r.k;
-  if (u) std::memcpy(u, r.U.data(), r.U.size() * sizeof(double));
+  if (u) {
+    if (r.dU) {
+      HC(hipSetDevice(device_));
+      HC(hipMemcpy(u, r.dU->p, r.dU->n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::memset(u, 0, (size_t)r.k * r.ng * sizeof(double));  // no case ran (failed set-up)
+    }
+  }
+  if (r.dU) result_bytes_ -= r.dU->n * sizeof(double);
   for (int i = 0; i < r.k; i++) {
     if (it) it[i] = r.it[i];
     if (status) status[i] = r.status[i];
@@ -398,6 +463,7 @@ void Server::cancel(int64_t id) {
     }
   }
   if (f->second.t_done < 0) st_.cancelled++;
+  if (f->second.dU) result_bytes_ -= f->second.dU->n * sizeof(double);
   req_.erase(f);
 }
 

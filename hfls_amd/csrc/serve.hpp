@@ -30,10 +30,14 @@ struct ServeRequest {
   int64_t id = 0;
   std::string key;
   int k = 0;
-  std::vector<double> F;               // k packed local loads
+  std::vector<double> F;               // k packed local loads (empty for seeded tractions)
+  bool seeded = false;                 // face tractions of load cases case0 .. case0 + k - 1
+  uint64_t seed = 0;
+  int case0 = 0;
   double t_submit = 0, t_admit = -1, t_done = -1;  // ms since the server started
   int started = 0, remaining = 0;
-  std::vector<double> U;               // k x 3*nPoints
+  int64_t ng = 0;                      // 3 * nPoints
+  std::shared_ptr<DVec<double>> dU;    // k x 3*nPoints displacements, device resident until fetched
   std::vector<int> it, status;
   std::vector<double> res;
   std::string error;                   // set-up failure of the request's mesh
@@ -54,12 +58,14 @@ struct ServeMesh {
   std::deque<ServeSlot> queue;         // cases admitted, waiting for a slot
   bool swept = false;
   double last_used = 0;
+  std::vector<int> waited;             // per slot: chunks since its case stopped (-1: not stopped)
 };
 
 struct ServeStats {
   int64_t cases_done = 0, requests_done = 0, setups = 0, evictions = 0, joins = 0, finishes = 0;
-  int64_t failed_setups = 0, cancelled = 0;
+  int64_t failed_setups = 0, cancelled = 0, steps = 0;
   double setup_ms = 0;
+  double join_ms = 0, sweep_ms = 0, finish_ms = 0;  // host wall time of the step phases (device work included)
 };
 
 class Server {
@@ -80,7 +86,7 @@ class Server {
   // slots (without recovery); the id is unknown afterwards
   void cancel(int64_t id);
   const ServeStats& stats() const { return st_; }
-  size_t bytes_in_use() const;
+  size_t bytes_in_use() const;  // cached meshes, a set-up in flight, results not yet fetched
   int meshes() const { return (int)mesh_.size(); }
 
  private:
@@ -101,6 +107,7 @@ class Server {
     std::future<std::unique_ptr<ServeMesh>> fut;
   };
   std::unique_ptr<PendingSetup> pend_;
+  size_t result_bytes_ = 0;                      // device bytes of results not yet fetched
   bool finish_setup(bool wait);                  // install a finished set-up; true if one was installed
   void fail_waiting(const std::string& key, const std::string& msg);  // requests of a failed set-up
   bool any_work() const;                         // a cached mesh has running or queued cases

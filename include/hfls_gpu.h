@@ -213,7 +213,9 @@ int hfls_server_state(hfls_server s, int64_t id, int* state);
 int hfls_server_result(hfls_server s, int64_t id, double* u, hfls_case_stats* cases, double* latency_ms);
 /* [0] cached meshes, [1] device bytes held by them, [2] mesh setups, [3] evictions,
  * [4] cases done, [5] requests done, [6] slot joins, [7] summed setup ms,
- * [8] failed set-ups, [9] cancelled requests (out holds 10) */
+ * [8] failed set-ups, [9] cancelled requests, [10] scheduling rounds, [11] [12]
+ * [13] host wall ms spent in joins, sweeps (to the stream sync) and finishes
+ * (out holds 14) */
 int hfls_server_stats(hfls_server s, double* out);
 /* drop a request (no counterpart in the reference): queued cases are removed,
  * running cases leave their slots at once without recovery; the id is unknown
