@@ -64,7 +64,10 @@ def cpu_sample():
     from threadpoolctl import threadpool_limits
     from oracle.mesh import opts
     from oracle.feti import HTFETI
-    h = HTFETI(opts(eps_iter="1e-8", **SAMPLE), sparse=True)
+    with threadpool_limits(limits=1):
+        ts = time.perf_counter()
+        h = HTFETI(opts(eps_iter="1e-8", **SAMPLE), sparse=True)
+        setup_s = time.perf_counter() - ts
     reps = 0
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()
@@ -74,7 +77,7 @@ def cpu_sample():
             if time.perf_counter() - t0 > 10.0:
                 break
         t = (time.perf_counter() - t0) / reps
-    print(json.dumps({"seconds": t, "iterations": r["iterations"], "nsub": h.nsub, "reps": reps}))
+    print(json.dumps({"seconds": t, "iterations": r["iterations"], "nsub": h.nsub, "reps": reps, "setup_s": setup_s}))
 
 
 def run_cpu_baseline(gpu_iterations, nsub_target):
@@ -98,7 +101,9 @@ def run_cpu_baseline(gpu_iterations, nsub_target):
            "sample": ("oracle (numpy/scipy, SuperLU K_reg, implicit K+ per iteration) on 2x2x2 subdomains of "
                       "10^3 elements: %.3f s per solve (mean of %d) for %d iterations; scaled per subdomain-iteration "
                       "to %d subdomains x %d iterations" % (s["seconds"], s["reps"], s["iterations"], nsub_target,
-                                                          gpu_iterations))}
+                                                          gpu_iterations)),
+           "setup_s_sample": s["setup_s"],
+           "setup_s_scaled": s["setup_s"] * nsub_target / s["nsub"]}
     ncores = min(16, len(os.sched_getaffinity(0)))
     if ncores > 1:
         t1 = time.time()
