@@ -10,6 +10,12 @@ sets; here the same sets are formed by grouping (DOF, subdomain) pairs, which
 yields identical ascending lists.  Clusters (an extension) restrict Bc to pairs
 inside one cluster; Bf is global over all clusters.
 
+Extension: typeBc="cor+edge" -- the `cor` rows plus, per cluster-interior edge of
+the subdomain grid, one averaged row (+-1/m over the m interior edge nodes) per
+component between the first subdomain holding the edge and each other one (every
+pair without Bc_fullRank).  Pinned by dim ker(Gc^T Gc) = 6, the displacement of
+`ker`, and a hand count of the edge rows (tests/test_options.py).
+
 Parity: row order, columns, signs, weights and compressed-row maps are pinned by
 closed-form counts (sum over DOFs of m(m-1)/2 plus Dirichlet rows), the per-row
 structure (+1/-1 pairs or a single +1), hand-enumerated 2-subdomain cases in

@@ -6,11 +6,14 @@ Preconditioning, mult_Ff, mult_Bf/BfT, Projection, scale (Cluster.cpp:961-1360),
 pcpg (Cluster.cpp:1363-1458) and the recovery in printVTK (Cluster.cpp:1648-1678).
 
 Choices (identical in the product):
-  * the cluster saddle-point matrix is the non-singular extended form
-    Ac = [Fc Gc 0; Gc^T 0 H; 0 H^T 0] (Cluster.cpp:592-660, option
-    Ac_extended_by_kerGc=true).  The reference default factors the singular
-    [Fc Gc; Gc^T 0] with PARDISO pivot perturbation; P F P and u do not depend on
-    which consistent generalised inverse is used.
+  * the cluster saddle-point matrix: with Ac_extended_by_kerGc=true the
+    non-singular extended form Ac = [Fc Gc 0; Gc^T 0 H; 0 H^T 0]
+    (Cluster.cpp:592-660); with false (the reference default, which factors the
+    singular [Fc Gc; Gc^T 0] with its kernel detection, Cluster.cpp:286-320) the
+    same matrix regularised along its kernel [0; H], [Fc Gc; Gc^T -sig H H^T], a
+    generalised inverse of the singular one.  P F P and u do not depend on which
+    consistent generalised inverse is used (pinned: both forms give the same u,
+    tests/test_options.py).  A cluster of one subdomain has H = I.
   * Dirichlet preconditioner is the explicit Schur complement (equal to the
     implicit one in exact arithmetic, Matrix.cpp:1853-1870).
 

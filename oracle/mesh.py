@@ -12,8 +12,16 @@ Extensions over the reference (documented, applied identically in the product):
   * material ``material``: "reference" (E*ratio_mat where remainder(partId,2)==0,
     Mesh.cpp:281, Data.cpp:47-48), "uniform", or "lognormal" per subdomain.
 
+  * ``metis=true`` with ``nparts=1`` (the reference's own main() map, Mesh.cpp:354-452):
+    one subdomain over the whole cube; corner DOFs and material ids stay those of the
+    geometric grid createMesh built (Mesh.cpp:413-421).
+
 Parity: node/element/partition numbering, corner and Dirichlet DOF lists are
-pinned by closed-form counts and hand-checked small cases in tests/test_oracle_mesh.py.
+pinned by closed-form counts and hand-checked small cases in
+tests/test_oracle_decomp.py (test_mesh_counts_and_partition, test_dirichlet_lists)
+and by the committed integer fixtures of tests/golden; the single-subdomain map by a
+global direct solve and by the geometric decomposition of the same cube
+(tests/test_options.py).
 """
 import math
 import numpy as np
