@@ -49,12 +49,17 @@ SAMPLE = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="un
 HBM_PEAK_GBS = 8000.0
 FP16_MFMA_PEAK_TFS = 2500.0  # dense FP16 matrix cores, published
 SEED = 20240611
-# HBM bytes per launch of k_tgemm_h16 on the default workload, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`, each scaled by
-# the counter's reading for kb_copy of known size (FETCH x2.000, WRITE x1.000):
-# 7.249e9 fetched + 2.222e8 written (FP32 results), K chunk depth 64.  Valid for the
-# default workload only.
-GEMM_TRAFFIC_BYTES = 7.47e9
+# HBM bytes per launch of the split-FP16 GEMM on the default workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`,
+# each scaled by the counter's reading for kb_copy of known size (FETCH x2.000,
+# WRITE x1.000).  Row form (k_tgemm_h16, K chunk depth 64): 7.249e9 fetched +
+# 2.222e8 written.  Band form: not measured yet (None).  Valid for the default
+# workload only.
+GEMM_TRAFFIC_BYTES = {"rows": 7.47e9, "bands": None}
+GEMM_KERNEL_NAME = {
+    "rows": "k_tgemm_h16 (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 columns, tiles read per row block)",
+    "bands": "k_tgemm_h16b + k_tgemm_h16b_reduce (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 "
+             "columns, every tile read once; time of both launches)"}
 
 
 def cpu_sample():
@@ -162,18 +167,19 @@ def fixture_error(fx, u, case):
     return float(np.linalg.norm(u[fx["sample"]] - ref) / np.linalg.norm(ref))
 
 
-def roofline_of(mesh):
+def roofline_of(mesh, gemm_kernel):
     """The split-FP16 tile GEMM over 64 columns with K+_bb (the dominant kernel):
     HIP-event time per launch on the library stream and its algorithmic bytes."""
     g_ms, g_bytes = mesh.time_stream(7, reps=10)
     _, g_flops = mesh.time_stream(2, reps=1)
     achieved = g_bytes / (g_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "kernel": "k_tgemm_h16", "kernel_ms": g_ms, "algorithmic_bytes": g_bytes,
+            "kernel": GEMM_KERNEL_NAME[gemm_kernel].split(" (")[0], "kernel_ms": g_ms, "algorithmic_bytes": g_bytes,
             "fp16_mfma_frac": 3.0 * g_flops / (g_ms * 1e-3) / 1e12 / FP16_MFMA_PEAK_TFS}
 
 
-def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fixture=None, fixture_cases=()):
+def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fixture=None, fixture_cases=(),
+                   gemm_kernel="rows"):
     """Set the mesh up, solve `warmup + steps` blocks of k load cases in mixed precision
     and `warmup + plain_steps` single cases in FP64; per-block device times, rates,
     iterations and, with a fixture, the error of the fixture's cases (columns
@@ -182,6 +188,7 @@ def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fi
     t0 = time.perf_counter()
     mesh = hfls_amd.Mesh(device=0, **cfg)
     setup_s = time.perf_counter() - t0
+    mesh.configure(gemm_kernel=gemm_kernel)
     nblk = warmup + steps
     if F is None:
         F = mesh.traction(SEED, 0, k * nblk)
@@ -212,7 +219,7 @@ def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fi
            "iterations_min_max": [int(its.min()), int(its.max())], "max_true_residual": max(res),
            "plain": {"solves_per_s": plain_rate, "ms_per_solve": float(np.mean(plain_ms)), "iterations": plain_it[-1]},
            "speedup_vs_plain": rate / plain_rate, "setup_s": round(setup_s, 3), "device_bytes": mesh.device_bytes,
-           "roofline": roofline_of(mesh)}
+           "roofline": roofline_of(mesh, gemm_kernel)}
     if fixture is not None:
         out["error_vs_fp64_oracle"] = {"cases": [str(c) for c in fixture_cases], "mixed": err_mixed,
                                        "plain_fp64": err_plain}
@@ -322,6 +329,8 @@ def main():
     ap.add_argument("--serve-saturated", type=int, default=48,
                     help="requests of the saturated serving run on the cached 512 x 10^3 cube (0: none)")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 tile format (block_gemm=f32) run")
+    ap.add_argument("--gemm-kernel", choices=("bands", "rows"), default="rows",
+                    help="form of the split-FP16 GEMM: every tile read once (bands) or once per row block (rows)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the iteration kernels directly instead of replaying captured HIP graphs")
     ap.add_argument("--kernel-only", action="store_true",
@@ -346,6 +355,7 @@ def main():
     setup_s = time.perf_counter() - t0
     if a.no_graph:
         mesh.configure(hip_graph=0)
+    mesh.configure(gemm_kernel=a.gemm_kernel)
     log("setup %.2f s (factor %.2f s), nlam %d, device bytes %.2f GB" %
         (setup_s, mesh.factor_ms / 1e3, mesh.nlam, mesh.device_bytes / 1e9))
     if a.kernel_only:
@@ -466,28 +476,29 @@ def main():
                                   "plain_fp64": err_plain, "max": max(err_mixed + err_plain)}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": GEMM_TRAFFIC_BYTES if (a.elements == 10 and not a.small) else None,
-                     "kernel": "k_tgemm_h16 (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 columns)",
+                     "traffic": GEMM_TRAFFIC_BYTES[a.gemm_kernel] if (a.elements == 10 and not a.small) else None,
+                     "kernel": GEMM_KERNEL_NAME[a.gemm_kernel],
                      "kernel_ms": g_ms, "algorithmic_bytes": g_bytes, "flops_per_launch": g_flops,
                      "algorithmic_TFs": g_flops / (g_ms * 1e-3) / 1e12,
                      "fp16_mfma_frac": 3.0 * g_flops / (g_ms * 1e-3) / 1e12 / FP16_MFMA_PEAK_TFS,
                      "S_gemm_ms": s_ms, "S_gemm_GBs": s_bytes / (s_ms * 1e-3) / 1e9,
                      "measured_mfma_ceiling_TFs": ceil,
                      "autotune": {"k_chunk": 32 * tune["row_tiles"],
-                                  "candidate_ms": [tune["ms"][1], tune["ms"][2]]}},
+                                  "candidate_ms": [tune["ms"][1], tune["ms"][2]], "band_ms": tune["ms"][3]}},
     }
     mesh.close()
     if not a.small and not a.no_f32:
         # the FP32 tile format (block_gemm=f32) of the same workload
         cf = dict(cfg, block_gemm="f32")
-        r32 = block_workload(hfls_amd, cf, a.k, 1, 2, 1, a.eps)
+        r32 = block_workload(hfls_amd, cf, a.k, 1, 2, 1, a.eps, gemm_kernel=a.gemm_kernel)
         out["block_gemm_f32"] = {"solves_per_s": r32["solves_per_s"], "ms_per_step": r32["ms_per_step"],
                                  "iterations_min_max": r32["iterations_min_max"]}
         log("block_gemm=f32: %.1f solves/s" % r32["solves_per_s"])
     extra = {}
     for c in [x for x in a.configs.split(",") if x] if not a.small else []:
         if c == "3":  # 512 x 12^3, 64 face tractions per block
-            extra["cfg3_512sub_12e3_64cases"] = block_workload(hfls_amd, cube(12), a.k, 1, a.config_steps, 1, a.eps)
+            extra["cfg3_512sub_12e3_64cases"] = block_workload(hfls_amd, cube(12), a.k, 1, a.config_steps, 1, a.eps,
+                                                               gemm_kernel=a.gemm_kernel)
         elif c == "2":  # 512 uneven subdomains, lognormal E; column 0 the body force (oracle fixture)
             fu = load_fixture("uneven512")
             if fu is not None:
@@ -503,7 +514,7 @@ def main():
             m2.close()
             extra["cfg2_512sub_uneven_64cases"] = block_workload(
                 hfls_amd, cu, a.k, 1, a.config_steps, 1, a.eps, F=Fu, fixture=fu,
-                fixture_cases=("body",) if fu is not None else ())
+                fixture_cases=("body",) if fu is not None else (), gemm_kernel=a.gemm_kernel)
         else:
             raise SystemExit("unknown extra config " + c)
         log("config", c, json.dumps(extra[list(extra)[-1]]))

@@ -222,7 +222,8 @@ class Mesh:
         self.close()
 
     def configure(self, **settings):
-        """Runtime settings: hip_graph=True/False, refine_tol=..., poll_chunk=..."""
+        """Runtime settings: hip_graph=True/False, refine_tol=..., poll_chunk=...,
+        gemm_rows=1|2|"auto", gemm_kernel="bands"|"rows"."""
         for k, v in settings.items():
             if isinstance(v, bool):
                 v = "1" if v else "0"
@@ -230,10 +231,11 @@ class Mesh:
 
     def tuning(self):
         """Autotuner result of the block GEMM: chosen row tiles per workgroup and the
-        measured ms of the candidates (after the first block call)."""
-        out = np.zeros(3)
+        measured ms of the candidates (after the first block call); ms[3] is the band
+        form of the split-FP16 GEMM (setting gemm_kernel)."""
+        out = np.zeros(4)
         _check(load().hfls_mesh_tuning(self._h, _ptr(out)))
-        return {"row_tiles": int(out[0]), "ms": {1: float(out[1]), 2: float(out[2])}}
+        return {"row_tiles": int(out[0]), "ms": {1: float(out[1]), 2: float(out[2]), 3: float(out[3])}}
 
     def write_vtk(self, u, folder, iter=-1):
         """Legacy ASCII VTK of the displacements u (3 * nPoints), as Problem.write_vtk."""

@@ -27,6 +27,7 @@ SOURCES = [
     ("symv.hip", HIP_FLAGS),
     ("tgemm.hip", HIP_FLAGS),
     ("tgemm16.hip", HIP_FLAGS),
+    ("tgemm16b.hip", HIP_FLAGS),
     ("lrr.hip", HIP_FLAGS),
     ("block.hip", HIP_FLAGS),
     ("noalloc.hip", HIP_FLAGS),

@@ -534,5 +534,6 @@ extern "C" int hfls_mesh_tuning(hfls_mesh m, double* out) {
     out[0] = eng(m)->tuned_rows();
     out[1] = eng(m)->tuned_ms(1);
     out[2] = eng(m)->tuned_ms(2);
+    out[3] = eng(m)->tuned_ms(3);
   });
 }

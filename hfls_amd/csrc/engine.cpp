@@ -301,6 +301,10 @@ void Engine::configure(const std::string& key, const std::string& value) {
     const int v = std::atoi(value.c_str());
     if (v < 0 || v > 120 * 1024) throw std::runtime_error("gemm_lds_pad must be 0..122880");
     gemm_lds_pad = v;
+  } else if (key == "gemm_kernel") {  // split-FP16 GEMM: bands (each tile read once) or rows
+    if (value == "bands") gemm_band = true;
+    else if (value == "rows") gemm_band = false;
+    else throw std::runtime_error("gemm_kernel must be bands or rows");
   } else if (key == "gemm_pf") {  // K chunks in flight in the split-FP16 GEMM: 1 or 2
     const int v = std::atoi(value.c_str());
     if (v != 1 && v != 2) throw std::runtime_error("gemm_pf must be 1 or 2");

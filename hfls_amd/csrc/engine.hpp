@@ -106,7 +106,7 @@ struct BlockCtx {
   hipGraphExec_t gx = nullptr;
   int64_t gx_key = -1;
   int k = 0, prec = 64;
-  DVec<float> xb32, yb32, sx;
+  DVec<float> xb32, yb32, sx, p16;
   DVec<double> gtp;
   DVec<double> xb, yb, cr, coef, kpart, v6, gc, la, h, hg, hgf, lam, g, w, Fw, Pg, z, Pz, d, g64, part, gt, Vr, Xr, Tr,
       V, V2, X, U;
@@ -213,7 +213,8 @@ class Engine {
   // workgroup 1 or 2; split-FP16 format: K chunk depth 32 or 64 as 1 or 2) and the
   // measured ms of each candidate
   int tuned_rows() const { return tune_rt32; }
-  double tuned_ms(int rt) const { return tune_ms32[rt - 1]; }
+  double tuned_ms(int rt) const { return tune_ms32[rt - 1]; }  // rt 3: the band kernel
+  bool band_gemm() const { return gemm16 && gemm_band; }
   // runtime settings: hip_graph (1/0: capture and replay the iteration chunks),
   // refine_tol (mixed-precision refinement factor), poll_chunk (iterations
   // between host polls), gemm_rows (1, 2 or auto: block GEMM tile shape, see tuned_rows)
@@ -278,7 +279,10 @@ class Engine {
   bool use_graph = true;
   int tune_rt32 = 1, gemm_rows_forced = 0;
   int gemm_lds_pad = 0, gemm_exact = 0, gemm_pf = 1;  // split-FP16 GEMM: residency cap, tile order, prefetch
-  double tune_ms32[2] = {0, 0};
+  // split-FP16 GEMM reading every tile once (bands of row blocks, partial sums in
+  // b_p16 from d_p16off[d]) instead of once per row block
+  bool gemm_band = false;
+  double tune_ms32[3] = {0, 0, 0};  // candidates 1, 2 (tuned_rows) and the band kernel
   int poll_chunk = 8;
   hipGraphExec_t gx_plain = nullptr, gx_blk = nullptr;
   int gx_plain_n = -1;
@@ -303,7 +307,9 @@ class Engine {
   // split-FP16 block GEMM (option block_gemm=f16x2): b_Kt32/b_St32 hold hi and lo FP16
   // planes of A_d / s_d, b_sK/b_sS the power-of-two s_d, b_sx the per-column scales of X
   bool gemm16 = true;
-  DVec<float> b_sK, b_sS, b_sx;
+  DVec<float> b_sK, b_sS, b_sx, b_p16;
+  DVec<int64_t> d_p16off;
+  int64_t p16_n = 0;
   DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
   int n_xob = 0, n_xor = 0;
   DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)

@@ -76,6 +76,15 @@ void Engine::block_prepare() {
     n_xob = (int)ob.size();
     n_xor = (int)orr.size();
   }
+  {  // partial sums of the band form of the split-FP16 GEMM, per subdomain
+    std::vector<int64_t> po(N);
+    p16_n = 0;
+    for (int d = 0; d < N; d++) {
+      po[d] = p16_n;
+      p16_n += tgemm_h16b_part_floats(h_ns[d], KP);
+    }
+    d_p16off.upload(po, st);
+  }
   {  // chunks of at most 256 b positions for G^T x
     std::vector<int> csub, cp0, cbeg(N + 1, 0);
     for (int d = 0; d < N; d++) {
@@ -261,6 +270,7 @@ void Engine::alloc_ctx_buffers() {
   zero_alloc(b_xb32, NBP * KP, st);
   zero_alloc(b_yb32, NBP * KP, st);
   zero_alloc(b_sx, (int64_t)N * KP, st);
+  if (gemm16) zero_alloc(b_p16, std::max<int64_t>(p16_n, 1), st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
   zero_alloc(b_kpart, (int64_t)N * 8 * 64 * KP, st);
@@ -293,6 +303,7 @@ void Engine::swap_ctx(BlockCtx& c) {
   b_xb32.swap(c.xb32);
   b_yb32.swap(c.yb32);
   b_sx.swap(c.sx);
+  b_p16.swap(c.p16);
   b_xb.swap(c.xb); b_yb.swap(c.yb); b_cr.swap(c.cr); b_coef.swap(c.coef); b_kpart.swap(c.kpart); b_v6.swap(c.v6);
   b_gc.swap(c.gc); b_la.swap(c.la); b_h.swap(c.h); b_hg.swap(c.hg); b_hgf.swap(c.hgf); b_lam.swap(c.lam);
   b_g.swap(c.g); b_w.swap(c.w); b_Fw.swap(c.Fw); b_Pg.swap(c.Pg); b_z.swap(c.z); b_Pz.swap(c.Pz); b_d.swap(c.d);
@@ -352,12 +363,26 @@ void Engine::autotune() {
     tune_ms32[rt - 1] = ms / 3;
   }
   tune_rt32 = gemm_rows_forced ? gemm_rows_forced : (tune_ms32[1] < tune_ms32[0] ? 2 : 1);
+  if (gemm16) {  // the band form, timed the same way (chosen by the gemm_kernel setting, not by time)
+    auto launch = [&] {
+      HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, b_Kt32.p, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f,
+                           nullptr, b_yb32.p, b_xob.p, n_xob, 1, b_p16.p, d_p16off.p));
+    };
+    launch();
+    HC(hipEventRecord(a, st));
+    for (int r = 0; r < 3; r++) launch();
+    HC(hipEventRecord(b, st));
+    HC(hipEventSynchronize(b));
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, a, b));
+    tune_ms32[2] = ms / 3;
+  }
 }
 
 size_t Engine::block_bytes() const {
   if (!blk_ready) return 0;
   size_t s = 0;
-  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n + b_sK.n + b_sS.n + b_sx.n) * sizeof(float);
+  s += (b_Kt32.n + b_St32.n + b_xb32.n + b_yb32.n + b_sK.n + b_sS.n + b_sx.n + b_p16.n) * sizeof(float);
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_gtp, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
@@ -393,7 +418,10 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
 void Engine::gemm_lo(int which, const int* gate) {
   const int N = P->nsub;
   const float* A = which == 0 ? b_Kt32.p : b_St32.p;
-  if (gemm16)
+  if (gemm16 && gemm_band)
+    HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
+                         KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 1, b_p16.p, d_p16off.p));
+  else if (gemm16)
     HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
                         KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 32 * tune_rt32, 1, gemm_lds_pad, gemm_exact, gemm_pf));
   else
@@ -547,6 +575,10 @@ double Engine::time_block_gemm(int which, int reps, double* flops) {
   auto launch = [&] {
     if (which == 2)
       HC(launch_kb_copy(st, ncopy, b_xb.p, b_yb.p));
+    else if (gemm16 && gemm_band)  // the band kernel and its partial-sum reduction
+      HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, which == 0 ? b_Kt32.p : b_St32.p, d_off_P.p,
+                           which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, nullptr, b_yb32.p,
+                           b_xob.p, n_xob, 0, b_p16.p, d_p16off.p));
     else if (gemm16)
       HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, which == 0 ? b_Kt32.p : b_St32.p, d_off_P.p,
                           which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, nullptr, b_yb32.p,

@@ -257,6 +257,19 @@ hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, c
                             int lds_pad = 0,  // unused dynamic LDS per workgroup (caps residency per CU)
                             int exact = 0,    // tile order rotated by the row block itself (else its pair)
                             int pf = 1);      // K chunks in flight while one is multiplied (1 or 2) // scales 0: sx already holds the scales of this X
+// The same product reading every stored tile once (tgemm16b.hip): workgroups own
+// bands of HFLS_H16B_M row blocks; the transposed contributions to the row blocks
+// above a band go through `part` (tgemm_h16b_part_floats(n_s, kp) floats per
+// subdomain from poff[d]) and are added in a fixed order.
+#define HFLS_H16B_M 4
+int64_t tgemm_h16b_part_floats(int ns, int kp);
+hipError_t launch_tgemm_h16b(hipStream_t st, int nsub, int maxT, const int* ns, const float* A, const int64_t* aoff,
+                             const float* sA, const float* X, const int64_t* xoff, float* sx, int kp, float alpha,
+                             const int* gate, float* Yf, const int* xcd_order, int norder, int scales, float* part,
+                             const int64_t* poff);
+// sx[d * kp + c]: power-of-two scale of column c of X_d (tgemm16.hip)
+hipError_t launch_kb_colscale(hipStream_t st, int nsub, const int* ns, const int64_t* xoff, int kp, const float* X,
+                              float* sx, const int* gate);
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
                                  const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
                                  const int* nlc, float* P32);

@@ -71,6 +71,7 @@ size_t Server::estimate(const Problem& p) const {
     b += 8.0 * (double)(TL * (TL + 1) / 2 * 4096);                        // L_rr tiles
     b += 2.0 * 12.0 * 81.0 * (double)s.n_s;                               // K_br, K_rb (stencil bound)
     b += (8.0 + 8.0 + 4.0 + 4.0) * (double)tiles;                         // K+_bb, S (FP64, FP32)
+    b += 4.0 * (double)tgemm_h16b_part_floats(s.n_s, KMAX);               // band GEMM partials
     b += 8.0 * (6.0 * s.n + 6.0 * s.n_s + 4.0 * s.n + 2.0 * s.n_s);       // R, R_b, vectors
     b += 8.0 * (2.0 * align32l(ldb * 36) + 54.0 * ldb);                   // C, Bc^T, [C|R_b|Q|Yh]
     b += 8.0 * (double)T * (T + 1) / 2 * 64;                              // tile-row partials

@@ -308,4 +308,12 @@ hipError_t launch_tgemm_h16(hipStream_t st, int nsub, int maxT, const int* ns, c
   return hipGetLastError();
 }
 
+// the per-column scales of X for the band form of the GEMM (tgemm16b.hip)
+hipError_t launch_kb_colscale(hipStream_t st, int nsub, const int* ns, const int64_t* xoff, int kp, const float* X,
+                              float* sx, const int* gate) {
+  if (nsub <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kb_colscale, dim3(nsub, kp / 64), dim3(1024), 0, st, ns, xoff, kp, X, sx, gate);
+  return hipGetLastError();
+}
+
 }  // namespace hfls

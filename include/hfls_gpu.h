@@ -86,11 +86,16 @@ int hfls_mesh_load(hfls_mesh m, double* f);
  * "poll_chunk": iterations between host polls of the stopping flags (default 8);
  * "gemm_rows": "1", "2" or "auto" (default): tile shape of the mixed-precision block
  * GEMM, overriding the autotuner (FP32 format: 1 or 2 output row tiles per workgroup;
- * split-FP16 format, the default, option block_gemm: K chunks of 32 or 64) */
+ * split-FP16 format, the default, option block_gemm: K chunks of 32 or 64);
+ * "gemm_kernel": "rows" (default) or "bands": the split-FP16 GEMM reads every stored
+ * tile once per row block that needs it, or once (workgroups own bands of 4 row
+ * blocks, partial sums added in a fixed order); the two forms round differently,
+ * each is bitwise reproducible */
 int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value);
 /* autotuner result (after the first block call): [0] tile shape chosen for the
  * mixed-precision block GEMM (1 or 2, as for "gemm_rows"), [1], [2] measured ms per
- * launch of candidates 1 and 2 */
+ * launch of candidates 1 and 2, [3] ms of the band form of the split-FP16 GEMM
+ * (0 in the FP32 format); out holds 4 */
 int hfls_mesh_tuning(hfls_mesh m, double* out);
 
 /* operators: Cluster::mult_Ff (Cluster.cpp:1114-1188), Projection (1314-1329,

@@ -48,19 +48,65 @@ def test_graph_replays_match_eager(name):
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_autotuned_gemm_shapes_agree(name):
-    """The autotuner times both block-GEMM tile shapes on the mesh and picks one;
-    either shape gives bit-identical solves."""
+    """The autotuner times both tile shapes of the row form of the block GEMM on the
+    mesh and picks one; either shape gives bit-identical solves."""
     m = hfls_amd.Mesh(**CASES[name])
+    m.configure(gemm_kernel="rows")
     F = m.traction(78, 0, 3)
     r_auto = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     t = m.tuning()
     print(name, "autotune", t, flush=True)
-    assert t["row_tiles"] in (1, 2) and t["ms"][1] > 0 and t["ms"][2] > 0
+    assert t["row_tiles"] in (1, 2) and t["ms"][1] > 0 and t["ms"][2] > 0 and t["ms"][3] > 0
     out = {}
     for rows in (1, 2):
         m.configure(gemm_rows=rows)
         out[rows] = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     assert np.array_equal(out[1]["u"], out[2]["u"]) and np.array_equal(out[1]["u"], r_auto["u"])
+    m.close()
+
+
+BAND_CASES = dict(CASES, cube8_8e3=dict(Nx=2, Ny=2, Nz=2, nx=8, ny=8, nz=8, bc="clamp_x0", material="lognormal",
+                                        young_modulus="2.1e5", seed="5", load="body"))
+
+
+@pytest.mark.parametrize("name", sorted(BAND_CASES))
+def test_band_gemm_matches_rows(name):
+    """The band form of the split-FP16 GEMM (every tile read once, partial sums of the
+    bands below a row block added in a fixed order) against the row form: the F and M
+    applies on a block of 64 columns within 2e-5 relative of FP64 in both forms and
+    within 1e-6 of each other; mixed solves certified with the same iteration counts
+    (+-1) and displacements within 1e-7; the band form bitwise reproducible.  cube8_8e3
+    has 19 tile rows per subdomain (5 bands, partials from 4 bands below)."""
+    kw = BAND_CASES[name]
+    print("band case", name, flush=True)
+    m = hfls_amd.Mesh(**kw)
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal((64, m.nlam))
+    ref = {op: m.apply_block(op, W, precision=64) for op in (0, 1)}
+    res = {}
+    for kern in ("rows", "bands"):
+        m.configure(gemm_kernel=kern)
+        for op in (0, 1):
+            y = m.apply_block(op, W, precision=32)
+            err = np.linalg.norm(y - ref[op], axis=1) / np.linalg.norm(ref[op], axis=1)
+            print(name, kern, "op", op, "max rel err vs FP64 %.2e" % err.max(), flush=True)
+            assert err.max() < 2e-5
+            res[(kern, op)] = y
+    for op in (0, 1):
+        d = np.linalg.norm(res[("bands", op)] - res[("rows", op)], axis=1) / np.linalg.norm(ref[op], axis=1)
+        assert d.max() < 1e-6
+    F = m.traction(81, 0, 4)
+    m.configure(gemm_kernel="rows")
+    br = m.pcpg_block(F, eps=1e-8, max_iter=800, precision=32)
+    m.configure(gemm_kernel="bands")
+    bb = m.pcpg_block(F, eps=1e-8, max_iter=800, precision=32)
+    bb2 = m.pcpg_block(F, eps=1e-8, max_iter=800, precision=32)
+    assert np.all(br["status"] == 2) and np.all(bb["status"] == 2)
+    assert np.all(np.abs(bb["iterations"] - br["iterations"]) <= 1)
+    du = np.linalg.norm(bb["u"] - br["u"], axis=1) / np.linalg.norm(br["u"], axis=1)
+    print(name, "iterations", bb["iterations"], br["iterations"], "u diff %.2e" % du.max(), flush=True)
+    assert du.max() < 1e-7
+    assert np.array_equal(bb["u"], bb2["u"]) and np.array_equal(bb["iterations"], bb2["iterations"])
     m.close()
 
 
