@@ -179,7 +179,7 @@ def roofline_of(mesh, gemm_kernel):
 
 
 def block_workload(hfls_amd, cfg, k, warmup, steps, plain_steps, eps, F=None, fixture=None, fixture_cases=(),
-                   gemm_kernel="rows"):
+                   gemm_kernel="bands"):
     """Set the mesh up, solve `warmup + steps` blocks of k load cases in mixed precision
     and `warmup + plain_steps` single cases in FP64; per-block device times, rates,
     iterations and, with a fixture, the error of the fixture's cases (columns
@@ -329,7 +329,7 @@ def main():
     ap.add_argument("--serve-saturated", type=int, default=48,
                     help="requests of the saturated serving run on the cached 512 x 10^3 cube (0: none)")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 tile format (block_gemm=f32) run")
-    ap.add_argument("--gemm-kernel", choices=("bands", "rows"), default="rows",
+    ap.add_argument("--gemm-kernel", choices=("bands", "rows"), default="bands",
                     help="form of the split-FP16 GEMM: every tile read once (bands) or once per row block (rows)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the iteration kernels directly instead of replaying captured HIP graphs")

@@ -281,7 +281,7 @@ class Engine {
   int gemm_lds_pad = 0, gemm_exact = 0, gemm_pf = 1;  // split-FP16 GEMM: residency cap, tile order, prefetch
   // split-FP16 GEMM reading every tile once (bands of row blocks, partial sums in
   // b_p16 from d_p16off[d]) instead of once per row block
-  bool gemm_band = false;
+  bool gemm_band = true;
   double tune_ms32[3] = {0, 0, 0};  // candidates 1, 2 (tuned_rows) and the band kernel
   int poll_chunk = 8;
   hipGraphExec_t gx_plain = nullptr, gx_blk = nullptr;

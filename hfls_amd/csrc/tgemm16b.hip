@@ -80,11 +80,19 @@ typedef __attribute__((__vector_size__(4 * sizeof(__fp16)))) __fp16 fp16x4;
 
 constexpr int BM = HFLS_H16B_M;  // row blocks per band
 
-// position of tile element (row r, column c) in an LDS tile plane: column major,
-// rows XORed in groups of 8 by f(c) = 16 (c mod 4) ^ 8 ((c / 4) mod 2), so that the
-// 4 columns one transposing load spans, and the 8 columns 8 lanes of a 16-byte
-// load span, fall in different bank groups
-static __device__ __forceinline__ int tpos(int c, int r) { return c * 64 + (r ^ (((c & 3) << 4) ^ (((c >> 2) & 1) << 3))); }
+// rows of LDS column c (64 halves each) are XORed in groups of 8 by
+// f(c) = (16 (c mod 4) + 8 ((c / 4) mod 2)) ^ 16 ((c / 8) mod 2): the 4 columns one
+// transposing load spans fall in 4 different 16-row blocks, and the 8 (or 16)
+// consecutive columns that 8 (or 16) lanes of a 16-byte load address at once fall
+// in 8 (or, with the column parity, 16) different bank groups
+static __device__ __forceinline__ int swz8(int c) {
+  return ((((c & 3) << 4) | (((c >> 2) & 1) << 3)) ^ (((c >> 3) & 1) << 4));
+}
+// position of tile element (row r, column c) in an LDS tile plane (column major)
+static __device__ __forceinline__ int tpos(int c, int r) { return c * 64 + (r ^ swz8(c)); }
+
+// position of X element (column n, row k) in an LDS X plane, swizzled like a tile
+static __device__ __forceinline__ int xpos(int n, int k) { return n * 64 + (k ^ swz8(n)); }
 
 static __device__ __forceinline__ half8 join8(fp16x4 a, fp16x4 b) {
   half8 o;
@@ -110,21 +118,18 @@ static __host__ __device__ __forceinline__ int64_t band_pbase(const BandGeom& g,
 
 template <int M>
 __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, const int* __restrict__ order, int norder,
-                                                    const int* __restrict__ mrows, const float* __restrict__ A,
-                                                    const int64_t* __restrict__ aoff, const float* __restrict__ sA,
-                                                    const float* __restrict__ X, const int64_t* __restrict__ xoff,
-                                                    const float* __restrict__ sx, int kp, float alpha,
-                                                    const int* __restrict__ gate, float* __restrict__ Yf,
-                                                    float* __restrict__ part, const int64_t* __restrict__ poff) {
-  static_assert(M == 4, "next-column staging: 4 steps x 4 waves x 4 rows");
-  constexpr int LDH = 72;       // padded row of an X plane (halves)
-  constexpr int XB = 64 * LDH;  // one X plane
-  constexpr int TP = 4096;      // one tile plane (swizzled)
-  __shared__ __attribute__((aligned(16))) _Float16 lds[2 * TP + 6 * XB];
-  _Float16* const Sh = lds;            // tile, hi
-  _Float16* const Sl = lds + TP;       //       lo
-  _Float16* const Xih = lds + 2 * TP;  // X_i of the step, hi; lo = +XB
-  auto xj_ptr = [&](int col) { return lds + 2 * TP + 2 * XB + (col & 1) * 2 * XB; };  // X_j, hi; lo = +XB
+                                                       const int* __restrict__ mrows, const float* __restrict__ A,
+                                                       const int64_t* __restrict__ aoff, const float* __restrict__ sA,
+                                                       const float* __restrict__ X, const int64_t* __restrict__ xoff,
+                                                       const float* __restrict__ sx, int kp, float alpha,
+                                                       const int* __restrict__ gate, float* __restrict__ Yf,
+                                                       float* __restrict__ part, const int64_t* __restrict__ poff) {
+  static_assert(M == 4, "two workgroups per CU: the tile and 4 X blocks in 80 KB of LDS");
+  constexpr int XP = 4096;  // one 64 x 64 FP16 plane (halves)
+  __shared__ __attribute__((aligned(16))) _Float16 lds[2 * XP * (M + 1)];
+  _Float16* const Sh = lds;           // tile (tpos), hi
+  _Float16* const Sl = lds + XP;      //             lo
+  _Float16* const XB = lds + 2 * XP;  // the band's X block m: XB + 2 XP m (hi), + XP (lo), at xpos()
 
   const int sl = blockIdx.x >> 3;
   const int gi = (sl / nslot) * 8 + (blockIdx.x & 7);
@@ -150,135 +155,151 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
 
   const _Float16* Ad = (const _Float16*)(A + aoff[d]);
   const float* Xd = X + xoff[d] + c0;
-  // tile staging: 16-byte units t and t + 256 of each plane, column (t >> 3) (+32),
-  // rows (t & 7) * 8 .. + 7
+  // tile staging: 16-byte units t and t + 256 of each plane (column (t >> 3) (+32),
+  // rows (t & 7) 8 .. +7)
   const int tc = t >> 3, tr8 = (t & 7) * 8;
   const int sp0 = tpos(tc, tr8), sp1 = tpos(tc + 32, tr8);
   const _Float16* const tsrc = Ad + tc * 64 + tr8;
-  // X staging: column xn, rows xk .. xk + 15
-  const int xn = t & 63, xk = w * 16;
-  const float xinv = 1.0f / sx[(int64_t)d * kp + c0 + xn];
-  const float* const xsrc = Xd + (int64_t)xk * kp + xn;
+  // X operands: column xcol = cb + lane % 32, rows kk + ck .. +7
+  const int xcol = cb + (lane & 31), ck = 8 * (lane >> 5);
+  const float xinv_op = 1.0f / sx[(int64_t)d * kp + c0 + xcol];
 
   struct Stage {
-    half8 h0, h1, l0, l1;  // tile
-    float xi[16];          // X_i rows xk .. xk + 15
-    float xn4[4];          // 4 values of the next off-band column's X block
+    half8 h0, h1, l0, l1;
   };
-  // loads of step (row block i, column j) and X rows xrow .. xrow + 3 (column xn)
-  auto fetch = [&](Stage& g, int i, int j, int xrow) {
+  auto fetch = [&](Stage& g, int i, int j) {
     const int64_t ti = (int64_t)j * T - (int64_t)j * (j - 1) / 2 + (i - j);
     const _Float16* tile = tsrc + ti * 8192;
     g.h0 = *(const half8*)tile;
     g.h1 = *(const half8*)(tile + 2048);
     g.l0 = *(const half8*)(tile + 4096);
     g.l1 = *(const half8*)(tile + 6144);
-    const float* xr = xsrc + (int64_t)i * 64 * kp;
-#pragma unroll
-    for (int u = 0; u < 16; u++) g.xi[u] = xr[(int64_t)u * kp];
-#pragma unroll
-    for (int u = 0; u < 4; u++) g.xn4[u] = Xd[(int64_t)(xrow + u) * kp + xn];
-  };
-  auto split8 = [&](const float* v, _Float16* h, int k) {  // 8 values -> rows k .. k+7 of column xn
-    half8 hh, ll;
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const float x = v[u] * xinv;
-      hh[u] = (_Float16)x;
-      ll[u] = (_Float16)(x - (float)hh[u]);
-    }
-    *(half8*)(h + xn * LDH + k) = hh;
-    *(half8*)(h + XB + xn * LDH + k) = ll;
   };
   auto stash = [&](const Stage& g) {
     *(half8*)(Sh + sp0) = g.h0;
     *(half8*)(Sh + sp1) = g.h1;
     *(half8*)(Sl + sp0) = g.l0;
     *(half8*)(Sl + sp1) = g.l1;
-    split8(g.xi, Xih, xk);
-    split8(g.xi + 8, Xih, xk + 8);
   };
-  auto stash_next = [&](const Stage& g, int col, int k) {  // rows k .. k+3 of column col's X_j
-    _Float16* h = xj_ptr(col);
-    half4 hh, ll;
+  // X_j of an off-band column, as this lane's B operands (4 K steps x hi, lo): loaded
+  // raw two steps ahead, split when the column starts
+  struct XjRaw {
+    float v[4][8];
+  };
+  struct XjOp {
+    half8 h[4], l[4];
+  };
+  auto xj_load = [&](XjRaw& r, int col) {
+    const float* xr = Xd + (int64_t)(col * 64 + ck) * kp + xcol;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const float x = g.xn4[u] * xinv;
-      hh[u] = (_Float16)x;
-      ll[u] = (_Float16)(x - (float)hh[u]);
-    }
-    *(half4*)(h + xn * LDH + k) = hh;
-    *(half4*)(h + XB + xn * LDH + k) = ll;
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) r.v[q][u] = xr[(int64_t)(16 * q + u) * kp];
   };
-  // operand offsets.  T X_j: lane group g4 = lane/16 reads rows m0 + li of columns
-  // kg .. kg+7 as two transposing loads (kg = kk + 8 (g4/2), m0 = rb + 16 (g4%2));
-  // lane li addresses column kg + li/4 (+4), rows m0 + 4 (li%4) .. +3.
+  auto xj_split = [&](const XjRaw& r, XjOp& o) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const float x = r.v[q][u] * xinv_op;
+        o.h[q][u] = (_Float16)x;
+        o.l[q][u] = (_Float16)(x - (float)o.h[q][u]);
+      }
+  };
+  // T X_j: lane group g4 = lane / 16 reads rows m0 + li of columns kg .. kg+7 with two
+  // transposing loads (kg = kk + 8 (g4 / 2), m0 = rb + 16 (g4 % 2)); lane li addresses
+  // column kg + li / 4 (+4), rows m0 + 4 (li % 4) .. +3
   const int g4 = lane >> 4, li = lane & 15;
   const int trow = rb + 16 * (g4 & 1) + 4 * (li & 3), tcol = 8 * (g4 >> 1) + (li >> 2);
   const int tro1 = tpos(tcol, trow), tro2 = tpos(tcol + 4, trow);  // + 64 kk
-  // T^T X_i: lane reads tile column rb + lane%32, rows kk + 8 (lane/32) .. +7
-  const int ccol = rb + (lane & 31), ck = 8 * (lane >> 5);
-  const int bo = (cb + (lane & 31)) * LDH + ck;  // X operand: column cb + lane%32, rows kk + 8 (lane/32)
+  const int ccol = rb + (lane & 31);  // T^T X_i: tile column rb + lane % 32, rows kk + ck .. +7
   auto tr4 = [](const _Float16* p) {
     return __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4*)p);
   };
-  auto mm = [&](floatx16& ar, const _Float16* xj, floatx16* ac, const bool col) {
-#pragma unroll
-    for (int kk = 0; kk < 64; kk += 16) {
-      const fp16x4 r1h = tr4(Sh + tro1 + 64 * kk), r2h = tr4(Sh + tro2 + 64 * kk);
-      const fp16x4 r1l = tr4(Sl + tro1 + 64 * kk), r2l = tr4(Sl + tro2 + 64 * kk);
-      const half8 ah = join8(r1h, r2h), al = join8(r1l, r2l);
-      const half8 bh = *(const half8*)(xj + bo + kk), bl = *(const half8*)(xj + XB + bo + kk);
-      ar = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, ar, 0, 0, 0);
-      ar = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, ar, 0, 0, 0);
-      ar = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, ar, 0, 0, 0);
-      if (col) {
-        const int o = tpos(ccol, kk + ck);
-        const half8 ch = *(const half8*)(Sh + o), cl = *(const half8*)(Sl + o);
-        const half8 xh = *(const half8*)(Xih + bo + kk), xl = *(const half8*)(Xih + XB + bo + kk);
-        *ac = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, xh, *ac, 0, 0, 0);
-        *ac = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, xl, *ac, 0, 0, 0);
-        *ac = __builtin_amdgcn_mfma_f32_32x32x16_f16(cl, xh, *ac, 0, 0, 0);
-      }
-    }
+  auto row_a = [&](int kk, half8& ah, half8& al) {
+    ah = join8(tr4(Sh + tro1 + 64 * kk), tr4(Sh + tro2 + 64 * kk));
+    al = join8(tr4(Sl + tro1 + 64 * kk), tr4(Sl + tro2 + 64 * kk));
+  };
+  auto mfma3 = [](floatx16& a, half8 ah, half8 al, half8 bh, half8 bl) {
+    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, a, 0, 0, 0);
+    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, a, 0, 0, 0);
+    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, a, 0, 0, 0);
+  };
+  auto col_prod = [&](floatx16& a, const _Float16* xi, int kk) {  // a += (T^T X_i), K step kk
+    const int o = tpos(ccol, kk + ck), xo = xpos(xcol, kk + ck);
+    mfma3(a, *(const half8*)(Sh + o), *(const half8*)(Sl + o), *(const half8*)(xi + xo),
+          *(const half8*)(xi + XP + xo));
   };
   constexpr int NTRI = M * (M + 1) / 2;
+  // step e of the in-band triangle -> tile; inactive (row >= nb, band 0 only) steps
+  // load a valid tile and do nothing else
   auto fetch_tri = [&](Stage& g, auto ec) {
     constexpr int e = decltype(ec)::value < NTRI ? decltype(ec)::value : NTRI - 1;
     constexpr int jj = tri_jj<M>(e), ii = tri_ii<M>(e);
     const bool act = ii < nb;
-    fetch(g, act ? i0 + ii : i0, act ? i0 + jj : i0, i0 * 64);
+    fetch(g, act ? i0 + ii : i0, act ? i0 + jj : i0);
   };
 
-  Stage g;
-  if (i0 > 0) {
-    {  // X_0 of the first off-band column
+  // the band's X blocks: thread column t % 64, rows 16 (t / 64) .. +15 of each
+  {
+    const int xn = t & 63, xk = 16 * w;
+    const float xinv = 1.0f / sx[(int64_t)d * kp + c0 + xn];
+    for (int m = 0; m < nb; m++) {
       float v[16];
+      const float* xr = Xd + (int64_t)((i0 + m) * 64 + xk) * kp + xn;
 #pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = xsrc[(int64_t)u * kp];
-      split8(v, xj_ptr(0), xk);
-      split8(v + 8, xj_ptr(0), xk + 8);
+      for (int u = 0; u < 16; u++) v[u] = xr[(int64_t)u * kp];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        half8 hh, ll;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const float x = v[8 * h + u] * xinv;
+          hh[u] = (_Float16)x;
+          ll[u] = (_Float16)(x - (float)hh[u]);
+        }
+        _Float16* blk = XB + 2 * XP * m;
+        *(half8*)(blk + xpos(xn, xk + 8 * h)) = hh;
+        *(half8*)(blk + XP + xpos(xn, xk + 8 * h)) = ll;
+      }
     }
-    // step (ii, j) stages rows 16 ii + 4 w .. + 3 of column j + 1
-    fetch(g, i0, 0, 64 + 4 * w);
-  } else {
-    fetch_tri(g, std::integral_constant<int, 0>{});
   }
-  // off-band columns j < i0: all M rows of the band each
+  Stage g[2];  // tiles of steps s (slot s mod 2) and s + 1, loaded two steps ahead
+  XjRaw xraw;
+  XjOp xop;
+  if (i0 > 0) {
+    fetch(g[0], i0, 0);
+    fetch(g[1], i0 + 1, 0);
+    xj_load(xraw, 0);
+  } else {
+    fetch_tri(g[0], std::integral_constant<int, 0>{});
+    fetch_tri(g[1], std::integral_constant<int, 1>{});
+  }
+  // off-band columns j < i0: the band's M rows each
   for (int j = 0; j < i0; j++) {
     static_for<0, M>([&](auto iic) {
       constexpr int ii = decltype(iic)::value;
-      stash(g);
-      stash_next(g, j + 1, 16 * ii + 4 * w);
+      constexpr int q = ii & 1;  // step 4 j + ii
+      if constexpr (ii == 0) xj_split(xraw, xop);  // this column's X_j, loaded two steps ago
+      stash(g[q]);
       __syncthreads();
-      // the next step: same column, the next column or the triangle's first
-      if constexpr (ii + 1 < M) {
-        fetch(g, i0 + ii + 1, j, (j + 1) * 64 + 16 * (ii + 1) + 4 * w);
+      // the tile two steps ahead: this column, the next one, or the triangle's first two
+      if constexpr (ii + 2 < M) {
+        fetch(g[q], i0 + ii + 2, j);
       } else {
+        constexpr int a = ii + 2 - M, tj = tri_jj<M>(a), tii = tri_ii<M>(a);
         const bool nxt = j + 1 < i0;
-        fetch(g, i0, nxt ? j + 1 : i0, nxt ? min(j + 2, T - 1) * 64 + 4 * w : i0 * 64);
+        fetch(g[q], nxt ? i0 + a : i0 + tii, nxt ? j + 1 : i0 + tj);
       }
-      mm(accr[ii], xj_ptr(j), &accc[0], true);
+      if constexpr (ii == M - 2) xj_load(xraw, min(j + 1, T - 1));
+      const _Float16* xi = XB + 2 * XP * ii;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        half8 ah, al;
+        row_a(16 * q, ah, al);
+        mfma3(accr[ii], ah, al, xop.h[q], xop.l[q]);
+        col_prod(accc[0], xi, 16 * q);
+      }
       if constexpr (ii == M - 1) {  // column j done: its partial out
         float* pq = part + poff[d] + ((band_pbase(geo, b) + j) * (kp / 64) + blockIdx.z) * 4096;
 #pragma unroll
@@ -290,21 +311,26 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
       __syncthreads();
     });
   }
-  // the band's own lower triangle, (jj, ii) column-major; inactive steps (ii >= nb,
-  // band 0 only) load a valid tile and do nothing else
+  // the band's own lower triangle, (jj, ii) column-major
   static_for<0, NTRI>([&](auto ec) {
     constexpr int e = decltype(ec)::value, jj = tri_jj<M>(e), ii = tri_ii<M>(e);
     const bool act = ii < nb;
+    constexpr int q = e & 1;  // step 4 i0 + e
+    if (act) stash(g[q]);
+    __syncthreads();
+    fetch_tri(g[q], std::integral_constant<int, e + 2>{});
     if (act) {
-      stash(g);
-      if constexpr (ii == jj) {  // the column's X_j is the diagonal tile's X_i
-        split8(g.xi, xj_ptr(i0 + jj), xk);
-        split8(g.xi + 8, xj_ptr(i0 + jj), xk + 8);
+      const _Float16* xj = XB + 2 * XP * jj;
+      const _Float16* xi = XB + 2 * XP * ii;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        half8 ah, al;
+        row_a(16 * q, ah, al);
+        const int xo = xpos(xcol, 16 * q + ck);
+        mfma3(accr[ii], ah, al, *(const half8*)(xj + xo), *(const half8*)(xj + XP + xo));
+        if constexpr (ii > jj) col_prod(accc[jj], xi, 16 * q);
       }
     }
-    __syncthreads();
-    fetch_tri(g, std::integral_constant<int, e + 1>{});
-    if (act) mm(accr[ii], xj_ptr(i0 + jj), &accc[jj], ii > jj);
     __syncthreads();
   });
   const int col = cb + (lane & 31);

@@ -87,9 +87,9 @@ int hfls_mesh_load(hfls_mesh m, double* f);
  * "gemm_rows": "1", "2" or "auto" (default): tile shape of the mixed-precision block
  * GEMM, overriding the autotuner (FP32 format: 1 or 2 output row tiles per workgroup;
  * split-FP16 format, the default, option block_gemm: K chunks of 32 or 64);
- * "gemm_kernel": "rows" (default) or "bands": the split-FP16 GEMM reads every stored
- * tile once per row block that needs it, or once (workgroups own bands of 4 row
- * blocks, partial sums added in a fixed order); the two forms round differently,
+ * "gemm_kernel": "bands" (default) or "rows": the split-FP16 GEMM reads every stored
+ * tile once (workgroups own bands of 4 row blocks, partial sums added in a fixed
+ * order), or once per row block that needs it; the two forms round differently,
  * each is bitwise reproducible */
 int hfls_mesh_configure(hfls_mesh m, const char* key, const char* value);
 /* autotuner result (after the first block call): [0] tile shape chosen for the
