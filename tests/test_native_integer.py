@@ -153,3 +153,56 @@ def test_mirror_shape_checks():
         _rows(np.zeros((0, 5)), 5, "F")
     with pytest.raises(ValueError):
         _rows(np.zeros((2, 4)), 5, "F")
+
+
+# Bf glue rows where 2, 4 and 8 subdomains share a DOF, derived by hand from the loop of
+# Cluster.cpp:846-903 and the geometry (not from the oracle's grouping): subdomain d visits
+# its interface DOFs in ascending global order (skipping Dirichlet DOFs) and, for each,
+# the subdomains above d that also hold it in ascending order (the data.interface lists
+# filled by the pairwise loop of Cluster.cpp:705-727): one row each, +1 in d, -1 in the
+# neighbour, weight 1 / (number of subdomains holding the DOF).  Cube of 2x2x2 subdomains
+# of 2x2x2 elements: global node (i, j, k), 0 <= i, j, k <= 4, is i + 5 j + 25 k;
+# subdomain (I, J, K) is I + 2 J + 4 K and holds nodes 2I <= i <= 2I + 2 (same in j, k);
+# clamp_x0 makes the DOFs of i = 0 Dirichlet.
+def _holders(i, j, k):
+    side = lambda x: [0] if x < 2 else ([0, 1] if x == 2 else [1])
+    return sorted(I + 2 * J + 4 * K for I in side(i) for J in side(j) for K in side(k))
+
+
+def _hand_rows(d):
+    I, J, K = d & 1, (d >> 1) & 1, d >> 2
+    nodes = sorted((i + 5 * j + 25 * k, (i, j, k)) for i in range(2 * I, 2 * I + 3) for j in range(2 * J, 2 * J + 3)
+                   for k in range(2 * K, 2 * K + 3))
+    rows = []
+    for node, (i, j, k) in nodes:
+        h = _holders(i, j, k)
+        if len(h) == 1 or i == 0:  # not on an interface, or Dirichlet
+            continue
+        for c in range(3):
+            rows += [(d, s, 3 * node + c, 1.0 / len(h)) for s in h if s > d]
+    return rows
+
+
+def test_bf_rows_of_dofs_shared_by_4_and_8_subdomains():
+    """Row order, signs and weights of the Bf glue rows of subdomains 0 and 1 (where DOFs
+    are shared by 2, 4 and 8 subdomains) against the hand derivation above, for the
+    native library and the oracle; spot check of the rows of the centre node (all 8
+    subdomains hold it: 7 rows from subdomain 0, 6 from subdomain 1, weight 1/8)."""
+    kw = dict(Nx=2, Ny=2, Nz=2, nx=2, ny=2, nz=2, bc="clamp_x0")
+    hand = _hand_rows(0) + _hand_rows(1)
+    n0 = len(_hand_rows(0))
+    centre = 3 * (2 + 5 * 2 + 25 * 2)
+    assert [r for r in hand if r[2] == centre] == [(0, s, centre, 0.125) for s in range(1, 8)] + \
+        [(1, s, centre, 0.125) for s in range(2, 8)]
+    p = hfls_amd.Problem(**kw)
+    a, b, dof, w = p.rows()
+    got = list(zip(a[:len(hand)].tolist(), b[:len(hand)].tolist(), dof[:len(hand)].tolist(), w[:len(hand)].tolist()))
+    assert got[:n0] == hand[:n0] and got == hand
+    assert a[len(hand)] == 2  # the rows of subdomain 2 follow
+    m = OMesh(oopts(**kw))
+    g = Gluing(m, Subdomains(m))
+    ogot = list(zip(g.row_a[:len(hand), 0].tolist(), g.row_b[:len(hand), 0].tolist(),
+                    g.row_a[:len(hand), 1].tolist(), g.weight[:len(hand)].tolist()))
+    assert ogot == hand
+    # every pair of the 8 holders of the centre DOF glued once: 28 rows
+    assert int(np.sum(dof[:p.n_glue] == centre)) == 28
