@@ -385,7 +385,7 @@ def main():
             plain_res.append(r["residual"])
             plain_pg.append(r["pg_rel"])
     log("plain fp64: %s ms, iterations %s" % (["%.1f" % x for x in plain_ms], plain_it))
-    plain_rate = 1e3 / float(np.mean(plain_ms))
+    plain_rate = 1e3 / float(np.mean(plain_ms)) if plain_ms else None  # --plain-steps 0: not timed
 
     # ---- batched mixed precision: blocks of k load cases
     blk_ms, blk_wall, its, res, certs, sweeps = [], [], [], [], [], []
@@ -462,10 +462,11 @@ def main():
                    "iterations_min_max": [int(its.min()), int(its.max())], "max_true_residual": float(max(res)),
                    "fp64_refinement_points_per_block": float(np.mean(certs)), "sweeps_per_block": float(np.mean(sweeps)),
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3)},
-        "plain": {"value": plain_rate, "unit": "solves/s", "ms_per_solve": float(np.mean(plain_ms)),
-                  "iterations": plain_it[-1], "dtype": "f64", "residual_max": float(max(plain_res)),
-                  "projected_gradient_rel_max": float(max(plain_pg))},
-        "speedup_vs_plain": value / plain_rate,
+        "plain": (None if plain_rate is None else
+                  {"value": plain_rate, "unit": "solves/s", "ms_per_solve": float(np.mean(plain_ms)),
+                   "iterations": plain_it[-1], "dtype": "f64", "residual_max": float(max(plain_res)),
+                   "projected_gradient_rel_max": float(max(plain_pg))}),
+        "speedup_vs_plain": value / plain_rate if plain_rate else None,
         "concurrent_blocks": conc,
         "host_inclusive_solves_per_s": a.k * len(blk_wall) / float(np.sum(blk_wall)),
         "block_vs_plain_displacement_rel_diff": agree,
@@ -525,7 +526,7 @@ def main():
         # faster than they are served (Poisson, 200 per second), 1..64 cases each
         sat = run_stream(hfls_amd, [cfg], a.serve_saturated, 200.0, SEED + 1, a.eps, 250.0, True)
         sat["meshes"] = [workload]
-        sat["speedup_vs_plain"] = sat["solves_per_s"] / plain_rate
+        sat["speedup_vs_plain"] = sat["solves_per_s"] / plain_rate if plain_rate else None
         out["serving_saturated"] = sat
         log("saturated serving:", json.dumps(sat))
     if a.serve_requests > 0:
