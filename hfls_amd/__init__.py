@@ -25,7 +25,8 @@ EXPORTS = [
     "hfls_server_result", "hfls_server_stats", "hfls_mesh_configure",
     "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk", "hfls_solve_blocks",
     "hfls_server_cancel", "hfls_mult_Gf", "hfls_mult_GfT", "hfls_mult_RfT", "hfls_recover", "hfls_mesh_rf",
-    "hfls_mesh_counts", "hfls_project_noalloc",
+    "hfls_mesh_counts", "hfls_project_noalloc", "hfls_mesh_save", "hfls_mesh_restore", "hfls_server_host_cache",
+    "hfls_server_cache_stats",
 ]
 
 DEFAULTS = {
@@ -204,6 +205,30 @@ class Mesh:
         n, k, v = _opts(opts)
         self._h = ctypes.c_void_p()
         _check(lib.hfls_mesh_create(int(device), n, k, v, ctypes.byref(self._h)))
+        self.restore_ms = None
+        self._read_sizes()
+
+    @classmethod
+    def restore(cls, path, device=0):
+        """A mesh from the set-up state saved by save() (in this or another process),
+        without the factorisation; bit-identical results to the saved mesh.
+        ``restore_ms`` is the load time, ``setup_ms`` the original set-up's."""
+        lib = load()
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        ms = ctypes.c_double()
+        _check(lib.hfls_mesh_restore(int(device), os.fsencode(path), ctypes.byref(self._h), ctypes.byref(ms)))
+        self.restore_ms = ms.value
+        self._read_sizes()
+        return self
+
+    def save(self, path):
+        """Write the set-up state (options, factors, K+_bb and S_i tiles, kernels,
+        gluing, coarse inverses) to ``path`` (through path + ".part")."""
+        _check(load().hfls_mesh_save(self._h, os.fsencode(path)))
+
+    def _read_sizes(self):
+        lib = load()
         s = np.zeros(6, dtype=np.int64)
         _check(lib.hfls_mesh_sizes(self._h, _ptr(s)))
         self.nsub, self.nlam, self.ncol, self.npacked, self.nPoints, self.device_bytes = (int(x) for x in s)
@@ -410,12 +435,20 @@ class Server:
     result() returns the displacements and per-case stats of a finished request.
     """
 
-    def __init__(self, device=0, hbm_budget_gb=200.0, slots=64, precision=32, eps=1e-8, max_iter=1000, chunk=8):
+    def __init__(self, device=0, hbm_budget_gb=200.0, slots=64, precision=32, eps=1e-8, max_iter=1000, chunk=8,
+                 host_cache_gb=0.0):
         lib = load()
         self._h = ctypes.c_void_p()
         _check(lib.hfls_server_create(int(device), ctypes.c_double(hbm_budget_gb), int(slots), int(precision),
                                       ctypes.c_double(eps), int(max_iter), int(chunk), ctypes.byref(self._h)))
         self._k = {}
+        if host_cache_gb:
+            self.host_cache(host_cache_gb)
+
+    def host_cache(self, gb):
+        """Pinned host memory (GB, 0 = off) for the set-up state of evicted meshes: a
+        later request for an evicted mesh restores it instead of setting it up."""
+        _check(load().hfls_server_host_cache(self._h, ctypes.c_double(gb)))
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -482,5 +515,10 @@ class Server:
         _check(load().hfls_server_stats(self._h, _ptr(out)))
         keys = ["meshes", "bytes", "setups", "evictions", "cases_done", "requests_done", "joins", "setup_ms",
                 "failed_setups", "cancelled", "steps", "join_ms", "sweep_ms", "finish_ms"]
-        return dict(zip(keys, out.tolist()))
+        st = dict(zip(keys, out.tolist()))
+        c = np.zeros(7)
+        _check(load().hfls_server_cache_stats(self._h, _ptr(c)))
+        st.update(zip(["host_meshes", "host_bytes", "saves", "save_ms", "restores", "restore_ms", "host_drops"],
+                      c.tolist()))
+        return st
 

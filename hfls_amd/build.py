@@ -35,6 +35,7 @@ SOURCES = [
     ("engine.cpp", HOST_FLAGS),
     ("engine_block.cpp", HOST_FLAGS),
     ("serve.cpp", HOST_FLAGS),
+    ("state.cpp", HOST_FLAGS),
     ("vtk.cpp", HOST_FLAGS),
     ("capi.cpp", HOST_FLAGS),
 ]

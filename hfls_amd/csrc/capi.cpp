@@ -9,6 +9,7 @@
 #include "engine.hpp"
 #include "problem.hpp"
 #include "serve.hpp"
+#include "state.hpp"
 #include "vtk.hpp"
 
 struct hfls_problem_s {
@@ -201,6 +202,21 @@ extern "C" int hfls_mesh_create(int device, int nopt, const char* const* keys, c
     hfls::Options o = make_options(nopt, keys, vals);
     std::unique_ptr<hfls::Engine> e(new hfls::Engine(o, device));
     e->setup();
+    auto* h = new hfls_mesh_s;
+    h->e = e.release();
+    *out = h;
+  });
+}
+
+extern "C" int hfls_mesh_save(hfls_mesh m, const char* path) {
+  return guard([&] { hfls::save_mesh_file(*eng(m), need(path, "path")); });
+}
+
+extern "C" int hfls_mesh_restore(int device, const char* path, hfls_mesh* out, double* restore_ms) {
+  return guard([&] {
+    need(out, "out");
+    std::unique_ptr<hfls::Engine> e = hfls::load_mesh_file(need(path, "path"), device);
+    if (restore_ms) *restore_ms = e->setup_info().restore_ms;
     auto* h = new hfls_mesh_s;
     h->e = e.release();
     *out = h;
@@ -518,6 +534,27 @@ extern "C" int hfls_server_stats(hfls_server s, double* out) {
     out[11] = t.join_ms;
     out[12] = t.sweep_ms;
     out[13] = t.finish_ms;
+  });
+}
+
+extern "C" int hfls_server_host_cache(hfls_server s, double gb) {
+  return guard([&] {
+    if (!(gb >= 0.0) || gb > 1e6) throw std::runtime_error("host cache size must be 0..1e6 GB");
+    srv(s)->set_host_cache((size_t)(gb * 1e9));
+  });
+}
+
+extern "C" int hfls_server_cache_stats(hfls_server s, double* out) {
+  return guard([&] {
+    need(out, "out");
+    const hfls::ServeStats& t = srv(s)->stats();
+    out[0] = (double)t.host_meshes;
+    out[1] = (double)t.host_bytes;
+    out[2] = (double)t.saves;
+    out[3] = t.save_ms;
+    out[4] = (double)t.restores;
+    out[5] = t.restore_ms;
+    out[6] = (double)t.host_drops;
   });
 }
 

@@ -790,6 +790,50 @@ void Engine::factor_subdomains() {
     throw std::runtime_error("kernel detection found a defect other than 6");
 }
 
+// Layout of the cluster constraints from their counts (h_nlc per subdomain, h_nlcl
+// per cluster): offsets of Bc_i^T and C_i (ld = ldb), of the local constraint rows,
+// of the clusters' saddle-point systems, and the device copies of these counts and
+// offsets.  The same function serves the set-up and the restore of a saved state.
+void Engine::coarse_layout() {
+  const int N = P->nsub, ncl = P->ncl;
+  if ((int)h_nlc.size() != N || (int)h_nlcl.size() != ncl) throw std::runtime_error("constraint counts of the wrong size");
+  off_C.assign(N + 1, 0);
+  off_lc.assign(N + 1, 0);
+  maxnlc = 0;
+  for (int d = 0; d < N; d++) {
+    if (h_nlc[d] < 0 || h_nlc[d] > 256) throw std::runtime_error("too many cluster constraints on one subdomain");
+    maxnlc = std::max(maxnlc, h_nlc[d]);
+    off_C[d + 1] = off_C[d] + align32((int64_t)h_ldb[d] * std::max(1, h_nlc[d]));
+    off_lc[d + 1] = off_lc[d] + h_nlc[d];
+  }
+  h_la_off.assign(ncl + 1, 0);
+  h_Aoff.assign(ncl + 1, 0);
+  max_cl_len = 0;
+  std::vector<int64_t> gc_off(ncl);  // first constraint of each cluster in the G^T pairs
+  int64_t gcount = 0;
+  for (int cl = 0; cl < ncl; cl++) {
+    if (h_nlcl[cl] < 0) throw std::runtime_error("negative cluster constraint count");
+    const int len = h_nlcl[cl] + 6 * P->nsub_cl;
+    h_la_off[cl + 1] = h_la_off[cl] + len;
+    h_Aoff[cl + 1] = h_Aoff[cl] + align32((int64_t)len * len);
+    max_cl_len = std::max(max_cl_len, len);
+    gc_off[cl] = gcount;
+    gcount += h_nlcl[cl];
+  }
+  d_nlc.upload(h_nlc, st);
+  d_off_C.upload(std::vector<int64_t>(off_C.begin(), off_C.end() - 1), st);
+  d_off_lc.upload(std::vector<int64_t>(off_lc.begin(), off_lc.end() - 1), st);
+  d_gc_off.upload(gc_off, st);
+  std::vector<int> nrcl(ncl, 6 * P->nsub_cl), d0v(ncl), nlen(ncl);
+  for (int cl = 0; cl < ncl; cl++) { d0v[cl] = cl * P->nsub_cl; nlen[cl] = h_nlcl[cl] + 6 * P->nsub_cl; }
+  d_nlcl.upload(h_nlcl, st); d_nrcl.upload(nrcl, st); d_d0.upload(d0v, st); d_nlen.upload(nlen, st);
+  d_la_off.upload(std::vector<int64_t>(h_la_off.begin(), h_la_off.end() - 1), st);
+  d_Aoff.upload(std::vector<int64_t>(h_Aoff.begin(), h_Aoff.end() - 1), st);
+  std::vector<int64_t> aidx(N);
+  for (int d = 0; d < N; d++) aidx[d] = h_la_off[h_cl[d]] + h_nlcl[h_cl[d]] + 6 * (d - h_cl[d] * P->nsub_cl);
+  d_alpha_idx.upload(aidx, st);
+}
+
 void Engine::build_bc_and_coarse() {
   const int N = P->nsub;
   const Options& o = P->o;
@@ -940,30 +984,17 @@ void Engine::build_bc_and_coarse() {
   });
   // per-subdomain dense Bc^T (b-order, ld = ldb), row maps
   h_nlc.assign(N, 0);
-  off_C.assign(N + 1, 0);
-  off_lc.assign(N + 1, 0);
   for (int d = 0; d < N; d++) {
     std::sort(brow[d].begin(), brow[d].end(), [](const Row& a, const Row& b) { return a.l < b.l; });
     h_nlc[d] = (int)brow[d].size();
-    if (h_nlc[d] > 256) throw std::runtime_error("too many cluster constraints on one subdomain");
-    maxnlc = std::max(maxnlc, h_nlc[d]);
-    off_C[d + 1] = off_C[d] + align32((int64_t)h_ldb[d] * std::max(1, h_nlc[d]));
-    off_lc[d + 1] = off_lc[d] + h_nlc[d];
   }
   h_nlcl = nlcl;
-  h_la_off.assign(P->ncl + 1, 0);
-  h_Aoff.assign(P->ncl + 1, 0);
-  max_cl_len = 0;
-  for (int cl = 0; cl < P->ncl; cl++) {
-    int len = nlcl[cl] + 6 * P->nsub_cl;
-    h_la_off[cl + 1] = h_la_off[cl] + len;
-    h_Aoff[cl + 1] = h_Aoff[cl] + align32((int64_t)len * len);
-    max_cl_len = std::max(max_cl_len, len);
-  }
+  coarse_layout();
   std::unique_ptr<double[]> BcT(new double[std::max<int64_t>(off_C[N], 1)]);  // zeroed per subdomain below
-  std::vector<int> lcmap(off_lc[N]);
-  std::vector<int> gc_src;
-  std::vector<int64_t> gc_off(P->ncl);
+  std::vector<int>& lcmap = h_lcmap;
+  std::vector<int>& gc_src = h_gcsrc;
+  lcmap.assign(off_lc[N], 0);
+  gc_src.clear();
   std::vector<std::vector<int>> contrib(h_la_off[P->ncl]);
   parallel_for(N, [&](int d) {
     std::fill(BcT.get() + off_C[d], BcT.get() + off_C[d + 1], 0.0);
@@ -976,36 +1007,21 @@ void Engine::build_bc_and_coarse() {
   });
   for (int d = 0; d < N; d++)
     for (int k = 0; k < h_nlc[d]; k++) contrib[h_la_off[h_cl[d]] + brow[d][k].l].push_back((int)(off_lc[d] + k));
-  int64_t gcount = 0;
   for (int cl = 0; cl < P->ncl; cl++) {
-    gc_off[cl] = gcount;
     for (int l = 0; l < nlcl[cl]; l++) {
       auto& c = contrib[h_la_off[cl] + l];
       if (c.empty() || c.size() > 2) throw std::runtime_error("cluster constraint with unexpected support");
       gc_src.push_back(c[0]);
       gc_src.push_back(c.size() > 1 ? c[1] : -1);
-      gcount++;
     }
   }
+  if (gc_src.empty()) gc_src.push_back(-1);
   d_BcT.alloc(std::max<int64_t>(off_C[N], 1));
   HC(hipMemcpyAsync(d_BcT.p, BcT.get(), off_C[N] * sizeof(double), hipMemcpyHostToDevice, st));
   HC(hipStreamSynchronize(st));
   BcT.reset();
   d_lcmap.upload(lcmap, st);
-  d_nlc.upload(h_nlc, st);
-  d_off_C.upload(std::vector<int64_t>(off_C.begin(), off_C.end() - 1), st);
-  d_off_lc.upload(std::vector<int64_t>(off_lc.begin(), off_lc.end() - 1), st);
-  if (gc_src.empty()) gc_src.push_back(-1);
   d_gc_src.upload(gc_src, st);
-  d_gc_off.upload(gc_off, st);
-  std::vector<int> nrcl(P->ncl, 6 * P->nsub_cl), d0v(P->ncl), nlen(P->ncl);
-  for (int cl = 0; cl < P->ncl; cl++) { d0v[cl] = cl * P->nsub_cl; nlen[cl] = nlcl[cl] + 6 * P->nsub_cl; }
-  d_nlcl.upload(nlcl, st); d_nrcl.upload(nrcl, st); d_d0.upload(d0v, st); d_nlen.upload(nlen, st);
-  d_la_off.upload(std::vector<int64_t>(h_la_off.begin(), h_la_off.end() - 1), st);
-  d_Aoff.upload(std::vector<int64_t>(h_Aoff.begin(), h_Aoff.end() - 1), st);
-  std::vector<int64_t> aidx(N);
-  for (int d = 0; d < N; d++) aidx[d] = h_la_off[h_cl[d]] + nlcl[h_cl[d]] + 6 * (d - h_cl[d] * P->nsub_cl);
-  d_alpha_idx.upload(aidx, st);
 
   // C_i = K+_bb Bc_i^T ; Fc_i = Bc_i C_i ; Gc_i = -Bc_i R_b
   d_C.alloc(off_C[N]);
@@ -1256,7 +1272,12 @@ void Engine::setup() {
   build_bc_and_coarse();
   HC(hipStreamSynchronize(st));
   sinfo.coarse_ms = ms_since(t1);
-  // work vectors
+  alloc_work();
+  sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// work vectors of the solves, and the set-up's device footprint
+void Engine::alloc_work() {
   const int64_t nl = P->nlam;
   const int N = P->nsub;
   for (DVec<double>* v : {&lam, &g, &w, &Fw, &Pg, &z, &Pz, &dvec}) v->alloc(nl);
@@ -1285,9 +1306,6 @@ void Engine::setup() {
   // the scratch array of the allocation-free projection: its device address is
   // looked up here, once per device, so that the entry point itself does no set-up
   if (12 * (int64_t)N <= NOALLOC_SCRATCH_DOUBLES) (void)noalloc_scratch(device, 0);
-  sinfo.setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  size_t freeb = 0, totb = 0;
-  HC(hipMemGetInfo(&freeb, &totb));
   sinfo.device_bytes = (off_L[N] + 2 * off_P[N] + off_R[N] + off_Rb[N] + 2 * off_C[N]) * sizeof(double) +
                        (d_brv.n + d_rbv.n) * (sizeof(double) + sizeof(int));
 }

@@ -68,6 +68,7 @@ struct SetupInfo {
   double meta_ms = 0.0;      // index structures
   double assemble_ms = 0.0;  // scratch allocation and element assembly
   double coarse_ms = 0.0;    // Bc, cluster and coarse problems
+  double restore_ms = 0.0;   // a mesh loaded from a saved state: the load (the others: the original set-up)
   size_t device_bytes = 0;
   int min_defect = 0, max_defect = 0;
 };
@@ -113,11 +114,23 @@ struct BlockCtx {
   DVec<BScalars> sc;
 };
 
+class StateStream;
+
 class Engine {
  public:
   Engine(const Options& o, int device);
   ~Engine();
   void setup();
+  // The set-up state (the products of setup(): the factors W = L_rr^-1, the K+_bb and
+  // S_i tiles, kernels and fixing DOFs, the gluing and the coarse inverses) through a
+  // byte stream: save_state after setup(); load_state on an Engine constructed with the
+  // options the state was saved with, in place of setup().  The index structures are
+  // rebuilt from the options, every loaded size and index is checked against them, and
+  // a loaded mesh computes bit-identical results to the mesh that was saved.
+  // copy: the stream of the device-to-host copies of a save (null: the engine's own;
+  // the saved state is immutable, so another stream may copy it while the engine runs)
+  void save_state(StateStream& s, hipStream_t copy = nullptr);
+  void load_state(StateStream& s);
   const Problem& problem() const { return *P; }
   int64_t nlam() const { return P->nlam; }
   int ncol_G() const { return 6 * P->ncl; }
@@ -263,6 +276,7 @@ class Engine {
   // clusters
   std::vector<int> h_nlcl;
   std::vector<int64_t> h_la_off, h_Aoff;
+  std::vector<int> h_lcmap, h_gcsrc;  // host copies of d_lcmap, d_gc_src (saved state)
   DVec<int> d_gc_src, d_nlcl, d_nrcl, d_d0, d_nlen;
   DVec<int64_t> d_gc_off, d_la_off, d_Aoff;
   DVec<double> d_Ainv;
@@ -343,6 +357,9 @@ class Engine {
   void build_meta();
   void factor_subdomains();
   void build_bc_and_coarse();
+  void coarse_layout();  // constraint offsets and device copies from h_nlc, h_nlcl
+  void alloc_work();     // solve work vectors (end of set-up and of a state load)
+  void state_fields(StateStream& s, hipStream_t cs);
   void partial_cholesky(double* M, const std::vector<int64_t>& moff, const std::vector<int>& ld,
                         const std::vector<int>& start, const std::vector<int>& ncols, const std::vector<int>& end);
   // solve pieces (device vectors)

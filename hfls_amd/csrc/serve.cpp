@@ -34,6 +34,7 @@ Server::Server(int device, size_t budget, int slots, int prec, double eps, int m
 
 Server::~Server() {
   if (pend_ && pend_->fut.valid()) pend_->fut.wait();
+  if (save_ && save_->fut.valid()) save_->fut.wait();
 }
 
 double Server::now() const {
@@ -115,12 +116,103 @@ void Server::fail_waiting(const std::string& key, const std::string& msg) {
   }
 }
 
+// with host_mu_ held
+int64_t Server::host_trim(size_t budget) {
+  int64_t dropped = 0;
+  while (saved_bytes_ > budget && !saved_.empty()) {
+    auto lru = saved_.begin();
+    for (auto it = saved_.begin(); it != saved_.end(); ++it)
+      if (it->second.last_used < lru->second.last_used) lru = it;
+    saved_bytes_ -= lru->second.h->bytes;
+    saved_.erase(lru);
+    dropped++;
+  }
+  return dropped;
+}
+
+void Server::set_host_cache(size_t bytes) {
+  finish_save(true);  // no copy in flight while the budget changes
+  std::lock_guard<std::mutex> lk(host_mu_);
+  host_budget_ = bytes;
+  st_.host_drops += host_trim(bytes);
+  st_.host_bytes = saved_bytes_;
+  st_.host_meshes = (int64_t)saved_.size();
+}
+
+// Start copying the set-up state of a cached mesh that has no saved copy yet (the
+// most recently used first) to pinned host memory, on a thread and a stream of its
+// own: the state is immutable after set-up, so the copy overlaps the mesh's own
+// iterations, and evicting the mesh later costs nothing.
+void Server::start_save() {
+  if (save_ || host_budget_ == 0) return;
+  ServeMesh* pick = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(host_mu_);
+    for (auto& kv : mesh_) {
+      ServeMesh* m = kv.second.get();
+      if (m->no_save || saved_.count(m->key)) continue;
+      if (!pick || m->last_used > pick->last_used) pick = m;
+    }
+  }
+  if (!pick) return;
+  save_.reset(new PendingSave);
+  save_->m = pick;
+  save_->t0 = now();
+  const int dev = device_;
+  const std::string key = pick->key;
+  Engine* e = pick->e.get();
+  save_->fut = std::async(std::launch::async, [this, dev, key, e]() -> int64_t {
+    HC(hipSetDevice(dev));
+    hipStream_t cs = nullptr;  // the engine's own streams belong to the server's thread
+    HC(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    int64_t dropped = 0;
+    std::shared_ptr<HostMesh> h;
+    try {
+      const size_t n = saved_bytes(*e, cs);  // host-side counting pass
+      {
+        std::lock_guard<std::mutex> lk(host_mu_);
+        if (n > host_budget_) throw std::runtime_error("saved mesh larger than the host cache");
+        dropped = host_trim(host_budget_ - n);
+      }
+      h = save_mesh_host(*e, cs);
+    } catch (...) {
+      (void)hipStreamSynchronize(cs);
+      (void)hipStreamDestroy(cs);
+      throw;
+    }
+    HC(hipStreamDestroy(cs));
+    std::lock_guard<std::mutex> lk(host_mu_);
+    saved_bytes_ += h->bytes;
+    SavedMesh& s = saved_[key];
+    s.h = std::move(h);
+    s.last_used = now();
+    return dropped;
+  });
+}
+
+bool Server::finish_save(bool wait) {
+  if (!save_) return false;
+  if (!wait && save_->fut.wait_for(std::chrono::seconds(0)) != std::future_status::ready) return false;
+  try {
+    st_.host_drops += save_->fut.get();
+    st_.saves++;
+    st_.save_ms += now() - save_->t0;
+  } catch (const std::exception&) {
+    save_->m->no_save = true;  // does not fit (or no pinned memory): not tried again
+  }
+  save_.reset();
+  std::lock_guard<std::mutex> lk(host_mu_);
+  st_.host_bytes = saved_bytes_;
+  st_.host_meshes = (int64_t)saved_.size();
+  return true;
+}
+
 bool Server::finish_setup(bool wait) {
   if (!pend_) return false;
   if (!wait && pend_->fut.wait_for(std::chrono::seconds(0)) != std::future_status::ready) return false;
-  std::unique_ptr<ServeMesh> m;
+  SetupOut out;
   try {
-    m = pend_->fut.get();  // rethrows a failed set-up
+    out = pend_->fut.get();  // rethrows a failed set-up
   } catch (const std::exception& ex) {
     // the waiting requests of this mesh end with status 7; the server goes on
     const std::string key = pend_->key;
@@ -130,8 +222,15 @@ bool Server::finish_setup(bool wait) {
     fail_waiting(key, std::string("set-up of the mesh failed: ") + ex.what());
     return false;
   }
-  st_.setup_ms += now() - pend_->t0;
-  st_.setups++;
+  std::unique_ptr<ServeMesh> m = std::move(out.m);
+  const double ms = now() - pend_->t0;
+  if (out.restored) {
+    st_.restores++;
+    st_.restore_ms += ms;
+  } else {
+    st_.setups++;
+    st_.setup_ms += ms;
+  }
   m->last_used = now();
   mesh_.emplace(pend_->key, std::move(m));
   pend_.reset();
@@ -184,7 +283,8 @@ bool Server::admit(ServeRequest& r) {
     if (pend_) return false;  // one set-up at a time (possibly this very mesh's)
     const Problem& p = *host_[r.key];
     const size_t need = estimate(p);
-    // evict idle meshes, least recently used first, until the new one fits
+    // evict idle meshes, least recently used first, until the new one fits (a mesh
+    // being copied to the host cache is evicted once its copy is done)
     while (bytes_in_use() + need > budget_) {
       ServeMesh* lru = nullptr;
       for (auto& kv : mesh_) {
@@ -194,10 +294,12 @@ bool Server::admit(ServeRequest& r) {
         if (idle && (!lru || m->last_used < lru->last_used)) lru = m;
       }
       if (!lru) return false;  // everything cached is busy: wait
+      if (save_ && save_->m == lru) finish_save(true);
       mesh_.erase(lru->key);
       st_.evictions++;
     }
-    // set the mesh up on the worker thread; the request is admitted once it is cached
+    // set the mesh up (or restore it from the host cache) on the worker thread; the
+    // request is admitted once it is cached
     pend_.reset(new PendingSetup);
     pend_->key = r.key;
     pend_->need = need;
@@ -207,15 +309,36 @@ bool Server::admit(ServeRequest& r) {
     const int dev = device_, slots = slots_, mi = max_iter_, prec = prec_;
     const double eps = eps_;
     const std::string key = r.key;
-    pend_->fut = std::async(std::launch::async, [o, ng, dev, slots, mi, prec, eps, key, need]() {
+    pend_->fut = std::async(std::launch::async, [this, o, ng, dev, slots, mi, prec, eps, key, need]() {
       HC(hipSetDevice(dev));
+      SetupOut out;
       std::unique_ptr<ServeMesh> m(new ServeMesh);
       m->key = key;
       m->opts = o;
       size_t f0 = 0, f1 = 0, tot = 0;
       HC(hipMemGetInfo(&f0, &tot));
-      m->e.reset(new Engine(o, dev));
-      m->e->setup();
+      std::shared_ptr<HostMesh> h;
+      {
+        std::lock_guard<std::mutex> lk(host_mu_);
+        auto sv = saved_.find(key);
+        if (sv != saved_.end()) {
+          h = sv->second.h;  // kept alive by this copy even if the saver drops the entry
+          sv->second.last_used = now();
+        }
+      }
+      if (h) {
+        try {
+          m->e = load_mesh_host(*h, dev);
+          out.restored = true;
+        } catch (const std::exception&) {
+          m->e.reset();  // set up instead
+        }
+        h.reset();
+      }
+      if (!m->e) {
+        m->e.reset(new Engine(o, dev));
+        m->e->setup();
+      }
       m->e->blk_begin(slots, eps, mi, prec);
       HC(hipStreamSynchronize(m->e->stream()));
       HC(hipMemGetInfo(&f1, &tot));
@@ -223,7 +346,8 @@ bool Server::admit(ServeRequest& r) {
       m->np = m->e->packed_n();
       m->ng = ng;
       m->slot.assign(slots, ServeSlot());
-      return m;
+      out.m = std::move(m);
+      return out;
     });
     return false;
   }
@@ -240,6 +364,8 @@ bool Server::step() {
   //    without blocking the requests behind it.  A finished set-up is installed
   //    first; with nothing else to run, the loop waits for the running set-up.
   finish_setup(false);
+  finish_save(false);
+  start_save();
   auto admission = [&] {
     // requests for cached meshes, in arrival order
     for (auto it = wait_.begin(); it != wait_.end();) {

@@ -12,17 +12,21 @@
 // and in the mix.  A cold mesh is set up on a worker thread (its own stream) while
 // the cached meshes keep iterating; one set-up runs at a time, in arrival order
 // except that a mesh with less than an eighth of the set-up work of the earliest
-// waiting one may go first.
+// waiting one may go first.  Optionally, evicted meshes are kept in pinned host
+// memory (their set-up state, see state.hpp): a later request restores the state
+// over the host link instead of setting the mesh up again.
 #pragma once
 #include <cstdint>
 #include <deque>
 #include <future>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "engine.hpp"
+#include "state.hpp"
 
 namespace hfls {
 
@@ -59,6 +63,7 @@ struct ServeMesh {
   bool swept = false;
   double last_used = 0;
   std::vector<int> waited;             // per slot: chunks since its case stopped (-1: not stopped)
+  bool no_save = false;                // its copy to the host cache failed (too large): not retried
 };
 
 struct ServeStats {
@@ -66,6 +71,12 @@ struct ServeStats {
   int64_t failed_setups = 0, cancelled = 0, steps = 0;
   double setup_ms = 0;
   double join_ms = 0, sweep_ms = 0, finish_ms = 0;  // host wall time of the step phases (device work included)
+  // host-memory cache of evicted meshes: saves of evicted meshes, restores in place
+  // of set-ups, saved meshes dropped to make room; the ms as setup_ms (admission to
+  // cached, on the worker thread); save_ms is the background copies' wall time
+  int64_t saves = 0, restores = 0, host_drops = 0, host_meshes = 0;
+  double save_ms = 0, restore_ms = 0;
+  size_t host_bytes = 0;
 };
 
 class Server {
@@ -85,6 +96,9 @@ class Server {
   // drop a request: its queued cases are removed, its running cases leave their
   // slots (without recovery); the id is unknown afterwards
   void cancel(int64_t id);
+  // pinned host memory for the set-up state of evicted meshes (0: off, the default);
+  // least recently used saved meshes are dropped when a new one does not fit
+  void set_host_cache(size_t bytes);
   const ServeStats& stats() const { return st_; }
   size_t bytes_in_use() const;  // cached meshes, a set-up in flight, results not yet fetched
   int meshes() const { return (int)mesh_.size(); }
@@ -100,12 +114,35 @@ class Server {
   std::map<std::string, std::unique_ptr<ServeMesh>> mesh_;
   std::map<std::string, std::unique_ptr<Problem>> host_;  // host problems (sizes, loads)
   ServeStats st_;
-  struct PendingSetup {  // a mesh being set up on the worker thread
+  struct SetupOut {  // what the worker thread hands back
+    std::unique_ptr<ServeMesh> m;
+    bool restored = false;
+  };
+  struct PendingSetup {  // a mesh being set up (or restored) on the worker thread
     std::string key;
     size_t need = 0;
     double t0 = 0;
-    std::future<std::unique_ptr<ServeMesh>> fut;
+    std::future<SetupOut> fut;
   };
+  // host-memory cache of set-up states: written by the saver thread, read by the
+  // set-up worker, guarded by host_mu_ (an entry in use by a restore stays alive
+  // through its shared pointer when the saver drops it)
+  struct SavedMesh {
+    std::shared_ptr<HostMesh> h;
+    double last_used = 0;
+  };
+  struct PendingSave {  // a cached mesh being copied to host memory
+    ServeMesh* m = nullptr;
+    double t0 = 0;
+    std::future<int64_t> fut;  // saved meshes dropped to make room
+  };
+  std::mutex host_mu_;
+  size_t host_budget_ = 0, saved_bytes_ = 0;
+  std::map<std::string, SavedMesh> saved_;
+  std::unique_ptr<PendingSave> save_;
+  int64_t host_trim(size_t budget);  // drop LRU saved meshes down to budget (host_mu_ held)
+  void start_save();                 // copy a cached mesh without a saved copy, in the background
+  bool finish_save(bool wait);       // collect a finished copy; true if one was collected
   std::unique_ptr<PendingSetup> pend_;
   size_t result_bytes_ = 0;                      // device bytes of results not yet fetched
   bool finish_setup(bool wait);                  // install a finished set-up; true if one was installed

@@ -73,6 +73,18 @@ int hfls_mesh_sizes(hfls_mesh m, int64_t* sizes);
  * detection, [2] index structures, [3] scratch allocation and element assembly,
  * [4] Bc, cluster and coarse problems */
 int hfls_mesh_setup_times(hfls_mesh m, double* t);
+/* ---- mesh cache (no counterpart in the reference, which assembles and factorises
+ *      on every run, Driver.cpp:20-38; its MatrixMarket dumps, Matrix.cpp:556-632,
+ *      are never read back).  hfls_mesh_save writes the set-up state of a mesh (its
+ *      options, the factors W = L_rr^-1, the K+_bb and S_i tiles, kernels, gluing and
+ *      coarse inverses) to `path` (through `path`.part, renamed when complete);
+ *      hfls_mesh_restore builds a mesh from such a file on `device` without the
+ *      factorisation: the index structures are rebuilt from the saved options and
+ *      every saved size and index is checked against them.  A restored mesh gives
+ *      bit-identical results to the saved one; hfls_mesh_setup_times reports the
+ *      original set-up and *restore_ms the load. */
+int hfls_mesh_save(hfls_mesh m, const char* path);
+int hfls_mesh_restore(int device, const char* path, hfls_mesh* out, double* restore_ms);
 /* kernel R (packed local, n x 6 column major per subdomain), fixing DOFs (6 local
  * DOFs per subdomain), detected defect per subdomain, body-force loads (packed local) */
 int hfls_mesh_kernel(hfls_mesh m, double* R, int* fix, int* defect);
@@ -227,6 +239,18 @@ int hfls_server_stats(hfls_server s, double* out);
  * afterwards.  A request whose mesh set-up failed ends with status 7 per case and
  * hfls_server_result returns the set-up error (after writing the statuses). */
 int hfls_server_cancel(hfls_server s, int64_t id);
+/* pinned host memory (GB) for evicted meshes (no counterpart in the reference;
+ * 0 = off, the default): an evicted mesh's set-up state (as hfls_mesh_save writes
+ * it) is copied to host memory on the set-up worker thread, least recently used
+ * saved meshes are dropped when it is full, and a later request for the mesh
+ * restores the state over the host link instead of setting the mesh up again
+ * (bit-identical results either way) */
+int hfls_server_host_cache(hfls_server s, double host_budget_gb);
+/* [0] meshes held in host memory, [1] their bytes, [2] saves, [3] summed save ms
+ * (worker thread), [4] restores, [5] summed restore ms (request for the mesh to
+ * mesh cached, as [7] of hfls_server_stats for set-ups), [6] saved meshes dropped
+ * to make room (out holds 7) */
+int hfls_server_cache_stats(hfls_server s, double* out);
 
 /* Measurement hook (no counterpart in the reference): average device time of
  * the dominant kernel over reps back-to-back launches timed with HIP events on
