@@ -240,22 +240,24 @@ def serve_mesh(nsub, e, seed):
     return o
 
 
-def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb, warm):
+def run_serving(hfls_amd, families, edges, nreq, rate, seed, eps, budget_gb, warm, host_cache_gb=0.0):
     """Request stream with Poisson arrivals: each request draws its mesh uniformly
     from families x edges and 1..64 load cases log-uniformly (seeded).  Meshes are
     set up on first use and kept on the device while they fit the HBM budget
     (least recently used idle meshes are evicted); with warm, one single-case
-    request per mesh runs before the stream starts.  Latency is submission to the
-    last case recovered, set-up time of cold meshes included."""
+    request per mesh runs before the stream starts.  With host_cache_gb, evicted
+    meshes are kept in that much pinned host memory and restored from it.  Latency
+    is submission to the last case recovered, set-up time of cold meshes included."""
     meshes = [serve_mesh(f, e, 100 + i) for i, (f, e) in enumerate((f, e) for f in families for e in edges)]
-    out = run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm)
+    out = run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm, host_cache_gb)
     out["meshes"] = ["%dx%d^3" % (f, e) for f in families for e in edges]
     return out
 
 
-def run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm):
+def run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm, host_cache_gb=0.0):
     rng = np.random.default_rng(seed)
-    srv = hfls_amd.Server(slots=64, precision=32, eps=eps, max_iter=1000, chunk=8, hbm_budget_gb=budget_gb)
+    srv = hfls_amd.Server(slots=64, precision=32, eps=eps, max_iter=1000, chunk=8, hbm_budget_gb=budget_gb,
+                          host_cache_gb=host_cache_gb)
     t0 = time.perf_counter()
     if warm:
         for m in meshes:
@@ -299,6 +301,11 @@ def run_stream(hfls_amd, meshes, nreq, rate, seed, eps, budget_gb, warm):
             "cache_warm_s": warm_s, "setups_in_stream": int(st["setups"] - st0["setups"]),
             "setup_s_in_stream": (st["setup_ms"] - st0["setup_ms"]) / 1e3,
             "evictions_in_stream": int(st["evictions"] - st0["evictions"]),
+            "host_cache": {"gb": host_cache_gb, "restores_in_stream": int(st["restores"] - st0["restores"]),
+                           "restore_s_in_stream": (st["restore_ms"] - st0["restore_ms"]) / 1e3,
+                           "saves": int(st["saves"]), "save_s": st["save_ms"] / 1e3,
+                           "meshes_held_at_end": int(st["host_meshes"]), "bytes_held_at_end": st["host_bytes"],
+                           "dropped": int(st["host_drops"])},
             "iterations_min_max": [int(min(iters)), int(max(iters))], "max_true_residual": worst,
             "slot_joins": int(st["joins"]), "rounds": int(st["steps"] - st0["steps"]),
             "host_ms": {"join": st["join_ms"] - st0["join_ms"], "sweeps": st["sweep_ms"] - st0["sweep_ms"],
@@ -323,6 +330,10 @@ def main():
     ap.add_argument("--serve-edges", default="8,9,10,11,12", help="elements per subdomain edge of the served meshes")
     ap.add_argument("--serve-budget-gb", type=float, default=250.0, help="HBM budget of the server")
     ap.add_argument("--serve-warm", action="store_true", help="set every mesh up before the stream starts")
+    ap.add_argument("--serve-host-cache-gb", type=float, default=128.0,
+                    help="pinned host memory for evicted meshes of the serving run (0: set evicted meshes up again)")
+    ap.add_argument("--serve-compare-no-cache", action="store_true",
+                    help="also run the serving stream without the host cache")
     ap.add_argument("--configs", default="3,2",
                     help="extra workloads: 3 = 512 x 12^3 cube, 2 = 512 uneven subdomains ('' for none)")
     ap.add_argument("--config-steps", type=int, default=2, help="timed blocks of each extra workload")
@@ -533,8 +544,12 @@ def main():
         fam = [int(x) for x in a.serve_families.split(",")]
         edg = [int(x) for x in a.serve_edges.split(",")]
         out["serving"] = run_serving(hfls_amd, fam, edg, a.serve_requests, a.serve_rate, SEED, a.eps,
-                                     a.serve_budget_gb, a.serve_warm)
+                                     a.serve_budget_gb, a.serve_warm, a.serve_host_cache_gb)
         log("serving:", json.dumps(out["serving"]))
+        if a.serve_compare_no_cache and a.serve_host_cache_gb > 0:
+            out["serving_no_host_cache"] = run_serving(hfls_amd, fam, edg, a.serve_requests, a.serve_rate, SEED,
+                                                       a.eps, a.serve_budget_gb, a.serve_warm, 0.0)
+            log("serving without host cache:", json.dumps(out["serving_no_host_cache"]))
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(plain_it[-1], out["config"]["subdomains"])
     print(json.dumps(out))
