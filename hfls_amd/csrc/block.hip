@@ -442,22 +442,33 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
   return hipGetLastError();
 }
 
+constexpr int GHR = 16;  // dual rows per wave
 // out[r*k+j] = x[r*k+j] + sgn * (G_r,a . h_a[:,j] - G_r,b . h_b[:,j]); partial dots, where
 // Gr[r*12 + m] = R_b,a[pos_a, m] and Gr[r*12 + 6 + m] = R_b,b[pos_b, m] (0 for Dirichlet
 // rows) are the rows of G_f restricted to the two subdomains of dual row r.
-// Wave g of a workgroup takes the 16 consecutive dual rows 64 blockIdx + 16 g + q; its
-// x and dotv loads are issued first, and h_a / h_b are reloaded only when the
-// subdomain pair changes from one row to the next (consecutive dual rows mostly join
-// the same two subdomains), so a row costs two vector loads and a store.
-constexpr int GHR = 16;  // dual rows per wave
+// Workgroup g takes the 64 dual rows 64 g .. 64 g + 63, wave w the 16 rows 64 g + 16 w + q.
+// The h rows of the (at most GH_SLOTS) subdomains of the group listed in
+// gsub[g*GH_SLOTS ..] are first staged in LDS; rslot[r] = slot_a | slot_b << 8 names the
+// LDS slots of row r's two subdomains (b = a for a row with one subdomain, whose
+// half of G is zero), 255 a subdomain without a slot, read from h directly.  The x
+// and dotv loads of a wave are issued first, so a row costs two vector loads, LDS
+// reads and a store.
 __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __restrict__ rsa,
-                                             const int* __restrict__ rsb, const double* __restrict__ Gr,
+                                             const int* __restrict__ rsb, const int* __restrict__ gsub,
+                                             const int* __restrict__ rslot, const double* __restrict__ Gr,
                                              const double* __restrict__ h, const double* __restrict__ x,
                                              double* __restrict__ out, double sgn, const double* __restrict__ dotv,
                                              double* __restrict__ part, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
+  __shared__ double hs[GH_SLOTS * 6 * 64];
   __shared__ double red[4 * 64];
   const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int t = threadIdx.x; t < GH_SLOTS * 6 * 64; t += 256) {
+    const int sl = t / (6 * 64), mj = t - sl * 6 * 64, m = mj >> 6, jj = mj & 63;
+    const int dsub = gsub[(int64_t)blockIdx.x * GH_SLOTS + sl];
+    hs[t] = (dsub >= 0 && jj < k) ? h[((int64_t)6 * dsub + m) * k + jj] : 0.0;
+  }
+  __syncthreads();
   double s = 0.0;
   if (j < k) {
     const int64_t r0 = (int64_t)blockIdx.x * (4 * GHR) + g * GHR;
@@ -468,23 +479,29 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
       xv[q] = (x && q < nr) ? x[(r0 + q) * k + j] : 0.0;
       dv[q] = (dotv && q < nr) ? dotv[(r0 + q) * k + j] : 0.0;
     }
-    int pa = -1, pb = -1;
-    double ha[6], hb[6];
 #pragma unroll
     for (int q = 0; q < GHR; q++) {
       if (q < nr) {
         const int64_t r = r0 + q;
-        const int a = rsa[r], b0 = rsb[r];
-        const int b = b0 >= 0 ? b0 : a;  // a row with one subdomain: zero half of its G row
-        if (a != pa) {
+        const int sl = rslot[r];
+        const int sa = sl & 255, sb = sl >> 8;
+        double ha[6], hb[6];
+        if (sa != 255) {
+#pragma unroll
+          for (int m = 0; m < 6; m++) ha[m] = hs[(sa * 6 + m) * 64 + j];
+        } else {
+          const int a = rsa[r];
 #pragma unroll
           for (int m = 0; m < 6; m++) ha[m] = h[((int64_t)6 * a + m) * k + j];
-          pa = a;
         }
-        if (b != pb) {
+        if (sb != 255) {
+#pragma unroll
+          for (int m = 0; m < 6; m++) hb[m] = hs[(sb * 6 + m) * 64 + j];
+        } else {
+          const int b0 = rsb[r];
+          const int b = b0 >= 0 ? b0 : rsa[r];
 #pragma unroll
           for (int m = 0; m < 6; m++) hb[m] = h[((int64_t)6 * b + m) * k + j];
-          pb = b;
         }
         const double* G = Gr + r * 12;
         double v = 0.0;
@@ -505,12 +522,12 @@ __global__ __launch_bounds__(256) void kb_gh(int64_t nlam, int k, const int* __r
   }
 }
 
-hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const double* Gr,
-                        const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
-                        const int* gate) {
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* gsub,
+                        const int* rslot, const double* Gr, const double* h, const double* x, double* out, double sgn,
+                        const double* dotv, double* part, const int* gate) {
   if (k > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 4 * GHR - 1) / (4 * GHR))), dim3(256), 0, st, nlam, k, rsa, rsb, Gr,
-                     h, x, out, sgn, dotv, part, gate);
+  hipLaunchKernelGGL(kb_gh, dim3((unsigned)((nlam + 4 * GHR - 1) / (4 * GHR))), dim3(256), 0, st, nlam, k, rsa, rsb,
+                     gsub, rslot, Gr, h, x, out, sgn, dotv, part, gate);
   return hipGetLastError();
 }
 

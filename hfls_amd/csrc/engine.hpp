@@ -311,11 +311,13 @@ class Engine {
   // the block format (split FP16 or FP32)
   void gemm_lo(int which, const int* gate);
   int maxTb = 0, maxTr = 0, maxTcr = 0, maxTcl = 0;
+  int sAc = 1;  // split-K parts of the Ac^-1 block product
   int64_t NBP = 0, NRP = 0, CRP = 0, GCP = 0;
   DVec<int64_t> b_obrow, b_orrow, b_xoff, b_roff, b_croff, b_cro, b_gcrow, b_gcoff, b_lcrow, b_lcpos, b_arow, b_abase_cr,
       b_v6off, b_v6row, b_crrow;
   DVec<int> b_crn, b_qrow;
   DVec<double> b_CR, b_CRt, b_Gr, b_kpart, b_gt;
+  DVec<int> b_ghsub, b_ghslot;  // kb_gh: subdomains staged per 64 dual rows, slots per row
   DVec<int64_t> b_crto;
   DVec<float> b_Kt32, b_St32, b_xb32, b_yb32;
   // split-FP16 block GEMM (option block_gemm=f16x2): b_Kt32/b_St32 hold hi and lo FP16

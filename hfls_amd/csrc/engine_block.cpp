@@ -111,6 +111,9 @@ void Engine::block_prepare() {
     maxTcl = std::max(maxTcl, (nlen + 63) / 64);
   }
   GCP = gr;
+  // the Ac^-1 product has few output blocks (maxTcl row blocks per cluster): split
+  // its inner dimension so that about 1024 workgroups run, keeping >= 2 K chunks each
+  sAc = std::max(1, std::min({8, 1024 / std::max(1, NC * maxTcl), maxTcl}));
   for (int d = 0; d < N; d++) {
     const int cl = h_cl[d];
     croff_el[d] = croff[d] * KP;
@@ -177,6 +180,35 @@ void Engine::block_prepare() {
         }
       }
       b_Gr.upload(Gr, st);
+      // per group of 64 dual rows, the subdomains whose h rows kb_gh stages in LDS
+      // (first GH_SLOTS distinct ones in row order) and each row's two slots
+      std::vector<int> ra(P->nlam, -1), rb(P->nlam, -1);
+      for (int d = 0; d < N; d++) {
+        const Subdomain& sd = P->subs[d];
+        for (size_t e = 0; e < sd.bf_row.size(); e++) (sd.bf_sign[e] > 0 ? ra : rb)[sd.bf_row[e]] = d;
+      }
+      const int64_t ngh = (P->nlam + 63) / 64;
+      std::vector<int> gsub((size_t)ngh * GH_SLOTS, -1), rslot(P->nlam, 0);
+      for (int64_t g = 0; g < ngh; g++) {
+        int* gs = gsub.data() + g * GH_SLOTS;
+        int cnt = 0;
+        auto slot = [&](int dd) {
+          for (int q = 0; q < cnt; q++)
+            if (gs[q] == dd) return q;
+          if (cnt < GH_SLOTS) {
+            gs[cnt] = dd;
+            return cnt++;
+          }
+          return 255;
+        };
+        for (int64_t r = 64 * g; r < std::min<int64_t>(P->nlam, 64 * g + 64); r++) {
+          const int sa = slot(ra[r]);
+          const int sb = slot(rb[r] >= 0 ? rb[r] : ra[r]);
+          rslot[r] = sa | (sb << 8);
+        }
+      }
+      b_ghsub.upload(gsub, st);
+      b_ghslot.upload(rslot, st);
     }
     for (int d = 0; d < N; d++) {  // modified Gram-Schmidt per subdomain
       const int n = h_ns[d];
@@ -273,7 +305,7 @@ void Engine::alloc_ctx_buffers() {
   if (gemm16) zero_alloc(b_p16, std::max<int64_t>(p16_n, 1), st);
   zero_alloc(b_cr, CRP * KP, st);
   zero_alloc(b_coef, CRP * KP, st);
-  zero_alloc(b_kpart, (int64_t)N * 8 * 64 * KP, st);
+  zero_alloc(b_kpart, std::max<int64_t>((int64_t)N * 8, (int64_t)P->ncl * maxTcl * sAc) * 64 * KP, st);
   zero_alloc(b_v6, (int64_t)64 * N * KP, st);
   zero_alloc(b_gc, GCP * KP, st);
   zero_alloc(b_la, GCP * KP, st);
@@ -386,6 +418,7 @@ size_t Engine::block_bytes() const {
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_gtp, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
+  s += (b_ghsub.n + b_ghslot.n) * sizeof(int);
   return s;
 }
 
@@ -409,7 +442,7 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
                           b_lcpos.p, b_cr.p, b_abase_cr.p, b_gc.p, b_gcrow.p, gate));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
-                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate));
+                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate, sAc, b_kpart.p));
   b_corr(k, 1, prec == 32 ? 1 : 0, gate, prec == 32 ? b_yb32.p : nullptr);
   HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
                        dotw, b_part.p, gate));
@@ -461,7 +494,7 @@ void Engine::b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate)
 // out = P x = x - G (G^T G)^-1 G^T x (Cluster.cpp:1314-1329); hbuf = H alpha per subdomain
 void Engine::bP(int k, const double* x, double* out, double* hbuf, const double* dotv, const int* gate) {
   b_gtx_coarse(k, x, hbuf, gate);
-  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, out, -1.0, dotv, b_part.p, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_ghsub.p, b_ghslot.p, b_Gr.p, hbuf, x, out, -1.0, dotv, b_part.p, gate));
 }
 
 // out = M P x (Cluster.cpp:1043-1112 applied to the projection of x); P x goes
@@ -469,7 +502,7 @@ void Engine::bP(int k, const double* x, double* out, double* hbuf, const double*
 void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, const int* gate) {
   const int N = P->nsub;
   b_gtx_coarse(k, x, hbuf, gate);
-  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_Gr.p, hbuf, x, b_Pg.p, -1.0, nullptr, nullptr, gate));
+  HC(launch_kb_gh(st, P->nlam, k, d_rsa.p, d_rsb.p, b_ghsub.p, b_ghslot.p, b_Gr.p, hbuf, x, b_Pg.p, -1.0, nullptr, nullptr, gate));
   HC(launch_kb_gather(st, NB, k, KP, b_qrow.p, d_gptr.p, d_gidx.p, b_Pg.p, d_wgt.p,
                       prec == 64 ? b_xb.p : nullptr, prec == 32 ? b_xb32.p : nullptr, gate));
   if (prec == 32) {
@@ -533,7 +566,7 @@ void Engine::b_kplus(int k, bool interior_too, double* V) {
   HC(launch_kb_cluster_gc(st, NC, max_cl_len, k, KP, d_gc_src.p, d_gc_off.p, d_nlcl.p, d_nrcl.p, d_d0.p, b_cr.p,
                           b_lcpos.p, b_v6.p, b_v6row.p, b_gc.p, b_gcrow.p));
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
-                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, nullptr));
+                      b_la.p, b_gcoff.p, nullptr, KP, 1.0, nullptr, sAc, b_kpart.p));
   b_corr(k, interior_too ? 0 : 1, 0, nullptr);
   if (!interior_too) return;
   // x_r = K_rr^-1 v_r - K_rr^-1 K_rb y_b
@@ -702,7 +735,7 @@ void Engine::blk_init(const std::vector<int>& cols) {
   HC(launch_kb_update(st, nl, k, 3, b_d.p, nullptr, b_Fw.p, nullptr, b_sc.p));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, b_v6row.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr,
                            1.0));
-  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, b_Gr.p, b_h.p, nullptr, b_Pg.p,
+  HC(launch_kb_gh(st, nl, k, d_rsa.p, d_rsb.p, b_ghsub.p, b_ghslot.p, b_Gr.p, b_h.p, nullptr, b_Pg.p,
                   1.0, nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_lam.p, nullptr, b_Pg.p, nullptr, b_sc.p));
   // g0 = F lam0 - d in FP64 ; w0 = P M P g0

@@ -940,53 +940,88 @@ hipError_t launch_gemv_gather(hipStream_t st, int nsub, int maxrows, const doubl
 }
 
 // ============================================================ cluster correction
-// per subdomain: c_i = C_i^T x_i (C_i = K+_bb Bc_i^T), a_i = R_b,i^T x_i
-__global__ __launch_bounds__(256) void k_corr_dots(const double* __restrict__ C, const int64_t* __restrict__ coff,
-                                                   const int* __restrict__ ldc, const int* __restrict__ nlc,
-                                                   const int64_t* __restrict__ lcoff, const double* __restrict__ Rb,
-                                                   const int64_t* __restrict__ rboff, const int* __restrict__ ns,
-                                                   const int64_t* __restrict__ boff, const int* __restrict__ gptr,
-                                                   const int* __restrict__ gidx, const double* __restrict__ v,
-                                                   const double* __restrict__ xb, double* __restrict__ cvec,
-                                                   double* __restrict__ avec, const int* __restrict__ done) {
+// per subdomain: c_i = C_i^T x_i (C_i = K+_bb Bc_i^T), a_i = R_b,i^T x_i.  One
+// 1024-thread workgroup per subdomain: wave w owns rq*64 consecutive rows of each
+// segment of 1024*rq rows (rq <= 4 rows per lane, held in registers), every lane
+// keeps the wave's sum of column k0 + lane over the segments, and the 16 wave sums
+// are added in wave order: a fixed reduction order, so the result is reproducible.
+// Columns go in passes of 64 (C_i then R_b,i).
+static DEV double xb_or_gather(const double* __restrict__ xb, const int* __restrict__ gptr,
+                               const int* __restrict__ gidx, const double* __restrict__ v, int64_t q) {
+  if (xb) return xb[q];
+  double s = 0.0;
+  const int e1 = gptr[q + 1];
+  for (int e = gptr[q]; e < e1; e++) {
+    const int code = gidx[e];
+    const int r = code >= 0 ? code : ~code;
+    s += code >= 0 ? v[r] : -v[r];
+  }
+  return s;
+}
+
+constexpr int CD_RQ = 4;
+
+__global__ __launch_bounds__(1024) void k_corr_dots(const double* __restrict__ C, const int64_t* __restrict__ coff,
+                                                    const int* __restrict__ ldc, const int* __restrict__ nlc,
+                                                    const int64_t* __restrict__ lcoff, const double* __restrict__ Rb,
+                                                    const int64_t* __restrict__ rboff, const int* __restrict__ ns,
+                                                    const int64_t* __restrict__ boff, const int* __restrict__ gptr,
+                                                    const int* __restrict__ gidx, const double* __restrict__ v,
+                                                    const double* __restrict__ xb, double* __restrict__ cvec,
+                                                    double* __restrict__ avec, const int* __restrict__ done) {
   if (done && *done) return;
-  extern __shared__ double xs[];
+  __shared__ double red[16 * 64];
   const int d = blockIdx.x;
   const int n = ns[d];
   const int64_t b0 = boff[d];
-  for (int p = threadIdx.x; p < n; p += 256) {
-    double s;
-    if (xb) {
-      s = xb[b0 + p];
-    } else {
-      s = 0.0;
-      const int e1 = gptr[b0 + p + 1];
-      for (int e = gptr[b0 + p]; e < e1; e++) {
-        const int code = gidx[e];
-        const int r = code >= 0 ? code : ~code;
-        s += code >= 0 ? v[r] : -v[r];
-      }
-    }
-    xs[p] = s;
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nc = nlc[d];
+  const int nc = nlc[d], ncol = nc + 6;
   const double* Cd = C + coff[d];
   const int64_t l = ldc[d];
-  for (int k = wv; k < nc + 6; k += 4) {
-    double s = 0.0;
-    if (k < nc) {
-      for (int p = lane; p < n; p += 64) s += Cd[p + (int64_t)k * l] * xs[p];
-    } else {
-      const double* Rd = Rb + rboff[d] + (int64_t)(k - nc) * n;
-      for (int p = lane; p < n; p += 64) s += Rd[p] * xs[p];
+  const double* Rd = Rb + rboff[d];
+  const int rq = min(CD_RQ, (n + 1023) / 1024);
+  for (int k0 = 0; k0 < ncol; k0 += 64) {
+    const int kn = min(64, ncol - k0);
+    double mine = 0.0;  // this wave's sum for column k0 + lane
+    for (int s0 = 0; s0 < n; s0 += 1024 * rq) {
+      double xr[CD_RQ];
+      int pr[CD_RQ];
+#pragma unroll
+      for (int q = 0; q < CD_RQ; q++) {
+        pr[q] = s0 + (wv * rq + q) * 64 + lane;
+        xr[q] = (q < rq && pr[q] < n) ? xb_or_gather(xb, gptr, gidx, v, b0 + pr[q]) : 0.0;
+      }
+      // 8 columns at a time: all their loads are issued before the reductions
+      for (int kk = 0; kk < kn; kk += 8) {
+        double sum[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          sum[u] = 0.0;
+          const int k = k0 + kk + u;
+          if (kk + u < kn) {
+            const double* col = k < nc ? Cd + (int64_t)k * l : Rd + (int64_t)(k - nc) * n;
+#pragma unroll
+            for (int q = 0; q < CD_RQ; q++)
+              if (q < rq && pr[q] < n) sum[u] += col[pr[q]] * xr[q];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const double w = wave_sum(sum[u]);
+          if (lane == kk + u) mine += w;
+        }
+      }
     }
-    s = wave_sum(s);
-    if (lane == 0) {
-      if (k < nc) cvec[lcoff[d] + k] = s;
-      else avec[6 * d + (k - nc)] = s;
+    red[wv * 64 + lane] = mine;
+    __syncthreads();
+    if ((int)threadIdx.x < kn) {
+      double sum = 0.0;
+      for (int w = 0; w < 16; w++) sum += red[w * 64 + threadIdx.x];
+      const int k = k0 + threadIdx.x;
+      if (k < nc) cvec[lcoff[d] + k] = sum;
+      else avec[6 * d + (k - nc)] = sum;
     }
+    __syncthreads();
   }
 }
 
@@ -995,8 +1030,9 @@ hipError_t launch_corr_dots(hipStream_t st, int nsub, int maxns, const double* C
                                const int64_t* rboff, const int* ns, const int64_t* boff, const int* gptr,
                                const int* gidx, const double* v, const double* xb, double* cvec, double* avec,
                                const int* done) {
-  hipLaunchKernelGGL(k_corr_dots, dim3(nsub), dim3(256), (size_t)maxns * sizeof(double), st, C, coff, ldc, nlc,
-                     lcoff, Rb, rboff, ns, boff, gptr, gidx, v, xb, cvec, avec, done);
+  (void)maxns;
+  hipLaunchKernelGGL(k_corr_dots, dim3(nsub), dim3(1024), 0, st, C, coff, ldc, nlc, lcoff, Rb, rboff, ns, boff, gptr,
+                     gidx, v, xb, cvec, avec, done);
   return hipGetLastError();
 }
 
@@ -1226,45 +1262,51 @@ hipError_t launch_axpy2(hipStream_t st, int64_t nlam, double* lam, double* g, co
   return hipGetLastError();
 }
 
-// v_i = R_b,i^T (B_i^T x)   (one workgroup per subdomain)
-__global__ __launch_bounds__(256) void k_proj_dots(const double* __restrict__ Rb, const int64_t* __restrict__ rboff,
-                                                   const int* __restrict__ ns, const int64_t* __restrict__ boff,
-                                                   const int* __restrict__ gptr, const int* __restrict__ gidx,
-                                                   const double* __restrict__ x, double* __restrict__ v,
-                                                   const int* __restrict__ done) {
+// v_i = R_b,i^T (B_i^T x): one 1024-thread workgroup per subdomain, rows strided
+// over the threads, the six sums reduced per wave and then in wave order (fixed)
+__global__ __launch_bounds__(1024) void k_proj_dots(const double* __restrict__ Rb, const int64_t* __restrict__ rboff,
+                                                    const int* __restrict__ ns, const int64_t* __restrict__ boff,
+                                                    const int* __restrict__ gptr, const int* __restrict__ gidx,
+                                                    const double* __restrict__ x, double* __restrict__ v,
+                                                    const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double sh[8];
+  __shared__ double red[16 * 6];
   const int d = blockIdx.x;
   const int n = ns[d];
   const int64_t b0 = boff[d];
   const double* Rd = Rb + rboff[d];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int p = threadIdx.x; p < n; p += 256) {
-    double s = 0.0;
-    const int e1 = gptr[b0 + p + 1];
-    for (int e = gptr[b0 + p]; e < e1; e++) {
-      const int code = gidx[e];
-      const int r = code >= 0 ? code : ~code;
-      s += code >= 0 ? x[r] : -x[r];
-    }
+  for (int p = threadIdx.x; p < n; p += 1024) {
+    const double s = xb_or_gather(nullptr, gptr, gidx, x, b0 + p);
+#pragma unroll
     for (int k = 0; k < 6; k++) acc[k] += Rd[p + (int64_t)k * n] * s;
   }
+#pragma unroll
   for (int k = 0; k < 6; k++) {
-    const double s = block_sum<256>(acc[k], sh);
-    if (threadIdx.x == 0) v[6 * d + k] = s;
+    const double s = wave_sum(acc[k]);
+    if (lane == 0) red[wv * 6 + k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double s = 0.0;
+    for (int w = 0; w < 16; w++) s += red[w * 6 + threadIdx.x];
+    v[6 * d + threadIdx.x] = s;
   }
 }
 
 hipError_t launch_proj_dots(hipStream_t st, int nsub, const double* Rb, const int64_t* rboff, const int* ns,
                             const int64_t* boff, const int* gptr, const int* gidx, const double* x, double* v,
                             const int* done) {
-  hipLaunchKernelGGL(k_proj_dots, dim3(nsub), dim3(256), 0, st, Rb, rboff, ns, boff, gptr, gidx, x, v, done);
+  hipLaunchKernelGGL(k_proj_dots, dim3(nsub), dim3(1024), 0, st, Rb, rboff, ns, boff, gptr, gidx, x, v, done);
   return hipGetLastError();
 }
 
 // y = Gf^T x = [sum_{i in cl} H_i^T v_i]_cl ; a = (Gf^T Gf)^-1 y ; alpha = asgn * a ;
 // h_i = sgn * H_i a_cl(i).  One workgroup; subdomains of a cluster are the
 // contiguous ids [cl*nsub_cl, (cl+1)*nsub_cl).  (Cluster.cpp:1314-1329)
+// The ncol sums of y and of a are each split into P = 1024 / ncol contiguous parts
+// (thread c + ncol*part), added part by part in a fixed order.
 __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const double* __restrict__ v,
                                                       const double* __restrict__ H, const int* __restrict__ cl,
                                                       const double* __restrict__ invGtG, double* __restrict__ h,
@@ -1273,29 +1315,50 @@ __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const 
   if (done && *done) return;
   __shared__ double y[1024];
   __shared__ double al[1024];
+  __shared__ double red[1024];
   const int ncl = ncol / 6;
   const int nsc = nsub / ncl;
-  // y[c]: thread per column, ascending subdomains of its cluster (fixed order)
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+  const int P = 1024 / ncol;
+  const int t = threadIdx.x;
+  const int c = t % ncol, part = t / ncol;
+  // y[c] = sum over the cluster's subdomains of (H_d^T v_d)[k]
+  if (part < P) {
     const int ccl = c / 6, k = c % 6;
+    const int per = (nsc + P - 1) / P;
+    const int d0 = ccl * nsc + part * per, d1 = min(ccl * nsc + (part + 1) * per, (ccl + 1) * nsc);
     double s = 0.0;
-    for (int d = ccl * nsc; d < (ccl + 1) * nsc; d++) {
+    for (int d = d0; d < d1; d++) {
       const double* Hd = H + 36 * d;
       const double* vd = v + 6 * d;
       s += Hd[0 * 6 + k] * vd[0] + Hd[1 * 6 + k] * vd[1] + Hd[2 * 6 + k] * vd[2] + Hd[3 * 6 + k] * vd[3] +
            Hd[4 * 6 + k] * vd[4] + Hd[5 * 6 + k] * vd[5];
     }
-    y[c] = s;
+    red[t] = s;
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+  if (t < ncol) {
     double s = 0.0;
-    for (int j = 0; j < ncol; j++) s += invGtG[c + (int64_t)j * ncol] * y[j];
-    al[c] = s;
-    if (alpha) alpha[c] = asgn * s;
+    for (int q = 0; q < P; q++) s += red[t + q * ncol];
+    y[t] = s;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < 6 * nsub; q += blockDim.x) {
+  // a = (Gf^T Gf)^-1 y
+  if (part < P) {
+    const int per = (ncol + P - 1) / P;
+    const int j0 = part * per, j1 = min(ncol, (part + 1) * per);
+    double s = 0.0;
+    for (int j = j0; j < j1; j++) s += invGtG[c + (int64_t)j * ncol] * y[j];
+    red[t] = s;
+  }
+  __syncthreads();
+  if (t < ncol) {
+    double s = 0.0;
+    for (int q = 0; q < P; q++) s += red[t + q * ncol];
+    al[t] = s;
+    if (alpha) alpha[t] = asgn * s;
+  }
+  __syncthreads();
+  for (int q = t; q < 6 * nsub; q += blockDim.x) {
     const int d = q / 6, a = q % 6;
     const double* Hd = H + 36 * d + a * 6;
     const double* ad = al + 6 * cl[d];
@@ -1306,7 +1369,7 @@ __global__ __launch_bounds__(1024) void k_proj_coarse(int nsub, int ncol, const 
 hipError_t launch_proj_coarse(hipStream_t st, int nsub, int ncol, const double* v, const double* H, const int* cl,
                               const double* invGtG, double* h, double* alpha, double sgn, const int* done,
                               double asgn) {
-  if (ncol > 1024) return hipErrorInvalidValue;
+  if (ncol > 1024 || ncol < 6) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_proj_coarse, dim3(1), dim3(1024), 0, st, nsub, ncol, v, H, cl, invGtG, h, alpha, sgn, done,
                      asgn);
   return hipGetLastError();

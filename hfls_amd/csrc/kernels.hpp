@@ -190,7 +190,7 @@ hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int o
 hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
                             const double* A, const int64_t* aoff, const int* lda, const double* X,
                             const int64_t* xoff, double* Y, const int64_t* yoff, const double* Z, int kp,
-                            double alpha, const int* gate, int S = 1, double* Ypart = nullptr,  // S: split K (maxRo 1)
+                            double alpha, const int* gate, int S = 1, double* Ypart = nullptr,  // S: split K
                             const float* Zf = nullptr,  // Zf: an FP32 addend in place of Z
                             const int* xcd_order = nullptr, int norder = 0);  // see xcd_order()
 hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
@@ -277,9 +277,11 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
                              const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
                              const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr,
                              const float* yf = nullptr);  // yf: FP32 b-block in place of y
-hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const double* Gr,
-                        const double* h, const double* x, double* out, double sgn, const double* dotv, double* part,
-                        const int* gate = nullptr);
+// subdomains whose h rows kb_gh stages in LDS per group of 64 dual rows
+constexpr int GH_SLOTS = 12;
+hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* gsub,
+                        const int* rslot, const double* Gr, const double* h, const double* x, double* out, double sgn,
+                        const double* dotv, double* part, const int* gate = nullptr);
 hipError_t launch_kb_proj_coarse(hipStream_t st, int nsub, int ncol, int k, int kp, const double* v,
                                  const int64_t* vrow, const double* H, const int* cl, const double* invGtG, double* h,
                                  double* alpha, double sgn, const int* gate = nullptr);

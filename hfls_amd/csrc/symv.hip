@@ -6,8 +6,8 @@
 // tile contributes T x_tj to y_ti and T^T x_ti to y_tj, halving the bytes of the
 // dense F_i / S_i stream of every PCPG iteration.
 //
-// k_symv_tiles: one workgroup per (tile column tj, subdomain).  Thread t holds a
-//   2-row x 8-column piece of each tile (rows 2(t%32), +1; columns t/32 + 8k), read
+// k_symv_tiles: one workgroup per (subdomain, tile column tj), longest columns
+//   first.  Thread t holds a 2-row x 8-column piece of each tile (rows 2(t%32), +1; columns t/32 + 8k), read
 //   with one 16-byte load per column (coalesced 4 KB per instruction across the
 //   workgroup); the next tile is loaded before the current one is reduced.
 //   Row parts are reduced across the 8 column groups through LDS and written as
@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void k_symv_tiles(const int* __restrict__ ns, 
                                                     const int64_t* __restrict__ partoff,
                                                     const int* __restrict__ done) {
   if (done && *done) return;
-  const int d = blockIdx.y;
+  const int d = blockIdx.x;
   const int n = ns[d];
   const int T = (n + 63) / 64;
-  const int tj = blockIdx.x;
+  const int tj = blockIdx.y;
   if (tj >= T) return;
   __shared__ double red[8 * 64];
   __shared__ double xrow[64];
@@ -160,7 +160,11 @@ __global__ __launch_bounds__(256) void k_symv_tiles(const int* __restrict__ ns, 
 hipError_t launch_symv_tiles(hipStream_t st, int nsub, int maxT, const int* ns, const double* P, const int64_t* poff,
                              const double* x, const int64_t* xoff, double* part, const int64_t* partoff,
                              const int* done) {
-  hipLaunchKernelGGL(k_symv_tiles, dim3(maxT, nsub), dim3(256), 0, st, ns, P, poff, x, xoff, part, partoff, done);
+  // subdomains fastest: every subdomain's longest tile column (tj = 0) is dispatched
+  // first and the one-tile columns last, and with the hardware handing consecutive
+  // workgroups to the 8 XCDs in turn, all columns of subdomain d run on XCD d mod 8
+  // (nsub a multiple of 8), sharing one L2 for x
+  hipLaunchKernelGGL(k_symv_tiles, dim3(nsub, maxT), dim3(256), 0, st, ns, P, poff, x, xoff, part, partoff, done);
   return hipGetLastError();
 }
 

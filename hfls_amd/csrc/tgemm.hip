@@ -315,9 +315,10 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
   }
 }
 
-// S > 1 (split K, one output row block): workgroup s of a subdomain takes K chunks
-// [s*nch/S, (s+1)*nch/S) and writes its partial 64 x kp block to Ypart at
-// (d*S + s)*64*kp; k_splitk_sum adds the partials in a fixed order.
+// S > 1 (split K): slot q of a matrix is row block i = q / S and K part s = q % S;
+// the workgroup takes K chunks [s*nch/S, (s+1)*nch/S) and writes its partial
+// 64 x kp block to Ypart at ((d*Rmax + i)*S + s)*64*kp (Rmax = nslot / S, the
+// largest row-block count); k_splitk_sum adds the S partials in a fixed order.
 // Workgroups are grouped per XCD as in k_tgemm_f32 (nslot = row blocks, or S);
 // with symmetric tiles and no split, row block i takes the tile columns in the
 // order (t - i) mod Ko, so the two reads of every off-diagonal tile meet in L2.
@@ -341,8 +342,8 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
   const int Ra = (m + 63) / 64, Ca = (n + 63) / 64;
   const int Ro = (mode != 0 && op == 1) ? Ca : Ra;
   const int Ko = (mode != 0 && op == 1) ? Ra : Ca;
-  const int i = S > 1 ? 0 : sl % nslot;
-  const int sp = S > 1 ? sl % nslot : 0;
+  const int i = S > 1 ? (sl % nslot) / S : sl % nslot;
+  const int sp = S > 1 ? (sl % nslot) % S : 0;
   if (i >= Ro) return;
   const int rot = (mode == 0 && S == 1) ? i : 0;  // K chunk c covers tile column (c/2 - rot) mod Ko
   const int c0 = blockIdx.z * 64;
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
     __syncthreads();
   }
   if (S > 1) {
-    double* Pd = Ypart + ((int64_t)d * S + sp) * 64 * kp + c0;
+    double* Pd = Ypart + (((int64_t)d * (nslot / S) + i) * S + sp) * 64 * kp + c0;
 #pragma unroll
     for (int cc = 0; cc < 4; cc++)
 #pragma unroll
@@ -417,19 +418,23 @@ __global__ __launch_bounds__(256) void k_tgemm_f64(int mode, int op, int nsub, i
     }
 }
 
-// Y[yoff[d] + r*kp + j] = sum_{s < S} Ypart[((d*S + s)*64 + r)*kp + j] (+ Z), fixed order
-__global__ __launch_bounds__(256) void k_splitk_sum(int S, int kp, const double* __restrict__ Ypart,
-                                                    double* __restrict__ Y, const int64_t* __restrict__ yoff,
-                                                    const double* __restrict__ Z, const int* __restrict__ gate) {
+// Y[yoff[d] + (64 i + r)*kp + j] = sum_{s < S} Ypart[(((d*Rmax + i)*S + s)*64 + r)*kp + j]
+// (+ Z), fixed order, for the row blocks i < ceil(orows[d] / 64)
+__global__ __launch_bounds__(256) void k_splitk_sum(int S, int Rmax, int kp, const int* __restrict__ orows,
+                                                    const double* __restrict__ Ypart, double* __restrict__ Y,
+                                                    const int64_t* __restrict__ yoff, const double* __restrict__ Z,
+                                                    const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int d = blockIdx.y;
+  const int d = blockIdx.y / Rmax, i = blockIdx.y % Rmax;
+  if (i >= (orows[d] + 63) / 64) return;
   const int e = blockIdx.x * 256 + threadIdx.x;  // 64 * kp entries
   if (e >= 64 * kp) return;
-  const double* p = Ypart + (int64_t)d * S * 64 * kp + e;
+  const double* p = Ypart + ((int64_t)d * Rmax + i) * S * 64 * kp + e;
   double v = 0.0;
   for (int s = 0; s < S; s++) v += p[(int64_t)s * 64 * kp];
-  if (Z) v += Z[yoff[d] + e];
-  Y[yoff[d] + e] = v;
+  const int64_t o = yoff[d] + (int64_t)i * 64 * kp + e;
+  if (Z) v += Z[o];
+  Y[o] = v;
 }
 
 hipError_t launch_tgemm_f32(hipStream_t st, int nsub, int maxRo, int mode, int op, const int* mrows, const int* mcols,
@@ -458,9 +463,9 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
                             double* Ypart, const float* Zf, const int* order, int norder) {
   if (kp % 64) return hipErrorInvalidValue;
   if (mode == 2 && !lda) return hipErrorInvalidValue;
-  if (S > 1 && (maxRo != 1 || !Ypart || mode == 3 || Zf)) return hipErrorInvalidValue;
+  if (S > 1 && (!Ypart || mode == 3 || Zf)) return hipErrorInvalidValue;
   if (order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
-  const int nslot = S > 1 ? S : maxRo;
+  const int nslot = S > 1 ? S * maxRo : maxRo;
   const dim3 grid(8 * nslot * (order ? norder / 8 : (nsub + 7) / 8), 1, kp / 64);
   if (mode == 2)
     hipLaunchKernelGGL(k_tgemm_f64<true>, grid, dim3(256), 0, st, mode, op, nsub, nslot, order, norder, mrows, mcols, A,
@@ -468,9 +473,9 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
   else
     hipLaunchKernelGGL(k_tgemm_f64<false>, grid, dim3(256), 0, st, mode, op, nsub, nslot, order, norder, mrows, mcols,
                        A, aoff, lda, X, xoff, Y, yoff, Z, kp, alpha, gate, S, Ypart, Zf);
-  if (S > 1)
-    hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub), dim3(256), 0, st, S, kp, Ypart, Y, yoff, Z,
-                       gate);
+  if (S > 1)  // output rows: mrows, or mcols for the transpose of a general or compact A
+    hipLaunchKernelGGL(k_splitk_sum, dim3((64 * kp + 255) / 256, nsub * maxRo), dim3(256), 0, st, S, maxRo, kp,
+                       (mode != 0 && op == 1) ? mcols : mrows, Ypart, Y, yoff, Z, gate);
   return hipGetLastError();
 }
 

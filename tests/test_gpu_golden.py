@@ -1,8 +1,13 @@
 """GPU solves of the 2x2x2 x 10^3 cube (the subdomain size of the 512-subdomain cube) against the
 committed oracle displacement (tests/golden/cube8_10e3_u.npz, eps 1e-10):
 
-* plain FP64 path at eps 1e-10: displacement within 1e-8 relative and |P g|/|P g0|
-  below 1e-6; at eps 1e-8 the iteration count within +-2 of the oracle's;
+* plain FP64 path at eps 1e-9: displacement within 1e-8 relative and |P g|/|P g0|
+  below 1e-6; at eps 1e-8 the iteration count within +-2 of the oracle's.  The
+  FP64 iteration on this cube reaches about 1e-10 and no further: the residual g
+  keeps its large range(G) part, whose rounding in P g = g - G h sets the floor, and
+  whether a solve stops just below 1e-10 or turns back just above it depends on the
+  summation order of the reductions (the oracle's own PCPG reaches 1e-10 in 18
+  iterations and never 1e-11), so the test stops one decade above that floor;
 * block FP64 at eps 1e-10: within 1e-8;
 * block mixed precision at eps 1e-8 (body force as one of three load cases):
   certified below 1e-8 and within 1e-7 of the FP64 fixture.
@@ -37,8 +42,8 @@ def rel(a, b):
 
 def test_plain_fp64(cube):
     m, u, it8 = cube
-    r = m.pcpg(eps=1e-10, max_iter=1000)
-    print("plain 1e-10: it", r["iterations"], "err %.2e" % rel(r["u"], u), "|Pg|/|Pg0| %.2e" % r["pg_rel"],
+    r = m.pcpg(eps=1e-9, max_iter=1000)
+    print("plain 1e-9: it", r["iterations"], "err %.2e" % rel(r["u"], u), "|Pg|/|Pg0| %.2e" % r["pg_rel"],
           flush=True)
     assert r["converged"] and rel(r["u"], u) <= 1e-8
     assert 0.0 < r["pg_rel"] < 1e-6  # the projected gradient falls with the preconditioned residual

@@ -110,14 +110,21 @@ def test_pcpg_matches_oracle_and_direct(pair):
     print(name, "iterations gpu", r["iterations"], "oracle", ro["iterations"], flush=True)
     assert r["converged"]
     assert abs(r["iterations"] - ro["iterations"]) <= 2
-    assert np.linalg.norm(r["u"] - ro["u"]) <= 1e-8 * np.linalg.norm(ro["u"])
+    # two iterates stopped at a 1e-8 residual differ by about 1e-8 relative (rounding
+    # in the reductions moves them apart at that level): 1e-7 here, and the 1e-8
+    # parity bar on the converged solutions below
+    assert np.linalg.norm(r["u"] - ro["u"]) <= 1e-7 * np.linalg.norm(ro["u"])
     u_ref, _, _ = global_solve(ora.mesh)
     err = np.linalg.norm(r["u"] - u_ref) / np.linalg.norm(u_ref)
     print(name, "error vs FP64 global direct solve %.3e" % err, flush=True)
     assert err <= 1e-6
-    # tight tolerance: displacement converges to the direct solve
+    # tight tolerance: displacement within 1e-8 of the oracle's and of the direct solve
     r10 = gpu.pcpg(eps=1e-10, max_iter=500)
+    ro10 = ora.pcpg(eps=1e-10, max_iter=500)
     assert r10["converged"]
+    e10 = np.linalg.norm(r10["u"] - ro10["u"]) / np.linalg.norm(ro10["u"])
+    print(name, "eps 1e-10: error vs oracle %.3e" % e10, flush=True)
+    assert e10 <= 1e-8
     assert np.linalg.norm(r10["u"] - u_ref) <= 1e-8 * np.linalg.norm(u_ref)
 
 
