@@ -42,10 +42,6 @@ def cube(e):
                 preconditioner="Dirichlet_explicit")
 
 
-# CPU sample: the same 10^3-element subdomains, one cluster of 2x2x2
-SAMPLE = dict(Nx=2, Ny=2, Nz=2, nx=10, ny=10, nz=10, bc="clamp_x0", material="uniform", young_modulus="2.1e5",
-              poissons_ratio="0.3", load="body", body_force="-1.0", typeBc="ker",
-              preconditioner="Dirichlet_explicit")
 HBM_PEAK_GBS = 8000.0
 FP16_MFMA_PEAK_TFS = 2500.0  # dense FP16 matrix cores, published
 SEED = 20240611
@@ -62,68 +58,71 @@ GEMM_KERNEL_NAME = {
              "columns, every tile read once; time of both launches)"}
 
 
-def cpu_sample():
-    """Time the oracle (numpy/scipy restatement of the reference method: sparse LU
-    of every K_reg, implicit K^+ in every PCPG iteration as in Cluster.cpp:1126)
-    solving one load case of SAMPLE on one core."""
-    from threadpoolctl import threadpool_limits
+def cpu_port_run(cfg):
+    """CPU baseline process (one per bench run): the C++ FP64 restatement of the
+    reference's PARDISO path (oracle/cpu/htfeti_cpu.cpp: analytic kernel, fixing
+    DOFs, band Cholesky of K_reg and K_rr, Dirichlet_implicit, implicit K_c^+ in
+    every PCPG iteration as in Cluster.cpp:1126, OpenMP over subdomains) on the
+    bench's own mesh and first load case.
+      * all cores: set-up and one solve of the whole mesh, both measured;
+      * one core: the same solve on 1 thread stopped after 3 PCPG iterations
+        (initialisation, 3 iterations, recovery measured; the iterations scaled to
+        the all-core run's count), and the 1-thread set-up of one cluster of the
+        mesh scaled by the number of clusters."""
     from oracle.mesh import opts
-    from oracle.feti import HTFETI
-    with threadpool_limits(limits=1):
-        ts = time.perf_counter()
-        h = HTFETI(opts(eps_iter="1e-8", **SAMPLE), sparse=True)
-        setup_s = time.perf_counter() - ts
-    reps = 0
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        while True:  # about 10 s of single-core solve time
-            r = h.pcpg(eps=1e-8)
-            reps += 1
-            if time.perf_counter() - t0 > 10.0:
-                break
-        t = (time.perf_counter() - t0) / reps
-    print(json.dumps({"seconds": t, "iterations": r["iterations"], "nsub": h.nsub, "reps": reps, "setup_s": setup_s}))
+    from oracle.cpu_port import CpuHTFETI
+    from oracle.loads import traction_loads
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    o = opts(**dict(cfg, preconditioner="Dirichlet_implicit"))
+    t = time.perf_counter()
+    c = CpuHTFETI(o, threads=ncores)
+    prep_setup = time.perf_counter() - t
+    f = traction_loads(c.mesh, c.lite, SEED, [0])[0]
+    ra = c.solve(f, eps=1e-8, max_iter=1000, threads=ncores)
+    r1 = c.solve(f, eps=1e-8, max_iter=3, threads=1)
+    c.close()
+    one = r1["times"]
+    one_s = one["init"] + one["recover"] + one["pcpg"] * ra["iterations"] / 3.0
+    cl = dict(cfg, Cx=1, Cy=1, Cz=1)
+    ncl = int(cfg.get("Cx", 1)) * int(cfg.get("Cy", 1)) * int(cfg.get("Cz", 1))
+    s1 = CpuHTFETI(opts(**dict(cl, preconditioner="Dirichlet_implicit")), threads=1)
+    set1 = s1.setup_times
+    s1.close()
+    print(json.dumps({"ncores": ncores, "iterations": ra["iterations"], "residual": ra["residual"],
+                      "all_setup": c.setup_times, "all_solve": ra["times"], "one_solve": one,
+                      "one_solve_s": one_s, "one_setup_cluster": set1, "clusters": ncl,
+                      "index_structures_s": prep_setup - c.setup_times["total"], "nsub": c.nsub}))
 
 
-def run_cpu_baseline(gpu_iterations, nsub_target):
-    """One single-threaded oracle process, then one per host core (at most 16)
-    solving concurrently: the all-cores rate is the sum of the per-process rates."""
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample"]
+def run_cpu_baseline(cfg):
+    """Runs cpu_port_run in a child process and reports it as `cpu_baseline`."""
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    env.pop("OMP_NUM_THREADS", None)
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample", "--cpu-config", json.dumps(cfg)]
     t0 = time.time()
-    p = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
+    p = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=1200)
     if p.returncode != 0:
         return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port",
                 "sample": "failed: " + p.stderr[-300:]}
     s = json.loads(p.stdout.strip().splitlines()[-1])
-
-    def scaled(seconds, iterations):
-        # per subdomain-iteration cost of the sample, scaled to the workload
-        per = seconds / (s["nsub"] * max(1, iterations + 3))
-        return 1.0 / (per * nsub_target * (gpu_iterations + 3))
-
-    out = {"value": scaled(s["seconds"], s["iterations"]), "unit": "solves/s", "cores": 1, "kind": "port",
-           "sample": ("oracle (numpy/scipy, SuperLU K_reg, implicit K+ per iteration) on 2x2x2 subdomains of "
-                      "10^3 elements: %.3f s per solve (mean of %d) for %d iterations; scaled per subdomain-iteration "
-                      "to %d subdomains x %d iterations" % (s["seconds"], s["reps"], s["iterations"], nsub_target,
-                                                          gpu_iterations)),
-           "setup_s_sample": s["setup_s"],
-           "setup_s_scaled": s["setup_s"] * nsub_target / s["nsub"]}
-    ncores = min(16, len(os.sched_getaffinity(0)))
-    if ncores > 1:
-        t1 = time.time()
-        procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
-                 for _ in range(ncores)]
-        res = [pr.communicate(timeout=900) for pr in procs]
-        if all(pr.returncode == 0 for pr in procs):
-            per = [json.loads(o.strip().splitlines()[-1]) for o, _ in res]
-            out["all_cores"] = {"value": sum(scaled(q["seconds"], q["iterations"]) for q in per), "cores": ncores,
-                                "sample": "%d concurrent single-threaded oracle processes on the same sample, "
-                                          "%.3f s per solve on average" % (ncores, sum(q["seconds"] for q in per) / ncores),
-                                "wall_s": round(time.time() - t1, 1)}
-        else:
-            out["all_cores"] = {"value": None, "cores": ncores,
-                                "sample": "failed: " + next(e for (_, e), pr in zip(res, procs) if pr.returncode)[-300:]}
+    nsub = s["nsub"]
+    out = {"value": 1.0 / s["one_solve_s"], "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": ("C++ FP64 restatement of the reference's PARDISO path (oracle/cpu/htfeti_cpu.cpp; band "
+                      "Cholesky in l2g order, no MKL on the host) on the bench mesh (%d subdomains) and its first "
+                      "load case: one thread for initialisation, 3 PCPG iterations and recovery, measured; the "
+                      "iteration time scaled to the %d iterations of the all-core solve" % (nsub, s["iterations"])),
+           "solve_s": s["one_solve_s"], "solve_phases_s": s["one_solve"],
+           "setup_s": s["one_setup_cluster"]["total"] * s["clusters"],
+           "setup_sample": "1-thread set-up of one cluster (%d subdomains) x %d clusters: %.2f s per cluster"
+                           % (nsub // max(1, s["clusters"]), s["clusters"], s["one_setup_cluster"]["total"]),
+           "all_cores": {"value": 1.0 / s["all_solve"]["total"], "cores": s["ncores"],
+                         "sample": "the whole bench mesh on %d OpenMP threads, one solve of its first load case, "
+                                   "measured" % s["ncores"],
+                         "solve_s": s["all_solve"]["total"], "solve_phases_s": s["all_solve"],
+                         "iterations": s["iterations"], "setup_s": s["all_setup"]["total"],
+                         "setup_phases_s": s["all_setup"]},
+           "index_structures_s": s["index_structures_s"],
+           "preconditioner": "Dirichlet_implicit (the reference default; same operator as the GPU's explicit S)"}
     out["host"] = {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": cpu_model()}
     out["wall_s"] = round(time.time() - t0, 1)
     return out
@@ -323,6 +322,7 @@ def main():
     ap.add_argument("--elements", type=int, default=10, help="elements per subdomain edge (10: 512 x 10^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", action="store_true")
+    ap.add_argument("--cpu-config", default="{}", help="mesh options of the CPU baseline process (internal)")
     ap.add_argument("--small", action="store_true", help="2x2x2 x 5^3 smoke-sized workload")
     ap.add_argument("--serve-requests", type=int, default=32, help="requests in the serving run (0: none)")
     ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate of requests per second")
@@ -349,7 +349,7 @@ def main():
                          "known size (4 launches; calibration of HBM traffic counters)")
     a = ap.parse_args()
     if a.cpu_sample:
-        cpu_sample()
+        cpu_port_run(json.loads(a.cpu_config))
         return 0
     if a.gpus != 1:
         print("bench.py runs on exactly one GPU: the library has no multi-GPU path.",
@@ -551,7 +551,7 @@ def main():
                                                        a.eps, a.serve_budget_gb, a.serve_warm, 0.0)
             log("serving without host cache:", json.dumps(out["serving_no_host_cache"]))
     if not a.no_cpu:
-        out["cpu_baseline"] = run_cpu_baseline(plain_it[-1], out["config"]["subdomains"])
+        out["cpu_baseline"] = run_cpu_baseline(cfg)
     print(json.dumps(out))
     return 0
 
