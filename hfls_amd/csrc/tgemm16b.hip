@@ -23,10 +23,12 @@
 // (ds_read_b64_tr_b16: in each group of 16 lanes, lane l receives column l of the
 // 4 x 16 block of 16-bit values whose rows the 16 lanes address).  Tile columns are
 // XOR-swizzled in groups of 8 rows (see tpos) so that the staging stores, the
-// 16-byte loads and the transposing loads are all free of bank conflicts.  X_i is
-// staged per step; X_j of the next off-band column is staged a quarter per step
-// during the current one (two column buffers).  70 KB of LDS: two workgroups per
-// CU.
+// 16-byte loads and the transposing loads are all free of bank conflicts.  The
+// band's M X blocks stay in LDS as hi/lo FP16 for the whole workgroup (80 KB of LDS
+// with the tile: two workgroups per CU); X_j of an off-band column is held in
+// registers as MFMA operands.  The tiles of the next M steps are in flight in
+// registers (one per band row of an off-band column), and the next column's X_j
+// is loaded M steps ahead.
 //
 // Bands are aligned to the last tile row (band b covers rows T-(B-b)M ..
 // T-(B-1-b)M-1, band 0 the rest), so every band with off-band columns has exactly
@@ -147,11 +149,16 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int rb = (w >> 1) * 32, cb = (w & 1) * 32;
-  floatx16 accr[M], accc[M];  // row block i0 + m: T X_j sums; column i0 + m (accc[0]: the off-band partial)
+  // row block i0 + m: T X_j sums; column i0 + m (accc[0]: also the off-band partial).
+  // accc[1..M-1] are first used in the in-band triangle and set to zero only there,
+  // so their registers are free during the off-band columns
+  floatx16 accr[M], accc[M];
   static_for<0, M>([&](auto mc) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) accr[mc.value][r] = accc[mc.value][r] = 0.0f;
+    for (int r = 0; r < 16; r++) accr[mc.value][r] = 0.0f;
   });
+#pragma unroll
+  for (int r = 0; r < 16; r++) accc[0][r] = 0.0f;
 
   const _Float16* Ad = (const _Float16*)(A + aoff[d]);
   const float* Xd = X + xoff[d] + c0;
@@ -264,34 +271,31 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
       }
     }
   }
-  Stage g[2];  // tiles of steps s (slot s mod 2) and s + 1, loaded two steps ahead
+  // tiles of steps s .. s + M - 1 in flight, slot s mod M: with M steps per off-band
+  // column the slot of a step is its row ii in the band, and the in-band triangle
+  // starts at step M i0, so every slot index is a compile-time constant
+  Stage g[M];
   XjRaw xraw;
   XjOp xop;
   if (i0 > 0) {
-    fetch(g[0], i0, 0);
-    fetch(g[1], i0 + 1, 0);
+    static_for<0, M>([&](auto uc) { fetch(g[uc.value], i0 + uc.value, 0); });
     xj_load(xraw, 0);
   } else {
-    fetch_tri(g[0], std::integral_constant<int, 0>{});
-    fetch_tri(g[1], std::integral_constant<int, 1>{});
+    static_for<0, M>([&](auto uc) { fetch_tri(g[uc.value], uc); });
   }
   // off-band columns j < i0: the band's M rows each
   for (int j = 0; j < i0; j++) {
     static_for<0, M>([&](auto iic) {
-      constexpr int ii = decltype(iic)::value;
-      constexpr int q = ii & 1;  // step 4 j + ii
-      if constexpr (ii == 0) xj_split(xraw, xop);  // this column's X_j, loaded two steps ago
-      stash(g[q]);
+      constexpr int ii = decltype(iic)::value;  // step M j + ii, slot ii
+      if constexpr (ii == 0) xj_split(xraw, xop);  // this column's X_j, loaded M steps ago
+      stash(g[ii]);
       __syncthreads();
-      // the tile two steps ahead: this column, the next one, or the triangle's first two
-      if constexpr (ii + 2 < M) {
-        fetch(g[q], i0 + ii + 2, j);
-      } else {
-        constexpr int a = ii + 2 - M, tj = tri_jj<M>(a), tii = tri_ii<M>(a);
+      {  // the tile M steps ahead: row ii of the next column, or step ii of the triangle
+        constexpr int tj = tri_jj<M>(ii), tii = tri_ii<M>(ii);
         const bool nxt = j + 1 < i0;
-        fetch(g[q], nxt ? i0 + a : i0 + tii, nxt ? j + 1 : i0 + tj);
+        fetch(g[ii], nxt ? i0 + ii : i0 + tii, nxt ? j + 1 : i0 + tj);
       }
-      if constexpr (ii == M - 2) xj_load(xraw, min(j + 1, T - 1));
+      if constexpr (ii == M - 2) xj_load(xraw, min(j + 1, T - 1));  // the next column's X_j, two steps ahead
       const _Float16* xi = XB + 2 * XP * ii;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -311,14 +315,18 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
       __syncthreads();
     });
   }
+  static_for<1, M>([&](auto mc) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) accc[mc.value][r] = 0.0f;
+  });
   // the band's own lower triangle, (jj, ii) column-major
   static_for<0, NTRI>([&](auto ec) {
     constexpr int e = decltype(ec)::value, jj = tri_jj<M>(e), ii = tri_ii<M>(e);
     const bool act = ii < nb;
-    constexpr int q = e & 1;  // step 4 i0 + e
+    constexpr int q = e % M;  // step M i0 + e
     if (act) stash(g[q]);
     __syncthreads();
-    fetch_tri(g[q], std::integral_constant<int, e + 2>{});
+    if constexpr (e + M < NTRI) fetch_tri(g[q], std::integral_constant<int, e + M>{});
     if (act) {
       const _Float16* xj = XB + 2 * XP * jj;
       const _Float16* xi = XB + 2 * XP * ii;
