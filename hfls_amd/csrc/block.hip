@@ -110,45 +110,75 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
 // G^T x of the projector without an interface block in HBM, in chunks of at most
 // GTC b positions of one subdomain (chunk c: subdomain csub[c], positions
 // cp0[c] .. min(cp0[c] + GTC, n_s)), so large and small subdomains spread evenly over
-// the device: part[(6c + m)*kp + j] = sum_p R_b[p][m] * (sum_e sign_e x[r_e][j]) over the
-// chunk's positions (R_b column major at Rb[offRb[d] + p + m*n_s]).  Each wave takes
-// GQ positions per step; the 4 wave sums are added in a fixed order.  kb_gtx_sum
-// then adds a subdomain's chunks in ascending order: out[(6d + m)*kp + j].  A
-// column's value does not depend on the other columns.
+// the device: part[(6c + m)*kp + j] = sum over the chunk's Bf^T entries e of
+// R_b[p_e][m] * sign_e * x[r_e][j] (R_b column major at Rb[offRb[d] + p + m*n_s], p_e =
+// gpos[e] the entry's b position).  The chunk's R_b rows, entry codes and positions
+// are first staged in LDS with coalesced loads; its entries are then split into 4
+// contiguous ranges, one per wave, walked GXB at a time with every x load of a batch
+// issued before any is used (codes, positions and R_b rows are LDS broadcasts).  The
+// 4 wave sums are added in a fixed order; kb_gtx_sum then adds a subdomain's chunks
+// in ascending order: out[(6d + m)*kp + j].  A column's value does not depend on the
+// other columns.
 constexpr int GTC = 256;
+constexpr int GXB = 16;
+constexpr int GTE = 8 * GTC;  // entries of a chunk: at most 8 dual rows meet at a b position
 __global__ __launch_bounds__(256) void kb_gtx(int k, int kp, const int* __restrict__ ns,
                                               const int* __restrict__ csub, const int* __restrict__ cp0,
                                               const int64_t* __restrict__ boff, const int* __restrict__ gptr,
-                                              const int* __restrict__ gidx, const double* __restrict__ Rb,
-                                              const int64_t* __restrict__ offRb, const double* __restrict__ x,
-                                              double* __restrict__ part, const int* __restrict__ gate) {
+                                              const int* __restrict__ gidx, const int* __restrict__ gpos,
+                                              const double* __restrict__ Rb, const int64_t* __restrict__ offRb,
+                                              const double* __restrict__ x, double* __restrict__ part,
+                                              const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  __shared__ double red[4][6][64];
+  __shared__ double sR[GTC * 6];  // R_b row p - pbeg at 6 (p - pbeg); reused for the wave sums
+  __shared__ int scode[GTE];
+  __shared__ unsigned char spos[GTE];
   const int c = blockIdx.x, j = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int d = csub[c], n = ns[d];
-  const int pbeg = cp0[c], pend = std::min(n, pbeg + GTC);
+  const int pbeg = cp0[c], pend = std::min(n, pbeg + GTC), np = pend - pbeg;
   const double* R = Rb + offRb[d];
+  const int e0 = gptr[boff[d] + pbeg], e1 = gptr[boff[d] + pend];
+  const int ne = std::min(e1 - e0, GTE);
+  for (int t = threadIdx.x; t < 6 * np; t += 256) {
+    const int m = t / np, p = t - m * np;
+    sR[6 * p + m] = R[pbeg + p + (int64_t)m * n];
+  }
+  for (int t = threadIdx.x; t < ne; t += 256) {
+    scode[t] = gidx[e0 + t];
+    spos[t] = (unsigned char)(gpos[e0 + t] - pbeg);
+  }
+  __syncthreads();
+  const int ea = (int)((int64_t)ne * w / 4), eb = (int)((int64_t)ne * (w + 1) / 4);
+  const int jc = j < k ? j : 0;  // lanes past k read column 0 and drop it
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (j < k) {
-    for (int p0 = pbeg + w * GQ; p0 < pend; p0 += 4 * GQ) {
-      const int nq = std::min(GQ, pend - p0);
-      double xq[GQ];
-      bt_row_sums<GQ, 8>(boff[d] + p0, nq, gptr, gidx, [&](int r) { return x[(int64_t)r * k + j]; }, xq);
+  for (int e = ea; e < eb; e += GXB) {
+    double xv[GXB];
 #pragma unroll
-      for (int u = 0; u < GQ; u++) {
-        if (u >= nq) break;
+    for (int i = 0; i < GXB; i++) {
+      const int code = scode[std::min(e + i, eb - 1)];  // past eb: a valid entry, loaded and dropped
+      const double v = x[(int64_t)(code >= 0 ? code : ~code) * k + jc];
+      xv[i] = code >= 0 ? v : -v;
+    }
 #pragma unroll
-        for (int m = 0; m < 6; m++) acc[m] += R[p0 + u + (int64_t)m * n] * xq[u];
+    for (int i = 0; i < GXB; i++) {
+      if (e + i < eb) {
+        const double* r6 = sR + 6 * spos[e + i];
+#pragma unroll
+        for (int m = 0; m < 6; m++) acc[m] += r6[m] * xv[i];
       }
     }
   }
+  __syncthreads();
+  double* red = sR;  // [w][m][64]
 #pragma unroll
-  for (int m = 0; m < 6; m++) red[w][m][j] = acc[m];
+  for (int m = 0; m < 6; m++) red[(w * 6 + m) * 64 + j] = acc[m];
   __syncthreads();
   for (int t = threadIdx.x; t < 6 * 64; t += 256) {
     const int m = t >> 6, jj = t & 63;
     if (jj >= k) continue;
-    part[((int64_t)6 * c + m) * kp + jj] = red[0][m][jj] + red[1][m][jj] + red[2][m][jj] + red[3][m][jj];
+    part[((int64_t)6 * c + m) * kp + jj] =
+        red[(0 * 6 + m) * 64 + jj] + red[(1 * 6 + m) * 64 + jj] + red[(2 * 6 + m) * 64 + jj] +
+        red[(3 * 6 + m) * 64 + jj];
   }
 }
 
@@ -165,11 +195,11 @@ __global__ __launch_bounds__(384) void kb_gtx_sum(int k, int kp, const int* __re
 
 hipError_t launch_kb_gtx(hipStream_t st, int nsub, int nchunk, int k, int kp, const int* ns, const int* csub,
                          const int* cp0, const int* cbeg, const int64_t* boff, const int* gptr, const int* gidx,
-                         const double* Rb, const int64_t* offRb, const double* x, double* part, double* out,
-                         const int* gate) {
-  if (k > 64 || k > kp || nchunk < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(kb_gtx, dim3(nchunk), dim3(256), 0, st, k, kp, ns, csub, cp0, boff, gptr, gidx, Rb, offRb, x,
-                     part, gate);
+                         const int* gpos, const double* Rb, const int64_t* offRb, const double* x, double* part,
+                         double* out, const int* gate) {
+  if (k > 64 || k > kp || nchunk < 1 || !gpos) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kb_gtx, dim3(nchunk), dim3(256), 0, st, k, kp, ns, csub, cp0, boff, gptr, gidx, gpos, Rb, offRb,
+                     x, part, gate);
   hipLaunchKernelGGL(kb_gtx_sum, dim3(nsub), dim3(384), 0, st, k, kp, cbeg, part, out, gate);
   return hipGetLastError();
 }

@@ -509,7 +509,12 @@ void Engine::build_meta() {
     const Subdomain& s = P->subs[d];
     for (size_t e = 0; e < s.bf_row.size(); e++) gptr[off_b[d] + s.bpos[s.bf_dof[e]] + 1]++;
   });
-  for (int64_t q = 0; q < NB; q++) gptr[q + 1] += gptr[q];
+  for (int64_t q = 0; q < NB; q++) {
+    // kb_gtx stages the entries of 256 b positions in LDS: at most 8 per position
+    // (a DOF shared by 8 subdomains has 7 gluing rows in each, plus a Dirichlet row)
+    if (gptr[q + 1] > 8) throw std::runtime_error("more than 8 dual rows at one interface DOF");
+    gptr[q + 1] += gptr[q];
+  }
   // the batched Bf^T walks read up to 7 entries past a group's end
   std::vector<int> gidx((size_t)gptr[NB] + 8, 0);
   parallel_for(N, [&](int d) {
@@ -522,7 +527,14 @@ void Engine::build_meta() {
       else { rb[r] = (int)(off_b[d] + p); rsb[r] = d; rpb[r] = p; }
     }
   });
-  d_gptr.upload(gptr, st); d_gidx.upload(gidx, st);
+  // entry -> b position within its subdomain (the flat entry walk of kb_gtx)
+  std::vector<int> gpos(gidx.size(), 0);
+  parallel_for(N, [&](int d) {
+    const Subdomain& s = P->subs[d];
+    for (int p = 0; p < s.n_s; p++)
+      for (int e = gptr[off_b[d] + p]; e < gptr[off_b[d] + p + 1]; e++) gpos[e] = p;
+  });
+  d_gptr.upload(gptr, st); d_gidx.upload(gidx, st); d_gpos.upload(gpos, st);
   d_ra.upload(ra, st); d_rb.upload(rb, st); d_rsa.upload(rsa, st); d_rsb.upload(rsb, st);
   d_rpa.upload(rpa, st); d_rpb.upload(rpb, st);
   d_wgt.upload(P->weight, st);

@@ -265,6 +265,7 @@ class Engine {
   DVec<int64_t> d_brp, d_rbp;
   DVec<int> d_brc, d_rbc, d_bloc, d_rsub, d_rloc;
   DVec<int> d_perm1, d_gdof, d_gptr, d_gidx, d_lcmap, d_bsub;
+  DVec<int> d_gpos;  // b position (within its subdomain) of each gidx entry
   DVec<double> d_Kt, d_St, partbuf, xb;  // packed lower tiles of K+_bb and S_i, tile partials
   int maxT = 0, maxtiles = 0;
   DVec<unsigned char> d_owner;

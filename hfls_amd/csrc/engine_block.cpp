@@ -486,7 +486,7 @@ void Engine::crq_products(const int* gate) {
 void Engine::b_gtx_coarse(int k, const double* x, double* hbuf, const int* gate) {
   const int N = P->nsub;
   HC(launch_kb_gtx(st, N, n_gchunk, k, KP, d_ns.p, b_gcsub.p, b_gcp0.p, b_gcbeg.p, d_off_b.p, d_gptr.p, d_gidx.p,
-                   d_Rb.p, d_off_Rb.p, x, b_gtp.p, b_gt.p, gate));
+                   d_gpos.p, d_Rb.p, d_off_Rb.p, x, b_gtp.p, b_gt.p, gate));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_gt.p, nullptr, d_H.p, d_cl.p, d_invGtG.p, hbuf, nullptr, 1.0,
                            gate));
 }

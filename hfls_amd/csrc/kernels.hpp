@@ -228,8 +228,8 @@ hipError_t launch_kb_gather(hipStream_t st, int64_t nb, int k, int kp, const int
 // owns chunks cbeg[d] .. cbeg[d+1] - 1
 hipError_t launch_kb_gtx(hipStream_t st, int nsub, int nchunk, int k, int kp, const int* ns, const int* csub,
                          const int* cp0, const int* cbeg, const int64_t* boff, const int* gptr, const int* gidx,
-                         const double* Rb, const int64_t* offRb, const double* x, double* part, double* out,
-                         const int* gate = nullptr);
+                         const int* gpos, const double* Rb, const int64_t* offRb, const double* x, double* part,
+                         double* out, const int* gate = nullptr);
 hipError_t launch_kb_to_f32(hipStream_t st, int64_t n, const double* a, float* b);
 hipError_t launch_kb_copy(hipStream_t st, int64_t n, const double* a, double* b);
 hipError_t launch_kb_rtf(hipStream_t st, int nsub, int k, int kp, int64_t npacked, const int* n, const int64_t* noff,
