@@ -402,17 +402,18 @@ hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const i
 }
 
 // out[r*k+j] = wgt[r]*(y_a[pos_a][j] - y_b[pos_b][j]) - sub[r*k+j]; per-column
-// partial dots with dotw, 64 dual rows per workgroup: part[blk*k + j]
+// partial dots with dotw, 64 dual rows per workgroup: part[blk*k + j].  srow[2r] and
+// srow[2r+1] are the b-block rows of dual row r's + and - entries (-1: no - entry).
 // Wave g of a workgroup takes the 16 consecutive dual rows 64 blockIdx + 16 g + q in
-// two batches of 8, every load of a batch issued before any is used.  TY: the
-// b-block's element type (FP64, or the FP32 result of the mixed-precision GEMM).
+// two batches of 8: the batch's 16 row indices are one scalar load, then every value
+// load of the batch is issued before any is used.  TY: the b-block's element type
+// (FP64, or the FP32 result of the mixed-precision GEMM).
 template <typename TY>
-__global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, const int* __restrict__ rsa,
-                                                  const int* __restrict__ rsb, const int* __restrict__ rpa,
-                                                  const int* __restrict__ rpb, const int64_t* __restrict__ obrow,
+__global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, const int* __restrict__ srow,
                                                   const TY* __restrict__ y, const double* __restrict__ wgt,
                                                   const double* __restrict__ sub, double* __restrict__ out,
-                                                  const double* __restrict__ dotw, double* __restrict__ part, const int* __restrict__ gate) {
+                                                  const double* __restrict__ dotw, double* __restrict__ part,
+                                                  const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double red[4 * 64];
   const int j = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -422,16 +423,22 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
     const int64_t r0 = (int64_t)blockIdx.x * 64 + g * 16;
 #pragma unroll
     for (int q0 = 0; q0 < 16; q0 += SB) {
+      int sa[SB], sb[SB];
+#pragma unroll
+      for (int u = 0; u < SB; u++) {
+        const int64_t r = r0 + q0 + u;
+        sa[u] = r < nlam ? srow[2 * r] : -1;
+        sb[u] = r < nlam ? srow[2 * r + 1] : -1;
+      }
       double va[SB], vb[SB], ws[SB], sv[SB], dv[SB];
 #pragma unroll
       for (int u = 0; u < SB; u++) {
         const int64_t r = r0 + q0 + u;
         va[u] = vb[u] = sv[u] = dv[u] = 0.0;
         ws[u] = 1.0;
-        if (r < nlam) {
-          const int a = rsa[r], b = rsb[r];
-          va[u] = (double)y[(obrow[a] + rpa[r]) * kp + j];
-          if (b >= 0) vb[u] = (double)y[(obrow[b] + rpb[r]) * kp + j];
+        if (sa[u] >= 0) {
+          va[u] = (double)y[(int64_t)sa[u] * kp + j];
+          if (sb[u] >= 0) vb[u] = (double)y[(int64_t)sb[u] * kp + j];
           if (wgt) ws[u] = wgt[r];
           if (sub) sv[u] = sub[r * k + j];
           if (dotw) dv[u] = dotw[r * k + j];
@@ -442,7 +449,7 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
         const int64_t r = r0 + q0 + u;
         if (r >= nlam) break;
         double v = va[u];
-        if (rsb[r] >= 0) v -= vb[u];
+        if (sb[u] >= 0) v -= vb[u];
         if (wgt) v *= ws[u];
         if (sub) v -= sv[u];
         out[r * k + j] = v;
@@ -457,18 +464,17 @@ __global__ __launch_bounds__(256) void kb_scatter(int64_t nlam, int k, int kp, c
   }
 }
 
-hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
-                             const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
-                             const double* sub, double* out, const double* dotw, double* part, const int* gate,
-                             const float* yf) {
-  if (k > 64) return hipErrorInvalidValue;
+hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* srow, const double* y,
+                             const double* wgt, const double* sub, double* out, const double* dotw, double* part,
+                             const int* gate, const float* yf) {
+  if (k > 64 || !srow) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((nlam + 63) / 64));
   if (yf)
-    hipLaunchKernelGGL(kb_scatter<float>, grid, dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb, obrow, yf, wgt,
-                       sub, out, dotw, part, gate);
+    hipLaunchKernelGGL(kb_scatter<float>, grid, dim3(256), 0, st, nlam, k, kp, srow, yf, wgt, sub, out, dotw, part,
+                       gate);
   else
-    hipLaunchKernelGGL(kb_scatter<double>, grid, dim3(256), 0, st, nlam, k, kp, rsa, rsb, rpa, rpb, obrow, y, wgt,
-                       sub, out, dotw, part, gate);
+    hipLaunchKernelGGL(kb_scatter<double>, grid, dim3(256), 0, st, nlam, k, kp, srow, y, wgt, sub, out, dotw, part,
+                       gate);
   return hipGetLastError();
 }
 

@@ -330,6 +330,7 @@ class Engine {
   DVec<int> b_xob, b_xor;  // XCD orders of the tile GEMMs over b rows (n_s^2) and interior rows (n_r^2)
   int n_xob = 0, n_xor = 0;
   DVec<int> b_gcsub, b_gcp0, b_gcbeg;  // chunks of b positions for G^T x (kb_gtx)
+  DVec<int> b_srow;  // b-block rows of the + and - entry of every dual row (kb_scatter)
   DVec<int> d_face_meta;                // subdomains on the x = 1 face (traction lumping)
   DVec<int64_t> d_face_off;
   DVec<double> d_trac;                  // face traction fields of the loads being joined

@@ -137,6 +137,16 @@ void Engine::block_prepare() {
     for (int d = 0; d < N; d++)
       for (int p = 0; p < h_ns[d]; p++) qrow[off_b[d] + p] = (int)(obrow[d] + p);
     b_qrow.upload(qrow, st);
+    // b-block rows of each dual row's + and - entries (kb_scatter)
+    std::vector<int> ra, rb;
+    d_ra.download(ra, st);
+    d_rb.download(rb, st);
+    std::vector<int> srow(2 * P->nlam);
+    for (int64_t r = 0; r < P->nlam; r++) {
+      srow[2 * r] = qrow[ra[r]];
+      srow[2 * r + 1] = rb[r] >= 0 ? qrow[rb[r]] : -1;
+    }
+    b_srow.upload(srow, st);
   }
   b_orrow.upload(orrow, st);
   b_xoff.upload(xoff, st);
@@ -418,7 +428,7 @@ size_t Engine::block_bytes() const {
   for (const DVec<double>* v : {&b_CR, &b_CRt, &b_kpart, &b_Gr, &b_gt, &b_gtp, &b_xb, &b_yb, &b_cr, &b_coef, &b_v6, &b_gc, &b_la, &b_h, &b_hg, &b_hgf, &b_lam, &b_g, &b_w,
                                 &b_Fw, &b_Pg, &b_z, &b_Pz, &b_d, &b_g64, &b_part, &b_Vr, &b_Xr, &b_Tr, &b_V, &b_V2, &b_X, &b_U})
     s += v->n * sizeof(double);
-  s += (b_ghsub.n + b_ghslot.n) * sizeof(int);
+  s += (b_ghsub.n + b_ghslot.n + b_srow.n) * sizeof(int);
   return s;
 }
 
@@ -444,7 +454,7 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
   HC(launch_tgemm_f64(st, NC, maxTcl, 2, 0, d_nlen.p, d_nlen.p, d_Ainv.p, d_Aoff.p, d_nlen.p, b_gc.p, b_gcoff.p,
                       b_la.p, b_gcoff.p, nullptr, KP, 1.0, gate, sAc, b_kpart.p));
   b_corr(k, 1, prec == 32 ? 1 : 0, gate, prec == 32 ? b_yb32.p : nullptr);
-  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, sub, out,
+  HC(launch_kb_scatter(st, P->nlam, k, KP, b_srow.p, b_yb.p, nullptr, sub, out,
                        dotw, b_part.p, gate));
 }
 
@@ -511,7 +521,7 @@ void Engine::bPM(int k, int prec, const double* x, double* out, double* hbuf, co
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
   }
-  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
+  HC(launch_kb_scatter(st, P->nlam, k, KP, b_srow.p, b_yb.p, d_wgt.p, nullptr,
                        out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
 }
 
@@ -526,7 +536,7 @@ void Engine::bM(int k, int prec, const double* x, double* out, const int* gate) 
     HC(launch_tgemm_f64(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, d_St.p, d_off_P.p, nullptr, b_xb.p, b_xoff.p, b_yb.p,
                         b_xoff.p, nullptr, KP, 1.0, gate, 1, nullptr, nullptr, b_xob.p, n_xob));
   }
-  HC(launch_kb_scatter(st, P->nlam, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, d_wgt.p, nullptr,
+  HC(launch_kb_scatter(st, P->nlam, k, KP, b_srow.p, b_yb.p, d_wgt.p, nullptr,
                        out, nullptr, nullptr, gate, prec == 32 ? b_yb32.p : nullptr));
 }
 
@@ -730,7 +740,7 @@ void Engine::blk_init(const std::vector<int>& cols) {
   // d = Bf K_c^+ f ; e = R^T f ; lam0 = G (G^T G)^-1 e (Cluster.cpp:1385-1394); computed
   // for every column into scratch, copied into the state of the joining ones
   b_kplus(k, false, b_V.p);
-  HC(launch_kb_scatter(st, nl, k, KP, d_rsa.p, d_rsb.p, d_rpa.p, d_rpb.p, b_obrow.p, b_yb.p, nullptr, nullptr, b_Fw.p,
+  HC(launch_kb_scatter(st, nl, k, KP, b_srow.p, b_yb.p, nullptr, nullptr, b_Fw.p,
                        nullptr, nullptr));
   HC(launch_kb_update(st, nl, k, 3, b_d.p, nullptr, b_Fw.p, nullptr, b_sc.p));
   HC(launch_kb_proj_coarse(st, N, 6 * P->ncl, k, KP, b_v6.p, b_v6row.p, d_H.p, d_cl.p, d_invGtG.p, b_h.p, nullptr,

@@ -273,9 +273,10 @@ hipError_t launch_kb_colscale(hipStream_t st, int nsub, const int* ns, const int
 hipError_t launch_kb_deflate_f32(hipStream_t st, int nsub, int maxtiles, const int* ns, const double* P,
                                  const int64_t* poff, const double* CRQ, const int64_t* coff, const int* ldc,
                                  const int* nlc, float* P32);
-hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* rsa, const int* rsb,
-                             const int* rpa, const int* rpb, const int64_t* obrow, const double* y, const double* wgt,
-                             const double* sub, double* out, const double* dotw, double* part, const int* gate = nullptr,
+// srow[2r], srow[2r+1]: b-block rows of dual row r's + and - entries (-1: none)
+hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const int* srow, const double* y,
+                             const double* wgt, const double* sub, double* out, const double* dotw, double* part,
+                             const int* gate = nullptr,
                              const float* yf = nullptr);  // yf: FP32 b-block in place of y
 // subdomains whose h rows kb_gh stages in LDS per group of 64 dual rows
 constexpr int GH_SLOTS = 12;

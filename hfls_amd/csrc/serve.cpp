@@ -79,7 +79,7 @@ size_t Server::estimate(const Problem& p) const {
     b += (8.0 + 8.0 + 4.0) * 64.0 * 64.0 * T;                             // b-blocks
     b += 8.0 * 64.0 * (2.0 * ((s.n_r + 63) / 64 * 64) + 64.0 * 3);        // interior, cr/coef/v6 blocks
   }
-  b += 8.0 * (8.0 * nl + 9.0 * 64.0 * nl);                                // dual vectors
+  b += 8.0 * (10.0 * nl + 9.0 * 64.0 * nl);                               // dual vectors and row indices
   b += 8.0 * 64.0 * (3.0 * np + ng) + 8.0 * (ng + 4.0 * np);               // local blocks, U
   return (size_t)(1.1 * b);
 }
