@@ -1092,7 +1092,15 @@ void Engine::build_bc_and_coarse() {
   //  * false (the reference default, htfeti.cpp:39: the singular Ac factorised with
   //    its kernel regularised, Cluster.cpp:309-317): A_r^-1 itself, which satisfies
   //    Ac A_r^-1 Ac = Ac because A_r [0; H y] = [0; -sig H y].
-  const bool extended = opt(o, "Ac_extended_by_kerGc") == "true";
+  //  * Ac_clust_explicit=false (the Sc branch, Cluster.cpp:325-342, which the
+  //    reference leaves after forming Sc = Gc^T Fc^-1 Gc), completed as the two-stage
+  //    coarse solve of myModul.py:128-135: beta = (Sc + H H^T)^-1 (Gc^T Fc^-1 d - e),
+  //    mu = Fc^-1 (d - Gc beta).  That map is the inverse of [Fc Gc; Gc^T -H H^T]
+  //    (block inverse with Schur complement -(Sc + H H^T)), i.e. A_r^-1 with sig = 1:
+  //    the elimination below factors Fc, then Sc + H H^T, which is the two-stage
+  //    solve made explicit.
+  const bool two_stage = opt(o, "Ac_clust_explicit") == "false";
+  const bool extended = !two_stage && opt(o, "Ac_extended_by_kerGc") == "true";
   const std::string dump = opti(o, "print_matrices") > 0 ? opt(o, "path2data") : std::string();
   // clusters in parallel: each writes its own slice of Aaug and the Hall rows of its subdomains
   parallel_for(P->ncl, [&](int cl) {
@@ -1126,7 +1134,7 @@ void Engine::build_bc_and_coarse() {
     double tf = 0.0, tg = 0.0;
     for (int l = 0; l < nlc; l++) tf += Fc[(size_t)l * nlc + l];
     for (int c = 0; c < nr; c++) tg += GtG[(size_t)c * nr + c];
-    double sig = (nlc > 0 && tf > 0) ? (tg / nr) / (tf / nlc) : 1.0;
+    double sig = (nlc > 0 && tf > 0 && !two_stage) ? (tg / nr) / (tf / nlc) : 1.0;
     const int L2 = 2 * (nlc + nr);
     double* A = Aaug.get() + aug_off[cl];
     auto at = [&](int r, int c) -> double& { return A[r + (int64_t)c * L2]; };  // lower triangle, column major

@@ -157,12 +157,9 @@ static void check_reference_options(const Options& o) {
     throw std::runtime_error(
         "metis=true with nparts > 1 partitions the mesh with METIS_PartMeshDual (Mesh.cpp:392-412), which this "
         "library does not provide; use metis=false (geometric Nx x Ny x Nz decomposition) or nparts=1");
-  // Ac_clust_explicit=false forms Sc = Gc^T Fc^-1 Gc and exits (Cluster.cpp:325-342)
-  const std::string ace = opt(o, "Ac_clust_explicit");
-  if (ace == "false")
-    throw std::runtime_error("Ac_clust_explicit=false: the reference forms Sc_clust and exits (Cluster.cpp:325-342); "
-                             "there is no solve on that path");
-  if (ace != "true") throw std::runtime_error("Ac_clust_explicit must be true or false");
+  // Ac_clust_explicit=false: the Sc branch (Cluster.cpp:325-342, Sc = Gc^T Fc^-1 Gc),
+  // completed as the two-stage coarse solve (myModul.py:128-135)
+  if (!is_bool(opt(o, "Ac_clust_explicit"))) throw std::runtime_error("Ac_clust_explicit must be true or false");
   // GcTGc_assembl_block_by_block picks one of two assemblies of the same Gc^T Gc
   // (Cluster.cpp:236-245); create_analytic_ker_K is read by no solver branch
   // (Cluster.cpp:84-86 is `if (0)`), and htfeti.cpp:60-66 can leave it empty

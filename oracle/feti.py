@@ -6,7 +6,10 @@ Preconditioning, mult_Ff, mult_Bf/BfT, Projection, scale (Cluster.cpp:961-1360),
 pcpg (Cluster.cpp:1363-1458) and the recovery in printVTK (Cluster.cpp:1648-1678).
 
 Choices (identical in the product):
-  * the cluster saddle-point matrix: with Ac_extended_by_kerGc=true the
+  * the cluster saddle-point matrix: with Ac_clust_explicit=false the two-stage
+    Schur-complement solve (Fc^-1, Sc = Gc^T Fc^-1 Gc, (Sc + H H^T)^-1,
+    myModul.py:128-135 completing Cluster.cpp:325-342); otherwise, with
+    Ac_extended_by_kerGc=true the
     non-singular extended form Ac = [Fc Gc 0; Gc^T 0 H; 0 H^T 0]
     (Cluster.cpp:592-660); with false (the reference default, which factors the
     singular [Fc Gc; Gc^T 0] with its kernel detection, Cluster.cpp:286-320) the
@@ -119,6 +122,15 @@ class HTFETI:
             else:
                 H, dH, _ = kernel_from_K(GtG, None, seed=seed + 100000 + cl)
             nh = H.shape[1]
+            two_stage = None
+            if o.get("Ac_clust_explicit", "true") == "false":
+                # the Sc branch (Cluster.cpp:325-342: Fc factorised, Sc = Gc^T Fc^-1 Gc)
+                # completed as the two-stage coarse solve of the Python HTFETI
+                # (myModul.py:128-135, 500-510): iF0 = Fc^-1, iS0 = (Sc + H H^T)^-1,
+                # beta = iS0 (Gc^T iF0 d0 - e0), mu = iF0 (d0 - Gc beta)
+                iF0 = sla.cho_factor(Fc, lower=True)
+                Sc = Gc.T @ sla.cho_solve(iF0, Gc)
+                two_stage = (iF0, sla.cho_factor(Sc + H @ H.T, lower=True), Sc)
             if o.get("Ac_extended_by_kerGc", "true") == "true":
                 # extended non-singular form (Cluster.cpp:592-660)
                 Ac = np.zeros((nlc + nr + nh, nlc + nr + nh))
@@ -141,7 +153,7 @@ class HTFETI:
                 Ac[nlc + nr:, nlc + nr:] = np.eye(nh)  # decoupled rows: multiplier 0
             self.clusters.append({"d0": d0, "d1": d1, "nlc": nlc, "nr": nr, "roff": roff, "H": H,
                                   "Fc": Fc, "Gc": Gc, "Ac_lu": sla.lu_factor(Ac), "nh": nh,
-                                  "Ac": Ac})
+                                  "Ac": Ac, "two_stage": two_stage})
         # ---- FETI coarse: Rf, Gf, GfTGf (Cluster.cpp:1227-1273, 1332-1352) ----
         ncol = sum(c["nh"] for c in self.clusters)
         self.ncol_G = ncol
@@ -215,7 +227,14 @@ class HTFETI:
                 gc[self.bc_rows[d]] += self.KplusBcT[d].T @ xs[d]
                 ro = c["roff"][d - c["d0"]]
                 gc[nlc + ro:nlc + ro + self.R[d].shape[1]] = -(self.R[d].T @ xs[d])
-            la = sla.lu_solve(c["Ac_lu"], gc)
+            if c["two_stage"] is not None:
+                iF0, iS0, _ = c["two_stage"]
+                d0v, e0v = gc[:nlc], gc[nlc:nlc + nr]
+                beta = sla.cho_solve(iS0, c["Gc"].T @ sla.cho_solve(iF0, d0v) - e0v)
+                mu = sla.cho_solve(iF0, d0v - c["Gc"] @ beta)
+                la = np.concatenate([mu, beta, np.zeros(nh)])
+            else:
+                la = sla.lu_solve(c["Ac_lu"], gc)
             for d in range(c["d0"], c["d1"]):
                 ro = c["roff"][d - c["d0"]]
                 alpha = la[nlc + ro:nlc + ro + self.R[d].shape[1]]

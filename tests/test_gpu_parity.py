@@ -33,6 +33,8 @@ CASES = {"cube8_5e3": CUBE8_5E3, "multi": MULTI, "refdefault": dict(nx=3, ny=3, 
          "cube8_5e3_cor_edge": dict(CUBE8_5E3, typeBc="cor+edge"), "multi_cor_edge": dict(MULTI, typeBc="cor+edge"),
          # the reference's default cluster inverse: the singular Ac regularised by its kernel
          "ac_regularized": dict(MULTI, Ac_extended_by_kerGc="false"),
+         # Ac_clust_explicit=false: the Sc branch completed as the two-stage coarse solve
+         "ac_two_stage": dict(MULTI, Ac_clust_explicit="false"),
          # the reference's own map: metis=true with nparts=1 makes the 10x5x5 cube one
          # subdomain (Mesh.cpp:413-421), checkerboard material of the geometric 2x1x1
          # partition, symmetry planes, the reference body force

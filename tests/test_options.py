@@ -3,14 +3,17 @@ handles, and the oracle pins of the option-selected variants.
 
 * every key of htfeti.cpp:12-57 is accepted with the reference's values; the
   values a path cannot honour are rejected with the reason (METIS partitions,
-  the Sc branch of Ac_clust_explicit=false, Dissection, per-subdomain dumps);
+  Dissection, per-subdomain dumps); Ac_clust_explicit=false (the Sc branch) is
+  accepted as the two-stage coarse solve;
 * metis=true with nparts=1: one subdomain over the whole cube (Mesh.cpp:413-421),
   integer structures bit-exact vs the oracle, material ids of the geometric
   partition in the VTK output;
 * oracle pins: corner plus averaged-edge gluing has dim ker(Gc^T Gc) = 6 and the
-  displacement of `ker`; the regularised (non-extended) Ac gives the same
-  displacement as the extended one; the single-subdomain map equals the global
-  direct solve;
+  displacement of `ker`; the regularised (non-extended) Ac and the two-stage
+  Schur-complement solve of Ac_clust_explicit=false give the same displacement as
+  the extended Ac; the two-stage solve equals the inverse of [Fc Gc; Gc^T -H H^T]
+  (block-inverse identity, the form the device builds); the single-subdomain map
+  equals the global direct solve;
 * a NULL handle or output pointer is an error return, not a crash.
 """
 import ctypes
@@ -46,7 +49,7 @@ def test_reference_map_accepted():
     ("linear_solver", "dissection", "dissection"),
     ("linear_solver", "mumps", "unknown linear_solver"),
     ("nparts", "4", "METIS"),
-    ("Ac_clust_explicit", "false", "Sc_clust"),
+    ("Ac_clust_explicit", "0", "Ac_clust_explicit"),
     ("print_matrices", "2", "per-subdomain"),
     ("typeBc", "edges", "unknown typeBc"),
     ("vtkWithinIter", "yes", "vtkWithinIter"),
@@ -128,6 +131,31 @@ def test_regularised_ac_same_displacement():
     ue = HTFETI(opts(**kw)).pcpg(eps=1e-11, max_iter=300)["u"]
     ur = HTFETI(opts(Ac_extended_by_kerGc="false", **kw)).pcpg(eps=1e-11, max_iter=300)["u"]
     assert np.linalg.norm(ue - ur) <= 1e-10 * np.linalg.norm(ue)
+
+
+def test_two_stage_coarse_solve():
+    """Ac_clust_explicit=false: beta = (Sc + H H^T)^-1 (Gc^T Fc^-1 d - e), mu = Fc^-1 (d -
+    Gc beta) with Sc = Gc^T Fc^-1 Gc (myModul.py:128-135) is the inverse of
+    [Fc Gc; Gc^T -H H^T], and the solve gives the displacement of the extended Ac."""
+    kw = dict(SMALL, Cx=2, Nx=2)
+    h = HTFETI(opts(Ac_clust_explicit="false", **kw))
+    p = hfls_amd.Problem(**dict(kw, Ac_clust_explicit="false"))
+    assert p.nsub == h.nsub
+    rng = np.random.default_rng(11)
+    for c in h.clusters:
+        nlc, nr, Fc, Gc, H = c["nlc"], c["nr"], c["Fc"], c["Gc"], c["H"]
+        iF0, iS0, Sc = c["two_stage"]
+        assert np.allclose(Sc, Gc.T @ np.linalg.solve(Fc, Gc), rtol=1e-12, atol=1e-12 * np.abs(Sc).max())
+        A = np.block([[Fc, Gc], [Gc.T, -H @ H.T]])
+        b = rng.standard_normal(nlc + nr)
+        import scipy.linalg as sla
+        beta = sla.cho_solve(iS0, Gc.T @ sla.cho_solve(iF0, b[:nlc]) - b[nlc:])
+        mu = sla.cho_solve(iF0, b[:nlc] - Gc @ beta)
+        ref = np.linalg.solve(A, b)
+        assert np.linalg.norm(np.concatenate([mu, beta]) - ref) <= 1e-10 * np.linalg.norm(ref)
+    ue = HTFETI(opts(**kw)).pcpg(eps=1e-11, max_iter=300)["u"]
+    us = h.pcpg(eps=1e-11, max_iter=300)["u"]
+    assert np.linalg.norm(ue - us) <= 1e-10 * np.linalg.norm(ue)
 
 
 def test_metis_single_subdomain_oracle_vs_direct():
