@@ -1490,7 +1490,17 @@ double Engine::time_stream(int which, int reps, double* bytes) {
     if (bytes) *bytes = fl;
     return ms / reps;
   }
-  if (which < 0 || which > 6) throw std::runtime_error("time_stream: which must be 0..8");
+  if (which == 9) {  // the K+_bb GEMM inside one eager block solve; reps = load cases
+    int n = 0;
+    const double ms = time_gemm_in_solve(reps, &n);
+    if (bytes) {
+      double nb = 0;
+      for (int d = 0; d < P->nsub; d++) nb += h_ns[d];
+      *bytes = 4.0 * (double)off_P[P->nsub] + 2.0 * 4.0 * KMAX * nb;
+    }
+    return ms;
+  }
+  if (which < 0 || which > 6) throw std::runtime_error("time_stream: which must be 0..9");
   const int N = P->nsub;
   const int64_t nl = P->nlam;
   std::vector<double> ones(nl, 1.0);

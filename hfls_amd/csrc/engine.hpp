@@ -174,6 +174,9 @@ class Engine {
   // per launch and the algorithmic flops of one launch (2 * 64 * sum n_s^2).
   // which = 2: a plain device copy of known size (*flops = bytes read = written).
   double time_block_gemm(int which, int reps, double* flops);
+  // mean device time of the K+_bb GEMM (band kernel and its reduction) over its
+  // launches inside one eager mixed-precision block solve of k seeded face tractions
+  double time_gemm_in_solve(int k, int* nlaunch);
 
   // Block of k load cases (column major: column j of f_local at j*packed_n(), of u
   // at j*3*nPoints, of lam at j*nlam).  precision 64: F_i and S_i applied in FP64;
@@ -292,6 +295,9 @@ class Engine {
 
   // ---- block state (allocated on the first block call)
   bool use_graph = true;
+  // in-solve GEMM probe (time_gemm_in_solve): events around every K+_bb GEMM launch
+  Events* gemm_probe = nullptr;
+  size_t gemm_probe_n = 0;
   int tune_rt32 = 1, gemm_rows_forced = 0;
   int gemm_lds_pad = 0, gemm_exact = 0, gemm_pf = 1;  // split-FP16 GEMM: residency cap, tile order, prefetch
   // split-FP16 GEMM reading every tile once (bands of row blocks, partial sums in

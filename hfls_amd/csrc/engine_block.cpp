@@ -461,7 +461,14 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
 void Engine::gemm_lo(int which, const int* gate) {
   const int N = P->nsub;
   const float* A = which == 0 ? b_Kt32.p : b_St32.p;
-  if (gemm16 && gemm_band)
+  if (gemm_probe && which == 0 && gemm16 && gemm_band && gemm_probe_n + 2 <= gemm_probe->e.size()) {
+    // the column scales, then the band kernel and its reduction between two events
+    HC(launch_kb_colscale(st, N, d_ns.p, b_xoff.p, KP, b_xb32.p, b_sx.p, gate));
+    HC(hipEventRecord(gemm_probe->e[gemm_probe_n++], st));
+    HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, gate,
+                         b_yb32.p, b_xob.p, n_xob, 0, b_p16.p, d_p16off.p));
+    HC(hipEventRecord(gemm_probe->e[gemm_probe_n++], st));
+  } else if (gemm16 && gemm_band)
     HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
                          KP, 1.0f, gate, b_yb32.p, b_xob.p, n_xob, 1, b_p16.p, d_p16off.p));
   else if (gemm16)
@@ -607,6 +614,45 @@ void Engine::apply_block(int op, int k, int prec, const double* x_h, double* y_h
   HC(hipStreamSynchronize(st));
   for (int j = 0; j < k; j++)
     for (int64_t r = 0; r < nl; r++) y_h[(size_t)j * nl + r] = t[(size_t)r * k + j];
+}
+
+double Engine::time_gemm_in_solve(int k, int* nlaunch) {
+  if (!(gemm16 && gemm_band)) throw std::runtime_error("the in-solve GEMM probe times the band form of the split-FP16 GEMM");
+  k = std::max(1, std::min(k, KMAX));
+  int64_t np = 0;
+  for (const Subdomain& sd : P->subs) np += sd.n;
+  std::vector<double> f((size_t)k * np);
+  traction_loads(*P, 20240611ull, 0, k, f.data());
+  std::vector<double> u((size_t)k * 3 * P->nPoints);
+  Events ev(2 * 4096);
+  const bool g = use_graph;
+  use_graph = false;  // events between the launches need the eager launch sequence
+  gemm_probe = &ev;
+  gemm_probe_n = 0;
+  try {
+    solve_block(1e-8, 1000, k, 32, f.data(), u.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+  } catch (...) {
+    gemm_probe = nullptr;
+    use_graph = g;
+    throw;
+  }
+  gemm_probe = nullptr;
+  use_graph = g;
+  HC(hipStreamSynchronize(st));
+  // launches skipped through their device gate (no running column) take microseconds:
+  // the mean over the launches longer than a quarter of the longest one
+  std::vector<float> ms(gemm_probe_n / 2);
+  float mx = 0;
+  for (size_t i = 0; i < ms.size(); i++) {
+    HC(hipEventElapsedTime(&ms[i], ev[2 * i], ev[2 * i + 1]));
+    mx = std::max(mx, ms[i]);
+  }
+  double sum = 0;
+  int n = 0;
+  for (float x : ms)
+    if (x > 0.25f * mx) sum += x, n++;
+  if (nlaunch) *nlaunch = n;
+  return n ? sum / n : 0.0;
 }
 
 double Engine::time_block_gemm(int which, int reps, double* flops) {

@@ -263,7 +263,11 @@ int hfls_server_cache_stats(hfls_server s, double* out);
  * each moved once); which = 4:
  * a plain device copy (reference for traffic counters), *bytes = bytes read;
  * which = 5 / 6: the sustained FP64 / FP32 matrix-core rate on register data with
- * random operands (no memory traffic), *bytes = flops of one launch. */
+ * random operands (no memory traffic), *bytes = flops of one launch; which = 9: the
+ * K+_bb GEMM of the band form (band kernel and reduction) inside one mixed-precision
+ * block solve of reps seeded face-traction load cases, launched eagerly with events
+ * around every GEMM, *ms = mean over the launches that ran (not skipped through
+ * their device gate), *bytes = algorithmic HBM bytes of one launch. */
 int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
 
 #ifdef __cplusplus

@@ -114,12 +114,14 @@ def test_measurement_hooks():
     """The timing hook: every kind returns a positive time and work figure, the
     matrix-core ceilings report their flops, the block GEMM its algorithmic flops
     (2 * 64 * sum n_s^2) and HBM bytes (packed 64x64 tiles at 4 B per element, 64
-    FP32 columns of right-hand sides and results per interface DOF), and an unknown
+    FP32 columns of right-hand sides and results per interface DOF), the in-solve
+    GEMM probe (kind 9, an eager block solve with events around every K+_bb GEMM)
+    the same bytes and leaves the captured path bitwise unchanged, and an unknown
     kind is an error."""
     m = hfls_amd.Mesh(**CASES["cube8_5e3"])
     F = m.traction(79, 0, 2)
-    m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
-    for which in range(9):
+    r0 = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    for which in range(10):
         ms, work = m.time_stream(which, reps=2)
         print("hook", which, ms, work, flush=True)
         assert ms > 0 and work > 0
@@ -130,6 +132,9 @@ def test_measurement_hooks():
     tiles = sum(((n + 63) // 64) * ((n + 63) // 64 + 1) // 2 for n in ns)
     assert m.time_stream(2, reps=1)[1] == sum(2.0 * 64 * n * n for n in ns)
     assert m.time_stream(7, reps=1)[1] == 4.0 * 4096 * tiles + 2 * 4.0 * 64 * sum(ns)
+    assert m.time_stream(9, reps=4)[1] == m.time_stream(7, reps=1)[1]
+    r1 = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
+    assert np.array_equal(r0["u"], r1["u"]) and np.array_equal(r0["iterations"], r1["iterations"])
     with pytest.raises(hfls_amd.HflsError):
-        m.time_stream(9, reps=1)
+        m.time_stream(10, reps=1)
     m.close()
