@@ -485,8 +485,12 @@ void Engine::b_corr(int k, int with_R, int defl, const int* gate, const float* y
   const int N = P->nsub;
   HC(launch_kb_coef(st, N, maxnlc + 18, k, KP, d_nlc.p, d_off_lc.p, b_lcrow.p, b_arow.p, b_la.p, with_R, defl, b_cr.p,
                     b_crrow.p, b_coef.p, gate));
-  HC(launch_tgemm_f64(st, N, maxTb, 1, 0, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_coef.p, b_croff.p, b_yb.p,
-                      b_xoff.p, yf ? nullptr : b_yb.p, KP, 1.0, gate, 1, nullptr, yf, b_xob.p, n_xob));
+  if (maxTcr == 1)  // at most 64 correction columns: coefficients staged once per row block
+    HC(launch_corr_f64(st, N, maxTb, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, b_coef.p, b_croff.p, b_yb.p, b_xoff.p,
+                       yf ? nullptr : b_yb.p, yf, KP, gate, b_xob.p, n_xob));
+  else
+    HC(launch_tgemm_f64(st, N, maxTb, 1, 0, d_ns.p, b_crn.p, b_CRt.p, b_crto.p, nullptr, b_coef.p, b_croff.p, b_yb.p,
+                        b_xoff.p, yf ? nullptr : b_yb.p, KP, 1.0, gate, 1, nullptr, yf, b_xob.p, n_xob));
 }
 
 // cr = CRQ^T x_b per subdomain (c = C^T x, a = R_b^T x, Q^T x, Yh^T x), from the

@@ -193,6 +193,10 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
                             double alpha, const int* gate, int S = 1, double* Ypart = nullptr,  // S: split K
                             const float* Zf = nullptr,  // Zf: an FP32 addend in place of Z
                             const int* xcd_order = nullptr, int norder = 0);  // see xcd_order()
+// Y = A C (+ Z or + Zf) for tiled A of at most 64 columns (one tile column), C row major
+hipError_t launch_corr_f64(hipStream_t st, int nsub, int maxRo, const int* mrows, const int* mcols, const double* A,
+                           const int64_t* aoff, const double* C, const int64_t* coff, double* Y, const int64_t* yoff,
+                           const double* Z, const float* Zf, int kp, const int* gate, const int* order, int norder);
 hipError_t launch_pack_f32(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,
                            const double* A, const int64_t* aoff, const int* lda, float* P, const int64_t* poff);
 hipError_t launch_pack_f64(hipStream_t st, int nsub, int maxtiles, int mode, const int* mrows, const int* mcols,

@@ -479,6 +479,81 @@ hipError_t launch_tgemm_f64(hipStream_t st, int nsub, int maxRo, int mode, int o
   return hipGetLastError();
 }
 
+// Y_d = A_d C_d (+ Z_d or + Zf_d) for a general tiled A_d with at most 64 columns (one
+// tile column: tile i at A + aoff[d] + 4096 i) and C_d of as many rows (row major, ld
+// kp, C + coff[d]): the HTFETI correction y_b += [C|R_b|Q|Yh] coef of the block solve.
+// One workgroup per 64-row output block; C_d (the same for every row block of the
+// subdomain) is staged once in LDS, rows past mcols[d] as zeros; every wave takes 16
+// output rows and loads its A operands straight from the column-major tile (one
+// 128-byte row segment per 16 lanes and K step), all of them before the first matrix
+// core operation, so the workgroup has one barrier.  Same workgroup order and
+// results as k_tgemm_f64 mode 1 op 0 up to the summation order inside a 64-wide K.
+__global__ __launch_bounds__(256) void k_corr_f64(int nsub, int nslot, const int* __restrict__ order, int norder,
+                                                  const int* __restrict__ mrows, const int* __restrict__ mcols,
+                                                  const double* __restrict__ A, const int64_t* __restrict__ aoff,
+                                                  const double* __restrict__ C, const int64_t* __restrict__ coff,
+                                                  double* __restrict__ Y, const int64_t* __restrict__ yoff,
+                                                  const double* __restrict__ Z, const float* __restrict__ Zf, int kp,
+                                                  const int* __restrict__ gate) {
+  const int sl = blockIdx.x >> 3;
+  const int gi = (sl / nslot) * 8 + (blockIdx.x & 7);
+  const int d = order ? (gi < norder ? order[gi] : -1) : (gi < nsub ? gi : -1);
+  if (d < 0) return;
+  if (gate && *gate == 0) return;
+  const int m = mrows[d], nc = mcols[d];
+  const int i = sl % nslot;
+  if (i >= (m + 63) / 64) return;
+  const int c0 = blockIdx.z * 64;
+  __shared__ double Cs[64 * 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rb = 16 * w;
+  const double* Ai = A + aoff[d] + (int64_t)i * 4096 + rb + (lane & 15);
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) a[q] = Ai[(4 * q + (lane >> 4)) * 64];  // A[rb + l%16][4q + l/16]
+  const double* Cd = C + coff[d] + c0;
+  for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+    const int r = t >> 6, c = t & 63;
+    Cs[t] = r < nc ? Cd[(int64_t)r * kp + c] : 0.0;
+  }
+  __syncthreads();
+  doublex4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[c][r] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; q++)
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+      acc[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], Cs[(4 * q + (lane >> 4)) * 64 + 16 * cc + (lane & 15)],
+                                                     acc[cc], 0, 0, 0);
+  double* Yd = Y + yoff[d] + c0;
+  const double* Zd = Z ? Z + yoff[d] + c0 : nullptr;
+  const float* Zfd = Zf ? Zf + yoff[d] + c0 : nullptr;
+#pragma unroll
+  for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = i * 64 + rb + 4 * r + (lane >> 4);
+      const int col = 16 * cc + (lane & 15);
+      double v = acc[cc][r];
+      if (Zfd) v += (double)Zfd[(int64_t)row * kp + col];
+      else if (Zd) v += Zd[(int64_t)row * kp + col];
+      Yd[(int64_t)row * kp + col] = v;
+    }
+}
+
+hipError_t launch_corr_f64(hipStream_t st, int nsub, int maxRo, const int* mrows, const int* mcols, const double* A,
+                           const int64_t* aoff, const double* C, const int64_t* coff, double* Y, const int64_t* yoff,
+                           const double* Z, const float* Zf, int kp, const int* gate, const int* order, int norder) {
+  if (kp % 64) return hipErrorInvalidValue;
+  if (order && (norder < 8 || norder % 8)) return hipErrorInvalidValue;
+  const dim3 grid(8 * maxRo * (order ? norder / 8 : (nsub + 7) / 8), 1, kp / 64);
+  hipLaunchKernelGGL(k_corr_f64, grid, dim3(256), 0, st, nsub, maxRo, order, norder, mrows, mcols, A, aoff, C, coff, Y,
+                     yoff, Z, Zf, kp, gate);
+  return hipGetLastError();
+}
+
 // pack a compact column-major matrix (m x n, ld) into 64x64 tiles (mode 0: lower
 // tiles of a symmetric matrix, mode 1: all tiles), converting to T
 template <typename T>
