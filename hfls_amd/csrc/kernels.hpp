@@ -283,7 +283,7 @@ hipError_t launch_kb_scatter(hipStream_t st, int64_t nlam, int k, int kp, const 
                              const int* gate = nullptr,
                              const float* yf = nullptr);  // yf: FP32 b-block in place of y
 // subdomains whose h rows kb_gh stages in LDS per group of 64 dual rows
-constexpr int GH_SLOTS = 12;
+constexpr int GH_SLOTS = 8;
 hipError_t launch_kb_gh(hipStream_t st, int64_t nlam, int k, const int* rsa, const int* rsb, const int* gsub,
                         const int* rslot, const double* Gr, const double* h, const double* x, double* out, double sgn,
                         const double* dotv, double* part, const int* gate = nullptr);
