@@ -435,16 +435,34 @@ bool Server::step() {
       st_.joins += (int64_t)cols.size();
     }
   }
-  // 3. one chunk of sweeps for every mesh with work (different streams overlap)
+  // 3. sweeps for every mesh with work (different streams overlap): whole chunks until,
+  //    by the most iterations a finished case of the mesh has taken, every running
+  //    case will have stopped when cases wait for a slot (the first of them when
+  //    none waits), at most 8 chunks; one chunk while the mesh has no finished case.
+  //    Every round costs a whole-block initialisation for its joins and a recovery
+  //    for its finishes (each about as long as 5 to 9 sweeps of a full block), so
+  //    under load rounds that end when the block can change hands keep the slots
+  //    full without paying those for a few cases at a time.
   st_.join_ms += now() - tph;
   tph = now();
   for (auto& kv : mesh_) {
     ServeMesh& m = *kv.second;
     m.swept = false;
-    for (auto& s : m.slot)
-      if (s.req >= 0) m.swept = true;
+    std::vector<int64_t> rem;
+    if ((int)m.it_seen.size() != slots_) m.it_seen.assign(slots_, 0);
+    for (int j = 0; j < slots_; j++)
+      if (m.slot[j].req >= 0) {
+        m.swept = true;  // it_seen < 0: stopped, waiting for its recovery
+        if (m.it_cases > 0 && m.it_seen[j] >= 0) rem.push_back(std::max<int64_t>(1, m.it_max - m.it_seen[j]));
+      }
     if (m.swept) {
-      m.e->blk_sweeps(chunk_);
+      int nrep = 1;
+      if (!rem.empty()) {
+        std::sort(rem.begin(), rem.end());
+        const size_t q = m.queue.empty() ? 0 : rem.size() - 1;
+        nrep = (int)std::min<int64_t>(8, std::max<int64_t>(1, (rem[q] + chunk_ - 1) / chunk_));
+      }
+      for (int r = 0; r < nrep; r++) m.e->blk_sweeps(chunk_);
       m.last_used = now();
     }
   }
@@ -463,7 +481,9 @@ bool Server::step() {
     if ((int)m.waited.size() != slots_) m.waited.assign(slots_, -1);
     for (int j = 0; j < slots_; j++) {
       if (m.slot[j].req < 0) continue;
+      m.it_seen[j] = sh.it[j];
       if (sh.done[j] == 2 || sh.done[j] == 3 || sh.done[j] == 6) {
+        m.it_seen[j] = -1;
         cols.push_back(j);
         m.waited[j] = m.waited[j] < 0 ? 0 : m.waited[j] + 1;
         maxw = std::max(maxw, m.waited[j]);
@@ -474,7 +494,7 @@ bool Server::step() {
     if (cols.empty()) continue;
     // the recovery also runs over the whole block: stopped cases are recovered
     // together once a quarter of the slots have stopped, when nothing else runs,
-    // when queued cases need their slots, or after 4 chunks of waiting
+    // when queued cases need their slots, or after 4 rounds of waiting
     int nfree = 0;
     for (auto& sl : m.slot) nfree += sl.req < 0 ? 1 : 0;
     const bool need = !m.queue.empty() && nfree < std::min<int>((int)m.queue.size(), batch);
@@ -502,6 +522,8 @@ bool Server::step() {
       const int i = m.slot[j].idx;
       r.it[i] = sh.it[j];
       r.status[i] = sh.done[j];
+      if (sh.done[j] == 2) m.it_max = std::max(m.it_max, sh.it[j]), m.it_cases++;
+      m.it_seen[j] = 0;
       r.res[i] = s2.res[j];
       if (--r.remaining == 0) {
         r.t_done = t;

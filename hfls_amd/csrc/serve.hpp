@@ -62,7 +62,9 @@ struct ServeMesh {
   std::deque<ServeSlot> queue;         // cases admitted, waiting for a slot
   bool swept = false;
   double last_used = 0;
-  std::vector<int> waited;             // per slot: chunks since its case stopped (-1: not stopped)
+  std::vector<int> waited;             // per slot: rounds since its case stopped (-1: not stopped)
+  std::vector<int> it_seen;            // per slot: iterations of its case at the last sync
+  int it_max = 0, it_cases = 0;        // most iterations of a case this mesh has finished; cases
   bool no_save = false;                // its copy to the host cache failed (too large): not retried
 };
 
