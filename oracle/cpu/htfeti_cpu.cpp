@@ -38,6 +38,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -292,6 +293,25 @@ std::vector<int> null_pivots(const std::vector<double>& R, int n, int nc) {
 
 constexpr int NCHUNK = 256;  // fixed reduction chunks over dual rows
 
+// body(i) for i < n on the OpenMP threads (one index per dynamic chunk); an
+// exception of a body is rethrown after the loop, since none may leave an OpenMP
+// region
+template <class F>
+void par_for(int n, F body) {
+  bool bad = false;
+  std::string err;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n; i++) {
+    try {
+      body(i);
+    } catch (const std::exception& e) {
+#pragma omp critical(htc_err)
+      if (!bad) bad = true, err = e.what();
+    }
+  }
+  if (bad) throw std::runtime_error(err);
+}
+
 }  // namespace
 
 extern "C" {
@@ -395,15 +415,14 @@ double dot(const Problem& P, const double* a, const double* b) {
 // x_d = Bf_d^T lam (Cluster.cpp:1210-1224)
 void mult_BfT(const Problem& P, const double* lam, std::vector<std::vector<double>>& xs, const double* scale) {
   const int N = P.D.nsub;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     std::vector<double>& x = xs[d];
     std::fill(x.begin(), x.end(), 0.0);
     for (int64_t p = P.D.bf_off[d]; p < P.D.bf_off[d + 1]; p++) {
       const int64_t r = P.D.bf_row[p];
       x[P.D.bf_col[p]] += P.D.bf_val[p] * lam[r] * (scale ? scale[r] : 1.0);
     }
-  }
+  });
 }
 
 // lam = sum_d Bf_d x_d (Cluster.cpp:1193-1208), per dual row in subdomain order
@@ -422,8 +441,7 @@ void mult_Bf(const Problem& P, const std::vector<std::vector<double>>& xs, doubl
 void mult_Kplus_f(const Problem& P, const std::vector<std::vector<double>>& fs, std::vector<std::vector<double>>& xs) {
   const int N = P.D.nsub, NC = P.D.ncl;
   std::vector<std::vector<double>> la(NC);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int c = 0; c < NC; c++) {
+  par_for(NC, [&](int c) {
     const Cl& cl = P.C[c];
     std::vector<double> gc(cl.na, 0.0);
     for (int d = cl.d0; d < cl.d1; d++) {
@@ -444,9 +462,8 @@ void mult_Kplus_f(const Problem& P, const std::vector<std::vector<double>>& fs, 
     }
     cl.Ac.solve(gc.data());
     la[c] = std::move(gc);
-  }
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  });
+  par_for(N, [&](int d) {
     const Sub& s = P.S[d];
     const Cl& cl = P.C[s.cl];
     const std::vector<double>& g = la[s.cl];
@@ -462,7 +479,7 @@ void mult_Kplus_f(const Problem& P, const std::vector<std::vector<double>>& fs, 
       for (int q = 0; q < nb; q++) t -= kb[q] * g[s.bc_rows[q]];
       x[i] += t;
     }
-  }
+  });
 }
 
 struct Work {
@@ -483,14 +500,13 @@ void mult_Ff(const Problem& P, Work& W, const double* w, double* Fw) {
 void precondition(const Problem& P, Work& W, const double* w, double* z) {
   mult_BfT(P, w, W.xs, P.weight.data());
   const int N = P.D.nsub;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     const Sub& s = P.S[d];
     std::vector<double>& y = W.ys[d];
     const std::vector<double>& x = W.xs[d];
     if (P.D.lumped) {
       s.Kfull.apply(x.data(), y.data(), false);
-      continue;
+      return;
     }
     const int nb = (int)s.bdofs.size(), nr = (int)s.rdofs.size();
     std::vector<double> xb(nb), yb(nb), t(nr);
@@ -502,7 +518,7 @@ void precondition(const Problem& P, Work& W, const double* w, double* z) {
     s.Kbr.apply(t.data(), u.data(), false);
     std::fill(y.begin(), y.end(), 0.0);
     for (int q = 0; q < nb; q++) y[s.bdofs[q]] = yb[q] - u[q];
-  }
+  });
   mult_Bf(P, W.ys, z, P.weight.data());
 }
 
@@ -549,8 +565,7 @@ void setup(Problem& P, htc_times* tm) {
   P.weight.assign(D.weight, D.weight + D.nlam);
   auto t0 = clk::now();
   // ---- assembly of K (band, l2g order) and the body load
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     Sub& s = P.S[d];
     s.n = (int)(D.n_off[d + 1] - D.n_off[d]);
     const double* X = D.coords + D.n_off[d];
@@ -592,12 +607,11 @@ void setup(Problem& P, htc_times* tm) {
       }
       s.Kfull.ptr[i + 1] = (int)s.Kfull.col.size();
     }
-  }
+  });
   tm->assemble = secs(t0);
   // ---- analytic kernel (Data.cpp:61-109), fixing DOFs, K_reg factor
   auto t1 = clk::now();
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     Sub& s = P.S[d];
     const double* X = D.coords + D.n_off[d];
     s.R.assign((size_t)s.n * 6, 0.0);
@@ -621,12 +635,11 @@ void setup(Problem& P, htc_times* tm) {
     s.fix = null_pivots(s.R, s.n, 6);
     for (int p : s.fix) s.K.at(p, p) *= 2.0;
     s.K.factor();
-  }
+  });
   tm->kernel_factor = secs(t1);
   // ---- Dirichlet_implicit: K_rr factor and the K_rb, K_br, K_bb blocks
   auto t2 = clk::now();
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     Sub& s = P.S[d];
     std::vector<char> isb(s.n, 0);
     for (int64_t p = D.bf_off[d]; p < D.bf_off[d + 1]; p++) isb[D.bf_col[p]] = 1;
@@ -637,7 +650,7 @@ void setup(Problem& P, htc_times* tm) {
       else
         pos[i] = (int)s.rdofs.size(), s.rdofs.push_back(i);
     }
-    if (D.lumped) continue;
+    if (D.lumped) return;
     const int nr = (int)s.rdofs.size(), nb = (int)s.bdofs.size();
     int kd = 0;
     for (int a = 0; a < nr; a++) {
@@ -677,7 +690,7 @@ void setup(Problem& P, htc_times* tm) {
       s.Kbr.ptr.push_back((int)s.Kbr.col.size());
     }
     if (nr) s.Krr.factor();
-  }
+  });
   tm->dirichlet = secs(t2);
   // ---- cluster objects (Cluster.cpp:151-206) and the cluster matrices
   auto t3 = clk::now();
@@ -688,8 +701,7 @@ void setup(Problem& P, htc_times* tm) {
     cl.nr = 6 * (cl.d1 - cl.d0);
     for (int d = cl.d0; d < cl.d1; d++) P.S[d].cl = c, P.S[d].roff = 6 * (d - cl.d0);
   }
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     Sub& s = P.S[d];
     for (int64_t p = D.bc_off[d]; p < D.bc_off[d + 1]; p++) s.bc_rows.push_back(D.bc_row[p]);
     std::sort(s.bc_rows.begin(), s.bc_rows.end());
@@ -709,9 +721,8 @@ void setup(Problem& P, htc_times* tm) {
         for (int i = 0; i < s.n; i++) t += s.BcT[(size_t)i * nb + q] * s.R[(size_t)k * s.n + i];
         s.Gc[(size_t)q * 6 + k] = -t;
       }
-  }
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int c = 0; c < NC; c++) {
+  });
+  par_for(NC, [&](int c) {
     Cl& cl = P.C[c];
     // H: the global rigid motion k in every subdomain's R basis (R_d column k is the
     // restriction of motion k over its norm), orthonormalised
@@ -763,13 +774,12 @@ void setup(Problem& P, htc_times* tm) {
     for (int i = 0; i < nr; i++)
       for (int k = 0; k < 6; k++) A(nlc + i, nlc + nr + k) = A(nlc + nr + k, nlc + i) = cl.H[(size_t)i * 6 + k];
     cl.Ac.factor();
-  }
+  });
   tm->cluster = secs(t3);
   // ---- FETI coarse: Rf = R H, Gf = Bf Rf, (Gf^T Gf)^-1
   auto t4 = clk::now();
   P.ncol = 6 * NC;
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int d = 0; d < N; d++) {
+  par_for(N, [&](int d) {
     Sub& s = P.S[d];
     const Cl& cl = P.C[s.cl];
     s.RH.assign((size_t)s.n * 6, 0.0);
@@ -778,7 +788,7 @@ void setup(Problem& P, htc_times* tm) {
         const double h = cl.H[(size_t)(s.roff + j) * 6 + k];
         for (int i = 0; i < s.n; i++) s.RH[(size_t)k * s.n + i] += s.R[(size_t)j * s.n + i] * h;
       }
-  }
+  });
   // dual rows -> entries in subdomain order
   P.rptr.assign(D.nlam + 1, 0);
   for (int d = 0; d < N; d++)
@@ -957,10 +967,10 @@ const char* htc_last_error() { return g_err.c_str(); }
 void* htc_setup(const htc_desc* desc, int threads, htc_times* tm) {
   try {
     omp_set_num_threads(threads);
-    Problem* P = new Problem();
+    std::unique_ptr<Problem> P(new Problem());
     P->D = *desc;
     setup(*P, tm);
-    return P;
+    return P.release();
   } catch (const std::exception& e) {
     g_err = e.what();
     return nullptr;
