@@ -168,17 +168,13 @@ def fixture_error(fx, u, case):
 
 def gemm_time(mesh, gemm_kernel, k):
     """Device time of one K+_bb GEMM launch (HIP events on the library stream) and its
-    algorithmic HBM bytes.  Band form: the mean over the GEMM launches inside one
-    eager block solve of k load cases (events around each launch), the kernel as it
-    runs in the solve; also the mean of 10 back-to-back launches of the GEMM alone.
-    Row form: the back-to-back figure."""
+    algorithmic HBM bytes: the mean over the GEMM launches inside one eager block
+    solve of k load cases (events around each launch), the kernel as it runs in the
+    solve; beside it the mean of 10 back-to-back launches of the GEMM alone."""
     b2b_ms, g_bytes = mesh.time_stream(7, reps=10)
-    if gemm_kernel == "bands":
-        g_ms, _ = mesh.time_stream(9, reps=k)
-        how = ("HIP events around each K+_bb GEMM launch (band kernel + reduction) of one eager %d-case block "
-               "solve, mean over the launches that ran" % k)
-    else:
-        g_ms, how = b2b_ms, "HIP events over 10 back-to-back launches of the GEMM alone"
+    g_ms, _ = mesh.time_stream(9, reps=k)
+    how = ("HIP events around each K+_bb GEMM launch (%s) of one eager %d-case block solve, mean over the "
+           "launches that ran" % ("band kernel + reduction" if gemm_kernel == "bands" else "row form", k))
     return g_ms, g_bytes, b2b_ms, how
 
 

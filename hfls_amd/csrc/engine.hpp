@@ -174,8 +174,9 @@ class Engine {
   // per launch and the algorithmic flops of one launch (2 * 64 * sum n_s^2).
   // which = 2: a plain device copy of known size (*flops = bytes read = written).
   double time_block_gemm(int which, int reps, double* flops);
-  // mean device time of the K+_bb GEMM (band kernel and its reduction) over its
-  // launches inside one eager mixed-precision block solve of k seeded face tractions
+  // mean device time of the mixed-precision K+_bb GEMM (the band kernel and its
+  // reduction, the row form or the FP32 tile GEMM, as configured) over its launches
+  // inside one eager mixed-precision block solve of k seeded face tractions
   double time_gemm_in_solve(int k, int* nlaunch);
 
   // Block of k load cases (column major: column j of f_local at j*packed_n(), of u

@@ -461,12 +461,20 @@ void Engine::bF(int k, int prec, const double* win, double* out, const double* s
 void Engine::gemm_lo(int which, const int* gate) {
   const int N = P->nsub;
   const float* A = which == 0 ? b_Kt32.p : b_St32.p;
-  if (gemm_probe && which == 0 && gemm16 && gemm_band && gemm_probe_n + 2 <= gemm_probe->e.size()) {
-    // the column scales, then the band kernel and its reduction between two events
-    HC(launch_kb_colscale(st, N, d_ns.p, b_xoff.p, KP, b_xb32.p, b_sx.p, gate));
+  if (gemm_probe && which == 0 && gemm_probe_n + 2 <= gemm_probe->e.size()) {
+    // the column scales (split FP16), then the GEMM alone between two events: the
+    // band kernel and its reduction, the row form or the FP32 tile GEMM
+    if (gemm16) HC(launch_kb_colscale(st, N, d_ns.p, b_xoff.p, KP, b_xb32.p, b_sx.p, gate));
     HC(hipEventRecord(gemm_probe->e[gemm_probe_n++], st));
-    HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, gate,
-                         b_yb32.p, b_xob.p, n_xob, 0, b_p16.p, d_p16off.p));
+    if (gemm16 && gemm_band)
+      HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, gate,
+                           b_yb32.p, b_xob.p, n_xob, 0, b_p16.p, d_p16off.p));
+    else if (gemm16)
+      HC(launch_tgemm_h16(st, N, maxTb, d_ns.p, A, d_off_P.p, b_sK.p, b_xb32.p, b_xoff.p, b_sx.p, KP, 1.0f, gate,
+                          b_yb32.p, b_xob.p, n_xob, 32 * tune_rt32, 0, gemm_lds_pad, gemm_exact, gemm_pf));
+    else
+      HC(launch_tgemm_f32(st, N, maxTb, 0, 0, d_ns.p, d_ns.p, A, d_off_P.p, nullptr, b_xb32.p, b_xoff.p, nullptr,
+                          b_xoff.p, nullptr, KP, 1.0f, gate, tune_rt32, b_yb32.p, b_xob.p, n_xob));
     HC(hipEventRecord(gemm_probe->e[gemm_probe_n++], st));
   } else if (gemm16 && gemm_band)
     HC(launch_tgemm_h16b(st, N, maxTb, d_ns.p, A, d_off_P.p, which == 0 ? b_sK.p : b_sS.p, b_xb32.p, b_xoff.p, b_sx.p,
@@ -621,7 +629,6 @@ void Engine::apply_block(int op, int k, int prec, const double* x_h, double* y_h
 }
 
 double Engine::time_gemm_in_solve(int k, int* nlaunch) {
-  if (!(gemm16 && gemm_band)) throw std::runtime_error("the in-solve GEMM probe times the band form of the split-FP16 GEMM");
   k = std::max(1, std::min(k, KMAX));
   int64_t np = 0;
   for (const Subdomain& sd : P->subs) np += sd.n;

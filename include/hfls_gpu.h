@@ -264,10 +264,11 @@ int hfls_server_cache_stats(hfls_server s, double* out);
  * a plain device copy (reference for traffic counters), *bytes = bytes read;
  * which = 5 / 6: the sustained FP64 / FP32 matrix-core rate on register data with
  * random operands (no memory traffic), *bytes = flops of one launch; which = 9: the
- * K+_bb GEMM of the band form (band kernel and reduction) inside one mixed-precision
- * block solve of reps seeded face-traction load cases, launched eagerly with events
- * around every GEMM, *ms = mean over the launches that ran (not skipped through
- * their device gate), *bytes = algorithmic HBM bytes of one launch. */
+ * mixed-precision K+_bb GEMM (as configured: band form with its reduction, row form,
+ * or FP32 tiles) inside one mixed-precision block solve of reps seeded face-traction
+ * load cases, launched eagerly with events around every GEMM, *ms = mean over the
+ * launches that ran (not skipped through their device gate), *bytes = algorithmic
+ * HBM bytes of one launch. */
 int hfls_time_stream(hfls_mesh m, int which, int reps, double* ms, double* bytes);
 
 #ifdef __cplusplus

@@ -133,6 +133,9 @@ def test_measurement_hooks():
     assert m.time_stream(2, reps=1)[1] == sum(2.0 * 64 * n * n for n in ns)
     assert m.time_stream(7, reps=1)[1] == 4.0 * 4096 * tiles + 2 * 4.0 * 64 * sum(ns)
     assert m.time_stream(9, reps=4)[1] == m.time_stream(7, reps=1)[1]
+    m.configure(gemm_kernel="rows")  # the probe times whichever form is configured
+    assert m.time_stream(9, reps=4)[0] > 0
+    m.configure(gemm_kernel="bands")
     r1 = m.pcpg_block(F, eps=1e-8, max_iter=500, precision=32)
     assert np.array_equal(r0["u"], r1["u"]) and np.array_equal(r0["iterations"], r1["iterations"])
     with pytest.raises(hfls_amd.HflsError):
