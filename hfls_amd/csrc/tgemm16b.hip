@@ -296,6 +296,7 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
         fetch(g[ii], nxt ? i0 + ii : i0 + tii, nxt ? j + 1 : i0 + tj);
       }
       if constexpr (ii == M - 2) xj_load(xraw, min(j + 1, T - 1));  // the next column's X_j, two steps ahead
+      __builtin_amdgcn_iglp_opt(0);  // interleave the LDS operand reads with the MFMAs
       const _Float16* xi = XB + 2 * XP * ii;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
@@ -329,6 +330,7 @@ __global__ __launch_bounds__(256, 2) void k_tgemm_h16b(int nsub, int nslot, cons
     if constexpr (e + M < NTRI) fetch_tri(g[q], std::integral_constant<int, e + M>{});
     if (act) {
       const _Float16* xj = XB + 2 * XP * jj;
+      __builtin_amdgcn_iglp_opt(0);  // interleave the LDS operand reads with the MFMAs
       const _Float16* xi = XB + 2 * XP * ii;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
