@@ -49,9 +49,9 @@ SEED = 20240611
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py --kernel-only`,
 # each scaled by the counter's reading for kb_copy of known size (FETCH x2.000,
 # WRITE x1.000).  Row form (k_tgemm_h16, K chunk depth 64): 7.249e9 fetched +
-# 2.222e8 written.  Band form: k_tgemm_h16b 3.654e9 + 0.865e9, k_tgemm_h16b_reduce
+# 2.222e8 written.  Band form: k_tgemm_h16b 3.676e9 + 0.865e9, k_tgemm_h16b_reduce
 # 0.827e9 + 0.189e9.  Valid for the default workload only.
-GEMM_TRAFFIC_BYTES = {"rows": 7.47e9, "bands": 5.54e9}
+GEMM_TRAFFIC_BYTES = {"rows": 7.47e9, "bands": 5.56e9}
 GEMM_KERNEL_NAME = {
     "rows": "k_tgemm_h16 (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 columns, tiles read per row block)",
     "bands": "k_tgemm_h16b + k_tgemm_h16b_reduce (deflated K+_bb, split-FP16 packed symmetric 64x64 tiles, 64 "
