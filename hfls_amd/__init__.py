@@ -26,7 +26,7 @@ EXPORTS = [
     "hfls_mesh_tuning", "hfls_problem_write_vtk", "hfls_mesh_write_vtk", "hfls_solve_blocks",
     "hfls_server_cancel", "hfls_mult_Gf", "hfls_mult_GfT", "hfls_mult_RfT", "hfls_recover", "hfls_mesh_rf",
     "hfls_mesh_counts", "hfls_project_noalloc", "hfls_mesh_save", "hfls_mesh_restore", "hfls_server_host_cache",
-    "hfls_server_cache_stats",
+    "hfls_server_cache_stats", "hfls_host_alloc", "hfls_host_free",
 ]
 
 DEFAULTS = {
@@ -92,6 +92,34 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class _HostAlloc:
+    """Owner of one hfls_host_alloc block; frees it when the last array over it goes."""
+
+    def __init__(self, addr):
+        self.addr = addr
+
+    def __del__(self):
+        if self.addr and _lib is not None:
+            _lib.hfls_host_free(ctypes.c_void_p(self.addr))
+            self.addr = None
+
+
+def pinned_empty(shape, dtype=np.float64):
+    """Uninitialised array in page-locked host memory (hfls_host_alloc): loads and
+    results in such arrays move to and from the device as DMA at the link rate.
+    For load and result buffers that are reused across solves (pcpg_block's F and
+    out=)."""
+    shape = tuple(int(x) for x in np.atleast_1d(shape))
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    if n <= 0:
+        return np.empty(shape, dtype=dtype)
+    p = ctypes.c_void_p()
+    _check(load().hfls_host_alloc(ctypes.c_int64(n), ctypes.byref(p)))
+    buf = (ctypes.c_char * n).from_address(p.value)
+    buf._owner = _HostAlloc(p.value)  # every view's base chain ends at buf
+    return np.frombuffer(buf, dtype=dtype).reshape(shape)
 
 
 def _vec(x, n, name):
@@ -385,16 +413,25 @@ class Mesh:
     def projection_block(self, W):
         return self.apply_block(2, W, 64)
 
-    def pcpg_block(self, F=None, k=None, eps=1e-8, max_iter=200, precision=32, want_lambda=False):
+    def pcpg_block(self, F=None, k=None, eps=1e-8, max_iter=200, precision=32, want_lambda=False, out=None):
         """Block of load cases: F (k, npacked) packed local loads, or None for k
         copies of the body-force load.  Returns u (k, 3*nPoints), per-case
         iterations/status/residual and the block timings.  Status per case: 2
-        converged, 3 iteration limit, 6 stopped on a non-finite residual."""
+        converged, 3 iteration limit, 6 stopped on a non-finite residual.
+        out: a C-contiguous float64 (k, 3*nPoints) array that receives u (for
+        example a reused pinned_empty buffer); returned as "u"."""
         if F is not None:
             F = _rows(F, self.npacked, "F")
             k = F.shape[0]
         k = 1 if k is None else int(k)
-        u = np.zeros((k, 3 * self.nPoints))
+        if out is None:
+            u = np.zeros((k, 3 * self.nPoints))
+        else:
+            if (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.shape != (k, 3 * self.nPoints)
+                    or not out.flags.c_contiguous or not out.flags.writeable):
+                raise ValueError("out must be a writeable C-contiguous float64 array of shape (%d, %d)"
+                                 % (k, 3 * self.nPoints))
+            u = out
         lam = np.zeros((k, self.nlam)) if want_lambda else None
         cs = (CaseStats * k)()
         bs = BlockStats()

@@ -394,6 +394,28 @@ extern "C" int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, in
   });
 }
 
+// page-locked host memory for load and result arrays: device copies from and to it
+// are DMA transfers at the link rate, not staged through the runtime's buffers
+extern "C" int hfls_host_alloc(int64_t bytes, void** out) {
+  return guard([&] {
+    if (!out) throw std::runtime_error("hfls_host_alloc: null output pointer");
+    *out = nullptr;
+    if (bytes <= 0) throw std::runtime_error("hfls_host_alloc: bytes must be positive");
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    *out = p;
+  });
+}
+
+extern "C" int hfls_host_free(void* p) {
+  return guard([&] {
+    if (!p) return;
+    const hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipHostFree: ") + hipGetErrorString(e));
+  });
+}
+
 extern "C" int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblocks, int k, int precision,
                                  const double* f, double* u, hfls_case_stats* cases, hfls_block_stats* stats) {
   return guard([&] {

@@ -190,6 +190,13 @@ int hfls_solve_block(hfls_mesh m, double eps, int max_iter, int k, int precision
  * reference: a throughput form of hfls_solve_block). */
 int hfls_solve_blocks(hfls_mesh m, double eps, int max_iter, int nblocks, int k, int precision, const double* f_local,
                       double* u, hfls_case_stats* cases, hfls_block_stats* stats);
+/* page-locked host memory (hipHostMalloc) for the f_local and u arrays of the solve
+ * entry points: their copies to and from the device then run as DMA at the link
+ * rate instead of through the runtime's staging of pageable memory.  No counterpart
+ * in the reference (its arrays never leave the host); for callers that reuse their
+ * load and result buffers across solves.  Release with hfls_host_free. */
+int hfls_host_alloc(int64_t bytes, void** out);
+int hfls_host_free(void* p);
 /* block operators on k dual vectors (column major, ld nlam): op 0 mult_Ff
  * (Cluster.cpp:1114-1188), 1 Preconditioning (1043-1112), 2 Projection
  * (1314-1329); precision 32 or 64 selects the F_i / S_i arithmetic */

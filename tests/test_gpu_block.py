@@ -171,6 +171,26 @@ def test_same_alone_and_in_block(pair, precision):
     assert np.array_equal(rb["u"][K - 1], blk["u"][0])
 
 
+def test_block_into_pinned_buffers(pair):
+    """Loads in and displacements out of page-locked host memory (hfls_host_alloc),
+    the out buffer reused over two solves: bit-identical to pageable arrays; a
+    wrongly shaped out array is rejected before the solve."""
+    name, gpu, ora, F = pair
+    ref = gpu.pcpg_block(F, eps=1e-8, max_iter=300, precision=32)
+    Fp = hfls_amd.pinned_empty(F.shape)
+    Fp[:] = F
+    up = hfls_amd.pinned_empty((K, 3 * gpu.nPoints))
+    for rep in range(2):
+        up[:] = np.nan
+        r = gpu.pcpg_block(Fp, eps=1e-8, max_iter=300, precision=32, out=up)
+        assert r["u"] is up
+        assert np.array_equal(up, ref["u"])
+        assert np.array_equal(r["iterations"], ref["iterations"])
+    with pytest.raises(ValueError):
+        gpu.pcpg_block(Fp, eps=1e-8, max_iter=300, precision=32, out=up[:, :-1])
+    del Fp, up
+
+
 @pytest.mark.parametrize("nblocks", [2, 3])
 def test_concurrent_blocks_same_as_single(pair, nblocks):
     """Blocks solved concurrently in separate block contexts (own buffers and
