@@ -27,6 +27,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+def test_host_alloc_argument_checks():
+    """hfls_host_alloc rejects a size <= 0 and a null output pointer before it asks
+    the runtime for memory (no device needed); freeing a null pointer is a no-op."""
+    lib = hfls_amd.load()
+    p = ctypes.c_void_p(1)
+    assert lib.hfls_host_alloc(ctypes.c_int64(0), ctypes.byref(p)) != 0
+    assert b"positive" in lib.hfls_last_error()
+    assert p.value is None
+    assert lib.hfls_host_alloc(ctypes.c_int64(64), None) != 0
+    assert b"null" in lib.hfls_last_error()
+    assert lib.hfls_host_free(None) == 0
+
+
 def test_unknown_option_is_an_error():
     with pytest.raises(hfls_amd.HflsError, match="unknown option"):
         hfls_amd.Problem(not_an_option="1")
