@@ -273,6 +273,7 @@ __global__ __launch_bounds__(256) void k_tgemm_f32(int mode, int op, int nsub, i
   for (int c = 0; c < nch; c++) {
     const bool more = c + 1 < nch;
     if (more) fetch(c + 1);
+    __builtin_amdgcn_iglp_opt(0);  // interleave the LDS operand reads with the MFMAs
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 2) {
       const float b = Xs[(kk + (lane >> 5)) * 64 + cb + (lane & 31)];
